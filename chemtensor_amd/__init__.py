@@ -1,0 +1,319 @@
+"""chemtensor_amd: MI355X drop-in for chemtensor's two-site DMRG matvec path.
+
+The product is the C-ABI library libchemtensor_gpu.so (include/chemtensor_gpu.h). This module
+is a thin ctypes front end for Python callers (tests, bench); it mirrors the names and argument
+meaning of the reference's C API. It refuses to run without the compiled HIP library: there is
+no CPU fallback.
+"""
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libchemtensor_gpu.so")
+REPO = os.path.dirname(HERE)
+HEADER = os.path.join(REPO, "include", "chemtensor_gpu.h")
+
+ct_long = C.c_int64
+P = C.c_void_p
+DP = C.POINTER(C.c_double)
+LP = C.POINTER(C.c_int64)
+IP = C.POINTER(C.c_int32)
+FP = C.POINTER(C.c_float)
+
+LEADING, TRAILING = 0, 1
+AXIS_IN, AXIS_OUT = -1, 1
+PLAIN_FP64, GROUPED_FP64, GROUPED_SPLIT = 0, 1, 2
+
+ERRORS = {-1: "Krylov breakdown / factorisation failure", -2: "eigensolver failure", -100: "HIP runtime error",
+          -101: "unsupported input", -102: "out of device memory / budget", -103: "invalid argument"}
+
+
+class DenseTensor(C.Structure):
+    _fields_ = [("data", P), ("dim", LP), ("dtype", C.c_int), ("ndim", C.c_int)]
+
+
+class BlockSparseTensor(C.Structure):
+    _fields_ = [("blocks", C.POINTER(C.POINTER(DenseTensor))), ("dim_blocks", LP), ("dim_logical", LP),
+                ("axis_dir", IP), ("qnums_blocks", C.POINTER(IP)), ("qnums_logical", C.POINTER(IP)),
+                ("dtype", C.c_int), ("ndim", C.c_int)]
+
+
+class MPSStruct(C.Structure):
+    _fields_ = [("a", C.POINTER(BlockSparseTensor)), ("qsite", IP), ("d", ct_long), ("nsites", C.c_int)]
+
+
+MPOStruct = MPSStruct
+
+_lib = None
+
+
+def lib():
+    """The compiled HIP library; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with chemtensor_amd.build.build() (there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        bst = C.POINTER(BlockSparseTensor)
+        sig = {
+            "allocate_block_sparse_tensor": (None, [C.c_int, C.c_int, LP, IP, C.POINTER(IP), bst]),
+            "delete_block_sparse_tensor": (None, [bst]),
+            "copy_block_sparse_tensor": (None, [P, bst]),
+            "block_sparse_tensor_num_elements_blocks": (ct_long, [P]),
+            "block_sparse_tensor_serialize_entries": (None, [P, P]),
+            "block_sparse_tensor_deserialize_entries": (None, [P, P]),
+            "block_sparse_tensor_norm2": (C.c_double, [P]),
+            "block_sparse_tensor_dot": (None, [P, C.c_int, P, C.c_int, C.c_int, bst]),
+            "block_sparse_tensor_transpose": (None, [IP, P, bst]),
+            "apply_local_hamiltonian": (None, [P, P, P, P, bst]),
+            "mpo_merge_tensor_pair": (None, [P, P, bst]),
+            "ct_matvec_plan_create": (C.c_int, [P, P, P, P, C.c_int, C.POINTER(P)]),
+            "ct_matvec_plan_destroy": (C.c_int, [P]),
+            "ct_matvec_plan_dim": (ct_long, [P]),
+            "ct_matvec_plan_flops": (C.c_double, [P]),
+            "ct_matvec_plan_bytes": (C.c_double, [P]),
+            "ct_matvec_plan_stats": (C.c_int, [P, DP]),
+            "ct_matvec_plan_stage_flops": (C.c_int, [P, DP]),
+            "ct_matvec_apply": (C.c_int, [P, P, P]),
+            "ct_matvec_apply_timed": (C.c_int, [P, P, P, FP]),
+            "ct_matvec_plan_stream": (P, [P]),
+            "ct_lanczos_ground_state": (C.c_int, [P, P, C.c_int, DP, P]),
+            "ct_device_alloc": (C.c_int, [C.POINTER(P), ct_long]),
+            "ct_device_free": (C.c_int, [P]),
+            "ct_memcpy_h2d": (C.c_int, [P, P, ct_long]),
+            "ct_memcpy_d2h": (C.c_int, [P, P, ct_long]),
+            "ct_device_synchronize": (C.c_int, []),
+            "ct_device_count": (C.c_int, []),
+            "ct_event_create": (C.c_int, [C.POINTER(P)]),
+            "ct_event_record": (C.c_int, [P, P]),
+            "ct_event_elapsed_ms": (C.c_int, [P, P, FP]),
+            "ct_event_destroy": (C.c_int, [P]),
+            "ct_stream_synchronize": (C.c_int, [P]),
+            "ct_generate_matvec_problem": (C.c_int, [C.c_int, C.c_int, ct_long, C.c_uint64, bst, bst, bst, bst]),
+            "ct_construct_heisenberg_xxz_1d_mpo": (None, [C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(MPOStruct)]),
+            "ct_construct_fermi_hubbard_1d_mpo": (None, [C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(MPOStruct)]),
+            "delete_mpo": (None, [C.POINTER(MPOStruct)]),
+            "delete_mps": (None, [C.POINTER(MPSStruct)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(ret, what):
+    if ret != 0:
+        raise RuntimeError(f"{what} failed: {ERRORS.get(ret, ret)}")
+
+
+def gpu_available():
+    try:
+        return lib().ct_device_count() > 0
+    except (OSError, RuntimeError):
+        return False
+
+
+def _ptr(t):
+    """Address of a block-sparse tensor struct (product HostTensor or any object with .ptr)."""
+    if isinstance(t, HostTensor):
+        return C.cast(C.pointer(t.s), P)
+    return t.ptr
+
+
+class HostTensor:
+    """A struct block_sparse_tensor allocated by the product library (freed with its delete)."""
+
+    def __init__(self):
+        self.s = BlockSparseTensor()
+        self.live = False
+
+    @property
+    def ptr(self):
+        return C.cast(C.pointer(self.s), P)
+
+    @property
+    def sptr(self):
+        return C.pointer(self.s)
+
+    def __del__(self):
+        if self.live and _lib is not None:
+            _lib.delete_block_sparse_tensor(C.pointer(self.s))
+            self.live = False
+
+    @staticmethod
+    def zeros(dims, axis_dir, qnums):
+        t = HostTensor()
+        nd = len(dims)
+        dims = np.asarray(dims, dtype=np.int64)
+        ad = np.asarray(axis_dir, dtype=np.int32)
+        qarrs = [np.ascontiguousarray(q, dtype=np.int32) for q in qnums]
+        qp = (IP * nd)(*[q.ctypes.data_as(IP) for q in qarrs])
+        lib().allocate_block_sparse_tensor(1, nd, dims.ctypes.data_as(LP), ad.ctypes.data_as(IP), qp, t.sptr)
+        t.live = True
+        return t
+
+    def nelem(self):
+        return lib().block_sparse_tensor_num_elements_blocks(self.ptr)
+
+    def serialize(self):
+        out = np.zeros(self.nelem())
+        lib().block_sparse_tensor_serialize_entries(self.ptr, out.ctypes.data_as(P))
+        return out
+
+    def deserialize(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.size == self.nelem()
+        lib().block_sparse_tensor_deserialize_entries(self.ptr, v.ctypes.data_as(P))
+
+    def meta(self):
+        nd = self.s.ndim
+        dims = [int(self.s.dim_logical[i]) for i in range(nd)]
+        ad = [int(self.s.axis_dir[i]) for i in range(nd)]
+        qn = [np.array([self.s.qnums_logical[i][j] for j in range(dims[i])], dtype=np.int32) for i in range(nd)]
+        return dims, ad, qn
+
+
+def dot(s, axs, t, axt, ndim_mult):
+    r = HostTensor()
+    lib().block_sparse_tensor_dot(_ptr(s), axs, _ptr(t), axt, ndim_mult, r.sptr)
+    r.live = True
+    return r
+
+
+def transpose(perm, t):
+    r = HostTensor()
+    p = np.asarray(perm, dtype=np.int32)
+    lib().block_sparse_tensor_transpose(p.ctypes.data_as(IP), _ptr(t), r.sptr)
+    r.live = True
+    return r
+
+
+def apply_local_hamiltonian(a, w, l, r):
+    b = HostTensor()
+    lib().apply_local_hamiltonian(_ptr(a), _ptr(w), _ptr(l), _ptr(r), b.sptr)
+    b.live = True
+    return b
+
+
+def mpo_merge_tensor_pair(a0, a1):
+    r = HostTensor()
+    lib().mpo_merge_tensor_pair(_ptr(a0), _ptr(a1), r.sptr)
+    r.live = True
+    return r
+
+
+def generate_matvec_problem(model, nsites, chi, seed):
+    """theta, h2, left, right of the synthetic H_eff matvec at the centre bond pair (model 'xxz' or 'hubbard')."""
+    ts = [HostTensor() for _ in range(4)]
+    check(lib().ct_generate_matvec_problem(0 if model == "xxz" else 1, nsites, chi, seed, *[t.sptr for t in ts]), "ct_generate_matvec_problem")
+    for t in ts:
+        t.live = True
+    return tuple(ts)
+
+
+class DeviceArray:
+    """A device buffer of float64 values owned by Python (hipMalloc via the library)."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        self.ptr = P()
+        check(lib().ct_device_alloc(C.byref(self.ptr), max(self.n, 1) * 8), "ct_device_alloc")
+
+    @staticmethod
+    def from_numpy(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = DeviceArray(a.size)
+        check(lib().ct_memcpy_h2d(d.ptr, a.ctypes.data_as(P), a.size * 8), "ct_memcpy_h2d")
+        return d
+
+    def numpy(self):
+        out = np.zeros(self.n)
+        check(lib().ct_memcpy_d2h(out.ctypes.data_as(P), self.ptr, self.n * 8), "ct_memcpy_d2h")
+        return out
+
+    def __del__(self):
+        if self.ptr and _lib is not None:
+            _lib.ct_device_free(self.ptr)
+            self.ptr = None
+
+
+class MatvecPlan:
+    """Device-resident H_eff plan (ct_matvec_plan_*): w, l, r stay on the GPU."""
+
+    def __init__(self, a, w, l, r, mode=GROUPED_FP64):
+        self.h = P()
+        check(lib().ct_matvec_plan_create(_ptr(a), _ptr(w), _ptr(l), _ptr(r), mode, C.byref(self.h)), "ct_matvec_plan_create")
+        self.mode = mode
+
+    def __del__(self):
+        if self.h and _lib is not None:
+            _lib.ct_matvec_plan_destroy(self.h)
+            self.h = None
+
+    @property
+    def dim(self):
+        return lib().ct_matvec_plan_dim(self.h)
+
+    @property
+    def flops(self):
+        return lib().ct_matvec_plan_flops(self.h)
+
+    @property
+    def bytes(self):
+        return lib().ct_matvec_plan_bytes(self.h)
+
+    def stats(self):
+        s = np.zeros(4)
+        lib().ct_matvec_plan_stats(self.h, s.ctypes.data_as(DP))
+        return dict(terms=int(s[0]), tasks=int(s[1]), items=int(s[2]), intermediate_doubles=int(s[3]))
+
+    def stage_flops(self):
+        s = np.zeros(3)
+        lib().ct_matvec_plan_stage_flops(self.h, s.ctypes.data_as(DP))
+        return s
+
+    @property
+    def stream(self):
+        return lib().ct_matvec_plan_stream(self.h)
+
+    def apply(self, x, y):
+        check(lib().ct_matvec_apply(self.h, x.ptr, y.ptr), "ct_matvec_apply")
+
+    def apply_timed(self, x, y):
+        ms = np.zeros(3, dtype=np.float32)
+        check(lib().ct_matvec_apply_timed(self.h, x.ptr, y.ptr, ms.ctypes.data_as(FP)), "ct_matvec_apply_timed")
+        return ms
+
+    def synchronize(self):
+        check(lib().ct_stream_synchronize(self.stream), "ct_stream_synchronize")
+
+    def lanczos(self, vstart, maxiter):
+        lam = C.c_double(0)
+        u = DeviceArray(self.dim)
+        ret = lib().ct_lanczos_ground_state(self.h, vstart.ptr, maxiter, C.byref(lam), u.ptr)
+        check(ret, "ct_lanczos_ground_state")
+        return lam.value, u
+
+
+class Event:
+    def __init__(self):
+        self.e = P()
+        check(lib().ct_event_create(C.byref(self.e)), "ct_event_create")
+
+    def record(self, stream):
+        check(lib().ct_event_record(self.e, stream), "ct_event_record")
+
+    def elapsed_ms(self, later):
+        ms = C.c_float(0)
+        check(lib().ct_event_elapsed_ms(self.e, later.e, C.byref(ms)), "ct_event_elapsed_ms")
+        return ms.value
+
+    def __del__(self):
+        if self.e and _lib is not None:
+            _lib.ct_event_destroy(self.e)
+            self.e = None

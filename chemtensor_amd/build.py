@@ -1,0 +1,64 @@
+"""Build libchemtensor_gpu.so in-tree for gfx950 with hipcc (no JIT cache, no torch).
+
+    python -m chemtensor_amd.build        (or chemtensor_amd.build.build())
+"""
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libchemtensor_gpu.so")
+
+HIP_SOURCES = ["kernels.hip", "runtime.hip", "engine.hip", "lanczos.hip", "dmrg.hip", "capi.hip"]
+CPP_SOURCES = ["host_tensor.cpp", "host_linalg.cpp", "host_models.cpp"]
+
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
+# gfx950 device code; packed FP32 instructions are disabled for all HIP code
+HIP_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def hipcc():
+    for c in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _compile(src):
+    path = os.path.join(CSRC, src)
+    obj = os.path.join(BUILD, src.replace(".", "_") + ".o")
+    cmd = [hipcc(), "-c", path, "-o", obj] + COMMON
+    if src.endswith(".hip"):
+        cmd += HIP_FLAGS
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return obj
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = [s for s in HIP_SOURCES + CPP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    try:
+        with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+            objs = list(ex.map(_compile, srcs))
+    except subprocess.CalledProcessError as e:
+        sys.stderr.write(e.stderr or "")
+        raise
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "-shared", "-o", tmp] + objs + ["--offload-arch=gfx950", "-Wl,-Bsymbolic", "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stderr)
+        raise RuntimeError("link failed")
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)

@@ -1,0 +1,249 @@
+// extern "C" entry points of libchemtensor_gpu (declared in include/chemtensor_gpu.h).
+
+#include <cstring>
+#include <memory>
+
+#include "ct_internal.h"
+
+using namespace ctg;
+
+struct ct_matvec_plan
+{
+	std::unique_ptr<MatvecPlan> p;
+};
+
+// void entry points mirror the reference's contract checks (assert): a failure is fatal
+[[noreturn]] static void fatal(const char* fn, const char* what)
+{
+	fprintf(stderr, "chemtensor_gpu: %s failed: %s\n", fn, what);
+	abort();
+}
+
+extern "C" {
+
+void apply_local_hamiltonian(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l, const struct block_sparse_tensor* r, struct block_sparse_tensor* b)
+{
+	try {
+		std::unique_ptr<MatvecPlan> p(build_matvec_plan(a, w, l, r, CT_MATVEC_GROUPED_FP64));
+		DevTensor x = upload(a, p->stream);
+		// result axes: l axis 3, w axis 1, r axis 2 (chain_ops.c:388); equal to a's axes here
+		DevTensor y(p->theta_meta);
+		matvec_apply(*p, x.d, y.d, nullptr);
+		download(y, b, p->stream);
+	}
+	catch (const HipError& e) {
+		fatal("apply_local_hamiltonian", e.where.c_str());
+	}
+}
+
+void block_sparse_tensor_dot(const struct block_sparse_tensor* s, const enum tensor_axis_range axrange_s, const struct block_sparse_tensor* t, const enum tensor_axis_range axrange_t, const int ndim_mult, struct block_sparse_tensor* r)
+{
+	try {
+		hipStream_t st = default_stream();
+		DevTensor ds = upload(s, st), dt = upload(t, st);
+		DevTensor dr = dev_dot(ds, axrange_s, dt, axrange_t, ndim_mult, DotForm::Grouped, st);
+		download(dr, r, st);
+	}
+	catch (const HipError& e) {
+		fatal("block_sparse_tensor_dot", e.where.c_str());
+	}
+}
+
+void block_sparse_tensor_transpose(const int* perm, const struct block_sparse_tensor* t, struct block_sparse_tensor* r)
+{
+	try {
+		hipStream_t st = default_stream();
+		DevTensor dt = upload(t, st);
+		DevTensor dr = dev_transpose(perm, dt, st);
+		download(dr, r, st);
+	}
+	catch (const HipError& e) {
+		fatal("block_sparse_tensor_transpose", e.where.c_str());
+	}
+}
+
+void mpo_merge_tensor_pair(const struct block_sparse_tensor* a0, const struct block_sparse_tensor* a1, struct block_sparse_tensor* a)
+{
+	try {
+		hipStream_t st = default_stream();
+		DevTensor d0 = upload(a0, st), d1 = upload(a1, st);
+		DevTensor t = dev_dot(d0, TENSOR_AXIS_RANGE_TRAILING, d1, TENSOR_AXIS_RANGE_LEADING, 1, DotForm::Grouped, st);
+		const int perm[6] = { 0, 1, 3, 2, 4, 5 };
+		DevTensor tp = dev_transpose(perm, t, st);
+		DevTensor f1 = dev_flatten_axes(tp, 1, TENSOR_AXIS_OUT, st);
+		DevTensor f2 = dev_flatten_axes(f1, 2, TENSOR_AXIS_IN, st);
+		download(f2, a, st);
+	}
+	catch (const HipError& e) {
+		fatal("mpo_merge_tensor_pair", e.where.c_str());
+	}
+}
+
+int ct_matvec_plan_create(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l, const struct block_sparse_tensor* r,
+	const int mode, ct_matvec_plan** plan)
+{
+	if (!a || !w || !l || !r || !plan || mode < 0 || mode > 2) return CT_ERR_ARG;
+	try {
+		auto* h = new ct_matvec_plan();
+		h->p.reset(build_matvec_plan(a, w, l, r, mode));
+		*plan = h;
+		return 0;
+	}
+	catch (const HipError& e) {
+		return (e.code == hipErrorNotSupported) ? CT_ERR_UNSUPPORTED : (e.code == hipErrorInvalidValue ? CT_ERR_ARG : CT_ERR_HIP);
+	}
+}
+
+int ct_matvec_plan_destroy(ct_matvec_plan* plan)
+{
+	delete plan;
+	return 0;
+}
+
+ct_long ct_matvec_plan_dim(const ct_matvec_plan* plan) { return plan->p->theta_meta.nelem; }
+double ct_matvec_plan_flops(const ct_matvec_plan* plan) { return plan->p->flops; }
+double ct_matvec_plan_bytes(const ct_matvec_plan* plan) { return plan->p->bytes; }
+
+int ct_matvec_plan_stage_flops(const ct_matvec_plan* plan, double* flops3)
+{
+	const MatvecPlan& p = *plan->p;
+	for (int i = 0; i < 3; i++) flops3[i] = p.stage_flops[i];
+	return 0;
+}
+
+int ct_matvec_plan_stats(const ct_matvec_plan* plan, double* stats4)
+{
+	const MatvecPlan& p = *plan->p;
+	stats4[0] = p.nterms;
+	stats4[1] = p.ntasks;
+	stats4[2] = p.nitems;
+	stats4[3] = p.interm_elems;
+	return 0;
+}
+
+int ct_matvec_apply(ct_matvec_plan* plan, const double* d_x, double* d_y)
+{
+	try {
+		matvec_apply(*plan->p, d_x, d_y, nullptr);
+		return 0;
+	}
+	catch (const HipError&) {
+		return CT_ERR_HIP;
+	}
+}
+
+int ct_matvec_apply_timed(ct_matvec_plan* plan, const double* d_x, double* d_y, float* stage_ms3)
+{
+	try {
+		matvec_apply(*plan->p, d_x, d_y, stage_ms3);
+		return 0;
+	}
+	catch (const HipError&) {
+		return CT_ERR_HIP;
+	}
+}
+
+void* ct_matvec_plan_stream(ct_matvec_plan* plan) { return (void*)plan->p->stream; }
+
+int ct_lanczos_ground_state(ct_matvec_plan* plan, const double* d_vstart, const int maxiter, double* lambda, double* d_u)
+{
+	try {
+		return lanczos_ground_state(*plan->p, d_vstart, maxiter, lambda, d_u);
+	}
+	catch (const HipError&) {
+		return CT_ERR_HIP;
+	}
+}
+
+int ct_device_alloc(void** ptr, const ct_long nbytes) { return hipMalloc(ptr, (size_t)nbytes) == hipSuccess ? 0 : CT_ERR_NOMEM; }
+int ct_device_free(void* ptr) { return hipFree(ptr) == hipSuccess ? 0 : CT_ERR_HIP; }
+int ct_memcpy_h2d(void* dst, const void* src, const ct_long nbytes) { return hipMemcpy(dst, src, (size_t)nbytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : CT_ERR_HIP; }
+int ct_memcpy_d2h(void* dst, const void* src, const ct_long nbytes) { return hipMemcpy(dst, src, (size_t)nbytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : CT_ERR_HIP; }
+int ct_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : CT_ERR_HIP; }
+
+int ct_device_count(void)
+{
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess) { (void)hipGetLastError(); return 0; }
+	return n;
+}
+
+int ct_event_create(void** ev)
+{
+	hipEvent_t e;
+	if (hipEventCreate(&e) != hipSuccess) return CT_ERR_HIP;
+	*ev = (void*)e;
+	return 0;
+}
+
+int ct_event_record(void* ev, void* stream) { return hipEventRecord((hipEvent_t)ev, (hipStream_t)stream) == hipSuccess ? 0 : CT_ERR_HIP; }
+
+int ct_event_elapsed_ms(void* ev0, void* ev1, float* ms)
+{
+	if (hipEventSynchronize((hipEvent_t)ev1) != hipSuccess) return CT_ERR_HIP;
+	return hipEventElapsedTime(ms, (hipEvent_t)ev0, (hipEvent_t)ev1) == hipSuccess ? 0 : CT_ERR_HIP;
+}
+
+int ct_event_destroy(void* ev) { return hipEventDestroy((hipEvent_t)ev) == hipSuccess ? 0 : CT_ERR_HIP; }
+
+int ct_stream_synchronize(void* stream) { return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? 0 : CT_ERR_HIP; }
+
+}  // extern "C"
+
+extern "C" {
+
+void ct_fill_random_normal(struct block_sparse_tensor* t, double alpha, struct rng_state* rng);
+void ct_random_bond_qnums(const int nsites, const ct_long d, const qnumber* qsite, const qnumber sector, const ct_long max_vdim, const uint64_t seed, ct_long* dims_out, qnumber* qbonds_out);
+
+int ct_generate_matvec_problem(const int model, const int nsites, const ct_long chi, const uint64_t seed,
+	struct block_sparse_tensor* theta, struct block_sparse_tensor* h2, struct block_sparse_tensor* left, struct block_sparse_tensor* right)
+{
+	if (nsites < 4 || chi < 1 || (model != 0 && model != 1)) return CT_ERR_ARG;
+	struct mpo H;
+	qnumber sector;
+	if (model == 0) { ct_construct_heisenberg_xxz_1d_mpo(nsites, 1.0, 0.5, 0.0, &H); sector = 0; }
+	else { ct_construct_fermi_hubbard_1d_mpo(nsites, 1.0, 4.0, 0.0, &H); sector = nsites * 65536; }
+	const ct_long d = H.d;
+	std::vector<ct_long> dims((size_t)nsites + 1);
+	std::vector<qnumber> qall((size_t)(nsites + 1) * (size_t)chi + 8);
+	ct_random_bond_qnums(nsites, d, H.qsite, sector, chi, seed, dims.data(), qall.data());
+	std::vector<std::vector<qnumber>> qb((size_t)nsites + 1);
+	ct_long off = 0;
+	for (int l = 0; l <= nsites; l++) { qb[l].assign(qall.begin() + off, qall.begin() + off + dims[l]); off += dims[l]; }
+	const int i = nsites / 2 - 1;
+	std::vector<qnumber> qp((size_t)(d * d));
+	for (ct_long a = 0; a < d; a++) for (ct_long b = 0; b < d; b++) qp[a * d + b] = H.qsite[a] + H.qsite[b];
+	struct rng_state rng;
+	{
+		const ct_long dim[3] = { dims[i], d * d, dims[i + 2] };
+		const tensor_axis_direction ad[3] = { TENSOR_AXIS_OUT, TENSOR_AXIS_OUT, TENSOR_AXIS_IN };
+		const qnumber* qn[3] = { qb[i].data(), qp.data(), qb[i + 2].data() };
+		allocate_block_sparse_tensor(CT_DOUBLE_REAL, 3, dim, ad, qn, theta);
+		seed_rng_state(seed + 1, &rng);
+		ct_fill_random_normal(theta, 1.0, &rng);
+	}
+	mpo_merge_tensor_pair(&H.a[i], &H.a[i + 1], h2);
+	const qnumber zero[1] = { 0 };
+	{
+		const block_sparse_tensor* wl = &H.a[i];
+		const ct_long dim[4] = { 1, dims[i], wl->dim_logical[0], dims[i] };
+		const tensor_axis_direction ad[4] = { TENSOR_AXIS_OUT, TENSOR_AXIS_IN, TENSOR_AXIS_IN, TENSOR_AXIS_OUT };
+		const qnumber* qn[4] = { zero, qb[i].data(), wl->qnums_logical[0], qb[i].data() };
+		allocate_block_sparse_tensor(CT_DOUBLE_REAL, 4, dim, ad, qn, left);
+		seed_rng_state(seed + 2, &rng);
+		ct_fill_random_normal(left, 1.0, &rng);
+	}
+	{
+		const block_sparse_tensor* wr = &H.a[i + 1];
+		const ct_long dim[4] = { dims[i + 2], wr->dim_logical[3], dims[i + 2], 1 };
+		const tensor_axis_direction ad[4] = { TENSOR_AXIS_OUT, TENSOR_AXIS_OUT, TENSOR_AXIS_IN, TENSOR_AXIS_IN };
+		const qnumber* qn[4] = { qb[i + 2].data(), wr->qnums_logical[3], qb[i + 2].data(), zero };
+		allocate_block_sparse_tensor(CT_DOUBLE_REAL, 4, dim, ad, qn, right);
+		seed_rng_state(seed + 3, &rng);
+		ct_fill_random_normal(right, 1.0, &rng);
+	}
+	delete_mpo(&H);
+	return 0;
+}
+
+}  // extern "C"

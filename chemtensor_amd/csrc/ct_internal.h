@@ -1,0 +1,234 @@
+// Internal declarations of libchemtensor_gpu: host-side block bookkeeping, device tensors,
+// grouped-GEMM descriptors and the matvec plan. Host code is C++17; device code is HIP for
+// gfx950 (64-wide wavefronts, FP64/FP32 MFMA).
+
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "../../include/chemtensor_gpu.h"
+
+namespace ctg {
+
+//________________________________________________________________________________________________________________________
+//
+// errors
+
+struct HipError
+{
+	hipError_t code;
+	std::string where;
+};
+
+void hip_check(hipError_t e, const char* what, const char* file, int line);
+#define CTG_HIP(x) ::ctg::hip_check((x), #x, __FILE__, __LINE__)
+
+//________________________________________________________________________________________________________________________
+//
+// block bookkeeping (the reference's allocate_block_sparse_tensor, bst.c:37-150)
+
+struct Meta
+{
+	int ndim = 0;
+	std::vector<ct_long> dim_logical;
+	std::vector<int> axis_dir;
+	std::vector<std::vector<qnumber>> qnums_logical;
+	std::vector<std::vector<qnumber>> qnums_blocks;   // sorted ascending, distinct
+	std::vector<std::vector<ct_long>> block_count;    // multiplicity of each block quantum number
+	std::vector<ct_long> dim_blocks;
+	ct_long nblocks = 1;                               // prod(dim_blocks) (1 for ndim 0)
+	std::vector<ct_long> offset;                       // per flat block index: packed offset or -1
+	ct_long nelem = 0;                                 // number of stored entries
+
+	bool present(ct_long k) const { return offset[k] >= 0; }
+	void block_index(ct_long k, ct_long* idx) const;
+	ct_long flat_index(const ct_long* idx) const;
+	void block_dims(ct_long k, ct_long* dims) const;
+	ct_long block_size(ct_long k) const;
+	ct_long find_qnum(int ax, qnumber q) const;
+};
+
+Meta make_meta(int ndim, const ct_long* dim, const int* axis_dir, const qnumber* const* qnums);
+Meta meta_of(const block_sparse_tensor* t);
+/// logical indices (in order) whose quantum number equals the block quantum number 'q' on axis 'ax'
+std::vector<ct_long> fanout(const Meta& m, int ax, qnumber q);
+
+//________________________________________________________________________________________________________________________
+//
+// device memory: caching allocator and device tensors
+
+void* dev_alloc(size_t nbytes);
+void dev_free(void* p);
+void dev_trim();
+size_t dev_bytes_in_use();
+
+struct DevTensor
+{
+	Meta meta;
+	double* d = nullptr;   // packed blocks in serialize order
+
+	DevTensor() = default;
+	explicit DevTensor(Meta m);
+	DevTensor(const DevTensor&) = delete;
+	DevTensor& operator=(const DevTensor&) = delete;
+	DevTensor(DevTensor&& o) noexcept;
+	DevTensor& operator=(DevTensor&& o) noexcept;
+	~DevTensor();
+	void zero(hipStream_t s);
+};
+
+hipStream_t default_stream();
+DevTensor upload(const block_sparse_tensor* t, hipStream_t s);
+void download(const DevTensor& t, block_sparse_tensor* out, hipStream_t s);
+
+//________________________________________________________________________________________________________________________
+//
+// grouped GEMM descriptors
+//
+// A task is one output matrix C (M x N) = sum over its terms of A_t (M x K_t) B_t (K_t x N)
+// (or C += ... when 'accumulate'). Column j of a task is split as (jb, jc) = (j / nci, j % nci);
+// addresses:  A(i, k) = A[i sa_i + k sa_k],  B(k, j) = B[jb sb_batch + jc sb_j + k sb_k],
+//             C(i, j) = C[jb sc_batch + jc + i sc_i].
+
+struct GemmTerm
+{
+	const double* A;
+	const double* B;
+	int64_t sb_batch;
+	int32_t K;
+	int32_t sa_i;
+	int32_t sa_k;
+	int32_t sb_k;
+	int32_t sb_j;
+	int32_t pad;
+};
+
+struct GemmTask
+{
+	double* C;
+	int64_t sc_batch;
+	int32_t M;
+	int32_t N;
+	int32_t nci;
+	int32_t sc_i;
+	int32_t t0;
+	int32_t t1;
+	int32_t accumulate;
+	int32_t pad;
+};
+
+struct WorkItem
+{
+	int32_t task;
+	int32_t m0;
+	int32_t n0;
+	int32_t cost;
+};
+
+// tile edge of one wavefront's output tile (2 x 2 MFMA 16x16 sub-tiles)
+constexpr int TILE = 32;
+
+/// host-built list of tasks/terms for one grouped launch
+struct GemmBatch
+{
+	std::vector<GemmTask> tasks;
+	std::vector<GemmTerm> terms;
+	double flops = 0;
+	void add_task(double* C, int M, int N, int nci, int sc_i, int64_t sc_batch, bool accumulate);
+	void add_term(const double* A, const double* B, int K, int sa_i, int sa_k, int sb_k, int sb_j, int64_t sb_batch);
+	std::vector<WorkItem> work_items() const;
+};
+
+/// a GemmBatch resident on the device
+struct DevGemm
+{
+	GemmTask* tasks = nullptr;
+	GemmTerm* terms = nullptr;
+	WorkItem* items = nullptr;
+	int ntasks = 0, nterms = 0, nitems = 0;
+	int grid = 0;
+	double flops = 0;
+	void upload(const GemmBatch& b, hipStream_t s);
+	void release();
+};
+
+// Operand pointers in tasks/terms are byte offsets added to per-launch base pointers
+// (absolute addresses when the base is null), so one descriptor list serves any input vector.
+struct GemmBases { const double* A; const double* B; double* C; };
+
+// kernel launchers (kernels.hip)
+void launch_grouped_gemm_f64(const DevGemm& g, GemmBases b, hipStream_t s);
+void launch_grouped_gemm_split(const DevGemm& g, GemmBases b, hipStream_t s);
+void launch_plain_gemm_f64(const GemmTask& task, const GemmTerm& term, GemmBases b, hipStream_t s);
+void launch_permute(const double* src, double* dst, int ndim, const ct_long* dims, const int* perm, hipStream_t s);
+struct PermBlock { int64_t src; int64_t dst; int64_t n; int64_t ddim[6]; int64_t sstride[6]; int32_t ndim; int32_t pad; };
+void launch_permute_blocks(const double* src, double* dst, const PermBlock* d_blocks, int nblocks, hipStream_t s);
+struct CopySeg { int64_t src; int64_t dst; int64_t len; };
+void launch_copy_segments(const double* src, double* dst, const CopySeg* d_segs, int nsegs, hipStream_t s);
+void launch_fill(double* p, double v, ct_long n, hipStream_t s);
+int num_compute_units();
+
+//________________________________________________________________________________________________________________________
+//
+// device tensor algebra (engine.hip)
+
+enum class DotForm { Grouped, Plain };
+
+/// r = s . t (bst.c:1826-2000) with r's block table planned on the host
+DevTensor dev_dot(const DevTensor& s, int axrange_s, const DevTensor& t, int axrange_t, int ndim_mult, DotForm form, hipStream_t st, double* flops = nullptr);
+DevTensor dev_transpose(const int* perm, const DevTensor& t, hipStream_t st);
+DevTensor dev_cyclic_partial_trace(const DevTensor& t, int ndim_trace, hipStream_t st);
+DevTensor dev_flatten_axes(const DevTensor& t, int i_ax, int new_dir, hipStream_t st);
+DevTensor dev_split_axis(const DevTensor& t, int i_ax, const ct_long* new_dims, const int* new_dirs, const qnumber* const* new_qnums, hipStream_t st);
+
+//________________________________________________________________________________________________________________________
+//
+// matvec plan
+
+struct MatvecPlan
+{
+	int mode = 1;
+	hipStream_t stream = nullptr;
+	Meta theta_meta;
+	DevTensor w, l, r;                          // resident operands
+	std::vector<DevTensor> interm;              // intermediates of the chosen form
+	// grouped forms: three launches; stage 1 reads x, stage 3 writes y
+	DevGemm stage[3];
+	GemmBases stage_bases[3] = {};
+	// plain form: the reference's operation sequence replayed as individual launches
+	std::vector<std::function<void(const double* x, double* y)>> plain_ops;
+	int plain_stage_end[3] = { 0, 0, 0 };       // index one past the last op of each dot stage
+	std::vector<void*> owned;                   // device descriptor buffers of the plain form
+	// bookkeeping
+	double flops = 0, bytes = 0;
+	double stage_flops[3] = { 0, 0, 0 };        // executed FP64 flops of each stage
+	double nterms = 0, ntasks = 0, nitems = 0, interm_elems = 0;
+	hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };
+	// Lanczos workspace
+	double* lz_v = nullptr;
+	int lz_cap = 0;
+	double* lz_w = nullptr;
+	double* lz_scal = nullptr;
+	double* lz_part = nullptr;
+	double* lz_z = nullptr;
+	~MatvecPlan();
+};
+
+MatvecPlan* build_matvec_plan(const block_sparse_tensor* a, const block_sparse_tensor* w, const block_sparse_tensor* l, const block_sparse_tensor* r, int mode);
+MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& l, DevTensor&& r, int mode, hipStream_t s);
+void matvec_apply(MatvecPlan& p, const double* x, double* y, float* stage_ms);
+int lanczos_ground_state(MatvecPlan& p, const double* vstart, int maxiter, double* lambda, double* u);
+
+// host FP64 linear algebra for the truncating split (host_linalg.cpp)
+int host_svd(ct_long m, ct_long n, const double* a, double* u, double* s, double* vh);
+int host_qr(ct_long m, ct_long n, const double* a, double* q, double* r);
+int host_rq(ct_long m, ct_long n, const double* a, double* r, double* q);
+int host_tridiag_eigh(int n, double* d, double* e, double* z);
+
+}  // namespace ctg

@@ -1,0 +1,737 @@
+// Device tensor algebra of libchemtensor_gpu: host-side contraction planning from the
+// quantum-number tables (the block pairing of src/tensor/block_sparse_tensor.c:1826-2000) and
+// the GPU execution of contractions, transposes and partial traces on packed device tensors;
+// the H_eff matvec plan of src/algorithm/chain_ops.c:353-390 in its grouped (three launches,
+// transposes folded into operand addressing) and plain (per-block launches) forms.
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+
+#include "ct_internal.h"
+
+namespace ctg {
+
+static const double* off_ptr(ct_long elems) { return (const double*)(intptr_t)(elems * (ct_long)sizeof(double)); }
+static double* off_ptr_mut(ct_long elems) { return (double*)(intptr_t)(elems * (ct_long)sizeof(double)); }
+
+//________________________________________________________________________________________________________________________
+//
+// grouped GEMM batches
+
+void GemmBatch::add_task(double* C, int M, int N, int nci, int sc_i, int64_t sc_batch, bool accumulate)
+{
+	GemmTask t;
+	t.C = C; t.sc_batch = sc_batch; t.M = M; t.N = N; t.nci = nci > 0 ? nci : 1; t.sc_i = sc_i;
+	t.t0 = (int32_t)terms.size(); t.t1 = t.t0; t.accumulate = accumulate ? 1 : 0; t.pad = 0;
+	tasks.push_back(t);
+}
+
+void GemmBatch::add_term(const double* A, const double* B, int K, int sa_i, int sa_k, int sb_k, int sb_j, int64_t sb_batch)
+{
+	GemmTerm m;
+	m.A = A; m.B = B; m.sb_batch = sb_batch; m.K = K; m.sa_i = sa_i; m.sa_k = sa_k; m.sb_k = sb_k; m.sb_j = sb_j; m.pad = 0;
+	terms.push_back(m);
+	GemmTask& t = tasks.back();
+	t.t1 = (int32_t)terms.size();
+	flops += 2.0 * (double)t.M * (double)t.N * (double)K;
+}
+
+std::vector<WorkItem> GemmBatch::work_items() const
+{
+	std::vector<WorkItem> items;
+	for (size_t ti = 0; ti < tasks.size(); ti++) {
+		const GemmTask& t = tasks[ti];
+		int64_t ksum = 0;
+		for (int q = t.t0; q < t.t1; q++) ksum += terms[q].K;
+		const int cost = (int)std::min<int64_t>(ksum + 4, (int64_t)1 << 30);
+		for (int m0 = 0; m0 < t.M; m0 += TILE)
+			for (int n0 = 0; n0 < t.N; n0 += TILE)
+				items.push_back(WorkItem{ (int32_t)ti, m0, n0, cost });
+	}
+	std::stable_sort(items.begin(), items.end(), [](const WorkItem& a, const WorkItem& b) { return a.cost > b.cost; });
+	return items;
+}
+
+void DevGemm::upload(const GemmBatch& b, hipStream_t s)
+{
+	release();
+	const std::vector<WorkItem> it = b.work_items();
+	ntasks = (int)b.tasks.size();
+	nterms = (int)b.terms.size();
+	nitems = (int)it.size();
+	flops = b.flops;
+	tasks = (GemmTask*)dev_alloc(std::max<size_t>(1, b.tasks.size()) * sizeof(GemmTask));
+	terms = (GemmTerm*)dev_alloc(std::max<size_t>(1, b.terms.size()) * sizeof(GemmTerm));
+	items = (WorkItem*)dev_alloc(std::max<size_t>(1, it.size()) * sizeof(WorkItem));
+	if (ntasks) CTG_HIP(hipMemcpyAsync(tasks, b.tasks.data(), b.tasks.size() * sizeof(GemmTask), hipMemcpyHostToDevice, s));
+	if (nterms) CTG_HIP(hipMemcpyAsync(terms, b.terms.data(), b.terms.size() * sizeof(GemmTerm), hipMemcpyHostToDevice, s));
+	if (nitems) CTG_HIP(hipMemcpyAsync(items, it.data(), it.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
+	CTG_HIP(hipStreamSynchronize(s));
+	const int cap = num_compute_units() * 8;
+	grid = std::max(1, std::min((nitems + 3) / 4, cap));
+}
+
+void DevGemm::release()
+{
+	dev_free(tasks); dev_free(terms); dev_free(items);
+	tasks = nullptr; terms = nullptr; items = nullptr;
+	ntasks = nterms = nitems = 0;
+}
+
+static void run_batch_plain(const GemmBatch& b, GemmBases bases, hipStream_t s)
+{
+	for (const GemmTask& t : b.tasks) {
+		for (int q = t.t0; q < t.t1; q++) {
+			GemmTask one = t;
+			one.accumulate = 1;
+			launch_plain_gemm_f64(one, b.terms[q], bases, s);
+		}
+	}
+}
+
+//________________________________________________________________________________________________________________________
+//
+// block_sparse_tensor_dot planning (bst.c:1826-2000)
+
+static Meta dot_result_meta(const Meta& s, int axs, const Meta& t, int axt, int nm)
+{
+	const int offset_s = (axs == TENSOR_AXIS_RANGE_LEADING ? nm : 0);
+	const int offset_t = (axt == TENSOR_AXIS_RANGE_LEADING ? nm : 0);
+	const int nds = s.ndim - nm, ndt = t.ndim - nm;
+	std::vector<ct_long> dl;
+	std::vector<int> ad;
+	std::vector<const qnumber*> qn;
+	for (int i = 0; i < nds; i++) { dl.push_back(s.dim_logical[offset_s + i]); ad.push_back(s.axis_dir[offset_s + i]); qn.push_back(s.qnums_logical[offset_s + i].data()); }
+	for (int i = 0; i < ndt; i++) { dl.push_back(t.dim_logical[offset_t + i]); ad.push_back(t.axis_dir[offset_t + i]); qn.push_back(t.qnums_logical[offset_t + i].data()); }
+	return make_meta(nds + ndt, dl.data(), ad.data(), qn.data());
+}
+
+/// plan r = s . t: tasks (one per output block) with terms (one per contracted block pair);
+/// operand pointers are element offsets into s, t and r (as byte offsets)
+static GemmBatch plan_dot(const Meta& s, int axs, const Meta& t, int axt, int nm, const Meta& r, bool accumulate)
+{
+	const int shift_s = (axs == TENSOR_AXIS_RANGE_LEADING ? 0 : s.ndim - nm);
+	const int shift_t = (axt == TENSOR_AXIS_RANGE_LEADING ? 0 : t.ndim - nm);
+	const int offset_s = (axs == TENSOR_AXIS_RANGE_LEADING ? nm : 0);
+	const int offset_t = (axt == TENSOR_AXIS_RANGE_LEADING ? nm : 0);
+	const int nds = s.ndim - nm, ndt = t.ndim - nm;
+	const int nlds = (axs == TENSOR_AXIS_RANGE_LEADING ? nm : s.ndim - nm);
+	const int nldt = (axt == TENSOR_AXIS_RANGE_LEADING ? nm : t.ndim - nm);
+	const bool transa = (axs == TENSOR_AXIS_RANGE_LEADING);
+	const bool transb = (axt == TENSOR_AXIS_RANGE_TRAILING);
+	ct_long ncontract = 1;
+	for (int i = 0; i < nm; i++) ncontract *= t.dim_blocks[shift_t + i];
+	GemmBatch b;
+	ct_long ir[16], is[16], it[16], ic[16], sd[16], td[16];
+	for (ct_long k = 0; k < r.nblocks; k++) {
+		if (!r.present(k)) continue;
+		if (r.ndim > 0) r.block_index(k, ir);
+		ct_long M = 1, N = 1;
+		if (r.ndim > 0) {
+			ct_long rd[16];
+			r.block_dims(k, rd);
+			for (int i = 0; i < nds; i++) M *= rd[i];
+			for (int i = 0; i < ndt; i++) N *= rd[nds + i];
+		}
+		b.add_task(off_ptr_mut(r.offset[k]), (int)M, (int)N, (int)N, (int)N, 0, accumulate);
+		for (int i = 0; i < nm; i++) ic[i] = 0;
+		for (ct_long mm = 0; mm < ncontract; mm++) {
+			for (int i = 0; i < nds; i++) is[offset_s + i] = ir[i];
+			for (int i = 0; i < nm; i++) is[shift_s + i] = ic[i];
+			qnumber qsum = 0;
+			for (int i = 0; i < s.ndim; i++) qsum += s.axis_dir[i] * s.qnums_blocks[i][is[i]];
+			if (qsum == 0) {
+				for (int i = 0; i < ndt; i++) it[offset_t + i] = ir[nds + i];
+				for (int i = 0; i < nm; i++) it[shift_t + i] = ic[i];
+				const ct_long sk = s.flat_index(is), tk = t.flat_index(it);
+				if (s.present(sk) && t.present(tk)) {
+					s.block_dims(sk, sd);
+					t.block_dims(tk, td);
+					ct_long lds = 1, tds = 1, ldt = 1, tdt = 1;
+					for (int i = 0; i < nlds; i++) lds *= sd[i];
+					for (int i = nlds; i < s.ndim; i++) tds *= sd[i];
+					for (int i = 0; i < nldt; i++) ldt *= td[i];
+					for (int i = nldt; i < t.ndim; i++) tdt *= td[i];
+					const ct_long K = transa ? lds : tds;
+					const int sa_i = transa ? 1 : (int)tds, sa_k = transa ? (int)tds : 1;
+					const int sb_k = transb ? 1 : (int)tdt, sb_j = transb ? (int)tdt : 1;
+					b.add_term(off_ptr(s.offset[sk]), off_ptr(t.offset[tk]), (int)K, sa_i, sa_k, sb_k, sb_j, 0);
+				}
+			}
+			for (int i = nm - 1; i >= 0; i--) { if (++ic[i] < t.dim_blocks[shift_t + i]) break; ic[i] = 0; }
+		}
+	}
+	return b;
+}
+
+DevTensor dev_dot(const DevTensor& s, int axs, const DevTensor& t, int axt, int nm, DotForm form, hipStream_t st, double* flops)
+{
+	DevTensor r(dot_result_meta(s.meta, axs, t.meta, axt, nm));
+	const GemmBatch b = plan_dot(s.meta, axs, t.meta, axt, nm, r.meta, form == DotForm::Plain);
+	if (flops) *flops = b.flops;
+	const GemmBases bases{ s.d, t.d, r.d };
+	if (form == DotForm::Plain) {
+		r.zero(st);
+		run_batch_plain(b, bases, st);
+	}
+	else {
+		DevGemm g;
+		g.upload(b, st);
+		launch_grouped_gemm_f64(g, bases, st);
+		CTG_HIP(hipStreamSynchronize(st));
+		g.release();
+	}
+	return r;
+}
+
+//________________________________________________________________________________________________________________________
+//
+// transpose (bst.c:785-870) and partial trace (bst.c:1560-1612)
+
+static Meta transpose_meta(const int* perm, const Meta& t)
+{
+	std::vector<ct_long> dl(t.ndim);
+	std::vector<int> ad(t.ndim);
+	std::vector<const qnumber*> qn(t.ndim);
+	for (int i = 0; i < t.ndim; i++) { dl[i] = t.dim_logical[perm[i]]; ad[i] = t.axis_dir[perm[i]]; qn[i] = t.qnums_logical[perm[i]].data(); }
+	return make_meta(t.ndim, dl.data(), ad.data(), qn.data());
+}
+
+static std::vector<PermBlock> transpose_blocks(const int* perm, const Meta& t, const Meta& r)
+{
+	std::vector<PermBlock> pbs;
+	ct_long it[16], ir[16], td[16];
+	for (ct_long k = 0; k < t.nblocks; k++) {
+		if (!t.present(k)) continue;
+		t.block_index(k, it);
+		for (int i = 0; i < t.ndim; i++) ir[i] = it[perm[i]];
+		const ct_long kr = r.flat_index(ir);
+		t.block_dims(k, td);
+		PermBlock pb;
+		memset(&pb, 0, sizeof(pb));
+		pb.src = t.offset[k];
+		pb.dst = r.offset[kr];
+		pb.ndim = t.ndim;
+		int64_t st[16], f = 1;
+		for (int i = t.ndim - 1; i >= 0; i--) { st[i] = f; f *= td[i]; }
+		pb.n = f;
+		for (int i = 0; i < t.ndim; i++) { pb.ddim[i] = td[perm[i]]; pb.sstride[i] = st[perm[i]]; }
+		pbs.push_back(pb);
+	}
+	return pbs;
+}
+
+DevTensor dev_transpose(const int* perm, const DevTensor& t, hipStream_t st)
+{
+	if (t.meta.ndim > 6) throw HipError{ hipErrorInvalidValue, "transpose supports at most 6 axes" };
+	DevTensor r(transpose_meta(perm, t.meta));
+	const std::vector<PermBlock> pbs = transpose_blocks(perm, t.meta, r.meta);
+	PermBlock* d = (PermBlock*)dev_alloc(std::max<size_t>(1, pbs.size()) * sizeof(PermBlock));
+	if (!pbs.empty()) CTG_HIP(hipMemcpyAsync(d, pbs.data(), pbs.size() * sizeof(PermBlock), hipMemcpyHostToDevice, st));
+	launch_permute_blocks(t.d, r.d, d, (int)pbs.size(), st);
+	CTG_HIP(hipStreamSynchronize(st));
+	dev_free(d);
+	return r;
+}
+
+struct TraceItem { int64_t r_off; int64_t nr; int64_t t_off; int64_t stride; };
+
+__global__ __launch_bounds__(256) void k_partial_trace(const double* __restrict__ t, double* __restrict__ r, const TraceItem* __restrict__ items,
+	const int* __restrict__ group_start, int ngroups)
+{
+	for (int g = blockIdx.x; g < ngroups; g += gridDim.x) {
+		const TraceItem first = items[group_start[g]];
+		for (int64_t j = threadIdx.x; j < first.nr; j += blockDim.x) {
+			double acc = 0;
+			for (int q = group_start[g]; q < group_start[g + 1]; q++) {
+				const TraceItem it = items[q];
+				for (int64_t kk = 0; kk < it.stride; kk++) acc += t[it.t_off + (kk * it.nr + j) * it.stride + kk];
+			}
+			r[first.r_off + j] = acc;
+		}
+	}
+}
+
+static void plan_trace(const Meta& t, int nt, const Meta& r, std::vector<TraceItem>& items, std::vector<int>& gstart)
+{
+	const int ndr = t.ndim - 2 * nt;
+	ct_long np = 1;
+	for (int i = 0; i < nt; i++) np *= t.dim_blocks[i];
+	ct_long it[16], td[16];
+	for (ct_long k = 0; k < r.nblocks; k++) {
+		if (!r.present(k)) continue;
+		gstart.push_back((int)items.size());
+		for (int i = 0; i < 16; i++) it[i] = 0;
+		if (ndr > 0) r.block_index(k, it + nt);
+		for (ct_long j = 0; j < np; j++) {
+			for (int i = 0; i < nt; i++) it[nt + ndr + i] = it[i];
+			const ct_long tk = t.flat_index(it);
+			if (t.present(tk)) {
+				t.block_dims(tk, td);
+				ct_long stride = 1;
+				for (int i = 0; i < nt; i++) stride *= td[i];
+				items.push_back(TraceItem{ r.offset[k], r.block_size(k), t.offset[tk], stride });
+			}
+			for (int i = nt - 1; i >= 0; i--) { if (++it[i] < t.dim_blocks[i]) break; it[i] = 0; }
+		}
+		if ((int)items.size() == gstart.back()) items.push_back(TraceItem{ r.offset[k], r.block_size(k), 0, 0 });
+	}
+	gstart.push_back((int)items.size());
+}
+
+DevTensor dev_cyclic_partial_trace(const DevTensor& t, int nt, hipStream_t st)
+{
+	const Meta& m = t.meta;
+	const int ndr = m.ndim - 2 * nt;
+	std::vector<const qnumber*> qn(ndr);
+	for (int i = 0; i < ndr; i++) qn[i] = m.qnums_logical[nt + i].data();
+	DevTensor r(make_meta(ndr, m.dim_logical.data() + nt, m.axis_dir.data() + nt, qn.data()));
+	std::vector<TraceItem> items;
+	std::vector<int> gs;
+	plan_trace(m, nt, r.meta, items, gs);
+	const int ng = (int)gs.size() - 1;
+	TraceItem* di = (TraceItem*)dev_alloc(std::max<size_t>(1, items.size()) * sizeof(TraceItem));
+	int* dg = (int*)dev_alloc(gs.size() * sizeof(int));
+	CTG_HIP(hipMemcpyAsync(di, items.data(), items.size() * sizeof(TraceItem), hipMemcpyHostToDevice, st));
+	CTG_HIP(hipMemcpyAsync(dg, gs.data(), gs.size() * sizeof(int), hipMemcpyHostToDevice, st));
+	if (ng > 0) {
+		hipLaunchKernelGGL(k_partial_trace, dim3(std::min(ng, 8192)), dim3(256), 0, st, t.d, r.d, di, dg, ng);
+		CTG_HIP(hipGetLastError());
+	}
+	CTG_HIP(hipStreamSynchronize(st));
+	dev_free(di);
+	dev_free(dg);
+	return r;
+}
+
+//________________________________________________________________________________________________________________________
+//
+// reshaping and slicing as batched segment copies (bst.c:950-1290, :1446-1552)
+
+static void run_segments(const double* src, double* dst, const std::vector<CopySeg>& segs, hipStream_t st)
+{
+	if (segs.empty()) return;
+	CopySeg* d = (CopySeg*)dev_alloc(segs.size() * sizeof(CopySeg));
+	CTG_HIP(hipMemcpyAsync(d, segs.data(), segs.size() * sizeof(CopySeg), hipMemcpyHostToDevice, st));
+	launch_copy_segments(src, dst, d, (int)segs.size(), st);
+	CTG_HIP(hipStreamSynchronize(st));
+	dev_free(d);
+}
+
+// logical index -> entry index within its block, for one axis (bst.c:882-903)
+static std::vector<ct_long> fanin_map(const Meta& m, int ax)
+{
+	std::vector<ct_long> out((size_t)m.dim_logical[ax]);
+	std::vector<ct_long> counter((size_t)m.dim_blocks[ax], 0);
+	for (ct_long i = 0; i < m.dim_logical[ax]; i++) out[i] = counter[(size_t)m.find_qnum(ax, m.qnums_logical[ax][i])]++;
+	return out;
+}
+
+DevTensor dev_flatten_axes(const DevTensor& t, int i_ax, int new_dir, hipStream_t st)
+{
+	const Meta& m = t.meta;
+	const int nd = m.ndim;
+	std::vector<ct_long> dl;
+	std::vector<int> ad;
+	std::vector<std::vector<qnumber>> qs;
+	for (int i = 0; i < i_ax; i++) { dl.push_back(m.dim_logical[i]); ad.push_back(m.axis_dir[i]); qs.push_back(m.qnums_logical[i]); }
+	const ct_long d0 = m.dim_logical[i_ax], d1 = m.dim_logical[i_ax + 1];
+	std::vector<qnumber> qf((size_t)(d0 * d1));
+	for (ct_long j = 0; j < d0; j++) for (ct_long k = 0; k < d1; k++)
+		qf[j * d1 + k] = new_dir * (m.axis_dir[i_ax] * m.qnums_logical[i_ax][j] + m.axis_dir[i_ax + 1] * m.qnums_logical[i_ax + 1][k]);
+	dl.push_back(d0 * d1); ad.push_back(new_dir); qs.push_back(qf);
+	for (int i = i_ax + 2; i < nd; i++) { dl.push_back(m.dim_logical[i]); ad.push_back(m.axis_dir[i]); qs.push_back(m.qnums_logical[i]); }
+	std::vector<const qnumber*> qp;
+	for (auto& q : qs) qp.push_back(q.data());
+	DevTensor r(make_meta(nd - 1, dl.data(), ad.data(), qp.data()));
+	const Meta& rm = r.meta;
+	const std::vector<ct_long> fin = fanin_map(rm, i_ax);
+	std::vector<CopySeg> segs;
+	ct_long it[16], ir[16], bt[16], br[16], is[16], isr[16];
+	for (ct_long k = 0; k < m.nblocks; k++) {
+		if (!m.present(k)) continue;
+		m.block_index(k, it);
+		m.block_dims(k, bt);
+		const qnumber q0 = m.qnums_blocks[i_ax][it[i_ax]], q1 = m.qnums_blocks[i_ax + 1][it[i_ax + 1]];
+		const qnumber qfl = new_dir * (m.axis_dir[i_ax] * q0 + m.axis_dir[i_ax + 1] * q1);
+		for (int i = 0; i < i_ax; i++) ir[i] = it[i];
+		ir[i_ax] = rm.find_qnum(i_ax, qfl);
+		for (int i = i_ax + 2; i < nd; i++) ir[i - 1] = it[i];
+		const ct_long kr = rm.flat_index(ir);
+		rm.block_dims(kr, br);
+		const std::vector<ct_long> f0 = fanout(m, i_ax, q0), f1 = fanout(m, i_ax + 1, q1);
+		ct_long stride = 1;
+		for (int i = i_ax + 2; i < nd; i++) stride *= bt[i];
+		ct_long nsl = 1;
+		for (int i = 0; i < i_ax + 2; i++) { nsl *= bt[i]; is[i] = 0; }
+		for (ct_long j = 0; j < nsl; j++) {
+			for (int i = 0; i < i_ax; i++) isr[i] = is[i];
+			isr[i_ax] = fin[f0[is[i_ax]] * d1 + f1[is[i_ax + 1]]];
+			ct_long l = 0;
+			for (int i = 0; i <= i_ax; i++) l = l * br[i] + isr[i];
+			segs.push_back(CopySeg{ m.offset[k] + j * stride, rm.offset[kr] + l * stride, stride });
+			for (int i = i_ax + 1; i >= 0; i--) { if (++is[i] < bt[i]) break; is[i] = 0; }
+		}
+	}
+	r.zero(st);
+	run_segments(t.d, r.d, segs, st);
+	return r;
+}
+
+DevTensor dev_split_axis(const DevTensor& t, int i_ax, const ct_long* nd2, const int* dir2, const qnumber* const* q2, hipStream_t st)
+{
+	const Meta& m = t.meta;
+	const int nd = m.ndim;
+	std::vector<ct_long> dl;
+	std::vector<int> ad;
+	std::vector<const qnumber*> qp;
+	for (int i = 0; i < i_ax; i++) { dl.push_back(m.dim_logical[i]); ad.push_back(m.axis_dir[i]); qp.push_back(m.qnums_logical[i].data()); }
+	for (int c = 0; c < 2; c++) { dl.push_back(nd2[c]); ad.push_back(dir2[c]); qp.push_back(q2[c]); }
+	for (int i = i_ax + 1; i < nd; i++) { dl.push_back(m.dim_logical[i]); ad.push_back(m.axis_dir[i]); qp.push_back(m.qnums_logical[i].data()); }
+	DevTensor r(make_meta(nd + 1, dl.data(), ad.data(), qp.data()));
+	const Meta& rm = r.meta;
+	const std::vector<ct_long> fin = fanin_map(m, i_ax);
+	std::vector<CopySeg> segs;
+	ct_long ir[16], it[16], br[16], bt[16], is[16], ist[16];
+	for (ct_long k = 0; k < rm.nblocks; k++) {
+		if (!rm.present(k)) continue;
+		rm.block_index(k, ir);
+		rm.block_dims(k, br);
+		const qnumber q0 = rm.qnums_blocks[i_ax][ir[i_ax]], q1 = rm.qnums_blocks[i_ax + 1][ir[i_ax + 1]];
+		const qnumber qfl = m.axis_dir[i_ax] * (rm.axis_dir[i_ax] * q0 + rm.axis_dir[i_ax + 1] * q1);
+		for (int i = 0; i < i_ax; i++) it[i] = ir[i];
+		it[i_ax] = m.find_qnum(i_ax, qfl);
+		for (int i = i_ax + 1; i < nd; i++) it[i] = ir[i + 1];
+		if (it[i_ax] < 0) continue;
+		const ct_long kt = m.flat_index(it);
+		if (!m.present(kt)) continue;
+		m.block_dims(kt, bt);
+		const std::vector<ct_long> f0 = fanout(rm, i_ax, q0), f1 = fanout(rm, i_ax + 1, q1);
+		ct_long stride = 1;
+		for (int i = i_ax + 2; i < nd + 1; i++) stride *= br[i];
+		ct_long nsl = 1;
+		for (int i = 0; i < i_ax + 2; i++) { nsl *= br[i]; is[i] = 0; }
+		for (ct_long j = 0; j < nsl; j++) {
+			for (int i = 0; i < i_ax; i++) ist[i] = is[i];
+			ist[i_ax] = fin[f0[is[i_ax]] * nd2[1] + f1[is[i_ax + 1]]];
+			ct_long l = 0;
+			for (int i = 0; i <= i_ax; i++) l = l * bt[i] + ist[i];
+			segs.push_back(CopySeg{ m.offset[kt] + l * stride, rm.offset[k] + j * stride, stride });
+			for (int i = i_ax + 1; i >= 0; i--) { if (++is[i] < br[i]) break; is[i] = 0; }
+		}
+	}
+	r.zero(st);
+	run_segments(t.d, r.d, segs, st);
+	return r;
+}
+
+//________________________________________________________________________________________________________________________
+//
+// H_eff matvec plan (chain_ops.c:353-390)
+
+MatvecPlan::~MatvecPlan()
+{
+	for (int i = 0; i < 3; i++) stage[i].release();
+	for (void* p : owned) dev_free(p);
+	for (int i = 0; i < 4; i++) if (ev[i]) (void)hipEventDestroy(ev[i]);
+	dev_free(lz_v); dev_free(lz_w); dev_free(lz_scal); dev_free(lz_part); dev_free(lz_z);
+}
+
+static bool same_axis(const Meta& a, int ia, const Meta& b, int ib)
+{
+	return a.dim_logical[ia] == b.dim_logical[ib] && a.qnums_logical[ia] == b.qnums_logical[ib];
+}
+
+// the reference's GEMM count of one matvec (every allocated block pair, zero blocks included)
+static double reference_flops(const Meta& th, const Meta& w, const Meta& l, const Meta& r)
+{
+	double f = 0;
+	// s1 = theta . R : [a p n c o]
+	Meta s1 = dot_result_meta(th, TENSOR_AXIS_RANGE_TRAILING, r, TENSOR_AXIS_RANGE_LEADING, 1);
+	f += plan_dot(th, TENSOR_AXIS_RANGE_TRAILING, r, TENSOR_AXIS_RANGE_LEADING, 1, s1, false).flops;
+	const int p0[5] = { 1, 2, 0, 3, 4 };
+	Meta t = transpose_meta(p0, s1);
+	Meta s2 = dot_result_meta(w, TENSOR_AXIS_RANGE_TRAILING, t, TENSOR_AXIS_RANGE_LEADING, 2);
+	f += plan_dot(w, TENSOR_AXIS_RANGE_TRAILING, t, TENSOR_AXIS_RANGE_LEADING, 2, s2, false).flops;
+	const int p1[5] = { 2, 0, 1, 3, 4 };
+	Meta t2 = transpose_meta(p1, s2);
+	const int p2[4] = { 0, 3, 1, 2 };
+	Meta k = transpose_meta(p2, l);
+	Meta s3 = dot_result_meta(k, TENSOR_AXIS_RANGE_TRAILING, t2, TENSOR_AXIS_RANGE_LEADING, 2);
+	f += plan_dot(k, TENSOR_AXIS_RANGE_TRAILING, t2, TENSOR_AXIS_RANGE_LEADING, 2, s3, false).flops;
+	return f;
+}
+
+static void build_grouped(MatvecPlan& p)
+{
+	const Meta& th = p.theta_meta;
+	const Meta& W = p.w.meta;
+	const Meta& L = p.l.meta;
+	const Meta& R = p.r.meta;
+	const qnumber qo = L.qnums_blocks[0][0];
+	const ct_long nA = th.dim_blocks[0], nP = th.dim_blocks[1], nB = th.dim_blocks[2];
+	const ct_long nN = R.dim_blocks[1], nC = R.dim_blocks[2];
+	const ct_long nM = W.dim_blocks[0], nQ = W.dim_blocks[1];
+	auto bc = [](const Meta& m, int ax, ct_long i) { return m.block_count[ax][i]; };
+	auto qb = [](const Meta& m, int ax, ct_long i) { return m.qnums_blocks[ax][i]; };
+
+	// stage 1: T1(A,P,N,C)[a p n c] = theta(A,P,B) R(B,N,C,O)
+	std::vector<ct_long> t1_off((size_t)(nA * nP * nN * nC), -1);
+	auto t1_key = [&](ct_long A, ct_long P, ct_long N, ct_long C) { return ((A * nP + P) * nN + N) * nC + C; };
+	GemmBatch s1;
+	ct_long t1_elems = 0;
+	{
+		std::vector<std::tuple<ct_long, ct_long, ct_long, ct_long, ct_long, ct_long>> blocks;  // A P N C Bidx off
+		for (ct_long A = 0; A < nA; A++) for (ct_long P = 0; P < nP; P++) {
+			const ct_long B = th.find_qnum(2, qb(th, 0, A) + qb(th, 1, P));
+			if (B < 0) continue;
+			const ct_long ib[3] = { A, P, B };
+			if (!th.present(th.flat_index(ib))) continue;
+			for (ct_long N = 0; N < nN; N++) for (ct_long C = 0; C < nC; C++) {
+				const ct_long irr[4] = { B, N, C, 0 };
+				if (!R.present(R.flat_index(irr))) continue;
+				t1_off[t1_key(A, P, N, C)] = t1_elems;
+				t1_elems += bc(th, 0, A) * bc(th, 1, P) * bc(R, 1, N) * bc(R, 2, C);
+				blocks.emplace_back(A, P, N, C, B, t1_off[t1_key(A, P, N, C)]);
+			}
+		}
+		for (auto& bl : blocks) {
+			ct_long A, P, N, C, B, off;
+			std::tie(A, P, N, C, B, off) = bl;
+			const ct_long dA = bc(th, 0, A), dP = bc(th, 1, P), dB = bc(th, 2, B), dN = bc(R, 1, N), dC = bc(R, 2, C);
+			const ct_long ith[3] = { A, P, B };
+			const ct_long irr[4] = { B, N, C, 0 };
+			s1.add_task(off_ptr_mut(off), (int)(dA * dP), (int)(dN * dC), (int)(dN * dC), (int)(dN * dC), 0, false);
+			s1.add_term(off_ptr(th.offset[th.flat_index(ith)]), off_ptr(R.offset[R.flat_index(irr)]), (int)dB, (int)dB, 1, (int)(dN * dC), 1, 0);
+		}
+	}
+	// stage 2: T2(A,M,Q,C)[a m q c] = sum_P W(M,Q,P,N) T1(A,P,N,C)
+	std::vector<ct_long> t2_off((size_t)(nA * nM * nQ * nC), -1);
+	auto t2_key = [&](ct_long A, ct_long M, ct_long Q, ct_long C) { return ((A * nM + M) * nQ + Q) * nC + C; };
+	GemmBatch s2;
+	ct_long t2_elems = 0;
+	for (ct_long A = 0; A < nA; A++) for (ct_long C = 0; C < nC; C++) for (ct_long M = 0; M < nM; M++) for (ct_long Q = 0; Q < nQ; Q++) {
+		if (qb(W, 0, M) + qb(W, 1, Q) + qb(th, 0, A) - qb(R, 2, C) - qo != 0) continue;
+		std::vector<std::pair<ct_long, ct_long>> tl;  // (P, N)
+		for (ct_long P = 0; P < nP; P++) {
+			const ct_long N = W.find_qnum(3, qb(W, 0, M) + qb(W, 1, Q) - qb(W, 2, P));
+			if (N < 0) continue;
+			const ct_long iw[4] = { M, Q, P, N };
+			if (!W.present(W.flat_index(iw))) continue;
+			if (t1_off[t1_key(A, P, N, C)] < 0) continue;
+			tl.emplace_back(P, N);
+		}
+		if (tl.empty()) continue;
+		const ct_long dA = bc(th, 0, A), dM = bc(W, 0, M), dQ = bc(W, 1, Q), dC = bc(R, 2, C);
+		const ct_long off = t2_elems;
+		t2_off[t2_key(A, M, Q, C)] = off;
+		t2_elems += dA * dM * dQ * dC;
+		s2.add_task(off_ptr_mut(off), (int)(dM * dQ), (int)(dA * dC), (int)dC, (int)dC, dM * dQ * dC, false);
+		for (auto& pn : tl) {
+			const ct_long P = pn.first, N = pn.second;
+			const ct_long dP = bc(W, 2, P), dN = bc(W, 3, N);
+			const ct_long iw[4] = { M, Q, P, N };
+			s2.add_term(off_ptr(W.offset[W.flat_index(iw)]), off_ptr(t1_off[t1_key(A, P, N, C)]), (int)(dP * dN), (int)(dP * dN), 1, (int)dC, 1, dP * dN * dC);
+		}
+	}
+	// stage 3: y(X,Q,C)[x q c] = sum_(A,M) L(O,A,M,X)^T T2(A,M,Q,C)
+	GemmBatch s3;
+	ct_long iy[3];
+	for (ct_long k = 0; k < th.nblocks; k++) {
+		if (!th.present(k)) continue;
+		th.block_index(k, iy);
+		const ct_long X = iy[0], Q = iy[1], C = iy[2];
+		const ct_long dX = bc(th, 0, X), dQ = bc(th, 1, Q), dC = bc(th, 2, C);
+		s3.add_task(off_ptr_mut(th.offset[k]), (int)dX, (int)(dQ * dC), (int)(dQ * dC), (int)(dQ * dC), 0, false);
+		for (ct_long A = 0; A < nA; A++) {
+			const ct_long M = L.find_qnum(2, qo - qb(L, 1, A) + qb(L, 3, X));
+			if (M < 0) continue;
+			const ct_long il[4] = { 0, A, M, X };
+			if (!L.present(L.flat_index(il))) continue;
+			const ct_long o2 = t2_off[t2_key(A, M, Q, C)];
+			if (o2 < 0) continue;
+			const ct_long dA = bc(L, 1, A), dM = bc(L, 2, M);
+			s3.add_term(off_ptr(L.offset[L.flat_index(il)]), off_ptr(o2), (int)(dA * dM), 1, (int)dX, (int)(dQ * dC), 1, 0);
+		}
+	}
+	Meta m1, m2;
+	m1.nelem = t1_elems;
+	m2.nelem = t2_elems;
+	p.interm.emplace_back(std::move(m1));
+	p.interm.emplace_back(std::move(m2));
+	p.stage[0].upload(s1, p.stream);
+	p.stage[1].upload(s2, p.stream);
+	p.stage[2].upload(s3, p.stream);
+	p.stage_flops[0] = s1.flops;
+	p.stage_flops[1] = s2.flops;
+	p.stage_flops[2] = s3.flops;
+	p.stage_bases[0] = GemmBases{ nullptr, p.r.d, p.interm[0].d };
+	p.stage_bases[1] = GemmBases{ p.w.d, p.interm[0].d, p.interm[1].d };
+	p.stage_bases[2] = GemmBases{ p.l.d, p.interm[1].d, nullptr };
+	p.nterms = s1.terms.size() + s2.terms.size() + s3.terms.size();
+	p.ntasks = s1.tasks.size() + s2.tasks.size() + s3.tasks.size();
+	p.nitems = p.stage[0].nitems + p.stage[1].nitems + p.stage[2].nitems;
+	p.interm_elems = (double)(t1_elems + t2_elems);
+}
+
+static void build_plain(MatvecPlan& p)
+{
+	hipStream_t st = p.stream;
+	const Meta& th = p.theta_meta;
+	// s1 = dot(theta, TRAILING, R, LEADING, 1)
+	Meta ms1 = dot_result_meta(th, TENSOR_AXIS_RANGE_TRAILING, p.r.meta, TENSOR_AXIS_RANGE_LEADING, 1);
+	GemmBatch b1 = plan_dot(th, TENSOR_AXIS_RANGE_TRAILING, p.r.meta, TENSOR_AXIS_RANGE_LEADING, 1, ms1, true);
+	const int p0[5] = { 1, 2, 0, 3, 4 };
+	Meta mt = transpose_meta(p0, ms1);
+	Meta ms2 = dot_result_meta(p.w.meta, TENSOR_AXIS_RANGE_TRAILING, mt, TENSOR_AXIS_RANGE_LEADING, 2);
+	GemmBatch b2 = plan_dot(p.w.meta, TENSOR_AXIS_RANGE_TRAILING, mt, TENSOR_AXIS_RANGE_LEADING, 2, ms2, true);
+	const int p1[5] = { 2, 0, 1, 3, 4 };
+	Meta mt2 = transpose_meta(p1, ms2);
+	const int p2[4] = { 0, 3, 1, 2 };
+	Meta mk = transpose_meta(p2, p.l.meta);
+	Meta ms3 = dot_result_meta(mk, TENSOR_AXIS_RANGE_TRAILING, mt2, TENSOR_AXIS_RANGE_LEADING, 2);
+	GemmBatch b3 = plan_dot(mk, TENSOR_AXIS_RANGE_TRAILING, mt2, TENSOR_AXIS_RANGE_LEADING, 2, ms3, true);
+	auto perm_desc = [&](const int* perm, const Meta& from, const Meta& to) {
+		std::vector<PermBlock> pbs = transpose_blocks(perm, from, to);
+		PermBlock* d = (PermBlock*)dev_alloc(std::max<size_t>(1, pbs.size()) * sizeof(PermBlock));
+		if (!pbs.empty()) CTG_HIP(hipMemcpyAsync(d, pbs.data(), pbs.size() * sizeof(PermBlock), hipMemcpyHostToDevice, st));
+		CTG_HIP(hipStreamSynchronize(st));
+		p.owned.push_back(d);
+		return std::make_pair(d, (int)pbs.size());
+	};
+	const auto pt = perm_desc(p0, ms1, mt);
+	const auto pt2 = perm_desc(p1, ms2, mt2);
+	const auto pk = perm_desc(p2, p.l.meta, mk);
+	// trace of s3 over the unit outer bonds: block (0, X, Q, C, 0) of s3 -> block (X, Q, C) of y
+	std::vector<CopySeg> segs;
+	{
+		ct_long iy[3];
+		for (ct_long k = 0; k < th.nblocks; k++) {
+			if (!th.present(k)) continue;
+			th.block_index(k, iy);
+			const ct_long is3[5] = { 0, iy[0], iy[1], iy[2], 0 };
+			const ct_long k3 = ms3.flat_index(is3);
+			if (ms3.present(k3)) segs.push_back(CopySeg{ ms3.offset[k3], th.offset[k], th.block_size(k) });
+		}
+	}
+	CopySeg* dseg = (CopySeg*)dev_alloc(std::max<size_t>(1, segs.size()) * sizeof(CopySeg));
+	if (!segs.empty()) CTG_HIP(hipMemcpyAsync(dseg, segs.data(), segs.size() * sizeof(CopySeg), hipMemcpyHostToDevice, st));
+	CTG_HIP(hipStreamSynchronize(st));
+	p.owned.push_back(dseg);
+	const int nseg = (int)segs.size();
+	// buffers
+	p.interm.emplace_back(std::move(ms1));   // 0 s1
+	p.interm.emplace_back(std::move(mt));    // 1 t
+	p.interm.emplace_back(std::move(ms2));   // 2 s2
+	p.interm.emplace_back(std::move(mt2));   // 3 t2
+	p.interm.emplace_back(std::move(mk));    // 4 k
+	p.interm.emplace_back(std::move(ms3));   // 5 s3
+	double* s1d = p.interm[0].d; double* td = p.interm[1].d; double* s2d = p.interm[2].d;
+	double* t2d = p.interm[3].d; double* kd = p.interm[4].d; double* s3d = p.interm[5].d;
+	const ct_long n1 = p.interm[0].meta.nelem, n2 = p.interm[2].meta.nelem, n3 = p.interm[5].meta.nelem;
+	const double* rd = p.r.d; const double* wd = p.w.d; const double* ld = p.l.d;
+	const ct_long ny = th.nelem;
+	// stage 1
+	p.plain_ops.push_back([=](const double*, double*) { launch_fill(s1d, 0.0, n1, st); });
+	for (const GemmTask& t : b1.tasks) for (int q = t.t0; q < t.t1; q++) {
+		const GemmTerm term = b1.terms[q];
+		p.plain_ops.push_back([=](const double* x, double*) { launch_plain_gemm_f64(t, term, GemmBases{ x, rd, s1d }, st); });
+	}
+	p.plain_stage_end[0] = (int)p.plain_ops.size();
+	// stage 2
+	p.plain_ops.push_back([=](const double*, double*) { launch_permute_blocks(s1d, td, pt.first, pt.second, st); });
+	p.plain_ops.push_back([=](const double*, double*) { launch_fill(s2d, 0.0, n2, st); });
+	for (const GemmTask& t : b2.tasks) for (int q = t.t0; q < t.t1; q++) {
+		const GemmTerm term = b2.terms[q];
+		p.plain_ops.push_back([=](const double*, double*) { launch_plain_gemm_f64(t, term, GemmBases{ wd, td, s2d }, st); });
+	}
+	p.plain_stage_end[1] = (int)p.plain_ops.size();
+	// stage 3
+	p.plain_ops.push_back([=](const double*, double*) { launch_permute_blocks(s2d, t2d, pt2.first, pt2.second, st); });
+	p.plain_ops.push_back([=](const double*, double*) { launch_permute_blocks(ld, kd, pk.first, pk.second, st); });
+	p.plain_ops.push_back([=](const double*, double*) { launch_fill(s3d, 0.0, n3, st); });
+	for (const GemmTask& t : b3.tasks) for (int q = t.t0; q < t.t1; q++) {
+		const GemmTerm term = b3.terms[q];
+		p.plain_ops.push_back([=](const double*, double*) { launch_plain_gemm_f64(t, term, GemmBases{ kd, t2d, s3d }, st); });
+	}
+	p.plain_ops.push_back([=](const double*, double* y) { launch_fill(y, 0.0, ny, st); launch_copy_segments(s3d, y, dseg, nseg, st); });
+	p.plain_stage_end[2] = (int)p.plain_ops.size();
+	p.nterms = b1.terms.size() + b2.terms.size() + b3.terms.size();
+	p.ntasks = b1.tasks.size() + b2.tasks.size() + b3.tasks.size();
+	p.nitems = p.nterms;
+	p.stage_flops[0] = b1.flops;
+	p.stage_flops[1] = b2.flops;
+	p.stage_flops[2] = b3.flops;
+	p.interm_elems = 0;
+	for (auto& t : p.interm) p.interm_elems += (double)t.meta.nelem;
+}
+
+MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& l, DevTensor&& r, int mode, hipStream_t s)
+{
+	if (theta.ndim != 3 || w.meta.ndim != 4 || l.meta.ndim != 4 || r.meta.ndim != 4)
+		throw HipError{ hipErrorInvalidValue, "apply_local_hamiltonian expects degrees 3, 4, 4, 4" };
+	if (l.meta.dim_logical[0] != 1 || r.meta.dim_logical[3] != 1 || l.meta.qnums_logical[0] != r.meta.qnums_logical[3])
+		throw HipError{ hipErrorNotSupported, "outer virtual bonds of the environment blocks must have dimension 1 and equal quantum numbers" };
+	if (!same_axis(theta, 2, r.meta, 0) || !same_axis(theta, 1, w.meta, 2) || !same_axis(theta, 0, l.meta, 1) ||
+		!same_axis(l.meta, 2, w.meta, 0) || !same_axis(r.meta, 1, w.meta, 3) ||
+		!same_axis(theta, 0, l.meta, 3) || !same_axis(theta, 1, w.meta, 1) || !same_axis(theta, 2, r.meta, 2))
+		throw HipError{ hipErrorInvalidValue, "incompatible dimensions or quantum numbers of apply_local_hamiltonian operands" };
+	MatvecPlan* p = new MatvecPlan();
+	p->mode = mode;
+	p->stream = s;
+	p->theta_meta = theta;
+	p->w = std::move(w);
+	p->l = std::move(l);
+	p->r = std::move(r);
+	try {
+		if (mode == CT_MATVEC_PLAIN_FP64) build_plain(*p);
+		else build_grouped(*p);
+		for (int i = 0; i < 4; i++) CTG_HIP(hipEventCreate(&p->ev[i]));
+	}
+	catch (...) {
+		delete p;
+		throw;
+	}
+	p->flops = reference_flops(theta, p->w.meta, p->l.meta, p->r.meta);
+	p->bytes = 8.0 * (2.0 * (double)theta.nelem + (double)p->w.meta.nelem + (double)p->l.meta.nelem + (double)p->r.meta.nelem);
+	return p;
+}
+
+MatvecPlan* build_matvec_plan(const block_sparse_tensor* a, const block_sparse_tensor* w, const block_sparse_tensor* l, const block_sparse_tensor* r, int mode)
+{
+	hipStream_t s = default_stream();
+	Meta th = meta_of(a);
+	DevTensor dw = upload(w, s), dl = upload(l, s), dr = upload(r, s);
+	return build_matvec_plan_dev(th, std::move(dw), std::move(dl), std::move(dr), mode, s);
+}
+
+void matvec_apply(MatvecPlan& p, const double* x, double* y, float* stage_ms)
+{
+	hipStream_t s = p.stream;
+	if (stage_ms) CTG_HIP(hipEventRecord(p.ev[0], s));
+	if (p.mode == CT_MATVEC_PLAIN_FP64) {
+		int i = 0;
+		for (int st = 0; st < 3; st++) {
+			for (; i < p.plain_stage_end[st]; i++) p.plain_ops[i](x, y);
+			if (stage_ms) CTG_HIP(hipEventRecord(p.ev[st + 1], s));
+		}
+	}
+	else {
+		for (int st = 0; st < 3; st++) {
+			GemmBases b = p.stage_bases[st];
+			if (st == 0) b.A = x;
+			if (st == 2) b.C = y;
+			if (p.mode == CT_MATVEC_GROUPED_SPLIT) launch_grouped_gemm_split(p.stage[st], b, s);
+			else launch_grouped_gemm_f64(p.stage[st], b, s);
+			if (stage_ms) CTG_HIP(hipEventRecord(p.ev[st + 1], s));
+		}
+	}
+	if (stage_ms) {
+		CTG_HIP(hipEventSynchronize(p.ev[3]));
+		for (int st = 0; st < 3; st++) CTG_HIP(hipEventElapsedTime(&stage_ms[st], p.ev[st], p.ev[st + 1]));
+	}
+}
+
+}  // namespace ctg

@@ -1,0 +1,272 @@
+// Host FP64 linear algebra of the DMRG sweep: the steps whose stability needs FP64 and whose
+// sizes are small (tridiagonal Lanczos eigenproblem, truncating block SVD, QR/RQ of the
+// orthonormalisation). Replaces the reference's LAPACK calls (dsteqr at src/util/krylov.c:230,
+// dgesvd at src/tensor/dense_tensor.c:3644, dgeqrf/dorgqr and dgerqf/dorgrq at
+// dense_tensor.c:2391-2879). QR and RQ use LAPACK's Householder sign convention (dlarfg), so
+// the gauge of the factors matches the reference; the SVD is Hestenes one-sided Jacobi.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "ct_internal.h"
+
+namespace ctg {
+
+#define CM(a, lda, i, j) ((a)[(size_t)(i) + (size_t)(lda) * (size_t)(j)])
+
+static double nrm2s(ct_long n, const double* x, ct_long inc)
+{
+	double scale = 0, ssq = 1;
+	for (ct_long i = 0; i < n; i++) {
+		const double v = x[i * inc];
+		if (v == 0) continue;
+		const double a = std::fabs(v);
+		if (scale < a) { ssq = 1 + ssq * (scale / a) * (scale / a); scale = a; }
+		else ssq += (a / scale) * (a / scale);
+	}
+	return scale * std::sqrt(ssq);
+}
+
+int host_tridiag_eigh(int n, double* d, double* offd, double* z)
+{
+	// implicit-shift QL iteration with eigenvector accumulation; eigenvalues ascending
+	std::vector<double> e((size_t)n + 1, 0.0);
+	for (int i = 0; i < n - 1; i++) e[i] = offd[i];
+	for (int i = 0; i < n * n; i++) z[i] = 0;
+	for (int i = 0; i < n; i++) z[i + n * i] = 1;
+	const double eps = 2.2204460492503131e-16;
+	for (int l = 0; l < n; l++) {
+		for (int iter = 0;; iter++) {
+			int m = l;
+			for (; m < n - 1; m++) if (std::fabs(e[m]) <= eps * (std::fabs(d[m]) + std::fabs(d[m + 1]))) break;
+			if (m == l) break;
+			if (iter >= 60) return -1;
+			double g = (d[l + 1] - d[l]) / (2 * e[l]);
+			double r = std::hypot(g, 1.0);
+			g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
+			double s = 1, c = 1, p = 0;
+			bool deflated = false;
+			for (int i = m - 1; i >= l; i--) {
+				double f = s * e[i];
+				const double b = c * e[i];
+				r = std::hypot(f, g);
+				e[i + 1] = r;
+				if (r == 0) { d[i + 1] -= p; e[m] = 0; deflated = true; break; }
+				s = f / r;
+				c = g / r;
+				g = d[i + 1] - p;
+				r = (d[i] - g) * s + 2 * c * b;
+				p = s * r;
+				d[i + 1] = g + p;
+				g = c * r - b;
+				for (int k = 0; k < n; k++) {
+					f = z[k + n * (i + 1)];
+					z[k + n * (i + 1)] = s * z[k + n * i] + c * f;
+					z[k + n * i] = c * z[k + n * i] - s * f;
+				}
+			}
+			if (deflated) continue;
+			d[l] -= p;
+			e[l] = g;
+			e[m] = 0;
+		}
+	}
+	std::vector<int> ord(n);
+	std::iota(ord.begin(), ord.end(), 0);
+	std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return d[a] < d[b]; });
+	std::vector<double> d2(n), z2((size_t)n * n);
+	for (int i = 0; i < n; i++) {
+		d2[i] = d[ord[i]];
+		for (int k = 0; k < n; k++) z2[k + (size_t)n * i] = z[k + (size_t)n * ord[i]];
+	}
+	std::copy(d2.begin(), d2.end(), d);
+	std::copy(z2.begin(), z2.end(), z);
+	return 0;
+}
+
+// Householder reflector (LAPACK dlarfg convention): H (alpha; x) = (beta; 0)
+static void larfg(ct_long n, double& alpha, double* x, ct_long inc, double& tau)
+{
+	if (n <= 1) { tau = 0; return; }
+	const double xn = nrm2s(n - 1, x, inc);
+	if (xn == 0) { tau = 0; return; }
+	const double beta = -std::copysign(std::hypot(alpha, xn), alpha);
+	tau = (beta - alpha) / beta;
+	const double sc = 1.0 / (alpha - beta);
+	for (ct_long i = 0; i < n - 1; i++) x[i * inc] *= sc;
+	alpha = beta;
+}
+
+static void apply_left(ct_long m, ct_long n, const double* v, ct_long incv, double tau, double* c, ct_long ldc)
+{
+	if (tau == 0) return;
+	for (ct_long j = 0; j < n; j++) {
+		double w = 0;
+		for (ct_long i = 0; i < m; i++) w += v[i * incv] * CM(c, ldc, i, j);
+		w *= tau;
+		for (ct_long i = 0; i < m; i++) CM(c, ldc, i, j) -= v[i * incv] * w;
+	}
+}
+
+static void apply_right(ct_long m, ct_long n, const double* v, ct_long incv, double tau, double* c, ct_long ldc)
+{
+	if (tau == 0) return;
+	for (ct_long i = 0; i < m; i++) {
+		double w = 0;
+		for (ct_long j = 0; j < n; j++) w += CM(c, ldc, i, j) * v[j * incv];
+		w *= tau;
+		for (ct_long j = 0; j < n; j++) CM(c, ldc, i, j) -= w * v[j * incv];
+	}
+}
+
+int host_qr(ct_long m, ct_long n, const double* a, double* q, double* r)
+{
+	const ct_long k = std::min(m, n);
+	std::vector<double> t((size_t)(m * n)), tau((size_t)k);
+	for (ct_long i = 0; i < m; i++) for (ct_long j = 0; j < n; j++) CM(t.data(), m, i, j) = a[i * n + j];
+	double* T = t.data();
+	for (ct_long i = 0; i < k; i++) {
+		larfg(m - i, CM(T, m, i, i), &CM(T, m, std::min(i + 1, m - 1), i), 1, tau[i]);
+		if (i < n - 1) {
+			const double aii = CM(T, m, i, i);
+			CM(T, m, i, i) = 1;
+			apply_left(m - i, n - i - 1, &CM(T, m, i, i), 1, tau[i], &CM(T, m, i, i + 1), m);
+			CM(T, m, i, i) = aii;
+		}
+	}
+	for (ct_long i = 0; i < k; i++) for (ct_long j = 0; j < n; j++) r[i * n + j] = (j < i ? 0 : CM(T, m, i, j));
+	for (ct_long i = k - 1; i >= 0; i--) {
+		if (i < k - 1) {
+			CM(T, m, i, i) = 1;
+			apply_left(m - i, k - i - 1, &CM(T, m, i, i), 1, tau[i], &CM(T, m, i, i + 1), m);
+		}
+		for (ct_long l = i + 1; l < m; l++) CM(T, m, l, i) *= -tau[i];
+		CM(T, m, i, i) = 1 - tau[i];
+		for (ct_long l = 0; l < i; l++) CM(T, m, l, i) = 0;
+	}
+	for (ct_long i = 0; i < m; i++) for (ct_long j = 0; j < k; j++) q[i * k + j] = CM(T, m, i, j);
+	return 0;
+}
+
+int host_rq(ct_long m, ct_long n, const double* a, double* r, double* q)
+{
+	const ct_long k = std::min(m, n);
+	std::vector<double> t((size_t)(m * n)), tau((size_t)k);
+	for (ct_long i = 0; i < m; i++) for (ct_long j = 0; j < n; j++) CM(t.data(), m, i, j) = a[i * n + j];
+	double* T = t.data();
+	for (ct_long i = k - 1; i >= 0; i--) {
+		const ct_long row = m - k + i, col = n - k + i;
+		larfg(col + 1, CM(T, m, row, col), &CM(T, m, row, 0), m, tau[i]);
+		const double aii = CM(T, m, row, col);
+		CM(T, m, row, col) = 1;
+		apply_right(row, col + 1, &CM(T, m, row, 0), m, tau[i], T, m);
+		CM(T, m, row, col) = aii;
+	}
+	const ct_long rshift = std::max<ct_long>(m - k, 0), cshift = std::max<ct_long>(n - m, 0);
+	for (ct_long i = 0; i < m; i++) {
+		const ct_long nz = std::max<ct_long>(i - rshift, 0);
+		for (ct_long j = 0; j < k; j++) r[i * k + j] = (j < nz ? 0 : CM(T, m, i, cshift + j));
+	}
+	std::vector<double> qt((size_t)(k * n));
+	double* Q = qt.data();
+	for (ct_long i = 0; i < k; i++) for (ct_long j = 0; j < n; j++) CM(Q, k, i, j) = CM(T, m, rshift + i, j);
+	for (ct_long i = 0; i < k; i++) {
+		const ct_long col = n - k + i;
+		CM(Q, k, i, col) = 1;
+		apply_right(i, col + 1, &CM(Q, k, i, 0), k, tau[i], Q, k);
+		for (ct_long j = 0; j < col; j++) CM(Q, k, i, j) *= -tau[i];
+		CM(Q, k, i, col) = 1 - tau[i];
+		for (ct_long j = col + 1; j < n; j++) CM(Q, k, i, j) = 0;
+	}
+	for (ct_long i = 0; i < k; i++) for (ct_long j = 0; j < n; j++) q[i * n + j] = CM(Q, k, i, j);
+	return 0;
+}
+
+// one-sided Jacobi on the columns of g (M x N column-major, M >= N); v accumulates rotations
+static int jacobi(ct_long M, ct_long N, double* g, double* v)
+{
+	std::fill(v, v + (size_t)(N * N), 0.0);
+	for (ct_long i = 0; i < N; i++) CM(v, N, i, i) = 1;
+	const double tol = 2.2204460492503131e-16 * std::max(std::sqrt((double)M), 4.0);
+	for (int sweep = 0; sweep < 80; sweep++) {
+		bool rotated = false;
+		for (ct_long p = 0; p < N - 1; p++) {
+			for (ct_long q = p + 1; q < N; q++) {
+				double* gp = &CM(g, M, 0, p);
+				double* gq = &CM(g, M, 0, q);
+				double al = 0, be = 0, ga = 0;
+				for (ct_long i = 0; i < M; i++) { al += gp[i] * gp[i]; be += gq[i] * gq[i]; ga += gp[i] * gq[i]; }
+				if (ga == 0 || std::fabs(ga) <= tol * std::sqrt(al * be)) continue;
+				rotated = true;
+				const double zeta = (be - al) / (2 * ga);
+				const double t = std::copysign(1.0, zeta) / (std::fabs(zeta) + std::sqrt(1 + zeta * zeta));
+				const double c = 1 / std::sqrt(1 + t * t), s = c * t;
+				for (ct_long i = 0; i < M; i++) { const double x = gp[i], y = gq[i]; gp[i] = c * x - s * y; gq[i] = s * x + c * y; }
+				double* vp = &CM(v, N, 0, p);
+				double* vq = &CM(v, N, 0, q);
+				for (ct_long i = 0; i < N; i++) { const double x = vp[i], y = vq[i]; vp[i] = c * x - s * y; vq[i] = s * x + c * y; }
+			}
+		}
+		if (!rotated) return 0;
+	}
+	return -1;
+}
+
+int host_svd(ct_long m, ct_long n, const double* a, double* u, double* s, double* vh)
+{
+	const ct_long k = std::min(m, n);
+	const bool tall = (m >= n);
+	const ct_long M = tall ? m : n, N = tall ? n : m;
+	std::vector<double> g((size_t)(M * N)), v((size_t)(N * N));
+	for (ct_long i = 0; i < m; i++) for (ct_long j = 0; j < n; j++) {
+		if (tall) CM(g.data(), M, i, j) = a[i * n + j];
+		else CM(g.data(), M, j, i) = a[i * n + j];
+	}
+	if (jacobi(M, N, g.data(), v.data()) != 0) return -1;
+	std::vector<double> sv((size_t)N);
+	for (ct_long j = 0; j < N; j++) sv[j] = nrm2s(M, &CM(g.data(), M, 0, j), 1);
+	std::vector<ct_long> ord((size_t)N);
+	std::iota(ord.begin(), ord.end(), 0);
+	std::stable_sort(ord.begin(), ord.end(), [&](ct_long x, ct_long y) { return sv[x] > sv[y]; });
+	std::vector<double> lu((size_t)(M * N), 0.0);
+	std::vector<char> valid((size_t)N, 0);
+	for (ct_long jj = 0; jj < N; jj++) {
+		const ct_long j = ord[jj];
+		s[jj] = sv[j];
+		valid[jj] = sv[j] > 0;
+		if (valid[jj]) for (ct_long i = 0; i < M; i++) CM(lu.data(), M, i, jj) = CM(g.data(), M, i, j) / sv[j];
+	}
+	// complete columns of zero singular values to an orthonormal set
+	for (ct_long j = 0; j < N; j++) {
+		if (valid[j]) continue;
+		double* x = &CM(lu.data(), M, 0, j);
+		for (ct_long e = 0; e < M; e++) {
+			for (ct_long i = 0; i < M; i++) x[i] = (i == e ? 1 : 0);
+			for (int pass = 0; pass < 2; pass++)
+				for (ct_long l = 0; l < N; l++) {
+					if (l == j || !valid[l]) continue;
+					const double* y = &CM(lu.data(), M, 0, l);
+					double d = 0;
+					for (ct_long i = 0; i < M; i++) d += x[i] * y[i];
+					for (ct_long i = 0; i < M; i++) x[i] -= d * y[i];
+				}
+			const double nx = nrm2s(M, x, 1);
+			if (nx > 0.5) { for (ct_long i = 0; i < M; i++) x[i] /= nx; break; }
+		}
+		valid[j] = 1;
+	}
+	if (tall) {
+		for (ct_long i = 0; i < m; i++) for (ct_long j = 0; j < k; j++) u[i * k + j] = CM(lu.data(), M, i, j);
+		for (ct_long jj = 0; jj < k; jj++) for (ct_long i = 0; i < n; i++) vh[jj * n + i] = CM(v.data(), N, i, ord[jj]);
+	}
+	else {
+		for (ct_long i = 0; i < m; i++) for (ct_long jj = 0; jj < k; jj++) u[i * k + jj] = CM(v.data(), N, i, ord[jj]);
+		for (ct_long jj = 0; jj < k; jj++) for (ct_long i = 0; i < n; i++) vh[jj * n + i] = CM(lu.data(), M, i, jj);
+	}
+	return 0;
+}
+
+}  // namespace ctg

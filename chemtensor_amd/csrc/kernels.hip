@@ -1,0 +1,374 @@
+// HIP kernels of libchemtensor_gpu for gfx950 (CDNA4, 64-wide wavefronts).
+//
+//  grouped_gemm_f64   persistent grouped FP64 GEMM: every wavefront owns one 32 x 32 output
+//                     tile at a time (2 x 2 v_mfma_f64_16x16x4 sub-tiles), sums all terms of
+//                     its task in registers and writes the tile once (no atomics, deterministic)
+//  grouped_gemm_split the same schedule with each FP64 operand split into FP32 terms on FP32
+//                     MFMA; every MFMA result is lifted into FP64 accumulators
+//  plain_gemm_f64     one output block x one contracted block per launch (the per-block form)
+//  permute / copy_segments / fill      data movement of transposes, reshapes and slices
+
+#include <hip/hip_runtime.h>
+#include "ct_internal.h"
+
+namespace ctg {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// column offset of task column j in B (term-dependent strides) and in C
+__device__ __forceinline__ int64_t b_col_offset(int j, int nci, int64_t sb_batch, int sb_j)
+{
+	const int jb = j / nci, jc = j - jb * nci;
+	return (int64_t)jb * sb_batch + (int64_t)jc * sb_j;
+}
+
+// One wavefront computes the 32 x 32 tile (m0, n0) of 'tk' summing terms [tk.t0, tk.t1).
+// MFMA 16x16x4 f64 operand layout: lane l supplies A(l % 16, l / 16) and B(l / 16, l % 16);
+// the result holds D(4 (l / 16) + r, l % 16) in register r.
+__device__ __forceinline__ const double* rebase(const double* base, const double* off)
+{
+	return (const double*)((const char*)base + (intptr_t)off);
+}
+
+__device__ __forceinline__ void wave_tile_f64(const GemmTask& tk, const GemmTerm* __restrict__ terms, int m0, int n0, int lane, GemmBases bs)
+{
+	const int i16 = lane & 15, kq = lane >> 4;
+	const bool two_m = (m0 + 16 < tk.M);
+	const bool two_n = (n0 + 16 < tk.N);
+	const int r0 = m0 + i16, r1 = m0 + 16 + i16;
+	const int c0 = n0 + i16, c1 = n0 + 16 + i16;
+	const bool vr0 = r0 < tk.M, vr1 = r1 < tk.M, vc0 = c0 < tk.N, vc1 = c1 < tk.N;
+	double4_t acc00 = { 0, 0, 0, 0 }, acc01 = { 0, 0, 0, 0 }, acc10 = { 0, 0, 0, 0 }, acc11 = { 0, 0, 0, 0 };
+	for (int t = tk.t0; t < tk.t1; t++) {
+		const GemmTerm tm = terms[t];
+		const double* __restrict__ A = rebase(bs.A, tm.A);
+		const double* __restrict__ B = rebase(bs.B, tm.B);
+		const int64_t ao0 = (int64_t)(vr0 ? r0 : 0) * tm.sa_i, ao1 = (int64_t)(vr1 ? r1 : 0) * tm.sa_i;
+		const int64_t bo0 = b_col_offset(vc0 ? c0 : 0, tk.nci, tm.sb_batch, tm.sb_j);
+		const int64_t bo1 = b_col_offset(vc1 ? c1 : 0, tk.nci, tm.sb_batch, tm.sb_j);
+		for (int k0 = 0; k0 < tm.K; k0 += 4) {
+			const int k = k0 + kq;
+			const bool vk = k < tm.K;
+			const int64_t ak = (int64_t)(vk ? k : 0) * tm.sa_k, bk = (int64_t)(vk ? k : 0) * tm.sb_k;
+			const double a0 = (vk && vr0) ? A[ao0 + ak] : 0.0;
+			const double b0 = (vk && vc0) ? B[bo0 + bk] : 0.0;
+			acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
+			if (two_n) {
+				const double b1 = (vk && vc1) ? B[bo1 + bk] : 0.0;
+				acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
+				if (two_m) {
+					const double a1 = (vk && vr1) ? A[ao1 + ak] : 0.0;
+					acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
+					acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+				}
+			}
+			else if (two_m) {
+				const double a1 = (vk && vr1) ? A[ao1 + ak] : 0.0;
+				acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
+			}
+		}
+	}
+	// store (D row = 4 kq + r, column i16 of each sub-tile)
+	const int nci = tk.nci;
+	#pragma unroll
+	for (int sub = 0; sub < 4; sub++) {
+		const int mi = sub >> 1, ni = sub & 1;
+		const double4_t acc = (sub == 0 ? acc00 : sub == 1 ? acc01 : sub == 2 ? acc10 : acc11);
+		const int col = n0 + 16 * ni + i16;
+		if (col >= tk.N) continue;
+		const int jb = col / nci, jc = col - jb * nci;
+		double* __restrict__ C = (double*)rebase(bs.C, tk.C) + (int64_t)jb * tk.sc_batch + jc;
+		#pragma unroll
+		for (int r = 0; r < 4; r++) {
+			const int row = m0 + 16 * mi + 4 * kq + r;
+			if (row < tk.M) {
+				if (tk.accumulate) C[(int64_t)row * tk.sc_i] += acc[r];
+				else C[(int64_t)row * tk.sc_i] = acc[r];
+			}
+		}
+	}
+}
+
+__global__ __launch_bounds__(256) void k_grouped_gemm_f64(const GemmTask* __restrict__ tasks, const GemmTerm* __restrict__ terms,
+	const WorkItem* __restrict__ items, int nitems, GemmBases bs)
+{
+	const int lane = threadIdx.x & 63;
+	const int nwaves = gridDim.x * 4;
+	for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < nitems; w += nwaves) {
+		const WorkItem it = items[w];
+		const GemmTask tk = tasks[it.task];
+		wave_tile_f64(tk, terms, it.m0, it.n0, lane, bs);
+	}
+}
+
+__global__ __launch_bounds__(256) void k_plain_gemm_f64(GemmTask tk, GemmTerm tm, GemmBases bs)
+{
+	const int lane = threadIdx.x & 63;
+	const int tiles_n = (tk.N + TILE - 1) / TILE;
+	const int tiles = tiles_n * ((tk.M + TILE - 1) / TILE);
+	const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+	if (w >= tiles) return;
+	tk.t0 = 0;
+	tk.t1 = 1;
+	wave_tile_f64(tk, &tm, (w / tiles_n) * TILE, (w % tiles_n) * TILE, lane, bs);
+}
+
+//________________________________________________________________________________________________________________________
+//
+// split-FP32 form (FP64 operands on FP32 MFMA)
+//
+// For every k-step of four products, the four A values of a row share one power-of-two scale
+// 2^EA (the largest |a| of the group is below 2^EA), likewise the four B values of a column
+// (2^EB). Each scaled operand is cut into five 10-bit integer slices, x = sum_t x_t 2^(-10(t+1)).
+// A slice product is an integer below 2^20, so one v_mfma_f32_16x16x4f32 sums its four products
+// exactly, and up to four slice pairs of one level t+u can share an FP32 accumulator (< 2^24).
+// Levels t+u <= 4 are kept (15 slice products); each level's exact FP32 sum is scaled by
+// 2^(EA+EB-10(t+u+2)) and added into FP64 accumulators. The dropped levels and the truncated
+// slices leave an error below 2^-49 |a||b| per product.
+
+constexpr int SPLIT_BITS = 10;
+constexpr int SPLIT_TERMS = 5;
+
+__device__ __forceinline__ int group_exponent(double x)
+{
+	double m = fabs(x);
+	m = fmax(m, __shfl_xor(m, 16));
+	m = fmax(m, __shfl_xor(m, 32));
+	return (m == 0.0) ? 0 : (ilogb(m) + 1);
+}
+
+__device__ __forceinline__ void split_slices(double x, int e, float* sl)
+{
+	double v = ldexp(x, -e);
+	#pragma unroll
+	for (int t = 0; t < SPLIT_TERMS; t++) {
+		v *= (double)(1 << SPLIT_BITS);
+		const double it = trunc(v);
+		sl[t] = (float)it;
+		v -= it;
+	}
+}
+
+__device__ __forceinline__ void wave_tile_split(const GemmTask& tk, const GemmTerm* __restrict__ terms, int m0, int n0, int lane, GemmBases bsp)
+{
+	const int i16 = lane & 15, kq = lane >> 4;
+	double acc[4][4];
+	#pragma unroll
+	for (int s = 0; s < 4; s++)
+		#pragma unroll
+		for (int r = 0; r < 4; r++) acc[s][r] = 0;
+	const int rr[2] = { m0 + i16, m0 + 16 + i16 };
+	const int cc[2] = { n0 + i16, n0 + 16 + i16 };
+	for (int t = tk.t0; t < tk.t1; t++) {
+		const GemmTerm tm = terms[t];
+		const double* __restrict__ A = rebase(bsp.A, tm.A);
+		const double* __restrict__ B = rebase(bsp.B, tm.B);
+		int64_t bo[2];
+		#pragma unroll
+		for (int h = 0; h < 2; h++) bo[h] = b_col_offset(cc[h] < tk.N ? cc[h] : 0, tk.nci, tm.sb_batch, tm.sb_j);
+		for (int k0 = 0; k0 < tm.K; k0 += 4) {
+			const int k = k0 + kq;
+			const bool vk = k < tm.K;
+			float as[2][SPLIT_TERMS], bs[2][SPLIT_TERMS];
+			int ea[2], eb[2];
+			#pragma unroll
+			for (int h = 0; h < 2; h++) {
+				const double av = (vk && rr[h] < tk.M) ? A[(int64_t)rr[h] * tm.sa_i + (int64_t)k * tm.sa_k] : 0.0;
+				const double bv = (vk && cc[h] < tk.N) ? B[bo[h] + (int64_t)k * tm.sb_k] : 0.0;
+				ea[h] = group_exponent(av);
+				eb[h] = group_exponent(bv);
+				split_slices(av, ea[h], as[h]);
+				split_slices(bv, eb[h], bs[h]);
+			}
+			#pragma unroll
+			for (int s = 0; s < 4; s++) {
+				const int mi = s >> 1, ni = s & 1;
+				float4_t lv[6];
+				#pragma unroll
+				for (int q = 0; q < 6; q++) lv[q] = float4_t{ 0, 0, 0, 0 };
+				// level L = t + u accumulates in lv[L] (level 4 uses lv[4] and lv[5])
+				#pragma unroll
+				for (int ta = 0; ta < SPLIT_TERMS; ta++) {
+					#pragma unroll
+					for (int ub = 0; ub < SPLIT_TERMS - ta; ub++) {
+						const int L = ta + ub;
+						const int slot = (L == 4 && ta >= 3) ? 5 : L;
+						lv[slot] = __builtin_amdgcn_mfma_f32_16x16x4f32(as[mi][ta], bs[ni][ub], lv[slot], 0, 0, 0);
+					}
+				}
+				#pragma unroll
+				for (int r = 0; r < 4; r++) {
+					double sum = 0;
+					#pragma unroll
+					for (int q = 5; q >= 0; q--) {
+						const int L = (q == 5 ? 4 : q);
+						sum += ldexp((double)lv[q][r], -SPLIT_BITS * (L + 2));
+					}
+					const int erow = __shfl(ea[mi], 4 * kq + r);
+					acc[s][r] += ldexp(sum, erow + eb[ni]);
+				}
+			}
+		}
+	}
+	#pragma unroll
+	for (int s = 0; s < 4; s++) {
+		const int mi = s >> 1, ni = s & 1;
+		const int col = n0 + 16 * ni + i16;
+		if (col >= tk.N) continue;
+		const int jb = col / tk.nci, jc = col - jb * tk.nci;
+		double* __restrict__ C = (double*)rebase(bsp.C, tk.C) + (int64_t)jb * tk.sc_batch + jc;
+		#pragma unroll
+		for (int r = 0; r < 4; r++) {
+			const int row = m0 + 16 * mi + 4 * kq + r;
+			if (row < tk.M) {
+				if (tk.accumulate) C[(int64_t)row * tk.sc_i] += acc[s][r];
+				else C[(int64_t)row * tk.sc_i] = acc[s][r];
+			}
+		}
+	}
+}
+
+__global__ __launch_bounds__(256) void k_grouped_gemm_split(const GemmTask* __restrict__ tasks, const GemmTerm* __restrict__ terms,
+	const WorkItem* __restrict__ items, int nitems, GemmBases bs)
+{
+	const int lane = threadIdx.x & 63;
+	const int nwaves = gridDim.x * 4;
+	for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < nitems; w += nwaves) {
+		const WorkItem it = items[w];
+		const GemmTask tk = tasks[it.task];
+		wave_tile_split(tk, terms, it.m0, it.n0, lane, bs);
+	}
+}
+
+//________________________________________________________________________________________________________________________
+//
+// data movement
+
+struct PermArgs
+{
+	int64_t sstride[8];   // source stride of each destination axis
+	int64_t ddim[8];      // destination dimensions
+	int ndim;
+	int64_t n;
+};
+
+__global__ __launch_bounds__(256) void k_permute(const double* __restrict__ src, double* __restrict__ dst, PermArgs a)
+{
+	for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.n; e += (int64_t)gridDim.x * blockDim.x) {
+		int64_t rem = e, so = 0;
+		for (int i = a.ndim - 1; i >= 0; i--) {
+			const int64_t q = rem / a.ddim[i];
+			so += (rem - q * a.ddim[i]) * a.sstride[i];
+			rem = q;
+		}
+		dst[e] = src[so];
+	}
+}
+
+__global__ __launch_bounds__(256) void k_permute_blocks(const double* __restrict__ src, double* __restrict__ dst, const PermBlock* __restrict__ blocks, int nblocks)
+{
+	for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+		const PermBlock pb = blocks[b];
+		for (int64_t e = threadIdx.x; e < pb.n; e += blockDim.x) {
+			int64_t rem = e, so = 0;
+			for (int i = pb.ndim - 1; i >= 0; i--) {
+				const int64_t q = rem / pb.ddim[i];
+				so += (rem - q * pb.ddim[i]) * pb.sstride[i];
+				rem = q;
+			}
+			dst[pb.dst + e] = src[pb.src + so];
+		}
+	}
+}
+
+__global__ __launch_bounds__(256) void k_copy_segments(const double* __restrict__ src, double* __restrict__ dst, const CopySeg* __restrict__ segs, int nsegs)
+{
+	for (int s = blockIdx.x; s < nsegs; s += gridDim.x) {
+		const CopySeg g = segs[s];
+		for (int64_t i = threadIdx.x; i < g.len; i += blockDim.x) dst[g.dst + i] = src[g.src + i];
+	}
+}
+
+__global__ __launch_bounds__(256) void k_fill(double* __restrict__ p, double v, int64_t n)
+{
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+//________________________________________________________________________________________________________________________
+//
+// launchers
+
+int num_compute_units()
+{
+	static int cus = 0;
+	if (cus == 0) {
+		int dev = 0;
+		CTG_HIP(hipGetDevice(&dev));
+		hipDeviceProp_t prop;
+		CTG_HIP(hipGetDeviceProperties(&prop, dev));
+		cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+	}
+	return cus;
+}
+
+void launch_grouped_gemm_f64(const DevGemm& g, GemmBases b, hipStream_t s)
+{
+	if (g.nitems == 0) return;
+	hipLaunchKernelGGL(k_grouped_gemm_f64, dim3(g.grid), dim3(256), 0, s, g.tasks, g.terms, g.items, g.nitems, b);
+	CTG_HIP(hipGetLastError());
+}
+
+void launch_grouped_gemm_split(const DevGemm& g, GemmBases b, hipStream_t s)
+{
+	if (g.nitems == 0) return;
+	hipLaunchKernelGGL(k_grouped_gemm_split, dim3(g.grid), dim3(256), 0, s, g.tasks, g.terms, g.items, g.nitems, b);
+	CTG_HIP(hipGetLastError());
+}
+
+void launch_plain_gemm_f64(const GemmTask& task, const GemmTerm& term, GemmBases b, hipStream_t s)
+{
+	const int tiles = ((task.M + TILE - 1) / TILE) * ((task.N + TILE - 1) / TILE);
+	if (tiles == 0) return;
+	hipLaunchKernelGGL(k_plain_gemm_f64, dim3((tiles + 3) / 4), dim3(256), 0, s, task, term, b);
+	CTG_HIP(hipGetLastError());
+}
+
+void launch_permute(const double* src, double* dst, int ndim, const ct_long* dims, const int* perm, hipStream_t s)
+{
+	PermArgs a;
+	a.ndim = ndim;
+	int64_t st[8];
+	int64_t f = 1;
+	for (int i = ndim - 1; i >= 0; i--) { st[i] = f; f *= dims[i]; }
+	a.n = f;
+	for (int i = 0; i < ndim; i++) { a.ddim[i] = dims[perm[i]]; a.sstride[i] = st[perm[i]]; }
+	if (a.n == 0) return;
+	const int64_t nb = (a.n + 255) / 256;
+	hipLaunchKernelGGL(k_permute, dim3((unsigned)(nb < 4096 ? nb : 4096)), dim3(256), 0, s, src, dst, a);
+	CTG_HIP(hipGetLastError());
+}
+
+void launch_permute_blocks(const double* src, double* dst, const PermBlock* d_blocks, int nblocks, hipStream_t s)
+{
+	if (nblocks == 0) return;
+	hipLaunchKernelGGL(k_permute_blocks, dim3(nblocks < 8192 ? nblocks : 8192), dim3(256), 0, s, src, dst, d_blocks, nblocks);
+	CTG_HIP(hipGetLastError());
+}
+
+void launch_copy_segments(const double* src, double* dst, const CopySeg* d_segs, int nsegs, hipStream_t s)
+{
+	if (nsegs == 0) return;
+	hipLaunchKernelGGL(k_copy_segments, dim3(nsegs < 8192 ? nsegs : 8192), dim3(256), 0, s, src, dst, d_segs, nsegs);
+	CTG_HIP(hipGetLastError());
+}
+
+void launch_fill(double* p, double v, ct_long n, hipStream_t s)
+{
+	if (n <= 0) return;
+	const int64_t nb = (n + 255) / 256;
+	hipLaunchKernelGGL(k_fill, dim3((unsigned)(nb < 8192 ? nb : 8192)), dim3(256), 0, s, p, v, (int64_t)n);
+	CTG_HIP(hipGetLastError());
+}
+
+}  // namespace ctg

@@ -1,0 +1,172 @@
+// Device-resident Lanczos ground state (src/util/krylov.c:24-89 and :172-251 with numeig = 1):
+// three-term recurrence without re-orthogonalisation, breakdown test beta < 100 n eps, all
+// vectors and the alpha / beta scalars on the GPU in FP64. Reductions are two-pass with a
+// fixed summation order (deterministic). The tridiagonal eigenproblem (at most maxiter x
+// maxiter) is solved on the host in FP64; the Ritz vector is formed on the device.
+
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+
+#include "ct_internal.h"
+
+namespace ctg {
+
+constexpr int LZ_THREADS = 256;
+
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+	sh[threadIdx.x] = v;
+	__syncthreads();
+	for (int s = LZ_THREADS / 2; s > 0; s >>= 1) {
+		if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+		__syncthreads();
+	}
+	const double r = sh[0];
+	__syncthreads();
+	return r;
+}
+
+// sum of the G partials in a fixed order, identical in every block
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int G, double* sh)
+{
+	double v = 0;
+	for (int b = threadIdx.x; b < G; b += LZ_THREADS) v += part[b];
+	return block_sum(v, sh);
+}
+
+// scal layout: [0, maxiter) alpha, [maxiter, 2 maxiter) beta, [2 maxiter] numiter, [2 maxiter + 1] stop flag
+__global__ __launch_bounds__(LZ_THREADS) void k_lz_dot_partial(const double* __restrict__ x, const double* __restrict__ y, int64_t n,
+	double* __restrict__ part, const double* __restrict__ stop)
+{
+	__shared__ double sh[LZ_THREADS];
+	if (stop != nullptr && *stop != 0) return;
+	double v = 0;
+	for (int64_t i = (int64_t)blockIdx.x * LZ_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * LZ_THREADS) v += x[i] * y[i];
+	const double s = block_sum(v, sh);
+	if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// v0 = vstart / ||vstart||
+__global__ __launch_bounds__(LZ_THREADS) void k_lz_normalize(const double* __restrict__ vs, double* __restrict__ v0, int64_t n, const double* __restrict__ part, int G)
+{
+	__shared__ double sh[LZ_THREADS];
+	const double nrm = sqrt(sum_partials(part, G, sh));
+	const double inv = 1 / nrm;
+	for (int64_t i = (int64_t)blockIdx.x * LZ_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * LZ_THREADS) v0[i] = inv * vs[i];
+}
+
+// alpha_j = sum(part); w -= alpha_j v_j + beta_{j-1} v_{j-1}; part2 = partial sums of w^2
+__global__ __launch_bounds__(LZ_THREADS) void k_lz_alpha_update(double* __restrict__ w, const double* __restrict__ vj, const double* __restrict__ vjm1,
+	int64_t n, int j, int maxiter, const double* __restrict__ part, int G, double* __restrict__ part2, double* __restrict__ scal)
+{
+	__shared__ double sh[LZ_THREADS];
+	if (scal[2 * maxiter + 1] != 0) return;
+	const double alpha = sum_partials(part, G, sh);
+	if (blockIdx.x == 0 && threadIdx.x == 0) scal[j] = alpha;
+	if (j == maxiter - 1) return;   // final iteration: alpha only
+	const double bprev = (j > 0) ? scal[maxiter + j - 1] : 0.0;
+	double v = 0;
+	for (int64_t i = (int64_t)blockIdx.x * LZ_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * LZ_THREADS) {
+		double wi = w[i];
+		if (j > 0) wi -= alpha * vj[i] + bprev * vjm1[i];
+		else wi -= alpha * vj[i];
+		w[i] = wi;
+		v += wi * wi;
+	}
+	const double s = block_sum(v, sh);
+	if (threadIdx.x == 0) part2[blockIdx.x] = s;
+}
+
+// beta_j = ||w||; stop if beta_j < thresh, else v_{j+1} = w / beta_j
+__global__ __launch_bounds__(LZ_THREADS) void k_lz_beta_scale(const double* __restrict__ w, double* __restrict__ vnext, int64_t n, int j, int maxiter,
+	const double* __restrict__ part2, int G, double thresh, double* __restrict__ scal)
+{
+	__shared__ double sh[LZ_THREADS];
+	if (scal[2 * maxiter + 1] != 0) return;
+	const double beta = sqrt(sum_partials(part2, G, sh));
+	// every block has read the stop flag before block 0 may set it (it is only read at kernel entry)
+	if (beta < thresh) {
+		if (blockIdx.x == 0 && threadIdx.x == 0) { scal[maxiter + j] = beta; scal[2 * maxiter] = j + 1; scal[2 * maxiter + 1] = 1; }
+		return;
+	}
+	if (blockIdx.x == 0 && threadIdx.x == 0) scal[maxiter + j] = beta;
+	const double inv = 1 / beta;
+	for (int64_t i = (int64_t)blockIdx.x * LZ_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * LZ_THREADS) vnext[i] = inv * w[i];
+}
+
+// u = sum_j z_j v_j
+__global__ __launch_bounds__(LZ_THREADS) void k_lz_ritz(const double* __restrict__ V, const double* __restrict__ z, int numiter, int64_t n, double* __restrict__ u)
+{
+	for (int64_t i = (int64_t)blockIdx.x * LZ_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * LZ_THREADS) {
+		double acc = 0;
+		for (int j = 0; j < numiter; j++) acc += V[(int64_t)j * n + i] * z[j];
+		u[i] = acc;
+	}
+}
+
+int lanczos_ground_state(MatvecPlan& p, const double* vstart, int maxiter, double* lambda, double* u)
+{
+	if (maxiter < 1) return CT_ERR_ARG;
+	hipStream_t s = p.stream;
+	const int64_t n = p.theta_meta.nelem;
+	const int G = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 4095) / 4096));
+	if (p.lz_cap < maxiter) {
+		dev_free(p.lz_v);
+		dev_free(p.lz_scal);
+		dev_free(p.lz_z);
+		p.lz_v = (double*)dev_alloc((size_t)maxiter * (size_t)n * sizeof(double));
+		p.lz_scal = (double*)dev_alloc((size_t)(2 * maxiter + 2) * sizeof(double));
+		p.lz_z = (double*)dev_alloc((size_t)maxiter * sizeof(double));
+		p.lz_cap = maxiter;
+	}
+	if (!p.lz_w) {
+		p.lz_w = (double*)dev_alloc((size_t)n * sizeof(double));
+		p.lz_part = (double*)dev_alloc(2 * 1024 * sizeof(double));
+	}
+	double* V = p.lz_v;
+	double* w = p.lz_w;
+	double* scal = p.lz_scal;
+	double* part = p.lz_part;
+	double* part2 = p.lz_part + 1024;
+	launch_fill(scal, 0.0, 2 * maxiter + 2, s);
+	const double thresh = 100.0 * (double)n * 2.2204460492503131e-16;
+	hipLaunchKernelGGL(k_lz_dot_partial, dim3(G), dim3(LZ_THREADS), 0, s, vstart, vstart, n, part, (const double*)nullptr);
+	hipLaunchKernelGGL(k_lz_normalize, dim3(G), dim3(LZ_THREADS), 0, s, vstart, V, n, part, G);
+	CTG_HIP(hipGetLastError());
+	for (int j = 0; j < maxiter; j++) {
+		const double* vj = V + (int64_t)j * n;
+		matvec_apply(p, vj, w, nullptr);
+		hipLaunchKernelGGL(k_lz_dot_partial, dim3(G), dim3(LZ_THREADS), 0, s, w, vj, n, part, scal + 2 * maxiter + 1);
+		hipLaunchKernelGGL(k_lz_alpha_update, dim3(G), dim3(LZ_THREADS), 0, s, w, vj, j > 0 ? vj - n : vj, n, j, maxiter, part, G, part2, scal);
+		if (j < maxiter - 1)
+			hipLaunchKernelGGL(k_lz_beta_scale, dim3(G), dim3(LZ_THREADS), 0, s, w, V + (int64_t)(j + 1) * n, n, j, maxiter, part2, G, thresh, scal);
+		CTG_HIP(hipGetLastError());
+	}
+	std::vector<double> hs((size_t)(2 * maxiter + 2));
+	CTG_HIP(hipMemcpyAsync(hs.data(), scal, hs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+	CTG_HIP(hipStreamSynchronize(s));
+	const int numiter = (hs[2 * maxiter + 1] != 0) ? (int)hs[2 * maxiter] : maxiter;
+	if (numiter < 1) {
+		fprintf(stderr, "Lanczos iteration stopped after %i iterations, cannot compute %i eigenvalues\n", numiter, 1);
+		return -1;
+	}
+	std::vector<double> d(hs.begin(), hs.begin() + numiter);
+	std::vector<double> e((size_t)numiter + 1, 0.0);
+	for (int i = 0; i < numiter - 1; i++) e[i] = hs[maxiter + i];
+	std::vector<double> z((size_t)numiter * numiter);
+	if (host_tridiag_eigh(numiter, d.data(), e.data(), z.data()) != 0) {
+		fprintf(stderr, "tridiagonal eigensolver failed\n");
+		return -2;
+	}
+	*lambda = d[0];
+	std::vector<double> z0(z.begin(), z.begin() + numiter);   // eigenvector of the smallest eigenvalue (column 0)
+	CTG_HIP(hipMemcpyAsync(p.lz_z, z0.data(), z0.size() * sizeof(double), hipMemcpyHostToDevice, s));
+	const int gr = (int)std::max<int64_t>(1, std::min<int64_t>(4096, (n + LZ_THREADS - 1) / LZ_THREADS));
+	hipLaunchKernelGGL(k_lz_ritz, dim3(gr), dim3(LZ_THREADS), 0, s, V, p.lz_z, numiter, n, u);
+	CTG_HIP(hipGetLastError());
+	CTG_HIP(hipStreamSynchronize(s));
+	return 0;
+}
+
+}  // namespace ctg

@@ -1,0 +1,170 @@
+// Device runtime of libchemtensor_gpu: HIP error handling, the library stream, a caching
+// device allocator (the DMRG sweep allocates and frees tensors of recurring sizes at every
+// bond, which must not turn into hipMalloc/hipFree calls) and host <-> device transfers of
+// block-sparse tensors in their packed (serialize-order) layout.
+
+#include <map>
+#include <mutex>
+#include <cstring>
+
+#include "ct_internal.h"
+
+namespace ctg {
+
+void hip_check(hipError_t e, const char* what, const char* file, int line)
+{
+	if (e == hipSuccess) return;
+	fprintf(stderr, "chemtensor_gpu: HIP error '%s' (%d) in %s at %s:%d\n", hipGetErrorString(e), (int)e, what, file, line);
+	throw HipError{ e, what };
+}
+
+hipStream_t default_stream()
+{
+	static hipStream_t s = nullptr;
+	static std::once_flag once;
+	std::call_once(once, [] { CTG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); });
+	return s;
+}
+
+//________________________________________________________________________________________________________________________
+//
+// caching allocator: freed blocks are kept per rounded size and handed out again
+
+namespace {
+
+struct Pool
+{
+	std::mutex mu;
+	std::multimap<size_t, void*> free_blocks;
+	std::map<void*, size_t> live;
+	size_t bytes_live = 0, bytes_cached = 0;
+};
+
+Pool& pool()
+{
+	static Pool* p = new Pool();
+	return *p;
+}
+
+size_t round_size(size_t n)
+{
+	if (n < 512) return 512;
+	// 1/8 binade granularity
+	size_t b = 1;
+	while ((b << 3) < n) b <<= 1;
+	return (n + b - 1) / b * b;
+}
+
+}  // namespace
+
+void* dev_alloc(size_t nbytes)
+{
+	Pool& p = pool();
+	const size_t sz = round_size(nbytes);
+	{
+		std::lock_guard<std::mutex> g(p.mu);
+		auto it = p.free_blocks.lower_bound(sz);
+		if (it != p.free_blocks.end() && it->first <= sz + sz / 4) {
+			void* ptr = it->second;
+			p.live[ptr] = it->first;
+			p.bytes_live += it->first;
+			p.bytes_cached -= it->first;
+			p.free_blocks.erase(it);
+			return ptr;
+		}
+	}
+	void* ptr = nullptr;
+	hipError_t e = hipMalloc(&ptr, sz);
+	if (e != hipSuccess) {
+		// release the cache and retry once
+		dev_trim();
+		(void)hipGetLastError();
+		CTG_HIP(hipMalloc(&ptr, sz));
+	}
+	std::lock_guard<std::mutex> g(p.mu);
+	p.live[ptr] = sz;
+	p.bytes_live += sz;
+	return ptr;
+}
+
+void dev_free(void* ptr)
+{
+	if (!ptr) return;
+	Pool& p = pool();
+	std::lock_guard<std::mutex> g(p.mu);
+	auto it = p.live.find(ptr);
+	if (it == p.live.end()) return;
+	p.free_blocks.emplace(it->second, ptr);
+	p.bytes_cached += it->second;
+	p.bytes_live -= it->second;
+	p.live.erase(it);
+}
+
+void dev_trim()
+{
+	Pool& p = pool();
+	std::lock_guard<std::mutex> g(p.mu);
+	if (!p.free_blocks.empty()) (void)hipDeviceSynchronize();
+	for (auto& kv : p.free_blocks) (void)hipFree(kv.second);
+	p.free_blocks.clear();
+	p.bytes_cached = 0;
+}
+
+size_t dev_bytes_in_use()
+{
+	Pool& p = pool();
+	std::lock_guard<std::mutex> g(p.mu);
+	return p.bytes_live;
+}
+
+//________________________________________________________________________________________________________________________
+//
+// device tensors
+
+DevTensor::DevTensor(Meta m) : meta(std::move(m))
+{
+	d = (double*)dev_alloc((size_t)(meta.nelem > 0 ? meta.nelem : 1) * sizeof(double));
+}
+
+DevTensor::DevTensor(DevTensor&& o) noexcept : meta(std::move(o.meta)), d(o.d) { o.d = nullptr; }
+
+DevTensor& DevTensor::operator=(DevTensor&& o) noexcept
+{
+	if (this != &o) {
+		dev_free(d);
+		meta = std::move(o.meta);
+		d = o.d;
+		o.d = nullptr;
+	}
+	return *this;
+}
+
+DevTensor::~DevTensor() { dev_free(d); }
+
+void DevTensor::zero(hipStream_t s) { launch_fill(d, 0.0, meta.nelem, s); }
+
+DevTensor upload(const block_sparse_tensor* t, hipStream_t s)
+{
+	if (t->dtype != CT_DOUBLE_REAL) throw HipError{ hipErrorInvalidValue, "only CT_DOUBLE_REAL tensors are supported" };
+	DevTensor dt(meta_of(t));
+	std::vector<double> host((size_t)dt.meta.nelem);
+	block_sparse_tensor_serialize_entries(t, host.data());
+	CTG_HIP(hipMemcpyAsync(dt.d, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, s));
+	CTG_HIP(hipStreamSynchronize(s));
+	return dt;
+}
+
+void download(const DevTensor& t, block_sparse_tensor* out, hipStream_t s)
+{
+	const Meta& m = t.meta;
+	std::vector<const qnumber*> qn(m.ndim);
+	std::vector<tensor_axis_direction> ad(m.ndim);
+	for (int i = 0; i < m.ndim; i++) { qn[i] = m.qnums_logical[i].data(); ad[i] = (tensor_axis_direction)m.axis_dir[i]; }
+	allocate_block_sparse_tensor(CT_DOUBLE_REAL, m.ndim, m.dim_logical.data(), ad.data(), qn.data(), out);
+	std::vector<double> host((size_t)m.nelem);
+	CTG_HIP(hipMemcpyAsync(host.data(), t.d, host.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+	CTG_HIP(hipStreamSynchronize(s));
+	block_sparse_tensor_deserialize_entries(out, host.data());
+}
+
+}  // namespace ctg

@@ -1,0 +1,194 @@
+/// \file chemtensor_gpu.h
+/// \brief C ABI of libchemtensor_gpu.so: an MI355X (gfx950) drop-in for chemtensor's two-site
+/// DMRG path (apply_local_hamiltonian over abelian block-sparse tensors inside the Lanczos
+/// ground-state iteration of dmrg_twosite).
+///
+/// Data structures have the reference's exact memory layout; functions named like the
+/// reference's keep its signature, argument meaning, ownership and error convention:
+/// results are allocated by the callee (free()-compatible, 16-byte aligned), inputs are const,
+/// int returns are 0 on success and negative on failure. Only CT_DOUBLE_REAL entries are
+/// supported. Errors of the HIP runtime are reported as CT_ERR_HIP (and on stderr).
+///
+/// Paths in citations are relative to the chemtensor source tree.
+
+#pragma once
+
+#include <stdint.h>
+#include <stdbool.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+//________________________________________________________________________________________________________________________
+//
+// types (byte-identical to the reference)
+
+typedef int64_t ct_long;   // include/util/util.h:12
+typedef int32_t qnumber;   // include/tensor/qnumber.h:17
+
+enum numeric_type { CT_SINGLE_REAL = 0, CT_DOUBLE_REAL = 1, CT_SINGLE_COMPLEX = 2, CT_DOUBLE_COMPLEX = 3, CT_NUM_NUMERIC_TYPES = 4 };  // include/numeric.h:15-22
+enum tensor_axis_direction { TENSOR_AXIS_IN = -1, TENSOR_AXIS_OUT = 1 };                                                           // include/tensor/qnumber.h:76-80
+enum tensor_axis_range { TENSOR_AXIS_RANGE_LEADING = 0, TENSOR_AXIS_RANGE_TRAILING = 1, TENSOR_AXIS_RANGE_NUM = 2 };                // include/tensor/dense_tensor.h:180-185
+enum singular_value_distr { SVD_DISTR_LEFT = 0, SVD_DISTR_RIGHT = 1 };                                                             // include/algorithm/bond_ops.h:15-19
+enum mps_orthonormalization_mode { MPS_ORTHONORMAL_LEFT = 0, MPS_ORTHONORMAL_RIGHT = 1 };                                          // include/state/mps.h:77-81
+
+/// include/tensor/dense_tensor.h:17-23
+struct dense_tensor
+{
+	void* data;
+	ct_long* dim;
+	enum numeric_type dtype;
+	int ndim;
+};
+
+/// include/tensor/block_sparse_tensor.h:18-28
+struct block_sparse_tensor
+{
+	struct dense_tensor** blocks;
+	ct_long* dim_blocks;
+	ct_long* dim_logical;
+	enum tensor_axis_direction* axis_dir;
+	qnumber** qnums_blocks;
+	qnumber** qnums_logical;
+	enum numeric_type dtype;
+	int ndim;
+};
+
+/// include/state/mps.h:14-20
+struct mps { struct block_sparse_tensor* a; qnumber* qsite; ct_long d; int nsites; };
+/// include/operator/mpo.h:34-40
+struct mpo { struct block_sparse_tensor* a; qnumber* qsite; ct_long d; int nsites; };
+/// include/algorithm/truncation.h:19-24
+struct trunc_info { double norm_sigma; double entropy; double tol_eff; };
+/// include/util/pcg_basic.h (struct pcg32_random) and include/util/rng.h:13-16
+struct pcg32_random { uint64_t state; uint64_t inc; };
+struct rng_state { struct pcg32_random gen[2]; };
+
+/// error codes beyond the reference's (-1 Krylov breakdown / factorisation failure, -2 eigensolver failure)
+#define CT_ERR_HIP        (-100)   ///< a HIP runtime call failed
+#define CT_ERR_UNSUPPORTED (-101)  ///< input outside the supported domain (dtype, outer bond dimension > 1, ...)
+#define CT_ERR_NOMEM      (-102)   ///< device memory exhausted or above the admission budget
+#define CT_ERR_ARG        (-103)   ///< invalid argument
+
+//________________________________________________________________________________________________________________________
+//
+// block-sparse tensors (host side; replaces src/tensor/block_sparse_tensor.c entry points)
+
+/// bst.c:37-150 (declared include/tensor/block_sparse_tensor.h:33)
+void allocate_block_sparse_tensor(const enum numeric_type dtype, const int ndim, const ct_long* dim, const enum tensor_axis_direction* axis_dir, const qnumber** qnums, struct block_sparse_tensor* t);
+/// bst.c:158-162 (block_sparse_tensor.h:34)
+void allocate_block_sparse_tensor_like(const struct block_sparse_tensor* s, struct block_sparse_tensor* t);
+/// bst.c:168-205 (block_sparse_tensor.h:35)
+void delete_block_sparse_tensor(struct block_sparse_tensor* t);
+/// bst.c:213-270 (block_sparse_tensor.h:36)
+void copy_block_sparse_tensor(const struct block_sparse_tensor* src, struct block_sparse_tensor* dst);
+/// bst.c:278-307 (block_sparse_tensor.h:40)
+struct dense_tensor* block_sparse_tensor_get_block(const struct block_sparse_tensor* t, const qnumber* qnums);
+/// bst.c:316-334 (block_sparse_tensor.h:46)
+double block_sparse_tensor_norm2(const struct block_sparse_tensor* t);
+/// bst.c:3111-3124 (block_sparse_tensor.h:219)
+ct_long block_sparse_tensor_num_elements_blocks(const struct block_sparse_tensor* t);
+/// bst.c:3131-3151 (block_sparse_tensor.h:220)
+void block_sparse_tensor_serialize_entries(const struct block_sparse_tensor* t, void* entries);
+/// bst.c:3157-3180 (block_sparse_tensor.h:221)
+void block_sparse_tensor_deserialize_entries(struct block_sparse_tensor* t, const void* entries);
+
+/// bst.c:1826-2000 (block_sparse_tensor.h:167): r = s . t, computed on the GPU by one grouped
+/// FP64 MFMA launch over all (output block, contracted block) pairs. Allocates r.
+void block_sparse_tensor_dot(const struct block_sparse_tensor* s, const enum tensor_axis_range axrange_s, const struct block_sparse_tensor* t, const enum tensor_axis_range axrange_t, const int ndim_mult, struct block_sparse_tensor* r);
+/// bst.c:785-870 (block_sparse_tensor.h:99), on the GPU. Allocates r.
+void block_sparse_tensor_transpose(const int* perm, const struct block_sparse_tensor* t, struct block_sparse_tensor* r);
+
+//________________________________________________________________________________________________________________________
+//
+// chain operations and DMRG (replaces src/algorithm/chain_ops.c, src/algorithm/dmrg.c)
+
+/// chain_ops.c:353-390 (include/algorithm/chain_ops.h:31): b = H_eff a on the GPU. Requires
+/// outer (dummy) bond dimension 1 on l and r; b is allocated. Host buffers in, host buffers out
+/// (the input tensors are copied to the device per call: use ct_matvec_plan_* to keep them resident).
+void apply_local_hamiltonian(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l, const struct block_sparse_tensor* r, struct block_sparse_tensor* b);
+
+/// dmrg.c:262-399 (include/algorithm/dmrg.h:12): two-site DMRG with device-resident MPS,
+/// environments and Lanczos; truncating block SVD/QR in FP64 on the host.
+int dmrg_twosite(const struct mpo* hamiltonian, const int num_sweeps, const int maxiter_lanczos, const double tol_split, const ct_long max_vdim,
+	struct mps* psi, double* en_sweeps, double* entropy);
+
+//________________________________________________________________________________________________________________________
+//
+// MPS, MPO and random numbers (replaces src/state/mps.c, src/operator/mpo.c, src/util/rng.c entry points)
+
+/// rng.c:15-27 (include/util/rng.h:20)
+void seed_rng_state(const uint64_t seed, struct rng_state* state);
+/// rng.c:116-125 (rng.h:50)
+double randn(struct rng_state* state);
+/// mps.c:93-178 (include/state/mps.h:36)
+void construct_random_mps(const enum numeric_type dtype, const int nsites, const ct_long d, const qnumber* qsite, const qnumber qnum_sector, const ct_long max_vdim, struct rng_state* rng_state, struct mps* mps);
+/// mps.c:62-74 (mps.h:32)
+void delete_mps(struct mps* mps);
+/// mpo.c:182-196 (include/operator/mpo.h:52)
+void delete_mpo(struct mpo* mpo);
+/// The operator of construct_heisenberg_xxz_1d_mpo_assembly + mpo_from_assembly
+/// (src/operator/hamiltonian.c:102-155, mpo.c:59-175) built directly as an MPO (bond dim 5).
+void ct_construct_heisenberg_xxz_1d_mpo(const int nsites, const double J, const double D, const double h, struct mpo* mpo);
+/// The operator of construct_fermi_hubbard_1d_mpo_assembly + mpo_from_assembly
+/// (src/operator/hamiltonian.c:240-366, mpo.c:59-175) built directly as an MPO (bond dim 6).
+void ct_construct_fermi_hubbard_1d_mpo(const int nsites, const double t, const double u, const double mu, struct mpo* mpo);
+/// mpo.c:255-282 (mpo.h:56), on the GPU
+void mpo_merge_tensor_pair(const struct block_sparse_tensor* a0, const struct block_sparse_tensor* a1, struct block_sparse_tensor* a);
+
+//________________________________________________________________________________________________________________________
+//
+// device-resident matvec plans (no counterpart in the reference: the reference re-plans and
+// re-allocates inside every block_sparse_tensor_dot call, bst.c:1826-2000)
+
+typedef struct ct_matvec_plan ct_matvec_plan;
+
+/// Precision / launch form of a plan.
+enum ct_matvec_mode
+{
+	CT_MATVEC_PLAIN_FP64   = 0,  ///< reference structure: per-block FP64 GEMM launches + transpose / trace kernels
+	CT_MATVEC_GROUPED_FP64 = 1,  ///< three persistent grouped FP64 MFMA launches, transposes folded into addressing
+	CT_MATVEC_GROUPED_SPLIT = 2, ///< as GROUPED_FP64, operands split into FP32 terms on FP32 MFMA with FP64 accumulation
+};
+
+/// Build a plan for H_eff = apply_local_hamiltonian(., w, l, r) on vectors with the block
+/// structure of 'a' (its entries are ignored). w, l and r are uploaded and stay resident.
+int ct_matvec_plan_create(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l, const struct block_sparse_tensor* r,
+	const int mode, ct_matvec_plan** plan);
+int ct_matvec_plan_destroy(ct_matvec_plan* plan);
+/// length of the packed vectors (= block_sparse_tensor_num_elements_blocks(a))
+ct_long ct_matvec_plan_dim(const ct_matvec_plan* plan);
+/// algorithmic FP64 flops of one matvec: sum over the reference's GEMM calls of 2 m n k
+double ct_matvec_plan_flops(const ct_matvec_plan* plan);
+/// algorithmic bytes of one matvec: 8 (|a| + |b| + |w| + |l| + |r|)
+double ct_matvec_plan_bytes(const ct_matvec_plan* plan);
+/// plan statistics: [0] GEMM terms (stage 1,2,3 summed), [1] output tasks, [2] work items, [3] intermediate doubles
+int ct_matvec_plan_stats(const ct_matvec_plan* plan, double* stats4);
+/// y = H_eff x on device vectors (packed in serialize order) on the plan's stream
+int ct_matvec_apply(ct_matvec_plan* plan, const double* d_x, double* d_y);
+/// as ct_matvec_apply, recording the device time (ms) of each of the three stages
+int ct_matvec_apply_timed(ct_matvec_plan* plan, const double* d_x, double* d_y, float* stage_ms3);
+/// the plan's HIP stream (hipStream_t) as an opaque pointer
+void* ct_matvec_plan_stream(ct_matvec_plan* plan);
+
+/// Lanczos ground state of the plan's H_eff (krylov.c:172-251 with numeig = 1) on the device:
+/// d_vstart / d_u are device vectors of length ct_matvec_plan_dim. Returns 0, -1 (breakdown
+/// before one iteration) or -2 (tridiagonal eigensolver failure).
+int ct_lanczos_ground_state(ct_matvec_plan* plan, const double* d_vstart, const int maxiter, double* lambda, double* d_u);
+
+//________________________________________________________________________________________________________________________
+//
+// device utilities for callers without their own HIP runtime bindings
+
+int ct_device_alloc(void** ptr, const ct_long nbytes);
+int ct_device_free(void* ptr);
+int ct_memcpy_h2d(void* dst, const void* src, const ct_long nbytes);
+int ct_memcpy_d2h(void* dst, const void* src, const ct_long nbytes);
+int ct_device_synchronize(void);
+/// number of visible GPUs (0 when none)
+int ct_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif

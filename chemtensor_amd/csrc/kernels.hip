@@ -25,7 +25,7 @@ __device__ __forceinline__ int64_t b_col_offset(int j, int nci, int64_t sb_batch
 
 // One wavefront computes the 32 x 32 tile (m0, n0) of 'tk' summing terms [tk.t0, tk.t1).
 // MFMA 16x16x4 f64 operand layout: lane l supplies A(l % 16, l / 16) and B(l / 16, l % 16);
-// the result holds D(4 (l / 16) + r, l % 16) in register r.
+// the result holds D(l / 16 + 4 r, l % 16) in register r.
 __device__ __forceinline__ const double* rebase(const double* base, const double* off)
 {
 	return (const double*)((const char*)base + (intptr_t)off);
@@ -69,7 +69,7 @@ __device__ __forceinline__ void wave_tile_f64(const GemmTask& tk, const GemmTerm
 			}
 		}
 	}
-	// store (D row = 4 kq + r, column i16 of each sub-tile)
+	// store: for v_mfma_f64_16x16x4f64 register r of lane l holds D(l / 16 + 4 r, l % 16)
 	const int nci = tk.nci;
 	#pragma unroll
 	for (int sub = 0; sub < 4; sub++) {
@@ -81,7 +81,7 @@ __device__ __forceinline__ void wave_tile_f64(const GemmTask& tk, const GemmTerm
 		double* __restrict__ C = (double*)rebase(bs.C, tk.C) + (int64_t)jb * tk.sc_batch + jc;
 		#pragma unroll
 		for (int r = 0; r < 4; r++) {
-			const int row = m0 + 16 * mi + 4 * kq + r;
+			const int row = m0 + 16 * mi + kq + 4 * r;
 			if (row < tk.M) {
 				if (tk.accumulate) C[(int64_t)row * tk.sc_i] += acc[r];
 				else C[(int64_t)row * tk.sc_i] = acc[r];

@@ -72,6 +72,10 @@ def main():
     ap.add_argument("--model", default="hubbard")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="grouped_fp64,split_fp32,plain_fp64",
+                    help="comma list of matvec forms to time (grouped_fp64 is always included)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB ten times (scale for HBM counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
@@ -82,13 +86,24 @@ def main():
         print("no GPU visible: the HIP product path cannot run here", file=sys.stderr)
         sys.exit(3)
 
+    if args.calibrate_copy:
+        n = args.calibrate_copy * 1024 * 1024 // 8
+        src, dst = ct.DeviceArray(n), ct.DeviceArray(n)
+        for _ in range(10):
+            ct.check(ct.lib().ct_device_copy(dst.ptr, src.ptr, n), "ct_device_copy")
+        print(json.dumps({"calibrate_copy_bytes_read": n * 8, "calibrate_copy_bytes_written": n * 8, "launches": 10}))
+        return
+
     t0 = time.perf_counter()
     theta, h2, left, right = ct.generate_matvec_problem(args.model, args.nsites, args.chi, args.seed)
     log(f"generated inputs in {time.perf_counter() - t0:.2f}s: |theta| = {theta.nelem()}")
     x = ct.DeviceArray.from_numpy(theta.serialize())
 
     forms = {}
+    wanted = set(args.forms.split(",")) | {"grouped_fp64"}
     for name, mode in (("grouped_fp64", ct.GROUPED_FP64), ("split_fp32", ct.GROUPED_SPLIT), ("plain_fp64", ct.PLAIN_FP64)):
+        if name not in wanted:
+            continue
         t0 = time.perf_counter()
         plan = ct.MatvecPlan(theta, h2, left, right, mode)
         y = ct.DeviceArray(plan.dim)
@@ -143,14 +158,16 @@ def main():
         "hbm_gbs_algorithmic": round(plan.bytes / (ms * 1e-3) / 1e9, 2),
         "forms": {k: {"ms_per_matvec": round(v["ms"], 5), "tflops": round(flops / (v["ms"] * 1e-3) / 1e12, 4),
                       "stage_ms": [round(float(s), 5) for s in v["stage_ms"]]} for k, v in forms.items()},
-        "speedup_grouped_vs_plain": round(forms["plain_fp64"]["ms"] / ms, 2),
-        "speedup_split_vs_plain": round(forms["plain_fp64"]["ms"] / forms["split_fp32"]["ms"], 2),
+        "speedup_grouped_vs_plain": round(forms["plain_fp64"]["ms"] / ms, 2) if "plain_fp64" in forms else None,
+        "speedup_split_vs_plain": round(forms["plain_fp64"]["ms"] / forms["split_fp32"]["ms"], 2) if {"plain_fp64", "split_fp32"} <= set(forms) else None,
         "plan_stats": plan.stats(),
+        "autotune": {"forms": plan.forms()[0], "ms_per_form": [[round(x, 5) for x in r] for r in plan.forms()[1]],
+                     "form_names": ["wave_simple", "wave_stream", "wg_chunk", "wg_stream"]},
     }
-    y_s = forms["split_fp32"]["y"].numpy()
-    y_p = forms["plain_fp64"]["y"].numpy()
-    result["rel_err_split_vs_grouped"] = float(np.linalg.norm(y_s - y_g) / np.linalg.norm(y_g))
-    result["rel_err_plain_vs_grouped"] = float(np.linalg.norm(y_p - y_g) / np.linalg.norm(y_g))
+    ys = {k: v["y"].numpy() for k, v in forms.items()}
+    for k in ys:
+        if k != "grouped_fp64":
+            result[f"rel_err_{k}_vs_grouped"] = float(np.linalg.norm(ys[k] - y_g) / np.linalg.norm(y_g))
 
     if not args.no_cpu:
         dt, n, threads, y_cpu = cpu_baseline(args.model, args.nsites, args.chi, args.seed)
@@ -158,7 +175,7 @@ def main():
             "value": round(flops / dt / 1e12, 6), "unit": "TFLOP/s", "cores": threads, "kind": "port",
             "sample": f"{n} oracle matvecs of the same workload ({dt * 1e3:.2f} ms each), OpenMP over output blocks",
         }
-        for k, yy in (("grouped_fp64", y_g), ("split_fp32", y_s), ("plain_fp64", y_p)):
+        for k, yy in ys.items():
             result["forms"][k]["rel_err_vs_cpu_fp64"] = float(np.linalg.norm(yy - y_cpu) / np.linalg.norm(y_cpu))
     print(json.dumps(result), flush=True)
 

@@ -77,6 +77,7 @@ def lib():
             "ct_matvec_plan_bytes": (C.c_double, [P]),
             "ct_matvec_plan_stats": (C.c_int, [P, DP]),
             "ct_matvec_plan_stage_flops": (C.c_int, [P, DP]),
+            "ct_matvec_plan_forms": (C.c_int, [P, IP, FP]),
             "ct_matvec_apply": (C.c_int, [P, P, P]),
             "ct_matvec_apply_timed": (C.c_int, [P, P, P, FP]),
             "ct_matvec_plan_stream": (P, [P]),
@@ -92,6 +93,7 @@ def lib():
             "ct_event_elapsed_ms": (C.c_int, [P, P, FP]),
             "ct_event_destroy": (C.c_int, [P]),
             "ct_stream_synchronize": (C.c_int, [P]),
+            "ct_device_copy": (C.c_int, [P, P, ct_long]),
             "ct_generate_matvec_problem": (C.c_int, [C.c_int, C.c_int, ct_long, C.c_uint64, bst, bst, bst, bst]),
             "ct_construct_heisenberg_xxz_1d_mpo": (None, [C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(MPOStruct)]),
             "ct_construct_fermi_hubbard_1d_mpo": (None, [C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(MPOStruct)]),
@@ -271,6 +273,12 @@ class MatvecPlan:
         s = np.zeros(4)
         lib().ct_matvec_plan_stats(self.h, s.ctypes.data_as(DP))
         return dict(terms=int(s[0]), tasks=int(s[1]), items=int(s[2]), intermediate_doubles=int(s[3]))
+
+    def forms(self):
+        f = np.zeros(3, dtype=np.int32)
+        t = np.zeros(12, dtype=np.float32)
+        lib().ct_matvec_plan_forms(self.h, f.ctypes.data_as(IP), t.ctypes.data_as(FP))
+        return f.tolist(), t.reshape(3, 4).tolist()
 
     def stage_flops(self):
         s = np.zeros(3)

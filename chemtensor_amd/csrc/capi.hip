@@ -24,7 +24,7 @@ extern "C" {
 void apply_local_hamiltonian(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l, const struct block_sparse_tensor* r, struct block_sparse_tensor* b)
 {
 	try {
-		std::unique_ptr<MatvecPlan> p(build_matvec_plan(a, w, l, r, CT_MATVEC_GROUPED_FP64));
+		std::unique_ptr<MatvecPlan> p(build_matvec_plan(a, w, l, r, CT_MATVEC_GROUPED_FP64, false));
 		DevTensor x = upload(a, p->stream);
 		// result axes: l axis 3, w axis 1, r axis 2 (chain_ops.c:388); equal to a's axes here
 		DevTensor y(p->theta_meta);
@@ -85,7 +85,9 @@ int ct_matvec_plan_create(const struct block_sparse_tensor* a, const struct bloc
 	if (!a || !w || !l || !r || !plan || mode < 0 || mode > 2) return CT_ERR_ARG;
 	try {
 		auto* h = new ct_matvec_plan();
-		h->p.reset(build_matvec_plan(a, w, l, r, mode));
+		const char* env = getenv("CT_AUTOTUNE");
+		const bool tune = !(env && env[0] == '0');
+		h->p.reset(build_matvec_plan(a, w, l, r, mode, tune));
 		*plan = h;
 		return 0;
 	}
@@ -108,6 +110,16 @@ int ct_matvec_plan_stage_flops(const ct_matvec_plan* plan, double* flops3)
 {
 	const MatvecPlan& p = *plan->p;
 	for (int i = 0; i < 3; i++) flops3[i] = p.stage_flops[i];
+	return 0;
+}
+
+int ct_matvec_plan_forms(const ct_matvec_plan* plan, int* forms3, float* tuned_ms12)
+{
+	const MatvecPlan& p = *plan->p;
+	for (int i = 0; i < 3; i++) {
+		forms3[i] = p.stage[i].form;
+		for (int f = 0; f < GEMM_NUM_FORMS; f++) tuned_ms12[i * GEMM_NUM_FORMS + f] = p.tuned_ms[i][f];
+	}
 	return 0;
 }
 
@@ -185,6 +197,18 @@ int ct_event_elapsed_ms(void* ev0, void* ev1, float* ms)
 }
 
 int ct_event_destroy(void* ev) { return hipEventDestroy((hipEvent_t)ev) == hipSuccess ? 0 : CT_ERR_HIP; }
+
+int ct_device_copy(double* dst, const double* src, const ct_long n)
+{
+	try {
+		launch_copy(src, dst, n, default_stream());
+		CTG_HIP(hipStreamSynchronize(default_stream()));
+		return 0;
+	}
+	catch (const HipError&) {
+		return CT_ERR_HIP;
+	}
+}
 
 int ct_stream_synchronize(void* stream) { return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? 0 : CT_ERR_HIP; }
 

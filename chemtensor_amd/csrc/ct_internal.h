@@ -125,10 +125,11 @@ struct GemmTask
 
 struct WorkItem
 {
-	int32_t task;
+	GemmTask task;         // the task descriptor travels with the item (one scalar load per item)
 	int32_t m0;
 	int32_t n0;
 	int32_t cost;
+	int32_t task_index;
 };
 
 // tile edge of one wavefront's output tile (2 x 2 MFMA 16x16 sub-tiles)
@@ -145,6 +146,16 @@ struct GemmBatch
 	std::vector<WorkItem> work_items() const;
 };
 
+/// launch forms of the grouped FP64 GEMM (chosen per stage by the plan-time autotuner)
+enum GemmForm
+{
+	GEMM_WAVE_SIMPLE = 0,  ///< one wavefront per 32x32 tile, one k-step per load round trip
+	GEMM_WAVE_STREAM = 1,  ///< one wavefront per tile, loads of 8 k-steps batched across terms
+	GEMM_WG_CHUNK    = 2,  ///< one workgroup per tile, chunks of 4 k-steps dealt round-robin to its wavefronts
+	GEMM_WG_STREAM   = 3,  ///< one workgroup per tile, batches of 8 k-steps dealt round-robin
+	GEMM_NUM_FORMS   = 4,
+};
+
 /// a GemmBatch resident on the device
 struct DevGemm
 {
@@ -153,6 +164,8 @@ struct DevGemm
 	WorkItem* items = nullptr;
 	int ntasks = 0, nterms = 0, nitems = 0;
 	int grid = 0;
+	int form = 1;          // one of GemmForm
+	double avg_steps = 0;  // mean k-steps per work item
 	double flops = 0;
 	void upload(const GemmBatch& b, hipStream_t s);
 	void release();
@@ -172,6 +185,7 @@ void launch_permute_blocks(const double* src, double* dst, const PermBlock* d_bl
 struct CopySeg { int64_t src; int64_t dst; int64_t len; };
 void launch_copy_segments(const double* src, double* dst, const CopySeg* d_segs, int nsegs, hipStream_t s);
 void launch_fill(double* p, double v, ct_long n, hipStream_t s);
+void launch_copy(const double* src, double* dst, ct_long n, hipStream_t s);
 int num_compute_units();
 
 //________________________________________________________________________________________________________________________
@@ -208,6 +222,7 @@ struct MatvecPlan
 	// bookkeeping
 	double flops = 0, bytes = 0;
 	double stage_flops[3] = { 0, 0, 0 };        // executed FP64 flops of each stage
+	float tuned_ms[3][GEMM_NUM_FORMS] = {};     // autotuner timings (ms per launch) of each stage and form
 	double nterms = 0, ntasks = 0, nitems = 0, interm_elems = 0;
 	hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };
 	// Lanczos workspace
@@ -220,8 +235,8 @@ struct MatvecPlan
 	~MatvecPlan();
 };
 
-MatvecPlan* build_matvec_plan(const block_sparse_tensor* a, const block_sparse_tensor* w, const block_sparse_tensor* l, const block_sparse_tensor* r, int mode);
-MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& l, DevTensor&& r, int mode, hipStream_t s);
+MatvecPlan* build_matvec_plan(const block_sparse_tensor* a, const block_sparse_tensor* w, const block_sparse_tensor* l, const block_sparse_tensor* r, int mode, bool autotune);
+MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& l, DevTensor&& r, int mode, hipStream_t s, bool autotune);
 void matvec_apply(MatvecPlan& p, const double* x, double* y, float* stage_ms);
 int lanczos_ground_state(MatvecPlan& p, const double* vstart, int maxiter, double* lambda, double* u);
 

@@ -48,7 +48,7 @@ std::vector<WorkItem> GemmBatch::work_items() const
 		const int cost = (int)std::min<int64_t>(ksum + 4, (int64_t)1 << 30);
 		for (int m0 = 0; m0 < t.M; m0 += TILE)
 			for (int n0 = 0; n0 < t.N; n0 += TILE)
-				items.push_back(WorkItem{ (int32_t)ti, m0, n0, cost });
+				items.push_back(WorkItem{ t, m0, n0, cost, (int32_t)ti });
 	}
 	std::stable_sort(items.begin(), items.end(), [](const WorkItem& a, const WorkItem& b) { return a.cost > b.cost; });
 	return items;
@@ -71,6 +71,15 @@ void DevGemm::upload(const GemmBatch& b, hipStream_t s)
 	CTG_HIP(hipStreamSynchronize(s));
 	const int cap = num_compute_units() * 8;
 	grid = std::max(1, std::min((nitems + 3) / 4, cap));
+	// workgroup-level split of the k-step stream only pays for long streams
+	double steps = 0;
+	for (const WorkItem& w : it) {
+		const GemmTask& t = w.task;
+		for (int q = t.t0; q < t.t1; q++) steps += (b.terms[q].K + 3) / 4;
+	}
+	const double avg = nitems ? steps / nitems : 0;
+	form = (avg > 8.0) ? GEMM_WG_CHUNK : GEMM_WAVE_STREAM;
+	avg_steps = avg;
 }
 
 void DevGemm::release()
@@ -668,7 +677,47 @@ static void build_plain(MatvecPlan& p)
 	for (auto& t : p.interm) p.interm_elems += (double)t.meta.nelem;
 }
 
-MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& l, DevTensor&& r, int mode, hipStream_t s)
+// Plan-time autotuner: every launch form of every stage runs on the plan's own operands (one
+// warm-up and three timed launches, HIP events on the plan stream); the fastest form is kept.
+static void autotune_stages(MatvecPlan& p)
+{
+	hipStream_t s = p.stream;
+	const ct_long n = p.theta_meta.nelem;
+	double* x = (double*)dev_alloc((size_t)std::max<ct_long>(n, 1) * sizeof(double));
+	double* y = (double*)dev_alloc((size_t)std::max<ct_long>(n, 1) * sizeof(double));
+	launch_fill(x, 1.0, n, s);
+	hipEvent_t e0, e1;
+	CTG_HIP(hipEventCreate(&e0));
+	CTG_HIP(hipEventCreate(&e1));
+	for (int st = 0; st < 3; st++) {
+		DevGemm& g = p.stage[st];
+		GemmBases b = p.stage_bases[st];
+		if (st == 0) b.A = x;
+		if (st == 2) b.C = y;
+		float best = 1e30f;
+		int best_form = g.form;
+		for (int f = 0; f < GEMM_NUM_FORMS; f++) {
+			g.form = f;
+			launch_grouped_gemm_f64(g, b, s);
+			CTG_HIP(hipEventRecord(e0, s));
+			for (int rep = 0; rep < 3; rep++) launch_grouped_gemm_f64(g, b, s);
+			CTG_HIP(hipEventRecord(e1, s));
+			CTG_HIP(hipEventSynchronize(e1));
+			float ms = 0;
+			CTG_HIP(hipEventElapsedTime(&ms, e0, e1));
+			p.tuned_ms[st][f] = ms / 3;
+			if (ms < best) { best = ms; best_form = f; }
+		}
+		g.form = best_form;
+	}
+	CTG_HIP(hipEventDestroy(e0));
+	CTG_HIP(hipEventDestroy(e1));
+	CTG_HIP(hipStreamSynchronize(s));
+	dev_free(x);
+	dev_free(y);
+}
+
+MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& l, DevTensor&& r, int mode, hipStream_t s, bool autotune)
 {
 	if (theta.ndim != 3 || w.meta.ndim != 4 || l.meta.ndim != 4 || r.meta.ndim != 4)
 		throw HipError{ hipErrorInvalidValue, "apply_local_hamiltonian expects degrees 3, 4, 4, 4" };
@@ -688,6 +737,7 @@ MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& 
 	try {
 		if (mode == CT_MATVEC_PLAIN_FP64) build_plain(*p);
 		else build_grouped(*p);
+		if (mode == CT_MATVEC_GROUPED_FP64 && autotune) autotune_stages(*p);
 		for (int i = 0; i < 4; i++) CTG_HIP(hipEventCreate(&p->ev[i]));
 	}
 	catch (...) {
@@ -699,12 +749,12 @@ MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& 
 	return p;
 }
 
-MatvecPlan* build_matvec_plan(const block_sparse_tensor* a, const block_sparse_tensor* w, const block_sparse_tensor* l, const block_sparse_tensor* r, int mode)
+MatvecPlan* build_matvec_plan(const block_sparse_tensor* a, const block_sparse_tensor* w, const block_sparse_tensor* l, const block_sparse_tensor* r, int mode, bool autotune)
 {
 	hipStream_t s = default_stream();
 	Meta th = meta_of(a);
 	DevTensor dw = upload(w, s), dl = upload(l, s), dr = upload(r, s);
-	return build_matvec_plan_dev(th, std::move(dw), std::move(dl), std::move(dr), mode, s);
+	return build_matvec_plan_dev(th, std::move(dw), std::move(dl), std::move(dr), mode, s, autotune);
 }
 
 void matvec_apply(MatvecPlan& p, const double* x, double* y, float* stage_ms)

@@ -9,6 +9,7 @@
 //  permute / copy_segments / fill      data movement of transposes, reshapes and slices
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "ct_internal.h"
 
 namespace ctg {
@@ -97,8 +98,233 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(const GemmTask* __rest
 	const int nwaves = gridDim.x * 4;
 	for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < nitems; w += nwaves) {
 		const WorkItem it = items[w];
-		const GemmTask tk = tasks[it.task];
+		const GemmTask& tk = it.task;
 		wave_tile_f64(tk, terms, it.m0, it.n0, lane, bs);
+	}
+}
+
+// Workgroup-per-tile form: the four wavefronts of a workgroup share one 32 x 32 output tile.
+// The k-steps of all terms of the task are cut into chunks of four; chunk g goes to wavefront
+// g % 4, which issues the loads of its four k-steps before their MFMAs (memory-level
+// parallelism). The four partial tiles are summed through LDS in a fixed order.
+constexpr int CHUNK = 4;
+
+__device__ __forceinline__ void wg_tile_f64(const GemmTask& tk, const GemmTerm* __restrict__ terms, int m0, int n0, GemmBases bs, double* __restrict__ red)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int i16 = lane & 15, kq = lane >> 4;
+	const bool two_m = (m0 + 16 < tk.M);
+	const bool two_n = (n0 + 16 < tk.N);
+	const int r0 = m0 + i16, r1 = m0 + 16 + i16;
+	const int c0 = n0 + i16, c1 = n0 + 16 + i16;
+	const bool vr0 = r0 < tk.M, vr1 = r1 < tk.M, vc0 = c0 < tk.N, vc1 = c1 < tk.N;
+	double4_t acc00 = { 0, 0, 0, 0 }, acc01 = { 0, 0, 0, 0 }, acc10 = { 0, 0, 0, 0 }, acc11 = { 0, 0, 0, 0 };
+	int g = 0;
+	for (int t = tk.t0; t < tk.t1; t++) {
+		const GemmTerm tm = terms[t];
+		const int nsteps = (tm.K + 3) >> 2;
+		const int nchunks = (nsteps + CHUNK - 1) / CHUNK;
+		const int first = (wave - g % 4 + 4) % 4;
+		g += nchunks;
+		if (first >= nchunks) continue;
+		const double* __restrict__ A = rebase(bs.A, tm.A);
+		const double* __restrict__ B = rebase(bs.B, tm.B);
+		const int64_t ao0 = (int64_t)(vr0 ? r0 : 0) * tm.sa_i, ao1 = (int64_t)(vr1 ? r1 : 0) * tm.sa_i;
+		const int64_t bo0 = b_col_offset(vc0 ? c0 : 0, tk.nci, tm.sb_batch, tm.sb_j);
+		const int64_t bo1 = b_col_offset(vc1 ? c1 : 0, tk.nci, tm.sb_batch, tm.sb_j);
+		for (int c = first; c < nchunks; c += 4) {
+			double a0[CHUNK], a1[CHUNK], b0[CHUNK], b1[CHUNK];
+			#pragma unroll
+			for (int u = 0; u < CHUNK; u++) {
+				const int k = (c * CHUNK + u) * 4 + kq;
+				const bool vk = k < tm.K;
+				const int64_t ak = (int64_t)(vk ? k : 0) * tm.sa_k, bk = (int64_t)(vk ? k : 0) * tm.sb_k;
+				a0[u] = (vk && vr0) ? A[ao0 + ak] : 0.0;
+				a1[u] = (vk && vr1) ? A[ao1 + ak] : 0.0;
+				b0[u] = (vk && vc0) ? B[bo0 + bk] : 0.0;
+				b1[u] = (vk && vc1) ? B[bo1 + bk] : 0.0;
+			}
+			#pragma unroll
+			for (int u = 0; u < CHUNK; u++) {
+				acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b0[u], acc00, 0, 0, 0);
+				if (two_n) acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b1[u], acc01, 0, 0, 0);
+				if (two_m) acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b0[u], acc10, 0, 0, 0);
+				if (two_m && two_n) acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b1[u], acc11, 0, 0, 0);
+			}
+		}
+	}
+	// reduction over the four wavefronts: red[wave][reg][lane], summed in wave order
+	#pragma unroll
+	for (int r = 0; r < 4; r++) {
+		red[((wave * 16) + r) * 64 + lane] = acc00[r];
+		red[((wave * 16) + 4 + r) * 64 + lane] = acc01[r];
+		red[((wave * 16) + 8 + r) * 64 + lane] = acc10[r];
+		red[((wave * 16) + 12 + r) * 64 + lane] = acc11[r];
+	}
+	__syncthreads();
+	// wavefront w stores sub-tile w: (mi, ni) = (w >> 1, w & 1)
+	const int mi = wave >> 1, ni = wave & 1;
+	const int col = n0 + 16 * ni + i16;
+	if (col < tk.N) {
+		const int jb = col / tk.nci, jc = col - jb * tk.nci;
+		double* __restrict__ C = (double*)rebase(bs.C, tk.C) + (int64_t)jb * tk.sc_batch + jc;
+		#pragma unroll
+		for (int r = 0; r < 4; r++) {
+			const int row = m0 + 16 * mi + kq + 4 * r;
+			if (row < tk.M) {
+				const int slot = wave * 4 + r;
+				double v = red[(0 * 16 + slot) * 64 + lane];
+				v += red[(1 * 16 + slot) * 64 + lane];
+				v += red[(2 * 16 + slot) * 64 + lane];
+				v += red[(3 * 16 + slot) * 64 + lane];
+				if (tk.accumulate) C[(int64_t)row * tk.sc_i] += v;
+				else C[(int64_t)row * tk.sc_i] = v;
+			}
+		}
+	}
+	__syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_grouped_gemm_f64_wg(const GemmTask* __restrict__ tasks, const GemmTerm* __restrict__ terms,
+	const WorkItem* __restrict__ items, int nitems, GemmBases bs)
+{
+	__shared__ double red[4 * 16 * 64];
+	for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+		const WorkItem it = items[w];
+		const GemmTask& tk = it.task;
+		wg_tile_f64(tk, terms, it.m0, it.n0, bs, red);
+	}
+}
+
+// k-step stream: the k-steps (four products each) of all terms of a task form one stream; it is
+// consumed in batches of SLOTS k-steps whose loads are all issued before their MFMAs, so a
+// batch costs one memory round trip even when it spans several short terms. With nparts > 1,
+// batch b of the stream belongs to part b % nparts (the wavefronts of a workgroup).
+constexpr int SLOTS = 8;
+
+struct Acc4 { double4_t a00, a01, a10, a11; };
+
+__device__ __forceinline__ void stream_tile_f64(const GemmTask& tk, const GemmTerm* __restrict__ terms, int m0, int n0, GemmBases bs,
+	int part, int nparts, Acc4& acc)
+{
+	const int lane = threadIdx.x & 63;
+	const int i16 = lane & 15, kq = lane >> 4;
+	const bool two_m = (m0 + 16 < tk.M);
+	const bool two_n = (n0 + 16 < tk.N);
+	const int r0 = m0 + i16, r1 = m0 + 16 + i16;
+	const int c0 = n0 + i16, c1 = n0 + 16 + i16;
+	const bool vr0 = r0 < tk.M, vr1 = r1 < tk.M, vc0 = c0 < tk.N, vc1 = c1 < tk.N;
+	int t = tk.t0, step = 0;
+	// advance the cursor by 'skip' k-steps
+	auto advance = [&](int skip) {
+		while (t < tk.t1 && skip > 0) {
+			const int ns = (terms[t].K + 3) >> 2;
+			const int rem = ns - step;
+			if (skip >= rem) { skip -= rem; t++; step = 0; }
+			else { step += skip; skip = 0; }
+		}
+	};
+	advance(part * SLOTS);
+	while (t < tk.t1) {
+		double a0[SLOTS], a1[SLOTS], b0[SLOTS], b1[SLOTS];
+		int filled = 0;
+		#pragma unroll
+		for (int u = 0; u < SLOTS; u++) {
+			if (t < tk.t1) {
+				const GemmTerm tm = terms[t];
+				const double* __restrict__ A = rebase(bs.A, tm.A);
+				const double* __restrict__ B = rebase(bs.B, tm.B);
+				const int k = step * 4 + kq;
+				const bool vk = k < tm.K;
+				const int64_t ak = (int64_t)(vk ? k : 0) * tm.sa_k, bk = (int64_t)(vk ? k : 0) * tm.sb_k;
+				a0[u] = (vk && vr0) ? A[(int64_t)r0 * tm.sa_i + ak] : 0.0;
+				a1[u] = (vk && vr1) ? A[(int64_t)r1 * tm.sa_i + ak] : 0.0;
+				b0[u] = (vk && vc0) ? B[b_col_offset(c0, tk.nci, tm.sb_batch, tm.sb_j) + bk] : 0.0;
+				b1[u] = (vk && vc1) ? B[b_col_offset(c1, tk.nci, tm.sb_batch, tm.sb_j) + bk] : 0.0;
+				filled = u + 1;
+				step++;
+				if (step * 4 >= tm.K) { t++; step = 0; }
+			}
+			else { a0[u] = 0; a1[u] = 0; b0[u] = 0; b1[u] = 0; }
+		}
+		#pragma unroll
+		for (int u = 0; u < SLOTS; u++) {
+			if (u < filled) {
+				acc.a00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b0[u], acc.a00, 0, 0, 0);
+				if (two_n) acc.a01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b1[u], acc.a01, 0, 0, 0);
+				if (two_m) acc.a10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b0[u], acc.a10, 0, 0, 0);
+				if (two_m && two_n) acc.a11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b1[u], acc.a11, 0, 0, 0);
+			}
+		}
+		advance((nparts - 1) * SLOTS);
+	}
+}
+
+__device__ __forceinline__ void store_sub(const GemmTask& tk, int m0, int n0, GemmBases bs, int mi, int ni, const double4_t& v, int lane)
+{
+	const int i16 = lane & 15, kq = lane >> 4;
+	const int col = n0 + 16 * ni + i16;
+	if (col >= tk.N) return;
+	const int jb = col / tk.nci, jc = col - jb * tk.nci;
+	double* __restrict__ C = (double*)rebase(bs.C, tk.C) + (int64_t)jb * tk.sc_batch + jc;
+	#pragma unroll
+	for (int r = 0; r < 4; r++) {
+		const int row = m0 + 16 * mi + kq + 4 * r;
+		if (row < tk.M) {
+			if (tk.accumulate) C[(int64_t)row * tk.sc_i] += v[r];
+			else C[(int64_t)row * tk.sc_i] = v[r];
+		}
+	}
+}
+
+// one wavefront per work item
+__global__ __launch_bounds__(256) void k_stream_gemm_f64_wave(const GemmTask* __restrict__ tasks, const GemmTerm* __restrict__ terms,
+	const WorkItem* __restrict__ items, int nitems, GemmBases bs)
+{
+	const int lane = threadIdx.x & 63;
+	const int nwaves = gridDim.x * 4;
+	for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < nitems; w += nwaves) {
+		const WorkItem it = items[w];
+		const GemmTask& tk = it.task;
+		Acc4 acc;
+		acc.a00 = acc.a01 = acc.a10 = acc.a11 = double4_t{ 0, 0, 0, 0 };
+		stream_tile_f64(tk, terms, it.m0, it.n0, bs, 0, 1, acc);
+		store_sub(tk, it.m0, it.n0, bs, 0, 0, acc.a00, lane);
+		store_sub(tk, it.m0, it.n0, bs, 0, 1, acc.a01, lane);
+		store_sub(tk, it.m0, it.n0, bs, 1, 0, acc.a10, lane);
+		store_sub(tk, it.m0, it.n0, bs, 1, 1, acc.a11, lane);
+	}
+}
+
+// one workgroup per work item: the four wavefronts take every fourth batch of the stream and
+// their partial tiles are summed through LDS in wavefront order
+__global__ __launch_bounds__(256) void k_stream_gemm_f64_wg(const GemmTask* __restrict__ tasks, const GemmTerm* __restrict__ terms,
+	const WorkItem* __restrict__ items, int nitems, GemmBases bs)
+{
+	__shared__ double red[4 * 16 * 64];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+		const WorkItem it = items[w];
+		const GemmTask& tk = it.task;
+		Acc4 acc;
+		acc.a00 = acc.a01 = acc.a10 = acc.a11 = double4_t{ 0, 0, 0, 0 };
+		stream_tile_f64(tk, terms, it.m0, it.n0, bs, wave, 4, acc);
+		#pragma unroll
+		for (int r = 0; r < 4; r++) {
+			red[((wave * 16) + r) * 64 + lane] = acc.a00[r];
+			red[((wave * 16) + 4 + r) * 64 + lane] = acc.a01[r];
+			red[((wave * 16) + 8 + r) * 64 + lane] = acc.a10[r];
+			red[((wave * 16) + 12 + r) * 64 + lane] = acc.a11[r];
+		}
+		__syncthreads();
+		double4_t v;
+		#pragma unroll
+		for (int r = 0; r < 4; r++) {
+			const int slot = wave * 4 + r;
+			v[r] = ((red[(0 * 16 + slot) * 64 + lane] + red[(1 * 16 + slot) * 64 + lane]) + red[(2 * 16 + slot) * 64 + lane]) + red[(3 * 16 + slot) * 64 + lane];
+		}
+		store_sub(tk, it.m0, it.n0, bs, wave >> 1, wave & 1, v, lane);
+		__syncthreads();
 	}
 }
 
@@ -236,7 +462,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_split(const GemmTask* __re
 	const int nwaves = gridDim.x * 4;
 	for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < nitems; w += nwaves) {
 		const WorkItem it = items[w];
-		const GemmTask tk = tasks[it.task];
+		const GemmTask& tk = it.task;
 		wave_tile_split(tk, terms, it.m0, it.n0, lane, bs);
 	}
 }
@@ -290,6 +516,11 @@ __global__ __launch_bounds__(256) void k_copy_segments(const double* __restrict_
 	}
 }
 
+__global__ __launch_bounds__(256) void k_copy_f64(const double* __restrict__ src, double* __restrict__ dst, int64_t n)
+{
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
 __global__ __launch_bounds__(256) void k_fill(double* __restrict__ p, double v, int64_t n)
 {
 	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
@@ -315,7 +546,13 @@ int num_compute_units()
 void launch_grouped_gemm_f64(const DevGemm& g, GemmBases b, hipStream_t s)
 {
 	if (g.nitems == 0) return;
-	hipLaunchKernelGGL(k_grouped_gemm_f64, dim3(g.grid), dim3(256), 0, s, g.tasks, g.terms, g.items, g.nitems, b);
+	const int wg_grid = std::max(1, std::min(g.nitems, num_compute_units() * 8));
+	switch (g.form) {
+		case GEMM_WAVE_STREAM: hipLaunchKernelGGL(k_stream_gemm_f64_wave, dim3(g.grid), dim3(256), 0, s, g.tasks, g.terms, g.items, g.nitems, b); break;
+		case GEMM_WG_CHUNK: hipLaunchKernelGGL(k_grouped_gemm_f64_wg, dim3(wg_grid), dim3(256), 0, s, g.tasks, g.terms, g.items, g.nitems, b); break;
+		case GEMM_WG_STREAM: hipLaunchKernelGGL(k_stream_gemm_f64_wg, dim3(wg_grid), dim3(256), 0, s, g.tasks, g.terms, g.items, g.nitems, b); break;
+		default: hipLaunchKernelGGL(k_grouped_gemm_f64, dim3(g.grid), dim3(256), 0, s, g.tasks, g.terms, g.items, g.nitems, b); break;
+	}
 	CTG_HIP(hipGetLastError());
 }
 
@@ -360,6 +597,14 @@ void launch_copy_segments(const double* src, double* dst, const CopySeg* d_segs,
 {
 	if (nsegs == 0) return;
 	hipLaunchKernelGGL(k_copy_segments, dim3(nsegs < 8192 ? nsegs : 8192), dim3(256), 0, s, src, dst, d_segs, nsegs);
+	CTG_HIP(hipGetLastError());
+}
+
+void launch_copy(const double* src, double* dst, ct_long n, hipStream_t s)
+{
+	if (n <= 0) return;
+	const int64_t nb = (n + 255) / 256;
+	hipLaunchKernelGGL(k_copy_f64, dim3((unsigned)(nb < 16384 ? nb : 16384)), dim3(256), 0, s, src, dst, (int64_t)n);
 	CTG_HIP(hipGetLastError());
 }
 

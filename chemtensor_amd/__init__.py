@@ -99,6 +99,9 @@ def lib():
             "ct_construct_fermi_hubbard_1d_mpo": (None, [C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(MPOStruct)]),
             "delete_mpo": (None, [C.POINTER(MPOStruct)]),
             "delete_mps": (None, [C.POINTER(MPSStruct)]),
+            "seed_rng_state": (None, [C.c_uint64, P]),
+            "construct_random_mps": (None, [C.c_int, C.c_int, ct_long, IP, C.c_int32, ct_long, P, C.POINTER(MPSStruct)]),
+            "dmrg_twosite": (C.c_int, [C.POINTER(MPOStruct), C.c_int, C.c_int, C.c_double, ct_long, C.POINTER(MPSStruct), DP, DP]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -325,3 +328,100 @@ class Event:
         if self.e and _lib is not None:
             _lib.ct_event_destroy(self.e)
             self.e = None
+
+
+def encode_pair(qa, qb):
+    return int(qa) * 65536 + int(qb)
+
+
+class MPO:
+    """struct mpo built by the product library (ct_construct_*_mpo)."""
+
+    def __init__(self):
+        self.s = MPOStruct()
+        self.live = False
+
+    @property
+    def ptr(self):
+        return C.cast(C.pointer(self.s), P)
+
+    @property
+    def nsites(self):
+        return self.s.nsites
+
+    @property
+    def d(self):
+        return self.s.d
+
+    @property
+    def qsite(self):
+        return np.array([self.s.qsite[i] for i in range(self.s.d)], dtype=np.int32)
+
+    def __del__(self):
+        if self.live and _lib is not None:
+            _lib.delete_mpo(C.pointer(self.s))
+            self.live = False
+
+    @staticmethod
+    def xxz(nsites, J=1.0, D=0.5, h=0.0):
+        m = MPO()
+        lib().ct_construct_heisenberg_xxz_1d_mpo(nsites, J, D, h, C.pointer(m.s))
+        m.live = True
+        return m
+
+    @staticmethod
+    def hubbard(nsites, t=1.0, u=4.0, mu=0.0):
+        m = MPO()
+        lib().ct_construct_fermi_hubbard_1d_mpo(nsites, t, u, mu, C.pointer(m.s))
+        m.live = True
+        return m
+
+
+class MPS:
+    """struct mps built by the product library (construct_random_mps with a seeded RNG)."""
+
+    def __init__(self):
+        self.s = MPSStruct()
+        self.live = False
+
+    @property
+    def ptr(self):
+        return C.cast(C.pointer(self.s), P)
+
+    @property
+    def nsites(self):
+        return self.s.nsites
+
+    def site_ptr(self, i):
+        return C.cast(C.pointer(self.s.a[i]), P)
+
+    def bond_dims(self):
+        dims = [int(self.s.a[0].dim_logical[0])]
+        for i in range(self.s.nsites):
+            dims.append(int(self.s.a[i].dim_logical[2]))
+        return dims
+
+    def __del__(self):
+        if self.live and _lib is not None:
+            _lib.delete_mps(C.pointer(self.s))
+            self.live = False
+
+    @staticmethod
+    def random(nsites, d, qsite, sector, max_vdim, seed):
+        m = MPS()
+        rng = C.create_string_buffer(64)
+        lib().seed_rng_state(seed, rng)
+        q = np.ascontiguousarray(qsite, dtype=np.int32)
+        lib().construct_random_mps(1, nsites, d, q.ctypes.data_as(IP), sector, max_vdim, rng, C.pointer(m.s))
+        m.live = True
+        return m
+
+
+def dmrg(mpo, psi, num_sweeps=5, maxiter_lanczos=20, tol_split=1e-10, max_vdim=256):
+    """dmrg_twosite (dmrg.c:262-399) on the GPU; returns (en_sweeps, entropy); psi is updated in place."""
+    en = np.zeros(num_sweeps)
+    ent = np.zeros(mpo.nsites - 1)
+    ret = lib().dmrg_twosite(C.pointer(mpo.s), num_sweeps, maxiter_lanczos, tol_split, max_vdim, C.pointer(psi.s),
+                             en.ctypes.data_as(DP), ent.ctypes.data_as(DP))
+    check(ret, "dmrg_twosite")
+    return en, ent

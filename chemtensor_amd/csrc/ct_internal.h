@@ -72,9 +72,12 @@ struct DevTensor
 {
 	Meta meta;
 	double* d = nullptr;   // packed blocks in serialize order
+	bool owner = true;     // false for views sharing another tensor's entries
 
 	DevTensor() = default;
 	explicit DevTensor(Meta m);
+	/// non-owning view of 'o' (same entries) with its own metadata
+	static DevTensor view(const DevTensor& o, Meta m);
 	DevTensor(const DevTensor&) = delete;
 	DevTensor& operator=(const DevTensor&) = delete;
 	DevTensor(DevTensor&& o) noexcept;
@@ -86,6 +89,24 @@ struct DevTensor
 hipStream_t default_stream();
 DevTensor upload(const block_sparse_tensor* t, hipStream_t s);
 void download(const DevTensor& t, block_sparse_tensor* out, hipStream_t s);
+
+/// a block-sparse tensor in packed form in host memory
+struct HostPacked
+{
+	Meta meta;
+	std::vector<double> v;
+};
+HostPacked download_packed(const DevTensor& t, hipStream_t s);
+DevTensor upload_packed(const HostPacked& h, hipStream_t s);
+
+// host block factorisations (host_bond.cpp)
+int host_block_svd(const HostPacked& a, HostPacked& u, std::vector<double>& s, HostPacked& vh);
+ct_long host_retained_bond_indices(const std::vector<double>& sigma, double tol, bool relative, ct_long max_vdim, std::vector<ct_long>& ind, trunc_info& info);
+HostPacked host_slice(const HostPacked& t, int i_ax, const std::vector<ct_long>& ind);
+void host_scale_axis(HostPacked& t, bool leading, const std::vector<double>& vec);
+int host_split_svd(const HostPacked& a, double tol, ct_long max_vdim, bool renormalize, int distr, HostPacked& a0, HostPacked& a1, trunc_info& info);
+int host_block_qr(const HostPacked& a, HostPacked& q, HostPacked& r);
+int host_block_rq(const HostPacked& a, HostPacked& r, HostPacked& q);
 
 //________________________________________________________________________________________________________________________
 //

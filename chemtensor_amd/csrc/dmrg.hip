@@ -1,0 +1,261 @@
+// Two-site DMRG with a device-resident sweep (src/algorithm/dmrg.c:262-399): MPS site tensors,
+// MPO tensors, merged two-site MPO tensors and all environment blocks live on the GPU as packed
+// block-sparse tensors; contractions, transposes and reshapes run on the GPU; the Lanczos
+// iteration runs on the GPU in FP64 with the grouped matvec plan of each bond. The truncating
+// block SVD and the QR/RQ orthonormalisation run in FP64 on the host (they are small next to
+// the maxiter matvecs of every bond).
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "ct_internal.h"
+
+namespace ctg {
+
+__global__ __launch_bounds__(256) void k_scale(double* __restrict__ p, double a, int64_t n)
+{
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] *= a;
+}
+
+static void dev_scale(DevTensor& t, double a, hipStream_t s)
+{
+	if (t.meta.nelem == 0) return;
+	const int64_t nb = (t.meta.nelem + 255) / 256;
+	hipLaunchKernelGGL(k_scale, dim3((unsigned)std::min<int64_t>(nb, 4096)), dim3(256), 0, s, t.d, a, (int64_t)t.meta.nelem);
+	CTG_HIP(hipGetLastError());
+}
+
+static Meta meta_with_dirs(const Meta& m, const std::vector<int>& dirs)
+{
+	std::vector<const qnumber*> qn(m.ndim);
+	for (int i = 0; i < m.ndim; i++) qn[i] = m.qnums_logical[i].data();
+	return make_meta(m.ndim, m.dim_logical.data(), dirs.data(), qn.data());
+}
+
+// a 6-axis tensor of unit dimensions with its single entry 1 (if the block exists)
+static DevTensor unit6(const qnumber* const* qn, hipStream_t s)
+{
+	const ct_long dim[6] = { 1, 1, 1, 1, 1, 1 };
+	const int ad[6] = { TENSOR_AXIS_OUT, TENSOR_AXIS_OUT, TENSOR_AXIS_IN, TENSOR_AXIS_IN, TENSOR_AXIS_IN, TENSOR_AXIS_OUT };
+	HostPacked h;
+	h.meta = make_meta(6, dim, ad, qn);
+	h.v.assign((size_t)h.meta.nelem, 1.0);
+	return upload_packed(h, s);
+}
+
+// chain_ops.c:14-46
+static DevTensor dummy_right(const Meta& a, const Meta& b, const Meta& w, hipStream_t s)
+{
+	const qnumber* qn[6] = { a.qnums_logical[2].data(), w.qnums_logical[3].data(), b.qnums_logical[2].data(),
+		a.qnums_logical[2].data(), w.qnums_logical[3].data(), b.qnums_logical[2].data() };
+	DevTensor u = unit6(qn, s);
+	DevTensor t = dev_flatten_axes(u, 4, TENSOR_AXIS_IN, s);
+	return dev_flatten_axes(t, 3, TENSOR_AXIS_IN, s);
+}
+
+// chain_ops.c:53-85
+static DevTensor dummy_left(const Meta& a, const Meta& b, const Meta& w, hipStream_t s)
+{
+	const qnumber* qn[6] = { a.qnums_logical[0].data(), w.qnums_logical[0].data(), b.qnums_logical[0].data(),
+		a.qnums_logical[0].data(), w.qnums_logical[0].data(), b.qnums_logical[0].data() };
+	DevTensor u = unit6(qn, s);
+	DevTensor t = dev_flatten_axes(u, 0, TENSOR_AXIS_OUT, s);
+	return dev_flatten_axes(t, 0, TENSOR_AXIS_OUT, s);
+}
+
+static DevTensor reversed(const DevTensor& b)
+{
+	std::vector<int> dirs(b.meta.axis_dir);
+	for (int& d : dirs) d = -d;
+	return DevTensor::view(b, meta_with_dirs(b.meta, dirs));
+}
+
+// chain_ops.c:116-165
+static DevTensor step_right(const DevTensor& a, const DevTensor& b, const DevTensor& w, const DevTensor& r, hipStream_t st)
+{
+	DevTensor s = dev_dot(a, TENSOR_AXIS_RANGE_TRAILING, r, TENSOR_AXIS_RANGE_LEADING, 1, DotForm::Grouped, st);
+	const int p0[5] = { 1, 2, 0, 3, 4 };
+	DevTensor t = dev_transpose(p0, s, st);
+	s = dev_dot(w, TENSOR_AXIS_RANGE_TRAILING, t, TENSOR_AXIS_RANGE_LEADING, 2, DotForm::Grouped, st);
+	const int p1[5] = { 4, 2, 0, 1, 3 };
+	t = dev_transpose(p1, s, st);
+	DevTensor br = reversed(b);
+	s = dev_dot(t, TENSOR_AXIS_RANGE_TRAILING, br, TENSOR_AXIS_RANGE_TRAILING, 2, DotForm::Grouped, st);
+	const int p2[4] = { 1, 2, 3, 0 };
+	return dev_transpose(p2, s, st);
+}
+
+// chain_ops.c:196-245
+static DevTensor step_left(const DevTensor& a, const DevTensor& b, const DevTensor& w, const DevTensor& l, hipStream_t st)
+{
+	DevTensor br = reversed(b);
+	DevTensor s = dev_dot(l, TENSOR_AXIS_RANGE_TRAILING, br, TENSOR_AXIS_RANGE_LEADING, 1, DotForm::Grouped, st);
+	const int p0[5] = { 0, 1, 4, 2, 3 };
+	DevTensor t = dev_transpose(p0, s, st);
+	s = dev_dot(t, TENSOR_AXIS_RANGE_TRAILING, w, TENSOR_AXIS_RANGE_LEADING, 2, DotForm::Grouped, st);
+	const int p1[5] = { 1, 3, 4, 2, 0 };
+	t = dev_transpose(p1, s, st);
+	s = dev_dot(a, TENSOR_AXIS_RANGE_LEADING, t, TENSOR_AXIS_RANGE_LEADING, 2, DotForm::Grouped, st);
+	const int p2[4] = { 3, 0, 1, 2 };
+	return dev_transpose(p2, s, st);
+}
+
+// mps.c:1166-1180
+static DevTensor merge_mps(const DevTensor& a0, const DevTensor& a1, hipStream_t st)
+{
+	DevTensor t = dev_dot(a0, TENSOR_AXIS_RANGE_TRAILING, a1, TENSOR_AXIS_RANGE_LEADING, 1, DotForm::Grouped, st);
+	return dev_flatten_axes(t, 1, TENSOR_AXIS_OUT, st);
+}
+
+// mpo.c:255-282
+static DevTensor merge_mpo(const DevTensor& a0, const DevTensor& a1, hipStream_t st)
+{
+	DevTensor t = dev_dot(a0, TENSOR_AXIS_RANGE_TRAILING, a1, TENSOR_AXIS_RANGE_LEADING, 1, DotForm::Grouped, st);
+	const int perm[6] = { 0, 1, 3, 2, 4, 5 };
+	DevTensor tp = dev_transpose(perm, t, st);
+	DevTensor f = dev_flatten_axes(tp, 1, TENSOR_AXIS_OUT, st);
+	return dev_flatten_axes(f, 2, TENSOR_AXIS_IN, st);
+}
+
+// mps.c:560-601: right-orthonormalise 'a' by RQ and multiply R into 'a_prev'
+static int local_orthonormalize_rq(DevTensor& a, DevTensor& a_prev, hipStream_t st)
+{
+	const ct_long dr[2] = { a.meta.dim_logical[1], a.meta.dim_logical[2] };
+	const std::vector<qnumber> q1 = a.meta.qnums_logical[1], q2 = a.meta.qnums_logical[2];
+	const int ad[2] = { TENSOR_AXIS_OUT, TENSOR_AXIS_IN };
+	DevTensor am = dev_flatten_axes(a, 1, TENSOR_AXIS_IN, st);
+	HostPacked hm = download_packed(am, st), hr, hq;
+	if (host_block_rq(hm, hr, hq) != 0) return -1;
+	DevTensor dq = upload_packed(hq, st), drr = upload_packed(hr, st);
+	const qnumber* qn[2] = { q1.data(), q2.data() };
+	a = dev_split_axis(dq, 1, dr, ad, qn, st);
+	a_prev = dev_dot(a_prev, TENSOR_AXIS_RANGE_TRAILING, drr, TENSOR_AXIS_RANGE_LEADING, 1, DotForm::Grouped, st);
+	return 0;
+}
+
+// mps.c:1119-1160
+static int split_svd(const DevTensor& a, ct_long d, const std::vector<qnumber>& qsite, double tol, ct_long max_vdim, int distr,
+	DevTensor& a0, DevTensor& a1, trunc_info& info, hipStream_t st)
+{
+	const ct_long dd[2] = { d, d };
+	const int ad[2] = { TENSOR_AXIS_OUT, TENSOR_AXIS_OUT };
+	const qnumber* qs[2] = { qsite.data(), qsite.data() };
+	DevTensor a2 = dev_split_axis(a, 1, dd, ad, qs, st);
+	DevTensor tmp = dev_flatten_axes(a2, 0, TENSOR_AXIS_OUT, st);
+	DevTensor am = dev_flatten_axes(tmp, 1, TENSOR_AXIS_IN, st);
+	HostPacked hm = download_packed(am, st), m0, m1;
+	const int ret = host_split_svd(hm, tol, max_vdim, false, distr, m0, m1, info);
+	if (ret < 0) return ret;
+	DevTensor d0 = upload_packed(m0, st), d1 = upload_packed(m1, st);
+	const Meta& m = a2.meta;
+	const ct_long dl0[2] = { m.dim_logical[0], m.dim_logical[1] }, dl1[2] = { m.dim_logical[2], m.dim_logical[3] };
+	const int ad0[2] = { m.axis_dir[0], m.axis_dir[1] }, ad1[2] = { m.axis_dir[2], m.axis_dir[3] };
+	const qnumber* qn0[2] = { m.qnums_logical[0].data(), m.qnums_logical[1].data() };
+	const qnumber* qn1[2] = { m.qnums_logical[2].data(), m.qnums_logical[3].data() };
+	a0 = dev_split_axis(d0, 0, dl0, ad0, qn0, st);
+	a1 = dev_split_axis(d1, 1, dl1, ad1, qn1, st);
+	return 0;
+}
+
+// dmrg.c:86-147 with the Lanczos iteration on the device
+static int minimize_local_energy(const DevTensor& w, const DevTensor& l, const DevTensor& r, const DevTensor& a_start, int maxiter,
+	double* en, DevTensor& a_opt, hipStream_t st)
+{
+	std::unique_ptr<MatvecPlan> p(build_matvec_plan_dev(a_start.meta, DevTensor::view(w, w.meta), DevTensor::view(l, l.meta), DevTensor::view(r, r.meta),
+		CT_MATVEC_GROUPED_FP64, st, false));
+	a_opt = DevTensor(a_start.meta);
+	return lanczos_ground_state(*p, a_start.d, maxiter, en, a_opt.d);
+}
+
+// the dummy tensor of dimensions (D, 1, D) with the outer bond quantum numbers and entry 1
+static DevTensor dummy_site(const Meta& a, int ax, hipStream_t st)
+{
+	const ct_long dim[3] = { a.dim_logical[ax], 1, a.dim_logical[ax] };
+	const int ad[3] = { TENSOR_AXIS_OUT, TENSOR_AXIS_OUT, TENSOR_AXIS_IN };
+	const qnumber qz[1] = { 0 };
+	const qnumber* qn[3] = { a.qnums_logical[ax].data(), qz, a.qnums_logical[ax].data() };
+	HostPacked h;
+	h.meta = make_meta(3, dim, ad, qn);
+	h.v.assign((size_t)h.meta.nelem, 0.0);
+	if (h.meta.nelem > 0 && h.meta.present(0)) h.v[0] = 1.0;
+	return upload_packed(h, st);
+}
+
+int dmrg_twosite_device(const mpo* H, int num_sweeps, int maxiter, double tol_split, ct_long max_vdim, mps* psi, double* en_sweeps, double* entropy)
+{
+	hipStream_t st = default_stream();
+	const int L = H->nsites;
+	if (L < 2 || psi->nsites != L) return CT_ERR_ARG;
+	std::vector<DevTensor> A, W;
+	for (int i = 0; i < L; i++) { A.push_back(upload(&psi->a[i], st)); W.push_back(upload(&H->a[i], st)); }
+	const std::vector<qnumber> qsite(psi->qsite, psi->qsite + psi->d);
+
+	// mps_orthonormalize_qr(psi, MPS_ORTHONORMAL_RIGHT) (mps.c:609-760)
+	for (int i = L - 1; i > 0; i--) if (local_orthonormalize_rq(A[i], A[i - 1], st) != 0) return -1;
+	double nrm = 0;
+	{
+		DevTensor head = dummy_site(A[0].meta, 0, st);
+		if (local_orthonormalize_rq(A[0], head, st) != 0) return -1;
+		HostPacked hh = download_packed(head, st);
+		if (hh.meta.nelem > 0) {
+			nrm = hh.v[0];
+			if (nrm < 0) { dev_scale(A[0], -1.0, st); nrm = -nrm; }
+		}
+	}
+	if (nrm == 0) printf("Warning: in 'dmrg_twosite': initial MPS has norm zero (possibly due to mismatching quantum numbers)\n");
+
+	std::vector<DevTensor> lb(L), rb(L);
+	rb[L - 1] = dummy_right(A[L - 1].meta, A[L - 1].meta, W[L - 1].meta, st);
+	for (int i = L - 1; i > 0; i--) rb[i - 1] = step_right(A[i], A[i], W[i], rb[i], st);
+	lb[0] = dummy_left(A[0].meta, A[0].meta, W[0].meta, st);
+	std::vector<DevTensor> h2;
+	for (int i = 0; i < L - 1; i++) h2.push_back(merge_mpo(W[i], W[i + 1], st));
+
+	for (int n = 0; n < num_sweeps; n++) {
+		double en = 0;
+		for (int i = 0; i < L - 2; i++) {
+			DevTensor ac = merge_mps(A[i], A[i + 1], st), ao;
+			int ret = minimize_local_energy(h2[i], lb[i], rb[i + 1], ac, maxiter, &en, ao, st);
+			if (ret < 0) return ret;
+			trunc_info info;
+			ret = split_svd(ao, psi->d, qsite, tol_split, max_vdim, SVD_DISTR_RIGHT, A[i], A[i + 1], info, st);
+			if (ret < 0) return ret;
+			lb[i + 1] = step_left(A[i], A[i], W[i], lb[i], st);
+		}
+		for (int i = L - 2; i >= 0; i--) {
+			DevTensor ac = merge_mps(A[i], A[i + 1], st), ao;
+			int ret = minimize_local_energy(h2[i], lb[i], rb[i + 1], ac, maxiter, &en, ao, st);
+			if (ret < 0) return ret;
+			trunc_info info;
+			ret = split_svd(ao, psi->d, qsite, tol_split, max_vdim, SVD_DISTR_LEFT, A[i], A[i + 1], info, st);
+			if (ret < 0) return ret;
+			entropy[i] = info.entropy;
+			rb[i] = step_right(A[i + 1], A[i + 1], W[i + 1], rb[i + 1], st);
+		}
+		{
+			DevTensor t = dummy_site(A[0].meta, 0, st);
+			if (local_orthonormalize_rq(A[0], t, st) != 0) return -1;
+		}
+		en_sweeps[n] = en;
+	}
+	for (int i = 0; i < L; i++) {
+		delete_block_sparse_tensor(&psi->a[i]);
+		download(A[i], &psi->a[i], st);
+	}
+	return 0;
+}
+
+}  // namespace ctg
+
+extern "C" int dmrg_twosite(const struct mpo* hamiltonian, const int num_sweeps, const int maxiter_lanczos, const double tol_split, const ct_long max_vdim,
+	struct mps* psi, double* en_sweeps, double* entropy)
+{
+	try {
+		return ctg::dmrg_twosite_device(hamiltonian, num_sweeps, maxiter_lanczos, tol_split, max_vdim, psi, en_sweeps, entropy);
+	}
+	catch (const ctg::HipError& e) {
+		fprintf(stderr, "chemtensor_gpu: dmrg_twosite failed: %s\n", e.where.c_str());
+		return (e.code == hipErrorNotSupported) ? CT_ERR_UNSUPPORTED : CT_ERR_HIP;
+	}
+}

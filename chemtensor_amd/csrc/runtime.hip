@@ -126,20 +126,30 @@ DevTensor::DevTensor(Meta m) : meta(std::move(m))
 	d = (double*)dev_alloc((size_t)(meta.nelem > 0 ? meta.nelem : 1) * sizeof(double));
 }
 
-DevTensor::DevTensor(DevTensor&& o) noexcept : meta(std::move(o.meta)), d(o.d) { o.d = nullptr; }
+DevTensor DevTensor::view(const DevTensor& o, Meta m)
+{
+	DevTensor v;
+	v.meta = std::move(m);
+	v.d = o.d;
+	v.owner = false;
+	return v;
+}
+
+DevTensor::DevTensor(DevTensor&& o) noexcept : meta(std::move(o.meta)), d(o.d), owner(o.owner) { o.d = nullptr; }
 
 DevTensor& DevTensor::operator=(DevTensor&& o) noexcept
 {
 	if (this != &o) {
-		dev_free(d);
+		if (owner) dev_free(d);
 		meta = std::move(o.meta);
 		d = o.d;
+		owner = o.owner;
 		o.d = nullptr;
 	}
 	return *this;
 }
 
-DevTensor::~DevTensor() { dev_free(d); }
+DevTensor::~DevTensor() { if (owner) dev_free(d); }
 
 void DevTensor::zero(hipStream_t s) { launch_fill(d, 0.0, meta.nelem, s); }
 
@@ -165,6 +175,24 @@ void download(const DevTensor& t, block_sparse_tensor* out, hipStream_t s)
 	CTG_HIP(hipMemcpyAsync(host.data(), t.d, host.size() * sizeof(double), hipMemcpyDeviceToHost, s));
 	CTG_HIP(hipStreamSynchronize(s));
 	block_sparse_tensor_deserialize_entries(out, host.data());
+}
+
+HostPacked download_packed(const DevTensor& t, hipStream_t s)
+{
+	HostPacked h;
+	h.meta = t.meta;
+	h.v.resize((size_t)t.meta.nelem);
+	if (!h.v.empty()) CTG_HIP(hipMemcpyAsync(h.v.data(), t.d, h.v.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+	CTG_HIP(hipStreamSynchronize(s));
+	return h;
+}
+
+DevTensor upload_packed(const HostPacked& h, hipStream_t s)
+{
+	DevTensor t(h.meta);
+	if (!h.v.empty()) CTG_HIP(hipMemcpyAsync(t.d, h.v.data(), h.v.size() * sizeof(double), hipMemcpyHostToDevice, s));
+	CTG_HIP(hipStreamSynchronize(s));
+	return t;
 }
 
 }  // namespace ctg

@@ -72,6 +72,10 @@ def main():
     ap.add_argument("--model", default="hubbard")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--hbm-traffic-kib", default="",
+                    help="FETCH_SIZE,WRITE_SIZE (KiB per launch) of the dominant kernel from rocprofv3 --pmc runs")
+    ap.add_argument("--fetch-scale", type=float, default=2.0,
+                    help="FETCH_SIZE scale from the plain-copy calibration (bench.py --calibrate-copy)")
     ap.add_argument("--forms", default="grouped_fp64,split_fp32,plain_fp64",
                     help="comma list of matvec forms to time (grouped_fp64 is always included)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -119,8 +123,15 @@ def main():
     sflops = plan.stage_flops()
     ms = g["ms"]
     value = flops / (ms * 1e-3) / 1e12
+    sbytes = plan.stage_bytes()
     dom = int(np.argmax(g["stage_ms"]))
-    achieved = sflops[dom] / (g["stage_ms"][dom] * 1e-3) / 1e12
+    # every stage is below the FP64 ridge point (about 10 flop/byte), so HBM is its roofline
+    intensity = sflops[dom] / sbytes[dom]
+    achieved = sbytes[dom] / (g["stage_ms"][dom] * 1e-3) / 1e9
+    traffic = None
+    if args.hbm_traffic_kib:
+        fetch_kib, write_kib = (float(v) for v in args.hbm_traffic_kib.split(","))
+        traffic = round((fetch_kib * args.fetch_scale + write_kib) * 1024.0)
     y_g = g["y"].numpy()
 
     result = {
@@ -143,16 +154,22 @@ def main():
             "form": "batched grouped FP64 MFMA (3 launches per matvec)",
         },
         "roofline": {
-            "bound": "mfma",
-            "achieved": round(achieved, 4),
-            "peak": FP64_MFMA_PEAK,
-            "unit": "TFLOP/s",
-            "frac": round(achieved / FP64_MFMA_PEAK, 5),
-            "traffic": None,
-            "kernel": f"k_grouped_gemm_f64 (stage {dom + 1})",
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": traffic,
+            "kernel": f"grouped FP64 GEMM, stage {dom + 1} ({['theta.R', 'W.T1', 'L.T2'][dom]}), form {plan.forms()[0][dom]}",
+            "kernel_algorithmic_bytes_per_launch": sbytes[dom],
             "kernel_flops_per_launch": sflops[dom],
+            "kernel_flop_per_byte": round(intensity, 3),
             "kernel_ms": round(float(g["stage_ms"][dom]), 5),
+            "kernel_tflops": round(sflops[dom] / (g["stage_ms"][dom] * 1e-3) / 1e12, 4),
+            "fp64_mfma_peak_tflops": FP64_MFMA_PEAK,
         },
+        "stage_flops": sflops.tolist(),
+        "stage_bytes": sbytes.tolist(),
         "flops_per_matvec_reference_count": flops,
         "flops_per_matvec_executed": float(np.sum(sflops)),
         "hbm_gbs_algorithmic": round(plan.bytes / (ms * 1e-3) / 1e9, 2),

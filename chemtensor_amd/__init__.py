@@ -77,6 +77,7 @@ def lib():
             "ct_matvec_plan_bytes": (C.c_double, [P]),
             "ct_matvec_plan_stats": (C.c_int, [P, DP]),
             "ct_matvec_plan_stage_flops": (C.c_int, [P, DP]),
+            "ct_matvec_plan_stage_bytes": (C.c_int, [P, DP]),
             "ct_matvec_plan_forms": (C.c_int, [P, IP, FP]),
             "ct_matvec_apply": (C.c_int, [P, P, P]),
             "ct_matvec_apply_timed": (C.c_int, [P, P, P, FP]),
@@ -282,6 +283,11 @@ class MatvecPlan:
         t = np.zeros(12, dtype=np.float32)
         lib().ct_matvec_plan_forms(self.h, f.ctypes.data_as(IP), t.ctypes.data_as(FP))
         return f.tolist(), t.reshape(3, 4).tolist()
+
+    def stage_bytes(self):
+        s = np.zeros(3)
+        lib().ct_matvec_plan_stage_bytes(self.h, s.ctypes.data_as(DP))
+        return s
 
     def stage_flops(self):
         s = np.zeros(3)

@@ -113,6 +113,13 @@ int ct_matvec_plan_stage_flops(const ct_matvec_plan* plan, double* flops3)
 	return 0;
 }
 
+int ct_matvec_plan_stage_bytes(const ct_matvec_plan* plan, double* bytes3)
+{
+	const MatvecPlan& p = *plan->p;
+	for (int i = 0; i < 3; i++) bytes3[i] = p.stage_bytes[i];
+	return 0;
+}
+
 int ct_matvec_plan_forms(const ct_matvec_plan* plan, int* forms3, float* tuned_ms12)
 {
 	const MatvecPlan& p = *plan->p;

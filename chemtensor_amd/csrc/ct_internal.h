@@ -243,6 +243,7 @@ struct MatvecPlan
 	// bookkeeping
 	double flops = 0, bytes = 0;
 	double stage_flops[3] = { 0, 0, 0 };        // executed FP64 flops of each stage
+	double stage_bytes[3] = { 0, 0, 0 };        // algorithmic bytes of each stage (operands read once, result written once)
 	float tuned_ms[3][GEMM_NUM_FORMS] = {};     // autotuner timings (ms per launch) of each stage and form
 	double nterms = 0, ntasks = 0, nitems = 0, interm_elems = 0;
 	hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };

@@ -576,6 +576,9 @@ static void build_grouped(MatvecPlan& p)
 	p.stage_flops[0] = s1.flops;
 	p.stage_flops[1] = s2.flops;
 	p.stage_flops[2] = s3.flops;
+	p.stage_bytes[0] = 8.0 * (double)(th.nelem + R.nelem + t1_elems);
+	p.stage_bytes[1] = 8.0 * (double)(t1_elems + W.nelem + t2_elems);
+	p.stage_bytes[2] = 8.0 * (double)(t2_elems + L.nelem + th.nelem);
 	p.stage_bases[0] = GemmBases{ nullptr, p.r.d, p.interm[0].d };
 	p.stage_bases[1] = GemmBases{ p.w.d, p.interm[0].d, p.interm[1].d };
 	p.stage_bases[2] = GemmBases{ p.l.d, p.interm[1].d, nullptr };

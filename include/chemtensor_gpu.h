@@ -165,6 +165,13 @@ double ct_matvec_plan_flops(const ct_matvec_plan* plan);
 double ct_matvec_plan_bytes(const ct_matvec_plan* plan);
 /// plan statistics: [0] GEMM terms (stage 1,2,3 summed), [1] output tasks, [2] work items, [3] intermediate doubles
 int ct_matvec_plan_stats(const ct_matvec_plan* plan, double* stats4);
+/// per-stage algorithmic flops (stage 1: theta.R, 2: W.T1, 3: L^T.T2) of the grouped forms
+int ct_matvec_plan_stage_flops(const ct_matvec_plan* plan, double* flops3);
+/// per-stage algorithmic bytes: 8 (|inputs| + |output|) of each stage's launch
+int ct_matvec_plan_stage_bytes(const ct_matvec_plan* plan, double* bytes3);
+/// launch form chosen per stage (0 wave/simple, 1 wave/stream, 2 workgroup/chunk, 3 workgroup/stream)
+/// and the autotuner's time (ms) of every form, [stage * 4 + form]; 0 where not timed
+int ct_matvec_plan_forms(const ct_matvec_plan* plan, int* forms3, float* tuned_ms12);
 /// y = H_eff x on device vectors (packed in serialize order) on the plan's stream
 int ct_matvec_apply(ct_matvec_plan* plan, const double* d_x, double* d_y);
 /// as ct_matvec_apply, recording the device time (ms) of each of the three stages
@@ -188,6 +195,32 @@ int ct_memcpy_d2h(void* dst, const void* src, const ct_long nbytes);
 int ct_device_synchronize(void);
 /// number of visible GPUs (0 when none)
 int ct_device_count(void);
+/// dst[0:n] = src[0:n] (device to device) on the library's stream
+int ct_device_copy(double* dst, const double* src, const ct_long n);
+int ct_stream_synchronize(void* stream);
+/// HIP events (opaque hipEvent_t) for timing work on a plan's stream
+int ct_event_create(void** ev);
+int ct_event_record(void* ev, void* stream);
+/// waits for ev1, then returns the time between the two events in ms
+int ct_event_elapsed_ms(void* ev0, void* ev1, float* ms);
+int ct_event_destroy(void* ev);
+
+//________________________________________________________________________________________________________________________
+//
+// seeded synthetic inputs (bit-identical to the CPU restatement used by the tests)
+
+/// Fill every block of t with alpha * N(0,1) draws, blocks in row-major block order (the order
+/// block_sparse_tensor_fill_random_normal of src/tensor/block_sparse_tensor.c:498 draws in).
+void ct_fill_random_normal(struct block_sparse_tensor* t, double alpha, struct rng_state* rng);
+/// Virtual-bond dimensions and quantum numbers exactly as construct_random_mps (src/state/mps.c)
+/// draws them for this seed; dims_out has nsites+1 entries, qbonds_out the concatenated bonds.
+void ct_random_bond_qnums(const int nsites, const ct_long d, const qnumber* qsite, const qnumber sector, const ct_long max_vdim, const uint64_t seed, ct_long* dims_out, qnumber* qbonds_out);
+/// One H_eff problem at the centre bond i = nsites/2 - 1 of a chain (model 0: XXZ J=1 D=0.5 h=0,
+/// sector 0; model 1: Fermi-Hubbard t=1 U=4 mu=0, half filling): bond quantum numbers as
+/// construct_random_mps(seed), theta / left / right N(0,1) with seeds seed+1 / +2 / +3, h2 the merged
+/// MPO pair. The caller frees the four tensors with delete_block_sparse_tensor.
+int ct_generate_matvec_problem(const int model, const int nsites, const ct_long chi, const uint64_t seed,
+	struct block_sparse_tensor* theta, struct block_sparse_tensor* h2, struct block_sparse_tensor* left, struct block_sparse_tensor* right);
 
 #ifdef __cplusplus
 }

@@ -25,7 +25,7 @@ FP = C.POINTER(C.c_float)
 
 LEADING, TRAILING = 0, 1
 AXIS_IN, AXIS_OUT = -1, 1
-PLAIN_FP64, GROUPED_FP64, GROUPED_SPLIT = 0, 1, 2
+PLAIN_FP64, GROUPED_FP64, GROUPED_SPLIT, FUSED_FP64 = 0, 1, 2, 3
 
 ERRORS = {-1: "Krylov breakdown / factorisation failure", -2: "eigensolver failure", -100: "HIP runtime error",
           -101: "unsupported input", -102: "out of device memory / budget", -103: "invalid argument"}
@@ -48,6 +48,10 @@ class MPSStruct(C.Structure):
 MPOStruct = MPSStruct
 
 _lib = None
+
+
+# include/util/krylov.h:10
+LINEAR_FUNC_D = C.CFUNCTYPE(None, ct_long, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double))
 
 
 def lib():
@@ -83,6 +87,7 @@ def lib():
             "ct_matvec_apply_timed": (C.c_int, [P, P, P, FP]),
             "ct_matvec_plan_stream": (P, [P]),
             "ct_lanczos_ground_state": (C.c_int, [P, P, C.c_int, DP, P]),
+            "eigensystem_krylov_symmetric": (C.c_int, [ct_long, LINEAR_FUNC_D, P, DP, C.c_int, C.c_int, DP, DP]),
             "ct_device_alloc": (C.c_int, [C.POINTER(P), ct_long]),
             "ct_device_free": (C.c_int, [P]),
             "ct_memcpy_h2d": (C.c_int, [P, P, ct_long]),
@@ -220,6 +225,27 @@ def generate_matvec_problem(model, nsites, chi, seed):
     for t in ts:
         t.live = True
     return tuple(ts)
+
+
+def eigensystem_krylov_symmetric(afunc, vstart, maxiter, numeig):
+    """Lanczos eigensystem of the symmetric map afunc(v) -> A v (numpy in, numpy out) through the
+    library's eigensystem_krylov_symmetric (src/util/krylov.c:172-251): returns (lambda[numeig],
+    u[n, numeig])."""
+    vstart = np.ascontiguousarray(vstart, dtype=np.float64)
+    n = vstart.size
+
+    def cb(nn, data, v, ret):
+        x = np.ctypeslib.as_array(v, shape=(nn,))
+        y = np.ctypeslib.as_array(ret, shape=(nn,))
+        y[:] = afunc(x.copy())
+
+    fn = LINEAR_FUNC_D(cb)
+    lam = np.zeros(numeig)
+    u = np.zeros((n, numeig))
+    ret = lib().eigensystem_krylov_symmetric(n, fn, None, vstart.ctypes.data_as(DP), maxiter, numeig,
+                                             lam.ctypes.data_as(DP), u.ctypes.data_as(DP))
+    check(ret, "eigensystem_krylov_symmetric")
+    return lam, u
 
 
 class DeviceArray:

@@ -82,7 +82,7 @@ void mpo_merge_tensor_pair(const struct block_sparse_tensor* a0, const struct bl
 int ct_matvec_plan_create(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l, const struct block_sparse_tensor* r,
 	const int mode, ct_matvec_plan** plan)
 {
-	if (!a || !w || !l || !r || !plan || mode < 0 || mode > 2) return CT_ERR_ARG;
+	if (!a || !w || !l || !r || !plan || mode < 0 || mode > 3) return CT_ERR_ARG;
 	try {
 		auto* h = new ct_matvec_plan();
 		const char* env = getenv("CT_AUTOTUNE");
@@ -168,6 +168,38 @@ int ct_lanczos_ground_state(ct_matvec_plan* plan, const double* d_vstart, const 
 {
 	try {
 		return lanczos_ground_state(*plan->p, d_vstart, maxiter, lambda, d_u);
+	}
+	catch (const HipError&) {
+		return CT_ERR_HIP;
+	}
+}
+
+// src/util/krylov.c:172-251 with a caller-supplied host matvec: the Krylov vectors, the recurrence
+// and the Ritz vectors live on the device; each step copies v_j to the host for 'afunc' and the
+// result back. lambda[numeig] ascending, u_ritz n x numeig row-major (host memory).
+int eigensystem_krylov_symmetric(const ct_long n, lanczos_linear_func_d afunc, const void* adata,
+	const double* vstart, const int maxiter, const int numeig, double* lambda, double* u_ritz)
+{
+	if (n < 1 || maxiter < 1 || numeig < 1 || numeig > maxiter) return CT_ERR_ARG;
+	try {
+		hipStream_t s = default_stream();
+		LanczosWork lw;
+		const auto dfree = [](double* q) { dev_free(q); };
+		std::unique_ptr<double, decltype(dfree)> dv((double*)dev_alloc((size_t)n * sizeof(double)), dfree);
+		std::unique_ptr<double, decltype(dfree)> du((double*)dev_alloc((size_t)n * numeig * sizeof(double)), dfree);
+		std::vector<double> hx((size_t)n), hy((size_t)n);
+		CTG_HIP(hipMemcpyAsync(dv.get(), vstart, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+		const DevMatvec mv = [&](const double* x, double* y) {
+			CTG_HIP(hipMemcpyAsync(hx.data(), x, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s));
+			CTG_HIP(hipStreamSynchronize(s));
+			afunc(n, adata, hx.data(), hy.data());
+			CTG_HIP(hipMemcpyAsync(y, hy.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+		};
+		const int ret = lanczos_eigensystem(lw, n, s, mv, dv.get(), maxiter, numeig, lambda, du.get());
+		if (ret != 0) return ret;
+		CTG_HIP(hipMemcpyAsync(u_ritz, du.get(), (size_t)n * numeig * sizeof(double), hipMemcpyDeviceToHost, s));
+		CTG_HIP(hipStreamSynchronize(s));
+		return 0;
 	}
 	catch (const HipError&) {
 		return CT_ERR_HIP;

@@ -226,6 +226,65 @@ DevTensor dev_split_axis(const DevTensor& t, int i_ax, const ct_long* new_dims, 
 //
 // matvec plan
 
+// Fused single-launch FP64 matvec. One workgroup owns a "unit": k consecutive columns c of one
+// right-bond sector C of the output. It stages the R columns it needs in LDS, computes
+// T1(A,P,N,C)[:,:,:,c] = theta(A,P,B) R(B,N,C)[:,:,c] and T2(A,M,Q,C)[:,:,:,c] = W T1 into LDS and
+// writes y(X,Q,C)[:,:,c] = sum L^T T2. LDS offsets are stored per column (multiplied by k at run time).
+struct FzCopy  { int64_t goff; int rowstart; int dB, dN, dC, rb, nb, j0; };         // R(B,N,C) -> LDS panel of B
+struct FzTask1 { int64_t th_off; int rowstart, dB, nb, nJ, rb, t1; };               // T1 rows (a p) of (A,P), j-chunks of 4
+struct FzTask2 { int rowstart, dMQ, t2, u0, u1, pad; };                              // T2 rows (a m q) of (A,M,Q)
+struct FzTerm2 { int64_t w_off; int dP, dN, t1, nb, j0, pad; };                      // W(M,Q,P,N) x T1(A,P)[.., j0 + n]
+struct FzTask3 { int64_t y_off; int rowstart, dX, dQ, dC, u0, u1; };                 // y rows (x q) of (X,Q,C)
+struct FzTerm3 { int64_t l_off; int dAM, t2; };                                      // L(0,A,M,X)^T x T2(A,M,Q)
+struct FzSector { int m0, n0, m1, rows1, m2, rows2, m3, rows3, split, lds_pc, dC, pad; };
+struct FzUnit  { int sector, c0, k, pad; };
+
+struct FusedPlan
+{
+	FzSector* sectors = nullptr;
+	FzUnit* units = nullptr;
+	int* order = nullptr;       // blockIdx -> unit (-1: idle), XCD-aware
+	int* map = nullptr;         // row -> task of every phase, per sector
+	FzCopy* copies = nullptr;
+	FzTask1* t1 = nullptr;
+	FzTask2* t2 = nullptr;
+	FzTerm2* u2 = nullptr;
+	FzTask3* t3 = nullptr;
+	FzTerm3* u3 = nullptr;
+	int grid = 0, nunits = 0;
+	size_t lds_bytes = 0;
+	double flops = 0;
+	void release();
+};
+
+// Plan the fused form; returns false when a sector's per-column working set exceeds the LDS budget.
+bool build_fused_plan(FusedPlan& f, const Meta& theta, const Meta& w, const Meta& l, const Meta& r, hipStream_t s);
+void launch_fused_matvec(const FusedPlan& f, const double* x, const double* w, const double* l, const double* r, double* y, hipStream_t s);
+
+// Device workspace of the Lanczos iteration: maxiter Krylov vectors, w, the alpha / beta scalars,
+// reduction partials and the Ritz coefficients. Grows on demand; freed with its owner.
+struct LanczosWork
+{
+	double* v = nullptr;
+	double* w = nullptr;
+	double* scal = nullptr;
+	double* part = nullptr;
+	double* z = nullptr;
+	int64_t n = 0;
+	int cap = 0, zcap = 0;
+	void reserve(int64_t n, int maxiter, int numeig);
+	~LanczosWork();
+};
+
+// y = A x on device vectors of the Lanczos length, enqueued on the Lanczos stream
+using DevMatvec = std::function<void(const double* x, double* y)>;
+
+// Lanczos eigensystem (src/util/krylov.c:24-89 and :172-251): lambda[0:numeig] ascending,
+// u (device, n x numeig row-major) the Ritz vectors. Returns 0, -1 (fewer Lanczos iterations than
+// numeig) or -2 (tridiagonal eigensolver failure).
+int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart,
+	int maxiter, int numeig, double* lambda, double* u);
+
 struct MatvecPlan
 {
 	int mode = 1;
@@ -236,6 +295,8 @@ struct MatvecPlan
 	// grouped forms: three launches; stage 1 reads x, stage 3 writes y
 	DevGemm stage[3];
 	GemmBases stage_bases[3] = {};
+	// fused form: one launch
+	FusedPlan fused;
 	// plain form: the reference's operation sequence replayed as individual launches
 	std::vector<std::function<void(const double* x, double* y)>> plain_ops;
 	int plain_stage_end[3] = { 0, 0, 0 };       // index one past the last op of each dot stage
@@ -246,14 +307,9 @@ struct MatvecPlan
 	double stage_bytes[3] = { 0, 0, 0 };        // algorithmic bytes of each stage (operands read once, result written once)
 	float tuned_ms[3][GEMM_NUM_FORMS] = {};     // autotuner timings (ms per launch) of each stage and form
 	double nterms = 0, ntasks = 0, nitems = 0, interm_elems = 0;
+	int nunits = 0;                             // fused form: workgroup units
 	hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };
-	// Lanczos workspace
-	double* lz_v = nullptr;
-	int lz_cap = 0;
-	double* lz_w = nullptr;
-	double* lz_scal = nullptr;
-	double* lz_part = nullptr;
-	double* lz_z = nullptr;
+	LanczosWork lz;                             // Lanczos workspace reused across solves
 	~MatvecPlan();
 };
 

@@ -443,9 +443,9 @@ DevTensor dev_split_axis(const DevTensor& t, int i_ax, const ct_long* nd2, const
 MatvecPlan::~MatvecPlan()
 {
 	for (int i = 0; i < 3; i++) stage[i].release();
+	fused.release();
 	for (void* p : owned) dev_free(p);
 	for (int i = 0; i < 4; i++) if (ev[i]) (void)hipEventDestroy(ev[i]);
-	dev_free(lz_v); dev_free(lz_w); dev_free(lz_scal); dev_free(lz_part); dev_free(lz_z);
 }
 
 static bool same_axis(const Meta& a, int ia, const Meta& b, int ib)
@@ -739,6 +739,13 @@ MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& 
 	p->r = std::move(r);
 	try {
 		if (mode == CT_MATVEC_PLAIN_FP64) build_plain(*p);
+		else if (mode == CT_MATVEC_FUSED_FP64) {
+			if (!build_fused_plan(p->fused, theta, p->w.meta, p->l.meta, p->r.meta, s))
+				throw HipError{ hipErrorNotSupported, "fused matvec: a right-bond sector column does not fit in LDS" };
+			p->stage_flops[0] = p->fused.flops;
+			p->stage_bytes[0] = 8.0 * (2.0 * (double)theta.nelem + (double)p->w.meta.nelem + (double)p->l.meta.nelem + (double)p->r.meta.nelem);
+			p->nunits = p->fused.nunits;
+		}
 		else build_grouped(*p);
 		if (mode == CT_MATVEC_GROUPED_FP64 && autotune) autotune_stages(*p);
 		for (int i = 0; i < 4; i++) CTG_HIP(hipEventCreate(&p->ev[i]));
@@ -764,7 +771,11 @@ void matvec_apply(MatvecPlan& p, const double* x, double* y, float* stage_ms)
 {
 	hipStream_t s = p.stream;
 	if (stage_ms) CTG_HIP(hipEventRecord(p.ev[0], s));
-	if (p.mode == CT_MATVEC_PLAIN_FP64) {
+	if (p.mode == CT_MATVEC_FUSED_FP64) {
+		launch_fused_matvec(p.fused, x, p.w.d, p.l.d, p.r.d, y, s);
+		if (stage_ms) for (int st = 1; st < 4; st++) CTG_HIP(hipEventRecord(p.ev[st], s));
+	}
+	else if (p.mode == CT_MATVEC_PLAIN_FP64) {
 		int i = 0;
 		for (int st = 0; st < 3; st++) {
 			for (; i < p.plain_stage_end[st]; i++) p.plain_ops[i](x, y);

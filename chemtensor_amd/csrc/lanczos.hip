@@ -1,8 +1,8 @@
-// Device-resident Lanczos ground state (src/util/krylov.c:24-89 and :172-251 with numeig = 1):
+// Device-resident Lanczos eigensystem (src/util/krylov.c:24-89 and :172-251):
 // three-term recurrence without re-orthogonalisation, breakdown test beta < 100 n eps, all
 // vectors and the alpha / beta scalars on the GPU in FP64. Reductions are two-pass with a
 // fixed summation order (deterministic). The tridiagonal eigenproblem (at most maxiter x
-// maxiter) is solved on the host in FP64; the Ritz vector is formed on the device.
+// maxiter) is solved on the host in FP64; the Ritz vectors are formed on the device.
 
 #include <cmath>
 #include <cstring>
@@ -95,48 +95,66 @@ __global__ __launch_bounds__(LZ_THREADS) void k_lz_beta_scale(const double* __re
 	for (int64_t i = (int64_t)blockIdx.x * LZ_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * LZ_THREADS) vnext[i] = inv * w[i];
 }
 
-// u = sum_j z_j v_j
-__global__ __launch_bounds__(LZ_THREADS) void k_lz_ritz(const double* __restrict__ V, const double* __restrict__ z, int numiter, int64_t n, double* __restrict__ u)
+// u[i, k] = sum_j V[j, i] z[j, k]  (u: n x numeig row-major, z: numiter x numeig row-major)
+__global__ __launch_bounds__(LZ_THREADS) void k_lz_ritz(const double* __restrict__ V, const double* __restrict__ z, int numiter, int numeig, int64_t n, double* __restrict__ u)
 {
 	for (int64_t i = (int64_t)blockIdx.x * LZ_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * LZ_THREADS) {
-		double acc = 0;
-		for (int j = 0; j < numiter; j++) acc += V[(int64_t)j * n + i] * z[j];
-		u[i] = acc;
+		for (int k = 0; k < numeig; k++) {
+			double acc = 0;
+			for (int j = 0; j < numiter; j++) acc += V[(int64_t)j * n + i] * z[j * numeig + k];
+			u[i * numeig + k] = acc;
+		}
 	}
 }
 
-int lanczos_ground_state(MatvecPlan& p, const double* vstart, int maxiter, double* lambda, double* u)
+void LanczosWork::reserve(int64_t nn, int maxiter, int numeig)
 {
-	if (maxiter < 1) return CT_ERR_ARG;
-	hipStream_t s = p.stream;
-	const int64_t n = p.theta_meta.nelem;
+	if (nn != n || cap < maxiter) {
+		dev_free(v);
+		dev_free(w);
+		dev_free(scal);
+		v = (double*)dev_alloc((size_t)maxiter * (size_t)nn * sizeof(double));
+		w = (double*)dev_alloc((size_t)nn * sizeof(double));
+		scal = (double*)dev_alloc((size_t)(2 * maxiter + 2) * sizeof(double));
+		n = nn;
+		cap = maxiter;
+	}
+	if (!part) part = (double*)dev_alloc(2 * 1024 * sizeof(double));
+	if (zcap < maxiter * numeig) {
+		dev_free(z);
+		z = (double*)dev_alloc((size_t)maxiter * numeig * sizeof(double));
+		zcap = maxiter * numeig;
+	}
+}
+
+LanczosWork::~LanczosWork()
+{
+	dev_free(v);
+	dev_free(w);
+	dev_free(scal);
+	dev_free(part);
+	dev_free(z);
+}
+
+int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart,
+	int maxiter, int numeig, double* lambda, double* u)
+{
+	if (maxiter < 1 || numeig < 1 || numeig > maxiter || n < 1) return CT_ERR_ARG;
 	const int G = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 4095) / 4096));
-	if (p.lz_cap < maxiter) {
-		dev_free(p.lz_v);
-		dev_free(p.lz_scal);
-		dev_free(p.lz_z);
-		p.lz_v = (double*)dev_alloc((size_t)maxiter * (size_t)n * sizeof(double));
-		p.lz_scal = (double*)dev_alloc((size_t)(2 * maxiter + 2) * sizeof(double));
-		p.lz_z = (double*)dev_alloc((size_t)maxiter * sizeof(double));
-		p.lz_cap = maxiter;
-	}
-	if (!p.lz_w) {
-		p.lz_w = (double*)dev_alloc((size_t)n * sizeof(double));
-		p.lz_part = (double*)dev_alloc(2 * 1024 * sizeof(double));
-	}
-	double* V = p.lz_v;
-	double* w = p.lz_w;
-	double* scal = p.lz_scal;
-	double* part = p.lz_part;
-	double* part2 = p.lz_part + 1024;
+	lw.reserve(n, maxiter, numeig);
+	double* V = lw.v;
+	double* w = lw.w;
+	double* scal = lw.scal;
+	double* part = lw.part;
+	double* part2 = lw.part + 1024;
 	launch_fill(scal, 0.0, 2 * maxiter + 2, s);
-	const double thresh = 100.0 * (double)n * 2.2204460492503131e-16;
+	const double thresh = 100.0 * (double)n * 2.2204460492503131e-16;   // krylov.c:58
 	hipLaunchKernelGGL(k_lz_dot_partial, dim3(G), dim3(LZ_THREADS), 0, s, vstart, vstart, n, part, (const double*)nullptr);
 	hipLaunchKernelGGL(k_lz_normalize, dim3(G), dim3(LZ_THREADS), 0, s, vstart, V, n, part, G);
 	CTG_HIP(hipGetLastError());
 	for (int j = 0; j < maxiter; j++) {
 		const double* vj = V + (int64_t)j * n;
-		matvec_apply(p, vj, w, nullptr);
+		matvec(vj, w);
 		hipLaunchKernelGGL(k_lz_dot_partial, dim3(G), dim3(LZ_THREADS), 0, s, w, vj, n, part, scal + 2 * maxiter + 1);
 		hipLaunchKernelGGL(k_lz_alpha_update, dim3(G), dim3(LZ_THREADS), 0, s, w, vj, j > 0 ? vj - n : vj, n, j, maxiter, part, G, part2, scal);
 		if (j < maxiter - 1)
@@ -147,26 +165,34 @@ int lanczos_ground_state(MatvecPlan& p, const double* vstart, int maxiter, doubl
 	CTG_HIP(hipMemcpyAsync(hs.data(), scal, hs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
 	CTG_HIP(hipStreamSynchronize(s));
 	const int numiter = (hs[2 * maxiter + 1] != 0) ? (int)hs[2 * maxiter] : maxiter;
-	if (numiter < 1) {
-		fprintf(stderr, "Lanczos iteration stopped after %i iterations, cannot compute %i eigenvalues\n", numiter, 1);
+	if (numiter < numeig) {
+		fprintf(stderr, "Lanczos iteration stopped after %i iterations, cannot compute %i eigenvalues\n", numiter, numeig);
 		return -1;
 	}
 	std::vector<double> d(hs.begin(), hs.begin() + numiter);
 	std::vector<double> e((size_t)numiter + 1, 0.0);
 	for (int i = 0; i < numiter - 1; i++) e[i] = hs[maxiter + i];
-	std::vector<double> z((size_t)numiter * numiter);
+	std::vector<double> z((size_t)numiter * numiter);   // column k (z[k numiter : (k+1) numiter]) = eigenvector k
 	if (host_tridiag_eigh(numiter, d.data(), e.data(), z.data()) != 0) {
 		fprintf(stderr, "tridiagonal eigensolver failed\n");
 		return -2;
 	}
-	*lambda = d[0];
-	std::vector<double> z0(z.begin(), z.begin() + numiter);   // eigenvector of the smallest eigenvalue (column 0)
-	CTG_HIP(hipMemcpyAsync(p.lz_z, z0.data(), z0.size() * sizeof(double), hipMemcpyHostToDevice, s));
+	for (int k = 0; k < numeig; k++) lambda[k] = d[k];
+	std::vector<double> zr((size_t)numiter * numeig);
+	for (int j = 0; j < numiter; j++)
+		for (int k = 0; k < numeig; k++) zr[(size_t)j * numeig + k] = z[(size_t)k * numiter + j];
+	CTG_HIP(hipMemcpyAsync(lw.z, zr.data(), zr.size() * sizeof(double), hipMemcpyHostToDevice, s));
 	const int gr = (int)std::max<int64_t>(1, std::min<int64_t>(4096, (n + LZ_THREADS - 1) / LZ_THREADS));
-	hipLaunchKernelGGL(k_lz_ritz, dim3(gr), dim3(LZ_THREADS), 0, s, V, p.lz_z, numiter, n, u);
+	hipLaunchKernelGGL(k_lz_ritz, dim3(gr), dim3(LZ_THREADS), 0, s, V, lw.z, numiter, numeig, n, u);
 	CTG_HIP(hipGetLastError());
 	CTG_HIP(hipStreamSynchronize(s));
 	return 0;
+}
+
+int lanczos_ground_state(MatvecPlan& p, const double* vstart, int maxiter, double* lambda, double* u)
+{
+	return lanczos_eigensystem(p.lz, p.theta_meta.nelem, p.stream,
+		[&p](const double* x, double* y) { matvec_apply(p, x, y, nullptr); }, vstart, maxiter, 1, lambda, u);
 }
 
 }  // namespace ctg

@@ -150,6 +150,7 @@ enum ct_matvec_mode
 	CT_MATVEC_PLAIN_FP64   = 0,  ///< reference structure: per-block FP64 GEMM launches + transpose / trace kernels
 	CT_MATVEC_GROUPED_FP64 = 1,  ///< three persistent grouped FP64 MFMA launches, transposes folded into addressing
 	CT_MATVEC_GROUPED_SPLIT = 2, ///< as GROUPED_FP64, operands split into FP32 terms on FP32 MFMA with FP64 accumulation
+	CT_MATVEC_FUSED_FP64   = 3,  ///< one launch per matvec: intermediates of a right-bond column chunk stay in LDS (FP64)
 };
 
 /// Build a plan for H_eff = apply_local_hamiltonian(., w, l, r) on vectors with the block
@@ -178,6 +179,17 @@ int ct_matvec_apply(ct_matvec_plan* plan, const double* d_x, double* d_y);
 int ct_matvec_apply_timed(ct_matvec_plan* plan, const double* d_x, double* d_y, float* stage_ms3);
 /// the plan's HIP stream (hipStream_t) as an opaque pointer
 void* ct_matvec_plan_stream(ct_matvec_plan* plan);
+
+/// include/util/krylov.h:10
+typedef void lanczos_linear_func_d(const ct_long n, const void* data, const double* v, double* ret);
+
+/// include/util/krylov.h:24 (src/util/krylov.c:172-251): Lanczos eigensystem of the symmetric linear
+/// map 'afunc' (host callback on host vectors). The Krylov vectors and the recurrence live on the
+/// GPU; every step copies v_j to the host for 'afunc' and the result back. lambda[numeig] ascending,
+/// u_ritz (n x numeig, row-major) the Ritz vectors. Returns 0, -1 (fewer iterations than numeig),
+/// -2 (tridiagonal eigensolver failure) or a CT_ERR_* code.
+int eigensystem_krylov_symmetric(const ct_long n, lanczos_linear_func_d afunc, const void* adata,
+	const double* vstart, const int maxiter, const int numeig, double* lambda, double* u_ritz);
 
 /// Lanczos ground state of the plan's H_eff (krylov.c:172-251 with numeig = 1) on the device:
 /// d_vstart / d_u are device vectors of length ct_matvec_plan_dim. Returns 0, -1 (breakdown

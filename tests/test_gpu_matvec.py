@@ -89,13 +89,14 @@ def test_mpo_merge_matches_oracle():
         assert np.array_equal(g.serialize(), o.serialize())
 
 
-@pytest.mark.parametrize("model,L,chi", [("xxz", 12, 16), ("hubbard", 8, 24), ("hubbard", 32, 64), ("xxz", 32, 128), ("hubbard", 32, 256)])
+@pytest.mark.parametrize("model,L,chi", [("xxz", 12, 16), ("hubbard", 8, 24), ("hubbard", 32, 64), ("xxz", 32, 128), ("hubbard", 32, 256),
+                                         ("hubbard", 32, 512), ("xxz", 32, 512)])
 def test_matvec_all_forms_match_oracle(model, L, chi):
     say("matvec", model, L, chi)
     theta, h2, left, right, _ = orc.matvec_problem(model, L, chi, 1234)
     want = orc.apply_local_hamiltonian(theta, h2, left, right).serialize()
     x = ct.DeviceArray.from_numpy(theta.serialize())
-    for mode in (ct.PLAIN_FP64, ct.GROUPED_FP64, ct.GROUPED_SPLIT):
+    for mode in (ct.PLAIN_FP64, ct.GROUPED_FP64, ct.GROUPED_SPLIT, ct.FUSED_FP64):
         say("  mode", mode)
         plan = ct.MatvecPlan(theta, h2, left, right, mode)
         assert plan.dim == theta.nelem()
@@ -141,9 +142,9 @@ def test_lanczos_ground_state_matches_oracle():
     theta, h2, left, right = orc.local_problem(mpo, psi, 3)
     n = theta.nelem()
     vstart = theta.serialize()
-    for maxiter in (3, 10, 25):
-        say("maxiter", maxiter)
-        plan = ct.MatvecPlan(theta, h2, left, right, ct.GROUPED_FP64)
+    for maxiter, mode in ((3, ct.GROUPED_FP64), (10, ct.GROUPED_FP64), (25, ct.GROUPED_FP64), (25, ct.FUSED_FP64)):
+        say("maxiter", maxiter, "mode", mode)
+        plan = ct.MatvecPlan(theta, h2, left, right, mode)
         lam, u = plan.lanczos(ct.DeviceArray.from_numpy(vstart), maxiter)
         lam_o = np.zeros(1)
         u_o = np.zeros(n)
@@ -162,3 +163,20 @@ def test_lanczos_ground_state_matches_oracle():
         ug = u.numpy()
         s = np.sign(np.dot(ug, u_o))
         assert rel(s * ug, u_o) < 1e-9
+
+
+def test_eigensystem_krylov_symmetric_fixture(golden):
+    """The exported eigensystem_krylov_symmetric (host callback, several eigenpairs) reproduces the
+    reference fixture test/util/data/test_eigensystem_krylov_symmetric.hdf5 (maxiter 35)."""
+    z = golden("test_eigensystem_krylov_symmetric")
+    a = np.ascontiguousarray(z["D_a"])
+    numeig = z["D_lambda"].size
+    say("eigensystem n", a.shape[0], "numeig", numeig)
+    lam, u = ct.eigensystem_krylov_symmetric(lambda v: a @ v, z["D_vstart"], 35, numeig)
+    assert np.max(np.abs(lam - z["D_lambda"])) < 1e-13
+    uref = z["D_u_ritz"]
+    for e in range(numeig):  # Ritz vectors are fixed up to sign
+        s = np.sign(np.dot(u[:, e], uref[:, e]))
+        assert np.max(np.abs(s * u[:, e] - uref[:, e])) < 1e-12
+    with pytest.raises(RuntimeError):
+        ct.eigensystem_krylov_symmetric(lambda v: a @ v, z["D_vstart"], 2, 3)   # numeig > maxiter

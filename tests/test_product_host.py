@@ -16,7 +16,7 @@ def declared_functions():
     text = re.sub(r"//[^\n]*", "", text)
     names = []
     for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(([^;{]*)\)\s*;", text, flags=re.M):
-        if m.group(1) not in ("if", "while", "for", "return", "sizeof"):
+        if m.group(1) not in ("if", "while", "for", "return", "sizeof") and not m.group(0).startswith("typedef"):
             names.append(m.group(1))
     return names
 

@@ -63,6 +63,125 @@ def cpu_baseline(model, nsites, chi, seed, budget_s=10.0):
     return dt, n, threads, y.serialize()
 
 
+def bench_batch(ct, nprob, seed, steps, warmup, lanczos_iter):
+    """Batched launch: nprob independent H_eff problems (XXZ or Hubbard, L in 16..64, chi drawn from
+    {64, 128, 256, 512, 1024}, per-instance seeds) applied in one fused launch, and their Lanczos
+    ground states with one recurrence per problem (one launch of every kind per Lanczos step)."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(nprob):
+        model = ["xxz", "hubbard"][int(rng.integers(2))]
+        L = int(rng.integers(16, 65)) // 2 * 2
+        chi = int(rng.choice([64, 128, 256, 512, 1024]))
+        specs.append((model, L, chi, seed + 1000 * i))
+    t0 = time.perf_counter()
+    probs = [ct.generate_matvec_problem(*sp) for sp in specs]
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batch = ct.MatvecBatch(probs)
+    t_plan = time.perf_counter() - t0
+    x = ct.DeviceArray.from_numpy(np.concatenate([p[0].serialize() for p in probs]))
+    y = ct.DeviceArray(batch.dim)
+    for _ in range(warmup):
+        batch.apply(x, y)
+    batch.synchronize()
+    e0, e1 = ct.Event(), ct.Event()
+    e0.record(batch.stream)
+    for _ in range(steps):
+        batch.apply(x, y)
+    e1.record(batch.stream)
+    ms = e0.elapsed_ms(e1) / steps
+    batch.lanczos(x, lanczos_iter)
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        lam, st, _u = batch.lanczos(x, lanczos_iter)
+    lz_ms = (time.perf_counter() - t0) / reps * 1e3
+    log(f"batch of {nprob}: {ms:.4f} ms per batched matvec ({batch.flops / (ms * 1e-3) / 1e12:.3f} TFLOP/s), "
+        f"Lanczos {lanczos_iter} steps {lz_ms:.3f} ms; generate {t_gen:.1f}s plan {t_plan:.2f}s")
+    return {
+        "workload": f"{nprob} independent two-site H_eff matvecs (XXZ / Fermi-Hubbard, L 16-64, chi from {{64,128,256,512,1024}}) in one fused launch",
+        "problems": [f"{m} L={L} chi={c}" for (m, L, c, _) in specs],
+        "elements": int(batch.dim),
+        "flops_per_batched_matvec": batch.flops,
+        "ms_per_batched_matvec": round(ms, 5),
+        "tflops": round(batch.flops / (ms * 1e-3) / 1e12, 4),
+        "hbm_gbs_algorithmic": round(batch.bytes / (ms * 1e-3) / 1e9, 2),
+        "lanczos_maxiter": lanczos_iter,
+        "lanczos_ms_per_batched_solve": round(lz_ms, 4),
+        "lanczos_ms_per_step": round(lz_ms / lanczos_iter, 5),
+        "lanczos_status_nonzero": int(np.count_nonzero(st)),
+        "plan_seconds": round(t_plan, 3),
+        "form": "fused (one launch)" if batch.fused else "grouped (three launches)",
+        "form_ms_grouped_fused": [round(v, 5) for v in batch.form_ms],
+    }
+
+
+def bench_serve(ct, nreq, rate, L_range, chi_range, seed, budget, max_active, timeout_s=240.0):
+    """Serving loop: nreq ground-state solve requests with Poisson arrivals at `rate` per second,
+    each an XXZ chain (Delta ~ U[0,2]) or a Fermi-Hubbard chain (U ~ U[0,8]) with L and chi drawn
+    uniformly from the given ranges, Lanczos 10 steps per bond, stop when a sweep changes the energy
+    by less than 1e-10 (at most 12 sweeps). Latency = finish - submit."""
+    import threading
+    rng = np.random.default_rng(seed)
+    reqs, desc = [], []
+    for i in range(nreq):
+        L = int(rng.integers(L_range[0], L_range[1] + 1)) // 2 * 2
+        chi = int(rng.integers(chi_range[0], chi_range[1] + 1))
+        if rng.integers(2) == 0:
+            delta = float(rng.uniform(0, 2))
+            reqs.append(ct.serve_request("xxz", L, (1.0, delta, 0.0), 0, chi, num_sweeps=12, maxiter_lanczos=10,
+                                         tol_split=1e-10, tol_energy=1e-10, seed=seed + i))
+            desc.append(f"xxz L={L} chi={chi} D={delta:.3f}")
+        else:
+            u = float(rng.uniform(0, 8))
+            reqs.append(ct.serve_request("hubbard", L, (1.0, u, 0.0), ct.encode_pair(L, 0), chi, num_sweeps=12,
+                                         maxiter_lanczos=10, tol_split=1e-10, tol_energy=1e-10, seed=seed + i))
+            desc.append(f"hubbard L={L} chi={chi} U={u:.3f}")
+    gaps = rng.exponential(1.0 / rate, nreq)
+    srv = ct.Server(budget, max_active)
+    tickets = [None] * nreq
+
+    def submitter():
+        for i in range(nreq):
+            time.sleep(gaps[i])
+            tickets[i] = srv.submit(reqs[i])
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    res = []
+    for i in range(nreq):
+        while True:
+            r = srv.poll(tickets[i], wait=False)
+            if r.status != 1 or time.perf_counter() - t0 > timeout_s:
+                break
+            time.sleep(0.01)
+        res.append(r)
+    makespan = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    done = [r for r in res if r.status == 0]
+    lat = np.array([r.t_done - r.t_submit for r in done]) if done else np.zeros(1)
+    log(f"serve: {len(done)}/{nreq} solved in {makespan:.1f}s, p50 {np.percentile(lat, 50):.2f}s p99 {np.percentile(lat, 99):.2f}s")
+    return {
+        "workload": f"{nreq} DMRG ground-state solves, Poisson arrivals at {rate}/s, XXZ (Delta~U[0,2]) or Hubbard (U~U[0,8]), "
+                    f"L~U{{{L_range[0]}..{L_range[1]}}}, chi~U{{{chi_range[0]}..{chi_range[1]}}}, stop at |dE_sweep| < 1e-10",
+        "requests": nreq, "solved": len(done), "failed": int(sum(1 for r in res if r.status < 0)),
+        "arrival_rate_per_s": rate,
+        "solves_per_s": round(len(done) / makespan, 4),
+        "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+        "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
+        "latency_mean_s": round(float(lat.mean()), 4),
+        "queue_wait_p50_s": round(float(np.percentile([r.t_admit - r.t_submit for r in done] or [0], 50)), 4),
+        "sweeps_mean": round(float(np.mean([r.sweeps for r in done])) if done else 0, 2),
+        "makespan_s": round(makespan, 3),
+        "hbm_budget_bytes": budget, "max_active": max_active,
+        "server_stats": st,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +192,13 @@ def main():
     ap.add_argument("--model", default="hubbard")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=64, help="problems of the batched-launch measurement (0: skip)")
+    ap.add_argument("--serve", type=int, default=24, help="requests of the serving measurement (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-L", default="16,32", help="chain length range")
+    ap.add_argument("--serve-chi", default="32,128", help="bond dimension range")
+    ap.add_argument("--serve-budget", type=float, default=200e9, help="HBM admission budget (bytes)")
+    ap.add_argument("--lanczos-iter", type=int, default=20, help="Lanczos steps of the timed ground-state solve")
     ap.add_argument("--hbm-traffic-kib", default="",
                     help="FETCH_SIZE,WRITE_SIZE (KiB per launch) of the dominant kernel from rocprofv3 --pmc runs")
     ap.add_argument("--fetch-scale", type=float, default=2.0,
@@ -195,6 +321,31 @@ def main():
     for k in ys:
         if k != "grouped_fp64":
             result[f"rel_err_{k}_vs_grouped"] = float(np.linalg.norm(ys[k] - y_g) / np.linalg.norm(y_g))
+
+    # Lanczos ground-state solve on the headline plan: recurrence captured in a HIP graph vs eager launches
+    lz = {}
+    v0 = ct.DeviceArray.from_numpy(np.random.default_rng(args.seed).standard_normal(plan.dim))
+    for label, flag in (("graph", "1"), ("eager", "0")):
+        os.environ["CT_LANCZOS_GRAPH"] = flag
+        plan.lanczos(v0, args.lanczos_iter)
+        t0 = time.perf_counter()
+        reps = 20
+        for _ in range(reps):
+            lam, _u = plan.lanczos(v0, args.lanczos_iter)
+        lz[label] = (time.perf_counter() - t0) / reps * 1e3
+    os.environ.pop("CT_LANCZOS_GRAPH", None)
+    result["lanczos"] = {"maxiter": args.lanczos_iter, "ms_per_solve_graph": round(lz["graph"], 4),
+                         "ms_per_solve_eager": round(lz["eager"], 4), "ms_per_step_graph": round(lz["graph"] / args.lanczos_iter, 5),
+                         "lambda0": lam, "form": head}
+    log(f"lanczos {args.lanczos_iter} steps: graph {lz['graph']:.3f} ms, eager {lz['eager']:.3f} ms")
+
+    if args.batch > 0:
+        result["batched"] = bench_batch(ct, args.batch, args.seed, max(10, args.steps // 5), 3, args.lanczos_iter)
+
+    if args.serve > 0:
+        lr = tuple(int(v) for v in args.serve_L.split(","))
+        cr = tuple(int(v) for v in args.serve_chi.split(","))
+        result["serving"] = bench_serve(ct, args.serve, args.serve_rate, lr, cr, args.seed, args.serve_budget, 64)
 
     if not args.no_cpu:
         dt, n, threads, y_cpu = cpu_baseline(args.model, args.nsites, args.chi, args.seed)

@@ -87,6 +87,22 @@ def lib():
             "ct_matvec_apply_timed": (C.c_int, [P, P, P, FP]),
             "ct_matvec_plan_stream": (P, [P]),
             "ct_lanczos_ground_state": (C.c_int, [P, P, C.c_int, DP, P]),
+            "ct_matvec_batch_create": (C.c_int, [C.c_int, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
+            "ct_matvec_batch_destroy": (C.c_int, [P]),
+            "ct_matvec_batch_dim": (ct_long, [P]),
+            "ct_matvec_batch_offsets": (C.c_int, [P, LP]),
+            "ct_matvec_batch_flops": (C.c_double, [P]),
+            "ct_matvec_batch_bytes": (C.c_double, [P]),
+            "ct_matvec_batch_form": (C.c_int, [P, IP, FP]),
+            "ct_matvec_batch_stream": (P, [P]),
+            "ct_matvec_batch_apply": (C.c_int, [P, P, P]),
+            "ct_lanczos_ground_state_batch": (C.c_int, [P, P, C.c_int, DP, IP, P]),
+            "ct_serve_create": (C.c_int, [C.c_double, C.c_int, C.POINTER(P)]),
+            "ct_serve_submit": (C.c_int, [P, P, C.POINTER(C.c_int64)]),
+            "ct_serve_poll": (C.c_int, [P, C.c_int64, C.c_int, P]),
+            "ct_serve_energies": (C.c_int, [P, C.c_int64, DP, C.c_int]),
+            "ct_serve_stats": (C.c_int, [P, DP]),
+            "ct_serve_destroy": (C.c_int, [P]),
             "eigensystem_krylov_symmetric": (C.c_int, [ct_long, LINEAR_FUNC_D, P, DP, C.c_int, C.c_int, DP, DP]),
             "ct_device_alloc": (C.c_int, [C.POINTER(P), ct_long]),
             "ct_device_free": (C.c_int, [P]),
@@ -341,6 +357,115 @@ class MatvecPlan:
         ret = lib().ct_lanczos_ground_state(self.h, vstart.ptr, maxiter, C.byref(lam), u.ptr)
         check(ret, "ct_lanczos_ground_state")
         return lam.value, u
+
+
+class MatvecBatch:
+    """Independent H_eff problems applied in one fused launch (ct_matvec_batch_*); problems are
+    (theta, w, l, r) tuples of block-sparse tensors."""
+
+    def __init__(self, problems):
+        n = len(problems)
+        arrs = [(P * n)(*[_ptr(p[i]) for p in problems]) for i in range(4)]
+        self.h = P()
+        check(lib().ct_matvec_batch_create(n, *arrs, C.byref(self.h)), "ct_matvec_batch_create")
+        self.nprob = n
+        self.dim = lib().ct_matvec_batch_dim(self.h)
+        off = np.zeros(n + 1, dtype=np.int64)
+        lib().ct_matvec_batch_offsets(self.h, off.ctypes.data_as(LP))
+        self.offsets = off
+        self.flops = lib().ct_matvec_batch_flops(self.h)
+        self.bytes = lib().ct_matvec_batch_bytes(self.h)
+        self.stream = lib().ct_matvec_batch_stream(self.h)
+        f = C.c_int(0)
+        ms = (C.c_float * 2)()
+        lib().ct_matvec_batch_form(self.h, C.byref(f), ms)
+        self.fused = bool(f.value)
+        self.form_ms = (ms[0], ms[1])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ct_matvec_batch_destroy(self.h)
+            self.h = None
+
+    def apply(self, x, y):
+        check(lib().ct_matvec_batch_apply(self.h, x.ptr, y.ptr), "ct_matvec_batch_apply")
+
+    def synchronize(self):
+        check(lib().ct_stream_synchronize(self.stream), "ct_stream_synchronize")
+
+    def lanczos(self, vstart, maxiter):
+        lam = np.zeros(self.nprob)
+        st = np.zeros(self.nprob, dtype=np.int32)
+        u = DeviceArray(self.dim)
+        lib().ct_lanczos_ground_state_batch(self.h, vstart.ptr, maxiter, lam.ctypes.data_as(DP), st.ctypes.data_as(IP), u.ptr)
+        return lam, st, u
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("model", C.c_int), ("nsites", C.c_int), ("param", C.c_double * 3), ("sector", C.c_int32), ("pad", C.c_int),
+                ("max_vdim", ct_long), ("num_sweeps", C.c_int), ("maxiter_lanczos", C.c_int), ("tol_split", C.c_double),
+                ("tol_energy", C.c_double), ("seed", C.c_uint64)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("sweeps", C.c_int), ("energy", C.c_double), ("energy_change", C.c_double),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+def serve_request(model, nsites, param, sector, max_vdim, num_sweeps=8, maxiter_lanczos=10, tol_split=1e-10,
+                  tol_energy=1e-10, seed=0):
+    r = ServeRequest()
+    r.model = 0 if model == "xxz" else 1
+    r.nsites = nsites
+    for i, v in enumerate(param):
+        r.param[i] = v
+    r.sector = sector
+    r.max_vdim = max_vdim
+    r.num_sweeps = num_sweeps
+    r.maxiter_lanczos = maxiter_lanczos
+    r.tol_split = tol_split
+    r.tol_energy = tol_energy
+    r.seed = seed
+    return r
+
+
+class Server:
+    """Continuous-batching server of DMRG ground-state solves (ct_serve_*)."""
+
+    def __init__(self, hbm_budget_bytes=200e9, max_active=64):
+        self.h = P()
+        check(lib().ct_serve_create(float(hbm_budget_bytes), max_active, C.byref(self.h)), "ct_serve_create")
+
+    def submit(self, req):
+        t = C.c_int64(0)
+        check(lib().ct_serve_submit(self.h, C.byref(req), C.byref(t)), "ct_serve_submit")
+        return t.value
+
+    def poll(self, ticket, wait=False):
+        r = ServeResult()
+        ret = lib().ct_serve_poll(self.h, ticket, 1 if wait else 0, C.byref(r))
+        if ret < 0:
+            raise RuntimeError(f"ct_serve_poll failed: {ERRORS.get(ret, ret)}")
+        return r
+
+    def energies(self, ticket):
+        buf = np.zeros(256)
+        n = lib().ct_serve_energies(self.h, ticket, buf.ctypes.data_as(DP), buf.size)
+        return buf[:max(n, 0)].copy()
+
+    def stats(self):
+        st = np.zeros(8)
+        lib().ct_serve_stats(self.h, st.ctypes.data_as(DP))
+        keys = ["ticks", "lanczos_batches", "completed", "queued", "reserved_bytes", "max_batch", "busy_s", "uptime_s"]
+        return dict(zip(keys, st.tolist()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ct_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Event:

@@ -12,6 +12,16 @@ struct ct_matvec_plan
 	std::unique_ptr<MatvecPlan> p;
 };
 
+struct ct_matvec_batch
+{
+	std::unique_ptr<BatchPlan> b;
+};
+
+static int map_error(const HipError& e)
+{
+	return (e.code == hipErrorNotSupported) ? CT_ERR_UNSUPPORTED : (e.code == hipErrorInvalidValue ? CT_ERR_ARG : CT_ERR_HIP);
+}
+
 // void entry points mirror the reference's contract checks (assert): a failure is fatal
 [[noreturn]] static void fatal(const char* fn, const char* what)
 {
@@ -84,11 +94,11 @@ int ct_matvec_plan_create(const struct block_sparse_tensor* a, const struct bloc
 {
 	if (!a || !w || !l || !r || !plan || mode < 0 || mode > 3) return CT_ERR_ARG;
 	try {
-		auto* h = new ct_matvec_plan();
+		std::unique_ptr<ct_matvec_plan> h(new ct_matvec_plan());
 		const char* env = getenv("CT_AUTOTUNE");
 		const bool tune = !(env && env[0] == '0');
 		h->p.reset(build_matvec_plan(a, w, l, r, mode, tune));
-		*plan = h;
+		*plan = h.release();
 		return 0;
 	}
 	catch (const HipError& e) {
@@ -203,6 +213,77 @@ int eigensystem_krylov_symmetric(const ct_long n, lanczos_linear_func_d afunc, c
 	}
 	catch (const HipError&) {
 		return CT_ERR_HIP;
+	}
+}
+
+int ct_matvec_batch_create(const int nprob, const struct block_sparse_tensor* const* a, const struct block_sparse_tensor* const* w,
+	const struct block_sparse_tensor* const* l, const struct block_sparse_tensor* const* r, ct_matvec_batch** batch)
+{
+	if (nprob < 1 || !a || !w || !l || !r || !batch) return CT_ERR_ARG;
+	try {
+		hipStream_t s = default_stream();
+		std::vector<BatchInput> in((size_t)nprob);
+		for (int p = 0; p < nprob; p++) {
+			if (!a[p] || !w[p] || !l[p] || !r[p]) return CT_ERR_ARG;
+			in[p].theta = meta_of(a[p]);
+			in[p].w = upload(w[p], s);
+			in[p].l = upload(l[p], s);
+			in[p].r = upload(r[p], s);
+		}
+		std::unique_ptr<ct_matvec_batch> h(new ct_matvec_batch());
+		h->b.reset(build_batch_plan(in, s));
+		*batch = h.release();
+		return 0;
+	}
+	catch (const HipError& e) {
+		return map_error(e);
+	}
+}
+
+int ct_matvec_batch_destroy(ct_matvec_batch* batch)
+{
+	delete batch;
+	return 0;
+}
+
+ct_long ct_matvec_batch_dim(const ct_matvec_batch* batch) { return batch->b->off.back(); }
+
+int ct_matvec_batch_offsets(const ct_matvec_batch* batch, ct_long* offsets)
+{
+	for (size_t i = 0; i < batch->b->off.size(); i++) offsets[i] = batch->b->off[i];
+	return 0;
+}
+
+double ct_matvec_batch_flops(const ct_matvec_batch* batch) { return batch->b->flops; }
+
+int ct_matvec_batch_form(const ct_matvec_batch* batch, int* fused, float* ms2)
+{
+	if (fused) *fused = batch->b->use_fused ? 1 : 0;
+	if (ms2) { ms2[0] = batch->b->form_ms[0]; ms2[1] = batch->b->form_ms[1]; }
+	return 0;
+}
+double ct_matvec_batch_bytes(const ct_matvec_batch* batch) { return batch->b->bytes; }
+void* ct_matvec_batch_stream(ct_matvec_batch* batch) { return (void*)batch->b->stream; }
+
+int ct_matvec_batch_apply(ct_matvec_batch* batch, const double* d_x, double* d_y)
+{
+	try {
+		batch_apply(*batch->b, d_x, d_y);
+		return 0;
+	}
+	catch (const HipError& e) {
+		return map_error(e);
+	}
+}
+
+int ct_lanczos_ground_state_batch(ct_matvec_batch* batch, const double* d_vstart, const int maxiter, double* lambda, int* status, double* d_u)
+{
+	try {
+		const char* env = getenv("CT_LANCZOS_GRAPH");
+		return lanczos_ground_state_batch(*batch->b, d_vstart, maxiter, lambda, status, d_u, !(env && env[0] == '0'));
+	}
+	catch (const HipError& e) {
+		return map_error(e);
 	}
 }
 

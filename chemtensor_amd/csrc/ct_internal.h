@@ -259,6 +259,10 @@ struct FusedPlan
 
 // Plan the fused form; returns false when a sector's per-column working set exceeds the LDS budget.
 bool build_fused_plan(FusedPlan& f, const Meta& theta, const Meta& w, const Meta& l, const Meta& r, hipStream_t s);
+// one problem of a batched fused plan: its metadata and the element offsets of its operands in the
+// concatenated theta / y, W, L and R buffers
+struct FusedProblem { const Meta* theta; const Meta* w; const Meta* l; const Meta* r; int64_t x_off, w_off, l_off, r_off; };
+bool build_fused_batch(FusedPlan& f, const std::vector<FusedProblem>& probs, hipStream_t s);
 void launch_fused_matvec(const FusedPlan& f, const double* x, const double* w, const double* l, const double* r, double* y, hipStream_t s);
 
 // Device workspace of the Lanczos iteration: maxiter Krylov vectors, w, the alpha / beta scalars,
@@ -270,9 +274,13 @@ struct LanczosWork
 	double* scal = nullptr;
 	double* part = nullptr;
 	double* z = nullptr;
+	double* vs = nullptr;              // staged start vector (input of the captured recurrence)
 	int64_t n = 0;
 	int cap = 0, zcap = 0;
+	hipGraphExec_t graph = nullptr;    // captured recurrence for 'graph_maxiter' steps
+	int graph_maxiter = 0;
 	void reserve(int64_t n, int maxiter, int numeig);
+	void drop_graph();
 	~LanczosWork();
 };
 
@@ -282,8 +290,11 @@ using DevMatvec = std::function<void(const double* x, double* y)>;
 // Lanczos eigensystem (src/util/krylov.c:24-89 and :172-251): lambda[0:numeig] ascending,
 // u (device, n x numeig row-major) the Ritz vectors. Returns 0, -1 (fewer Lanczos iterations than
 // numeig) or -2 (tridiagonal eigensolver failure).
+// With use_graph, the recurrence (maxiter matvecs and the vector updates) is captured once into a
+// HIP graph held by lw and replayed on later calls with the same maxiter; matvec must then only
+// enqueue kernels on s (no host synchronisation or copies).
 int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart,
-	int maxiter, int numeig, double* lambda, double* u);
+	int maxiter, int numeig, double* lambda, double* u, bool use_graph = false);
 
 struct MatvecPlan
 {
@@ -311,6 +322,112 @@ struct MatvecPlan
 	hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };
 	LanczosWork lz;                             // Lanczos workspace reused across solves
 	~MatvecPlan();
+};
+
+// grouped three-stage tables of one or several problems (operand offsets relative to the stage bases)
+struct GroupedTables
+{
+	GemmBatch s1, s2, s3;
+	int64_t t1_elems = 0, t2_elems = 0;
+	double stage_bytes[3] = { 0, 0, 0 };
+};
+void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const Meta& L, const Meta& R,
+	int64_t x_off, int64_t w_off, int64_t l_off, int64_t r_off);
+
+// contract checks of apply_local_hamiltonian (degrees, unit outer bonds, matching axes); throws HipError
+void validate_matvec_operands(const Meta& theta, const Meta& w, const Meta& l, const Meta& r);
+void autotune_grouped(DevGemm* stage, const GemmBases* bases, int64_t nx, hipStream_t s, float (*tuned_ms)[GEMM_NUM_FORMS]);
+
+// FP64 flops of the reference's unfused operation sequence: sum of 2 m n k over its GEMM calls
+double reference_flops(const Meta& th, const Meta& w, const Meta& l, const Meta& r);
+
+//________________________________________________________________________________________________________________________
+//
+// batch of independent H_eff problems: one fused launch applies all of them to the concatenated
+// vector; the batched Lanczos keeps one recurrence per problem (segments of the concatenated vectors)
+
+struct LzSeg  { int64_t begin, end; int prob, pad; };   // element range of one reduction block
+struct LzProb { int b0, b1; int64_t n; };               // reduction blocks and length of a problem
+
+struct BatchLanczosWork
+{
+	double* v = nullptr;      // maxiter x ntot
+	double* w = nullptr;
+	double* vs = nullptr;
+	double* scal = nullptr;   // per problem: alpha[maxiter], beta[maxiter], numiter, stop
+	double* part = nullptr;   // 2 x nblocks
+	double* z = nullptr;      // per problem: maxiter Ritz coefficients
+	int* nit = nullptr;       // per problem: Lanczos iterations used by the Ritz vector
+	int cap = 0;
+	hipGraphExec_t graph = nullptr;
+	int graph_maxiter = 0;
+	void drop_graph();
+	~BatchLanczosWork();
+};
+
+struct BatchPlan
+{
+	hipStream_t stream = nullptr;
+	int nprob = 0;
+	std::vector<int64_t> off;                   // nprob + 1 offsets of each problem in the concatenated vectors
+	double* w = nullptr;                        // concatenated operands
+	double* l = nullptr;
+	double* r = nullptr;
+	FusedPlan fused;
+	bool fused_ok = false;
+	// grouped three-stage form over the concatenated operands
+	DevGemm stage[3];
+	GemmBases stage_bases[3] = {};
+	double* t1 = nullptr;
+	double* t2 = nullptr;
+	bool use_fused = false;                     // the faster of the two forms, timed at plan creation
+	float form_ms[2] = { 0, 0 };                // ms per batched matvec: [0] grouped, [1] fused
+	LzSeg* segs = nullptr;
+	LzProb* probs = nullptr;
+	int nblocks = 0;
+	double flops = 0, bytes = 0;
+	BatchLanczosWork lz;
+	~BatchPlan();
+};
+
+struct BatchInput { Meta theta; DevTensor w, l, r; };
+// tune: time the grouped launch forms and the fused form and keep the fastest; otherwise the
+// grouped form with heuristic launch forms (no timing launches, for plans rebuilt often)
+BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tune = true);
+void batch_apply(BatchPlan& b, const double* x, double* y);
+// per-problem ground states: lambda[nprob], status[nprob] (0, -1 breakdown, -2 eigensolver), u concatenated
+int lanczos_ground_state_batch(BatchPlan& b, const double* vstart, int maxiter, double* lambda, int* status, double* u, bool use_graph);
+
+// Two-site DMRG sweep (src/algorithm/dmrg.c:262-399) as a resumable state: after init() the
+// current local eigenproblem is (h2[bond], lb[bond], rb[bond + 1]) with start vector 'theta';
+// advance() consumes its ground state (energy, eigenvector) and moves to the next bond. With
+// tol_energy > 0 the run also ends after a sweep whose energy changed by less than tol_energy.
+struct DmrgState
+{
+	hipStream_t st = nullptr;
+	int L = 0, d = 0, maxiter = 0, num_sweeps = 0;
+	double tol_split = 0, tol_energy = 0;
+	ct_long max_vdim = 0;
+	std::vector<qnumber> qsite;
+	std::vector<DevTensor> A, W, lb, rb, h2;
+	std::vector<double> en_sweeps, entropy;
+	int sweep = 0, dir = 0, bond = 0;
+	bool done = false;
+	DevTensor theta;
+	// split of the current bond: advance_begin (device, downloads the matrix), advance_host (host
+	// block SVD only, no HIP calls: callers may run it for several states in parallel threads),
+	// advance_end (device); advance() runs the three in sequence
+	Meta split_meta;
+	HostPacked split_in, split_m0, split_m1;
+	trunc_info split_info = {};
+	int split_ret = 0;
+	int init(const mpo* H, const mps* psi, int num_sweeps, int maxiter, double tol_split, ct_long max_vdim, double tol_energy);
+	void advance_begin(const DevTensor& a_opt);
+	void advance_host();
+	int advance_end(double en);
+	int advance(double en, DevTensor& a_opt);
+	void download_mps(mps* psi) const;
+	size_t device_bytes() const;
 };
 
 MatvecPlan* build_matvec_plan(const block_sparse_tensor* a, const block_sparse_tensor* w, const block_sparse_tensor* l, const block_sparse_tensor* r, int mode, bool autotune);

@@ -135,29 +135,6 @@ static int local_orthonormalize_rq(DevTensor& a, DevTensor& a_prev, hipStream_t 
 }
 
 // mps.c:1119-1160
-static int split_svd(const DevTensor& a, ct_long d, const std::vector<qnumber>& qsite, double tol, ct_long max_vdim, int distr,
-	DevTensor& a0, DevTensor& a1, trunc_info& info, hipStream_t st)
-{
-	const ct_long dd[2] = { d, d };
-	const int ad[2] = { TENSOR_AXIS_OUT, TENSOR_AXIS_OUT };
-	const qnumber* qs[2] = { qsite.data(), qsite.data() };
-	DevTensor a2 = dev_split_axis(a, 1, dd, ad, qs, st);
-	DevTensor tmp = dev_flatten_axes(a2, 0, TENSOR_AXIS_OUT, st);
-	DevTensor am = dev_flatten_axes(tmp, 1, TENSOR_AXIS_IN, st);
-	HostPacked hm = download_packed(am, st), m0, m1;
-	const int ret = host_split_svd(hm, tol, max_vdim, false, distr, m0, m1, info);
-	if (ret < 0) return ret;
-	DevTensor d0 = upload_packed(m0, st), d1 = upload_packed(m1, st);
-	const Meta& m = a2.meta;
-	const ct_long dl0[2] = { m.dim_logical[0], m.dim_logical[1] }, dl1[2] = { m.dim_logical[2], m.dim_logical[3] };
-	const int ad0[2] = { m.axis_dir[0], m.axis_dir[1] }, ad1[2] = { m.axis_dir[2], m.axis_dir[3] };
-	const qnumber* qn0[2] = { m.qnums_logical[0].data(), m.qnums_logical[1].data() };
-	const qnumber* qn1[2] = { m.qnums_logical[2].data(), m.qnums_logical[3].data() };
-	a0 = dev_split_axis(d0, 0, dl0, ad0, qn0, st);
-	a1 = dev_split_axis(d1, 1, dl1, ad1, qn1, st);
-	return 0;
-}
-
 // dmrg.c:86-147 with the Lanczos iteration on the device
 static int minimize_local_energy(const DevTensor& w, const DevTensor& l, const DevTensor& r, const DevTensor& a_start, int maxiter,
 	double* en, DevTensor& a_opt, hipStream_t st)
@@ -182,14 +159,19 @@ static DevTensor dummy_site(const Meta& a, int ax, hipStream_t st)
 	return upload_packed(h, st);
 }
 
-int dmrg_twosite_device(const mpo* H, int num_sweeps, int maxiter, double tol_split, ct_long max_vdim, mps* psi, double* en_sweeps, double* entropy)
+int DmrgState::init(const mpo* H, const mps* psi, int num_sweeps_, int maxiter_, double tol_split_, ct_long max_vdim_, double tol_energy_)
 {
-	hipStream_t st = default_stream();
-	const int L = H->nsites;
-	if (L < 2 || psi->nsites != L) return CT_ERR_ARG;
-	std::vector<DevTensor> A, W;
+	L = H->nsites;
+	if (L < 2 || psi->nsites != L || num_sweeps_ < 1 || maxiter_ < 1) return CT_ERR_ARG;
+	num_sweeps = num_sweeps_;
+	maxiter = maxiter_;
+	tol_split = tol_split_;
+	max_vdim = max_vdim_;
+	tol_energy = tol_energy_;
+	d = (int)psi->d;
+	qsite.assign(psi->qsite, psi->qsite + psi->d);
+	A.clear(); W.clear();
 	for (int i = 0; i < L; i++) { A.push_back(upload(&psi->a[i], st)); W.push_back(upload(&H->a[i], st)); }
-	const std::vector<qnumber> qsite(psi->qsite, psi->qsite + psi->d);
 
 	// mps_orthonormalize_qr(psi, MPS_ORTHONORMAL_RIGHT) (mps.c:609-760)
 	for (int i = L - 1; i > 0; i--) if (local_orthonormalize_rq(A[i], A[i - 1], st) != 0) return -1;
@@ -205,44 +187,130 @@ int dmrg_twosite_device(const mpo* H, int num_sweeps, int maxiter, double tol_sp
 	}
 	if (nrm == 0) printf("Warning: in 'dmrg_twosite': initial MPS has norm zero (possibly due to mismatching quantum numbers)\n");
 
-	std::vector<DevTensor> lb(L), rb(L);
+	lb.clear();
+	rb.clear();
+	lb.resize(L);
+	rb.resize(L);
 	rb[L - 1] = dummy_right(A[L - 1].meta, A[L - 1].meta, W[L - 1].meta, st);
 	for (int i = L - 1; i > 0; i--) rb[i - 1] = step_right(A[i], A[i], W[i], rb[i], st);
 	lb[0] = dummy_left(A[0].meta, A[0].meta, W[0].meta, st);
-	std::vector<DevTensor> h2;
+	h2.clear();
 	for (int i = 0; i < L - 1; i++) h2.push_back(merge_mpo(W[i], W[i + 1], st));
+	en_sweeps.clear();
+	entropy.assign(L - 1, 0.0);
+	sweep = 0;
+	// dmrg.c:296-357: left-to-right over bonds 0 .. L-3, then right-to-left over L-2 .. 0
+	dir = (L > 2) ? 0 : 1;
+	bond = (L > 2) ? 0 : L - 2;
+	done = false;
+	theta = merge_mps(A[bond], A[bond + 1], st);
+	CTG_HIP(hipStreamSynchronize(st));
+	return 0;
+}
 
-	for (int n = 0; n < num_sweeps; n++) {
-		double en = 0;
-		for (int i = 0; i < L - 2; i++) {
-			DevTensor ac = merge_mps(A[i], A[i + 1], st), ao;
-			int ret = minimize_local_energy(h2[i], lb[i], rb[i + 1], ac, maxiter, &en, ao, st);
-			if (ret < 0) return ret;
-			trunc_info info;
-			ret = split_svd(ao, psi->d, qsite, tol_split, max_vdim, SVD_DISTR_RIGHT, A[i], A[i + 1], info, st);
-			if (ret < 0) return ret;
-			lb[i + 1] = step_left(A[i], A[i], W[i], lb[i], st);
-		}
-		for (int i = L - 2; i >= 0; i--) {
-			DevTensor ac = merge_mps(A[i], A[i + 1], st), ao;
-			int ret = minimize_local_energy(h2[i], lb[i], rb[i + 1], ac, maxiter, &en, ao, st);
-			if (ret < 0) return ret;
-			trunc_info info;
-			ret = split_svd(ao, psi->d, qsite, tol_split, max_vdim, SVD_DISTR_LEFT, A[i], A[i + 1], info, st);
-			if (ret < 0) return ret;
-			entropy[i] = info.entropy;
-			rb[i] = step_right(A[i + 1], A[i + 1], W[i + 1], rb[i + 1], st);
-		}
-		{
+// split of the optimised two-site tensor (mps_split_tensor_svd, mps.c:1119-1160) in three parts:
+// device reshape and download, host block SVD with truncation, upload and device reshape
+void DmrgState::advance_begin(const DevTensor& a_opt)
+{
+	const ct_long dd[2] = { d, d };
+	const int ad[2] = { TENSOR_AXIS_OUT, TENSOR_AXIS_OUT };
+	const qnumber* qs[2] = { qsite.data(), qsite.data() };
+	DevTensor a2 = dev_split_axis(a_opt, 1, dd, ad, qs, st);
+	DevTensor tmp = dev_flatten_axes(a2, 0, TENSOR_AXIS_OUT, st);
+	DevTensor am = dev_flatten_axes(tmp, 1, TENSOR_AXIS_IN, st);
+	split_meta = a2.meta;
+	split_in = download_packed(am, st);
+}
+
+void DmrgState::advance_host()
+{
+	split_ret = host_split_svd(split_in, tol_split, max_vdim, false, dir == 0 ? SVD_DISTR_RIGHT : SVD_DISTR_LEFT, split_m0, split_m1, split_info);
+	split_in = HostPacked();
+}
+
+int DmrgState::advance_end(double en)
+{
+	if (split_ret < 0) return split_ret;
+	{
+		DevTensor d0 = upload_packed(split_m0, st), d1 = upload_packed(split_m1, st);
+		const Meta& m = split_meta;
+		const ct_long dl0[2] = { m.dim_logical[0], m.dim_logical[1] }, dl1[2] = { m.dim_logical[2], m.dim_logical[3] };
+		const int ad0[2] = { m.axis_dir[0], m.axis_dir[1] }, ad1[2] = { m.axis_dir[2], m.axis_dir[3] };
+		const qnumber* qn0[2] = { m.qnums_logical[0].data(), m.qnums_logical[1].data() };
+		const qnumber* qn1[2] = { m.qnums_logical[2].data(), m.qnums_logical[3].data() };
+		A[bond] = dev_split_axis(d0, 0, dl0, ad0, qn0, st);
+		A[bond + 1] = dev_split_axis(d1, 1, dl1, ad1, qn1, st);
+		split_m0 = HostPacked();
+		split_m1 = HostPacked();
+	}
+	const int i = bond;
+	if (dir == 0) {
+		lb[i + 1] = step_left(A[i], A[i], W[i], lb[i], st);
+		if (i + 1 < L - 2) bond = i + 1;
+		else { dir = 1; bond = L - 2; }
+	}
+	else {
+		entropy[i] = split_info.entropy;
+		rb[i] = step_right(A[i + 1], A[i + 1], W[i + 1], rb[i + 1], st);
+		if (i > 0) bond = i - 1;
+		else {
 			DevTensor t = dummy_site(A[0].meta, 0, st);
 			if (local_orthonormalize_rq(A[0], t, st) != 0) return -1;
+			en_sweeps.push_back(en);
+			sweep++;
+			const int ns = (int)en_sweeps.size();
+			const bool converged = tol_energy > 0 && ns >= 2 && std::fabs(en_sweeps[ns - 1] - en_sweeps[ns - 2]) < tol_energy;
+			if (sweep >= num_sweeps || converged) done = true;
+			dir = (L > 2) ? 0 : 1;
+			bond = (L > 2) ? 0 : L - 2;
 		}
-		en_sweeps[n] = en;
 	}
+	if (!done) theta = merge_mps(A[bond], A[bond + 1], st);
+	else theta = DevTensor();
+	CTG_HIP(hipStreamSynchronize(st));
+	return 0;
+}
+
+int DmrgState::advance(double en, DevTensor& a_opt)
+{
+	advance_begin(a_opt);
+	advance_host();
+	return advance_end(en);
+}
+
+void DmrgState::download_mps(mps* psi) const
+{
 	for (int i = 0; i < L; i++) {
 		delete_block_sparse_tensor(&psi->a[i]);
 		download(A[i], &psi->a[i], st);
 	}
+}
+
+size_t DmrgState::device_bytes() const
+{
+	size_t n = 0;
+	for (const auto* v : { &A, &W, &lb, &rb, &h2 })
+		for (const DevTensor& t : *v) n += (size_t)t.meta.nelem;
+	return n * sizeof(double);
+}
+
+int dmrg_twosite_device(const mpo* H, int num_sweeps, int maxiter, double tol_split, ct_long max_vdim, mps* psi, double* en_sweeps, double* entropy)
+{
+	DmrgState S;
+	S.st = default_stream();
+	int ret = S.init(H, psi, num_sweeps, maxiter, tol_split, max_vdim, 0.0);
+	if (ret != 0) return ret;
+	while (!S.done) {
+		DevTensor ao;
+		double en = 0;
+		ret = minimize_local_energy(S.h2[S.bond], S.lb[S.bond], S.rb[S.bond + 1], S.theta, maxiter, &en, ao, S.st);
+		if (ret < 0) return ret;
+		ret = S.advance(en, ao);
+		if (ret < 0) return ret;
+	}
+	for (int n = 0; n < num_sweeps; n++) en_sweeps[n] = S.en_sweeps[n];
+	for (int i = 0; i < S.L - 1; i++) entropy[i] = S.entropy[i];
+	S.download_mps(psi);
 	return 0;
 }
 

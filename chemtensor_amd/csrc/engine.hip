@@ -454,7 +454,7 @@ static bool same_axis(const Meta& a, int ia, const Meta& b, int ib)
 }
 
 // the reference's GEMM count of one matvec (every allocated block pair, zero blocks included)
-static double reference_flops(const Meta& th, const Meta& w, const Meta& l, const Meta& r)
+double reference_flops(const Meta& th, const Meta& w, const Meta& l, const Meta& r)
 {
 	double f = 0;
 	// s1 = theta . R : [a p n c o]
@@ -473,12 +473,15 @@ static double reference_flops(const Meta& th, const Meta& w, const Meta& l, cons
 	return f;
 }
 
-static void build_grouped(MatvecPlan& p)
+// Append the three grouped stages of one problem to g. Element offsets: theta / y at x_off, W at
+// w_off, L at l_off, R at r_off, its T1 / T2 blocks from the current ends of g's intermediates.
+void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const Meta& L, const Meta& R,
+	int64_t x_off, int64_t w_off, int64_t l_off, int64_t r_off)
 {
-	const Meta& th = p.theta_meta;
-	const Meta& W = p.w.meta;
-	const Meta& L = p.l.meta;
-	const Meta& R = p.r.meta;
+	GemmBatch& s1 = g.s1;
+	GemmBatch& s2 = g.s2;
+	GemmBatch& s3 = g.s3;
+	const int64_t t1_base = g.t1_elems, t2_base = g.t2_elems;
 	const qnumber qo = L.qnums_blocks[0][0];
 	const ct_long nA = th.dim_blocks[0], nP = th.dim_blocks[1], nB = th.dim_blocks[2];
 	const ct_long nN = R.dim_blocks[1], nC = R.dim_blocks[2];
@@ -489,7 +492,6 @@ static void build_grouped(MatvecPlan& p)
 	// stage 1: T1(A,P,N,C)[a p n c] = theta(A,P,B) R(B,N,C,O)
 	std::vector<ct_long> t1_off((size_t)(nA * nP * nN * nC), -1);
 	auto t1_key = [&](ct_long A, ct_long P, ct_long N, ct_long C) { return ((A * nP + P) * nN + N) * nC + C; };
-	GemmBatch s1;
 	ct_long t1_elems = 0;
 	{
 		std::vector<std::tuple<ct_long, ct_long, ct_long, ct_long, ct_long, ct_long>> blocks;  // A P N C Bidx off
@@ -512,14 +514,13 @@ static void build_grouped(MatvecPlan& p)
 			const ct_long dA = bc(th, 0, A), dP = bc(th, 1, P), dB = bc(th, 2, B), dN = bc(R, 1, N), dC = bc(R, 2, C);
 			const ct_long ith[3] = { A, P, B };
 			const ct_long irr[4] = { B, N, C, 0 };
-			s1.add_task(off_ptr_mut(off), (int)(dA * dP), (int)(dN * dC), (int)(dN * dC), (int)(dN * dC), 0, false);
-			s1.add_term(off_ptr(th.offset[th.flat_index(ith)]), off_ptr(R.offset[R.flat_index(irr)]), (int)dB, (int)dB, 1, (int)(dN * dC), 1, 0);
+			s1.add_task(off_ptr_mut(t1_base + off), (int)(dA * dP), (int)(dN * dC), (int)(dN * dC), (int)(dN * dC), 0, false);
+			s1.add_term(off_ptr(x_off + th.offset[th.flat_index(ith)]), off_ptr(r_off + R.offset[R.flat_index(irr)]), (int)dB, (int)dB, 1, (int)(dN * dC), 1, 0);
 		}
 	}
 	// stage 2: T2(A,M,Q,C)[a m q c] = sum_P W(M,Q,P,N) T1(A,P,N,C)
 	std::vector<ct_long> t2_off((size_t)(nA * nM * nQ * nC), -1);
 	auto t2_key = [&](ct_long A, ct_long M, ct_long Q, ct_long C) { return ((A * nM + M) * nQ + Q) * nC + C; };
-	GemmBatch s2;
 	ct_long t2_elems = 0;
 	for (ct_long A = 0; A < nA; A++) for (ct_long C = 0; C < nC; C++) for (ct_long M = 0; M < nM; M++) for (ct_long Q = 0; Q < nQ; Q++) {
 		if (qb(W, 0, M) + qb(W, 1, Q) + qb(th, 0, A) - qb(R, 2, C) - qo != 0) continue;
@@ -537,23 +538,22 @@ static void build_grouped(MatvecPlan& p)
 		const ct_long off = t2_elems;
 		t2_off[t2_key(A, M, Q, C)] = off;
 		t2_elems += dA * dM * dQ * dC;
-		s2.add_task(off_ptr_mut(off), (int)(dM * dQ), (int)(dA * dC), (int)dC, (int)dC, dM * dQ * dC, false);
+		s2.add_task(off_ptr_mut(t2_base + off), (int)(dM * dQ), (int)(dA * dC), (int)dC, (int)dC, dM * dQ * dC, false);
 		for (auto& pn : tl) {
 			const ct_long P = pn.first, N = pn.second;
 			const ct_long dP = bc(W, 2, P), dN = bc(W, 3, N);
 			const ct_long iw[4] = { M, Q, P, N };
-			s2.add_term(off_ptr(W.offset[W.flat_index(iw)]), off_ptr(t1_off[t1_key(A, P, N, C)]), (int)(dP * dN), (int)(dP * dN), 1, (int)dC, 1, dP * dN * dC);
+			s2.add_term(off_ptr(w_off + W.offset[W.flat_index(iw)]), off_ptr(t1_base + t1_off[t1_key(A, P, N, C)]), (int)(dP * dN), (int)(dP * dN), 1, (int)dC, 1, dP * dN * dC);
 		}
 	}
 	// stage 3: y(X,Q,C)[x q c] = sum_(A,M) L(O,A,M,X)^T T2(A,M,Q,C)
-	GemmBatch s3;
 	ct_long iy[3];
 	for (ct_long k = 0; k < th.nblocks; k++) {
 		if (!th.present(k)) continue;
 		th.block_index(k, iy);
 		const ct_long X = iy[0], Q = iy[1], C = iy[2];
 		const ct_long dX = bc(th, 0, X), dQ = bc(th, 1, Q), dC = bc(th, 2, C);
-		s3.add_task(off_ptr_mut(th.offset[k]), (int)dX, (int)(dQ * dC), (int)(dQ * dC), (int)(dQ * dC), 0, false);
+		s3.add_task(off_ptr_mut(x_off + th.offset[k]), (int)dX, (int)(dQ * dC), (int)(dQ * dC), (int)(dQ * dC), 0, false);
 		for (ct_long A = 0; A < nA; A++) {
 			const ct_long M = L.find_qnum(2, qo - qb(L, 1, A) + qb(L, 3, X));
 			if (M < 0) continue;
@@ -562,30 +562,39 @@ static void build_grouped(MatvecPlan& p)
 			const ct_long o2 = t2_off[t2_key(A, M, Q, C)];
 			if (o2 < 0) continue;
 			const ct_long dA = bc(L, 1, A), dM = bc(L, 2, M);
-			s3.add_term(off_ptr(L.offset[L.flat_index(il)]), off_ptr(o2), (int)(dA * dM), 1, (int)dX, (int)(dQ * dC), 1, 0);
+			s3.add_term(off_ptr(l_off + L.offset[L.flat_index(il)]), off_ptr(t2_base + o2), (int)(dA * dM), 1, (int)dX, (int)(dQ * dC), 1, 0);
 		}
 	}
+	g.t1_elems += t1_elems;
+	g.t2_elems += t2_elems;
+	g.stage_bytes[0] += 8.0 * (double)(th.nelem + R.nelem + t1_elems);
+	g.stage_bytes[1] += 8.0 * (double)(t1_elems + W.nelem + t2_elems);
+	g.stage_bytes[2] += 8.0 * (double)(t2_elems + L.nelem + th.nelem);
+}
+
+static void build_grouped(MatvecPlan& p)
+{
+	GroupedTables g;
+	plan_grouped_tables(g, p.theta_meta, p.w.meta, p.l.meta, p.r.meta, 0, 0, 0, 0);
 	Meta m1, m2;
-	m1.nelem = t1_elems;
-	m2.nelem = t2_elems;
+	m1.nelem = g.t1_elems;
+	m2.nelem = g.t2_elems;
 	p.interm.emplace_back(std::move(m1));
 	p.interm.emplace_back(std::move(m2));
-	p.stage[0].upload(s1, p.stream);
-	p.stage[1].upload(s2, p.stream);
-	p.stage[2].upload(s3, p.stream);
-	p.stage_flops[0] = s1.flops;
-	p.stage_flops[1] = s2.flops;
-	p.stage_flops[2] = s3.flops;
-	p.stage_bytes[0] = 8.0 * (double)(th.nelem + R.nelem + t1_elems);
-	p.stage_bytes[1] = 8.0 * (double)(t1_elems + W.nelem + t2_elems);
-	p.stage_bytes[2] = 8.0 * (double)(t2_elems + L.nelem + th.nelem);
+	p.stage[0].upload(g.s1, p.stream);
+	p.stage[1].upload(g.s2, p.stream);
+	p.stage[2].upload(g.s3, p.stream);
+	for (int i = 0; i < 3; i++) p.stage_bytes[i] = g.stage_bytes[i];
+	p.stage_flops[0] = g.s1.flops;
+	p.stage_flops[1] = g.s2.flops;
+	p.stage_flops[2] = g.s3.flops;
 	p.stage_bases[0] = GemmBases{ nullptr, p.r.d, p.interm[0].d };
 	p.stage_bases[1] = GemmBases{ p.w.d, p.interm[0].d, p.interm[1].d };
 	p.stage_bases[2] = GemmBases{ p.l.d, p.interm[1].d, nullptr };
-	p.nterms = s1.terms.size() + s2.terms.size() + s3.terms.size();
-	p.ntasks = s1.tasks.size() + s2.tasks.size() + s3.tasks.size();
+	p.nterms = g.s1.terms.size() + g.s2.terms.size() + g.s3.terms.size();
+	p.ntasks = g.s1.tasks.size() + g.s2.tasks.size() + g.s3.tasks.size();
 	p.nitems = p.stage[0].nitems + p.stage[1].nitems + p.stage[2].nitems;
-	p.interm_elems = (double)(t1_elems + t2_elems);
+	p.interm_elems = (double)(g.t1_elems + g.t2_elems);
 }
 
 static void build_plain(MatvecPlan& p)
@@ -682,19 +691,19 @@ static void build_plain(MatvecPlan& p)
 
 // Plan-time autotuner: every launch form of every stage runs on the plan's own operands (one
 // warm-up and three timed launches, HIP events on the plan stream); the fastest form is kept.
-static void autotune_stages(MatvecPlan& p)
+// time every launch form of each grouped stage on the plan's own descriptors (x = 1, y scratch)
+// and keep the fastest
+void autotune_grouped(DevGemm* stage, const GemmBases* bases, int64_t nx, hipStream_t s, float (*tuned_ms)[GEMM_NUM_FORMS])
 {
-	hipStream_t s = p.stream;
-	const ct_long n = p.theta_meta.nelem;
-	double* x = (double*)dev_alloc((size_t)std::max<ct_long>(n, 1) * sizeof(double));
-	double* y = (double*)dev_alloc((size_t)std::max<ct_long>(n, 1) * sizeof(double));
-	launch_fill(x, 1.0, n, s);
+	double* x = (double*)dev_alloc((size_t)std::max<int64_t>(nx, 1) * sizeof(double));
+	double* y = (double*)dev_alloc((size_t)std::max<int64_t>(nx, 1) * sizeof(double));
+	launch_fill(x, 1.0, nx, s);
 	hipEvent_t e0, e1;
 	CTG_HIP(hipEventCreate(&e0));
 	CTG_HIP(hipEventCreate(&e1));
 	for (int st = 0; st < 3; st++) {
-		DevGemm& g = p.stage[st];
-		GemmBases b = p.stage_bases[st];
+		DevGemm& g = stage[st];
+		GemmBases b = bases[st];
 		if (st == 0) b.A = x;
 		if (st == 2) b.C = y;
 		float best = 1e30f;
@@ -708,7 +717,7 @@ static void autotune_stages(MatvecPlan& p)
 			CTG_HIP(hipEventSynchronize(e1));
 			float ms = 0;
 			CTG_HIP(hipEventElapsedTime(&ms, e0, e1));
-			p.tuned_ms[st][f] = ms / 3;
+			if (tuned_ms) tuned_ms[st][f] = ms / 3;
 			if (ms < best) { best = ms; best_form = f; }
 		}
 		g.form = best_form;
@@ -720,16 +729,26 @@ static void autotune_stages(MatvecPlan& p)
 	dev_free(y);
 }
 
+static void autotune_stages(MatvecPlan& p)
+{
+	autotune_grouped(p.stage, p.stage_bases, p.theta_meta.nelem, p.stream, p.tuned_ms);
+}
+
+void validate_matvec_operands(const Meta& theta, const Meta& w, const Meta& l, const Meta& r)
+{
+	if (theta.ndim != 3 || w.ndim != 4 || l.ndim != 4 || r.ndim != 4)
+		throw HipError{ hipErrorInvalidValue, "apply_local_hamiltonian expects degrees 3, 4, 4, 4" };
+	if (l.dim_logical[0] != 1 || r.dim_logical[3] != 1 || l.qnums_logical[0] != r.qnums_logical[3])
+		throw HipError{ hipErrorNotSupported, "outer virtual bonds of the environment blocks must have dimension 1 and equal quantum numbers" };
+	if (!same_axis(theta, 2, r, 0) || !same_axis(theta, 1, w, 2) || !same_axis(theta, 0, l, 1) ||
+		!same_axis(l, 2, w, 0) || !same_axis(r, 1, w, 3) ||
+		!same_axis(theta, 0, l, 3) || !same_axis(theta, 1, w, 1) || !same_axis(theta, 2, r, 2))
+		throw HipError{ hipErrorInvalidValue, "incompatible dimensions or quantum numbers of apply_local_hamiltonian operands" };
+}
+
 MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& l, DevTensor&& r, int mode, hipStream_t s, bool autotune)
 {
-	if (theta.ndim != 3 || w.meta.ndim != 4 || l.meta.ndim != 4 || r.meta.ndim != 4)
-		throw HipError{ hipErrorInvalidValue, "apply_local_hamiltonian expects degrees 3, 4, 4, 4" };
-	if (l.meta.dim_logical[0] != 1 || r.meta.dim_logical[3] != 1 || l.meta.qnums_logical[0] != r.meta.qnums_logical[3])
-		throw HipError{ hipErrorNotSupported, "outer virtual bonds of the environment blocks must have dimension 1 and equal quantum numbers" };
-	if (!same_axis(theta, 2, r.meta, 0) || !same_axis(theta, 1, w.meta, 2) || !same_axis(theta, 0, l.meta, 1) ||
-		!same_axis(l.meta, 2, w.meta, 0) || !same_axis(r.meta, 1, w.meta, 3) ||
-		!same_axis(theta, 0, l.meta, 3) || !same_axis(theta, 1, w.meta, 1) || !same_axis(theta, 2, r.meta, 2))
-		throw HipError{ hipErrorInvalidValue, "incompatible dimensions or quantum numbers of apply_local_hamiltonian operands" };
+	validate_matvec_operands(theta, w.meta, l.meta, r.meta);
 	MatvecPlan* p = new MatvecPlan();
 	p->mode = mode;
 	p->stream = s;

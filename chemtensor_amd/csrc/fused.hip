@@ -16,6 +16,9 @@
 // LDS offsets in the tables are per column and are multiplied by the unit's k at run time.
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
 #include <numeric>
 #include <tuple>
 
@@ -26,8 +29,10 @@ namespace ctg {
 constexpr int FZ_THREADS = 1024;
 constexpr int FZ_KMAX = 4;      // columns per unit
 constexpr int FZ_JC = 4;        // T1 columns (N n) per phase-1 item
+constexpr int FZ_LB = 8;        // global loads in flight per thread in the contraction loops
 constexpr int FZ_NXCD = 8;      // MI355X: 8 XCDs; workgroup i runs on XCD i mod 8
-constexpr size_t FZ_LDS_BUDGET = 64 * 1024;
+constexpr size_t FZ_LDS_BUDGET = 64 * 1024;        // per workgroup: two units fit the 160 KB LDS of a CU
+constexpr size_t FZ_LDS_MAX = 160 * 1024;          // a sector whose column needs more runs one column per unit up to this
 
 struct FzArgs
 {
@@ -79,15 +84,23 @@ __global__ __launch_bounds__(FZ_THREADS) void k_fused_matvec(FzArgs a)
 			#pragma unroll
 			for (int cc = 0; cc < FZ_KMAX; cc++) acc[j][cc] = 0;
 		const double* th = a.x + t.th_off + (int64_t)ap * t.dB;
-		for (int b = 0; b < t.dB; b++) {
-			const double v = th[b];
-			const double* rp = lds + (t.rb + b * t.nb + jc) * k;
+		// FZ_LB theta loads in flight per thread, then their FMAs
+		for (int b0 = 0; b0 < t.dB; b0 += FZ_LB) {
+			double v[FZ_LB];
 			#pragma unroll
-			for (int j = 0; j < FZ_JC; j++) {
-				if (j < nj) {
+			for (int i = 0; i < FZ_LB; i++) v[i] = (b0 + i < t.dB) ? th[b0 + i] : 0.0;
+			#pragma unroll
+			for (int i = 0; i < FZ_LB; i++) {
+				if (b0 + i < t.dB) {
+					const double* rp = lds + (t.rb + (b0 + i) * t.nb + jc) * k;
 					#pragma unroll
-					for (int cc = 0; cc < FZ_KMAX; cc++)
-						if (cc < k) acc[j][cc] = fma(v, rp[j * k + cc], acc[j][cc]);
+					for (int j = 0; j < FZ_JC; j++) {
+						if (j < nj) {
+							#pragma unroll
+							for (int cc = 0; cc < FZ_KMAX; cc++)
+								if (cc < k) acc[j][cc] = fma(v[i], rp[j * k + cc], acc[j][cc]);
+						}
+					}
 				}
 			}
 		}
@@ -111,16 +124,26 @@ __global__ __launch_bounds__(FZ_THREADS) void k_fused_matvec(FzArgs a)
 		double acc[FZ_KMAX];
 		#pragma unroll
 		for (int cc = 0; cc < FZ_KMAX; cc++) acc[cc] = 0;
+		FzTerm2 nxt;
+		if (t.u0 < t.u1) nxt = a.u2[t.u0];
 		for (int q = t.u0; q < t.u1; q++) {
-			const FzTerm2 tm = a.u2[q];
+			const FzTerm2 tm = nxt;
+			if (q + 1 < t.u1) nxt = a.u2[q + 1];
 			const double* wp = a.w + tm.w_off + (int64_t)mq * tm.dP * tm.dN;
-			for (int p = 0; p < tm.dP; p++) {
-				const double* tp = lds + (tm.t1 + (ai * tm.dP + p) * tm.nb + tm.j0) * k;
-				for (int n = 0; n < tm.dN; n++) {
-					const double wv = wp[p * tm.dN + n];
-					#pragma unroll
-					for (int cc = 0; cc < FZ_KMAX; cc++)
-						if (cc < k) acc[cc] = fma(wv, tp[n * k + cc], acc[cc]);
+			const int pn = tm.dP * tm.dN;
+			for (int e0 = 0; e0 < pn; e0 += FZ_LB) {
+				double wv[FZ_LB];
+				#pragma unroll
+				for (int i = 0; i < FZ_LB; i++) wv[i] = (e0 + i < pn) ? wp[e0 + i] : 0.0;
+				#pragma unroll
+				for (int i = 0; i < FZ_LB; i++) {
+					if (e0 + i < pn) {
+						const int p = (e0 + i) / tm.dN, n = (e0 + i) - p * tm.dN;
+						const double* tp = lds + (tm.t1 + (ai * tm.dP + p) * tm.nb + tm.j0 + n) * k;
+						#pragma unroll
+						for (int cc = 0; cc < FZ_KMAX; cc++)
+							if (cc < k) acc[cc] = fma(wv[i], tp[cc], acc[cc]);
+					}
 				}
 			}
 		}
@@ -141,16 +164,26 @@ __global__ __launch_bounds__(FZ_THREADS) void k_fused_matvec(FzArgs a)
 		double acc[FZ_KMAX];
 		#pragma unroll
 		for (int cc = 0; cc < FZ_KMAX; cc++) acc[cc] = 0;
+		FzTerm3 nxt;
+		if (t.u0 + sidx < t.u1) nxt = a.u3[t.u0 + sidx];
 		for (int q = t.u0 + sidx; q < t.u1; q += split) {
-			const FzTerm3 tm = a.u3[q];
+			const FzTerm3 tm = nxt;
+			if (q + split < t.u1) nxt = a.u3[q + split];   // next descriptor in flight during this term
 			const double* lp = a.l + tm.l_off + xi;
 			const double* tp = lds + (tm.t2 + qi) * k;
-			for (int am = 0; am < tm.dAM; am++) {
-				const double lv = lp[(int64_t)am * t.dX];
-				const double* tq = tp + am * t.dQ * k;
+			for (int am0 = 0; am0 < tm.dAM; am0 += FZ_LB) {
+				double lv[FZ_LB];
 				#pragma unroll
-				for (int cc = 0; cc < FZ_KMAX; cc++)
-					if (cc < k) acc[cc] = fma(lv, tq[cc], acc[cc]);
+				for (int i = 0; i < FZ_LB; i++) lv[i] = (am0 + i < tm.dAM) ? lp[(int64_t)(am0 + i) * t.dX] : 0.0;
+				#pragma unroll
+				for (int i = 0; i < FZ_LB; i++) {
+					if (am0 + i < tm.dAM) {
+						const double* tq = tp + (am0 + i) * t.dQ * k;
+						#pragma unroll
+						for (int cc = 0; cc < FZ_KMAX; cc++)
+							if (cc < k) acc[cc] = fma(lv[i], tq[cc], acc[cc]);
+					}
+				}
 			}
 		}
 		if (split == 1) {
@@ -197,11 +230,12 @@ struct FusedHost
 	std::vector<FzTerm2> u2;
 	std::vector<FzTask3> t3;
 	std::vector<FzTerm3> u3;
+	std::vector<double> cost;   // estimated cost of each unit
 	size_t lds_doubles = 0;
 	double fma = 0;
 };
 
-static bool plan_fused_host(FusedHost& h, const Meta& th, const Meta& W, const Meta& L, const Meta& R, size_t lds_budget)
+static bool plan_fused_host(FusedHost& h, const Meta& th, const Meta& W, const Meta& L, const Meta& R, size_t lds_budget, size_t lds_max)
 {
 	const qnumber qo = L.qnums_blocks[0][0];
 	const ct_long nA = th.dim_blocks[0], nP = th.dim_blocks[1], nB = th.dim_blocks[2];
@@ -210,7 +244,6 @@ static bool plan_fused_host(FusedHost& h, const Meta& th, const Meta& W, const M
 	auto bc = [](const Meta& m, int ax, ct_long i) { return m.block_count[ax][i]; };
 	auto qb = [](const Meta& m, int ax, ct_long i) { return m.qnums_blocks[ax][i]; };
 	const size_t budget = lds_budget / sizeof(double);
-	std::vector<double> unit_cost;
 
 	for (ct_long C = 0; C < nC; C++) {
 		const int dC = (int)bc(R, 2, C);
@@ -332,36 +365,81 @@ static bool plan_fused_host(FusedHost& h, const Meta& th, const Meta& W, const M
 		sc.lds_pc = std::max(lds_pc, 1);
 		sc.split = 1;
 		while (sc.split < 8 && 2 * sc.split * sc.rows3 <= FZ_THREADS && 2 * sc.split * sc.rows3 <= r_pc + t1_pc) sc.split *= 2;
-		if ((size_t)sc.lds_pc > budget) return false;
-		const int kmax = (int)std::min<size_t>({ (size_t)FZ_KMAX, (size_t)dC, budget / (size_t)sc.lds_pc });
+		if ((size_t)sc.lds_pc * sizeof(double) > lds_max) return false;
+		const int kmax = (size_t)sc.lds_pc > budget ? 1 : (int)std::min<size_t>({ (size_t)FZ_KMAX, (size_t)dC, budget / (size_t)sc.lds_pc });
 		const int isec = (int)h.sectors.size();
 		h.sectors.push_back(sc);
 		for (int c0 = 0; c0 < dC; c0 += kmax) {
 			const int kk = std::min(kmax, dC - c0);
 			h.units.push_back(FzUnit{ isec, c0, kk, 0 });
-			unit_cost.push_back(kk * fma_pc + 64.0 * (sc.rows1 + sc.rows2 + sc.rows3));
+			h.cost.push_back(kk * fma_pc + 64.0 * (sc.rows1 + sc.rows2 + sc.rows3));
 			h.lds_doubles = std::max(h.lds_doubles, (size_t)sc.lds_pc * kk);
 		}
 		h.fma += fma_pc * dC;
 	}
-	// XCD-aware order: longest-processing-time assignment of units to the 8 XCDs, each XCD's
-	// units by decreasing cost; workgroup slot*8 + xcd runs the slot-th unit of that XCD
+	return true;
+}
+
+// XCD-aware order: longest-processing-time assignment of units to the 8 XCDs, each XCD's units by
+// decreasing cost; workgroup slot*8 + xcd runs the slot-th unit of that XCD
+static void order_units(FusedHost& h)
+{
 	std::vector<int> idx(h.units.size());
 	std::iota(idx.begin(), idx.end(), 0);
-	std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return unit_cost[a] > unit_cost[b]; });
+	std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return h.cost[a] > h.cost[b]; });
 	std::vector<std::vector<int>> bins(FZ_NXCD);
 	std::vector<double> load(FZ_NXCD, 0.0);
 	for (int i : idx) {
 		const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
 		bins[x].push_back(i);
-		load[x] += unit_cost[i];
+		load[x] += h.cost[i];
 	}
 	size_t slots = 0;
 	for (auto& b : bins) slots = std::max(slots, b.size());
 	h.order.assign(slots * FZ_NXCD, -1);
 	for (int x = 0; x < FZ_NXCD; x++)
 		for (size_t s = 0; s < bins[x].size(); s++) h.order[s * FZ_NXCD + x] = bins[x][s];
-	return true;
+}
+
+// Append problem b to a, its operands living at element offsets x_off (theta and y), w_off,
+// l_off and r_off of concatenated buffers.
+static void append_fused(FusedHost& a, const FusedHost& b, int64_t x_off, int64_t w_off, int64_t l_off, int64_t r_off)
+{
+	const int map0 = (int)a.map.size(), sec0 = (int)a.sectors.size();
+	const int cp0 = (int)a.copies.size(), t10 = (int)a.t1.size(), t20 = (int)a.t2.size(), u20 = (int)a.u2.size();
+	const int t30 = (int)a.t3.size(), u30 = (int)a.u3.size();
+	for (FzSector sc : b.sectors) { sc.m0 += map0; sc.m1 += map0; sc.m2 += map0; sc.m3 += map0; a.sectors.push_back(sc); }
+	for (FzUnit u : b.units) { u.sector += sec0; a.units.push_back(u); }
+	a.cost.insert(a.cost.end(), b.cost.begin(), b.cost.end());
+	// map entries index the table of their phase; the phase of a map range follows from the sectors
+	a.map.resize(map0 + b.map.size());
+	for (const FzSector& sc : b.sectors) {
+		for (int e = 0; e < sc.n0; e++) a.map[map0 + sc.m0 + e] = b.map[sc.m0 + e] + cp0;
+		for (int e = 0; e < sc.rows1; e++) a.map[map0 + sc.m1 + e] = b.map[sc.m1 + e] + t10;
+		for (int e = 0; e < sc.rows2; e++) a.map[map0 + sc.m2 + e] = b.map[sc.m2 + e] + t20;
+		for (int e = 0; e < sc.rows3; e++) a.map[map0 + sc.m3 + e] = b.map[sc.m3 + e] + t30;
+	}
+	for (FzCopy c : b.copies) { c.goff += r_off; a.copies.push_back(c); }
+	for (FzTask1 t : b.t1) { t.th_off += x_off; a.t1.push_back(t); }
+	for (FzTask2 t : b.t2) { t.u0 += u20; t.u1 += u20; a.t2.push_back(t); }
+	for (FzTerm2 t : b.u2) { t.w_off += w_off; a.u2.push_back(t); }
+	for (FzTask3 t : b.t3) { t.y_off += x_off; t.u0 += u30; t.u1 += u30; a.t3.push_back(t); }
+	for (FzTerm3 t : b.u3) { t.l_off += l_off; a.u3.push_back(t); }
+	a.lds_doubles = std::max(a.lds_doubles, b.lds_doubles);
+	a.fma += b.fma;
+}
+
+// largest dynamic LDS allocation of one workgroup on the current device (160 KB on MI355X)
+static size_t lds_limit()
+{
+	static size_t lim = 0;
+	if (lim == 0) {
+		int dev = 0, v = 0;
+		CTG_HIP(hipGetDevice(&dev));
+		CTG_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+		lim = std::min<size_t>(FZ_LDS_MAX, (size_t)v);
+	}
+	return lim;
 }
 
 template <typename T>
@@ -372,10 +450,8 @@ static T* upload_vec(const std::vector<T>& v, hipStream_t s)
 	return d;
 }
 
-bool build_fused_plan(FusedPlan& f, const Meta& theta, const Meta& w, const Meta& l, const Meta& r, hipStream_t s)
+static void upload_fused(FusedPlan& f, const FusedHost& h, hipStream_t s)
 {
-	FusedHost h;
-	if (!plan_fused_host(h, theta, w, l, r, FZ_LDS_BUDGET)) return false;
 	f.sectors = upload_vec(h.sectors, s);
 	f.units = upload_vec(h.units, s);
 	f.order = upload_vec(h.order, s);
@@ -391,6 +467,27 @@ bool build_fused_plan(FusedPlan& f, const Meta& theta, const Meta& w, const Meta
 	f.nunits = (int)h.units.size();
 	f.lds_bytes = std::max<size_t>(h.lds_doubles, 1) * sizeof(double);
 	f.flops = 2.0 * h.fma;
+}
+
+bool build_fused_plan(FusedPlan& f, const Meta& theta, const Meta& w, const Meta& l, const Meta& r, hipStream_t s)
+{
+	FusedHost h;
+	if (!plan_fused_host(h, theta, w, l, r, FZ_LDS_BUDGET, lds_limit())) return false;
+	order_units(h);
+	upload_fused(f, h, s);
+	return true;
+}
+
+bool build_fused_batch(FusedPlan& f, const std::vector<FusedProblem>& probs, hipStream_t s)
+{
+	FusedHost all;
+	for (const FusedProblem& pb : probs) {
+		FusedHost h;
+		if (!plan_fused_host(h, *pb.theta, *pb.w, *pb.l, *pb.r, FZ_LDS_BUDGET, lds_limit())) return false;
+		append_fused(all, h, pb.x_off, pb.w_off, pb.l_off, pb.r_off);
+	}
+	order_units(all);
+	upload_fused(f, all, s);
 	return true;
 }
 
@@ -406,8 +503,142 @@ void launch_fused_matvec(const FusedPlan& f, const double* x, const double* w, c
 {
 	if (f.grid == 0) return;
 	FzArgs a = { f.sectors, f.units, f.order, f.map, f.copies, f.t1, f.t2, f.u2, f.t3, f.u3, x, w, l, r, y };
+	if (f.lds_bytes > 64 * 1024) {
+		static size_t granted = 64 * 1024;
+		if (f.lds_bytes > granted) {
+			CTG_HIP(hipFuncSetAttribute((const void*)k_fused_matvec, hipFuncAttributeMaxDynamicSharedMemorySize, (int)f.lds_bytes));
+			granted = f.lds_bytes;
+		}
+	}
 	hipLaunchKernelGGL(k_fused_matvec, dim3(f.grid), dim3(FZ_THREADS), f.lds_bytes, s, a);
 	CTG_HIP(hipGetLastError());
+}
+
+//________________________________________________________________________________________________________________________
+//
+// batch of independent problems
+
+BatchPlan::~BatchPlan()
+{
+	fused.release();
+	for (int i = 0; i < 3; i++) stage[i].release();
+	dev_free(t1);
+	dev_free(t2);
+	dev_free(w);
+	dev_free(l);
+	dev_free(r);
+	dev_free(segs);
+	dev_free(probs);
+}
+
+BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tune)
+{
+	std::unique_ptr<BatchPlan> b(new BatchPlan());
+	b->stream = s;
+	b->nprob = (int)in.size();
+	int64_t nx = 0, nw = 0, nl = 0, nr = 0;
+	std::vector<FusedProblem> probs;
+	b->off.push_back(0);
+	for (BatchInput& p : in) {
+		const Meta& th = p.theta;
+		validate_matvec_operands(th, p.w.meta, p.l.meta, p.r.meta);
+		probs.push_back(FusedProblem{ &p.theta, &p.w.meta, &p.l.meta, &p.r.meta, nx, nw, nl, nr });
+		nx += th.nelem;
+		nw += p.w.meta.nelem;
+		nl += p.l.meta.nelem;
+		nr += p.r.meta.nelem;
+		b->off.push_back(nx);
+		b->flops += reference_flops(th, p.w.meta, p.l.meta, p.r.meta);
+		b->bytes += 8.0 * (2.0 * (double)th.nelem + (double)(p.w.meta.nelem + p.l.meta.nelem + p.r.meta.nelem));
+	}
+	b->w = (double*)dev_alloc((size_t)std::max<int64_t>(nw, 1) * sizeof(double));
+	b->l = (double*)dev_alloc((size_t)std::max<int64_t>(nl, 1) * sizeof(double));
+	b->r = (double*)dev_alloc((size_t)std::max<int64_t>(nr, 1) * sizeof(double));
+	for (size_t i = 0; i < in.size(); i++) {
+		if (in[i].w.meta.nelem) launch_copy(in[i].w.d, b->w + probs[i].w_off, in[i].w.meta.nelem, s);
+		if (in[i].l.meta.nelem) launch_copy(in[i].l.d, b->l + probs[i].l_off, in[i].l.meta.nelem, s);
+		if (in[i].r.meta.nelem) launch_copy(in[i].r.d, b->r + probs[i].r_off, in[i].r.meta.nelem, s);
+	}
+	// grouped three-stage tables over the concatenated operands
+	GroupedTables g;
+	for (size_t i = 0; i < in.size(); i++)
+		plan_grouped_tables(g, in[i].theta, in[i].w.meta, in[i].l.meta, in[i].r.meta, probs[i].x_off, probs[i].w_off, probs[i].l_off, probs[i].r_off);
+	b->t1 = (double*)dev_alloc((size_t)std::max<int64_t>(g.t1_elems, 1) * sizeof(double));
+	b->t2 = (double*)dev_alloc((size_t)std::max<int64_t>(g.t2_elems, 1) * sizeof(double));
+	b->stage[0].upload(g.s1, s);
+	b->stage[1].upload(g.s2, s);
+	b->stage[2].upload(g.s3, s);
+	b->stage_bases[0] = GemmBases{ nullptr, b->r, b->t1 };
+	b->stage_bases[1] = GemmBases{ b->w, b->t1, b->t2 };
+	b->stage_bases[2] = GemmBases{ b->l, b->t2, nullptr };
+	if (tune) {
+		autotune_grouped(b->stage, b->stage_bases, nx, s, nullptr);
+		b->fused_ok = build_fused_batch(b->fused, probs, s);
+	}
+	else {
+		// one fixed launch form: the summation order of a problem does not depend on the batch
+		for (int i = 0; i < 3; i++) b->stage[i].form = GEMM_WAVE_STREAM;
+	}
+	// reduction blocks of the batched Lanczos: up to 64 blocks of >= 4096 elements per problem
+	std::vector<LzSeg> segs;
+	std::vector<LzProb> pr;
+	for (int p = 0; p < b->nprob; p++) {
+		const int64_t n = b->off[p + 1] - b->off[p];
+		const int G = (int)std::max<int64_t>(1, std::min<int64_t>(64, (n + 4095) / 4096));
+		LzProb q = { (int)segs.size(), (int)segs.size() + G, n };
+		for (int g = 0; g < G; g++) {
+			const int64_t lo = b->off[p] + n * g / G, hi = b->off[p] + n * (g + 1) / G;
+			segs.push_back(LzSeg{ lo, hi, p, 0 });
+		}
+		pr.push_back(q);
+	}
+	b->nblocks = (int)segs.size();
+	b->segs = upload_vec(segs, s);
+	b->probs = upload_vec(pr, s);
+	CTG_HIP(hipStreamSynchronize(s));
+	// time both forms on the batch itself and keep the faster
+	if (tune) {
+		double* x = (double*)dev_alloc((size_t)std::max<int64_t>(nx, 1) * sizeof(double));
+		double* y = (double*)dev_alloc((size_t)std::max<int64_t>(nx, 1) * sizeof(double));
+		launch_fill(x, 1.0, nx, s);
+		hipEvent_t e0, e1;
+		CTG_HIP(hipEventCreate(&e0));
+		CTG_HIP(hipEventCreate(&e1));
+		for (int f = 0; f < (b->fused_ok ? 2 : 1); f++) {
+			b->use_fused = (f == 1);
+			batch_apply(*b, x, y);
+			CTG_HIP(hipEventRecord(e0, s));
+			for (int rep = 0; rep < 3; rep++) batch_apply(*b, x, y);
+			CTG_HIP(hipEventRecord(e1, s));
+			CTG_HIP(hipEventSynchronize(e1));
+			CTG_HIP(hipEventElapsedTime(&b->form_ms[f], e0, e1));
+			b->form_ms[f] /= 3;
+		}
+		b->use_fused = b->fused_ok && b->form_ms[1] < b->form_ms[0];
+		// CT_BATCH_FORM=grouped|fused overrides the timing-based choice
+		const char* force = getenv("CT_BATCH_FORM");
+		if (force && strcmp(force, "grouped") == 0) b->use_fused = false;
+		if (force && strcmp(force, "fused") == 0 && b->fused_ok) b->use_fused = true;
+		CTG_HIP(hipEventDestroy(e0));
+		CTG_HIP(hipEventDestroy(e1));
+		dev_free(x);
+		dev_free(y);
+	}
+	return b.release();
+}
+
+void batch_apply(BatchPlan& b, const double* x, double* y)
+{
+	if (b.use_fused) {
+		launch_fused_matvec(b.fused, x, b.w, b.l, b.r, y, b.stream);
+		return;
+	}
+	for (int st = 0; st < 3; st++) {
+		GemmBases g = b.stage_bases[st];
+		if (st == 0) g.A = x;
+		if (st == 2) g.C = y;
+		launch_grouped_gemm_f64(b.stage[st], g, b.stream);
+	}
 }
 
 }  // namespace ctg

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <algorithm>
+#include <limits>
 
 #include "ct_internal.h"
 
@@ -110,12 +111,15 @@ __global__ __launch_bounds__(LZ_THREADS) void k_lz_ritz(const double* __restrict
 void LanczosWork::reserve(int64_t nn, int maxiter, int numeig)
 {
 	if (nn != n || cap < maxiter) {
+		drop_graph();
 		dev_free(v);
 		dev_free(w);
 		dev_free(scal);
+		dev_free(vs);
 		v = (double*)dev_alloc((size_t)maxiter * (size_t)nn * sizeof(double));
 		w = (double*)dev_alloc((size_t)nn * sizeof(double));
 		scal = (double*)dev_alloc((size_t)(2 * maxiter + 2) * sizeof(double));
+		vs = nullptr;
 		n = nn;
 		cap = maxiter;
 	}
@@ -127,21 +131,28 @@ void LanczosWork::reserve(int64_t nn, int maxiter, int numeig)
 	}
 }
 
+void LanczosWork::drop_graph()
+{
+	if (graph) (void)hipGraphExecDestroy(graph);
+	graph = nullptr;
+	graph_maxiter = 0;
+}
+
 LanczosWork::~LanczosWork()
 {
+	drop_graph();
 	dev_free(v);
 	dev_free(w);
 	dev_free(scal);
 	dev_free(part);
 	dev_free(z);
+	dev_free(vs);
 }
 
-int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart,
-	int maxiter, int numeig, double* lambda, double* u)
+// v0 = vstart / |vstart|, then maxiter steps of the recurrence; only kernel launches on s
+static void enqueue_recurrence(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart, int maxiter)
 {
-	if (maxiter < 1 || numeig < 1 || numeig > maxiter || n < 1) return CT_ERR_ARG;
 	const int G = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 4095) / 4096));
-	lw.reserve(n, maxiter, numeig);
 	double* V = lw.v;
 	double* w = lw.w;
 	double* scal = lw.scal;
@@ -161,8 +172,40 @@ int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatv
 			hipLaunchKernelGGL(k_lz_beta_scale, dim3(G), dim3(LZ_THREADS), 0, s, w, V + (int64_t)(j + 1) * n, n, j, maxiter, part2, G, thresh, scal);
 		CTG_HIP(hipGetLastError());
 	}
+}
+
+int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart,
+	int maxiter, int numeig, double* lambda, double* u, bool use_graph)
+{
+	if (maxiter < 1 || numeig < 1 || numeig > maxiter || n < 1) return CT_ERR_ARG;
+	lw.reserve(n, maxiter, numeig);
+	if (use_graph) {
+		if (!lw.vs) lw.vs = (double*)dev_alloc((size_t)n * sizeof(double));
+		launch_copy(vstart, lw.vs, n, s);
+		if (!lw.graph || lw.graph_maxiter != maxiter) {
+			lw.drop_graph();
+			hipGraph_t g = nullptr;
+			CTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+			try {
+				enqueue_recurrence(lw, n, s, matvec, lw.vs, maxiter);
+			}
+			catch (...) {
+				(void)hipStreamEndCapture(s, &g);
+				if (g) (void)hipGraphDestroy(g);
+				throw;
+			}
+			CTG_HIP(hipStreamEndCapture(s, &g));
+			const hipError_t e = hipGraphInstantiate(&lw.graph, g, nullptr, nullptr, 0);
+			(void)hipGraphDestroy(g);
+			CTG_HIP(e);
+			lw.graph_maxiter = maxiter;
+		}
+		CTG_HIP(hipGraphLaunch(lw.graph, s));
+	}
+	else enqueue_recurrence(lw, n, s, matvec, vstart, maxiter);
+	double* V = lw.v;
 	std::vector<double> hs((size_t)(2 * maxiter + 2));
-	CTG_HIP(hipMemcpyAsync(hs.data(), scal, hs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+	CTG_HIP(hipMemcpyAsync(hs.data(), lw.scal, hs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
 	CTG_HIP(hipStreamSynchronize(s));
 	const int numiter = (hs[2 * maxiter + 1] != 0) ? (int)hs[2 * maxiter] : maxiter;
 	if (numiter < numeig) {
@@ -189,10 +232,231 @@ int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatv
 	return 0;
 }
 
+static bool graph_enabled()
+{
+	const char* e = getenv("CT_LANCZOS_GRAPH");
+	return !(e && e[0] == '0');
+}
+
 int lanczos_ground_state(MatvecPlan& p, const double* vstart, int maxiter, double* lambda, double* u)
 {
+	// the plain form replays thousands of per-block launches; it stays eager
+	const bool graph = p.mode != CT_MATVEC_PLAIN_FP64 && graph_enabled();
 	return lanczos_eigensystem(p.lz, p.theta_meta.nelem, p.stream,
-		[&p](const double* x, double* y) { matvec_apply(p, x, y, nullptr); }, vstart, maxiter, 1, lambda, u);
+		[&p](const double* x, double* y) { matvec_apply(p, x, y, nullptr); }, vstart, maxiter, 1, lambda, u, graph);
+}
+
+//________________________________________________________________________________________________________________________
+//
+// batched Lanczos: one recurrence per problem over its segment of the concatenated vectors. Each
+// reduction block covers a range of one problem; every block of a problem sums that problem's
+// partials in the same order. Per-problem scalars: scal[p * S + ...] with S = 2 maxiter + 2.
+
+__device__ __forceinline__ double sum_range(const double* __restrict__ part, int b0, int b1, double* sh)
+{
+	double v = 0;
+	for (int b = b0 + (int)threadIdx.x; b < b1; b += LZ_THREADS) v += part[b];
+	return block_sum(v, sh);
+}
+
+__global__ __launch_bounds__(LZ_THREADS) void k_blz_dot_partial(const double* __restrict__ x, const double* __restrict__ y, const LzSeg* __restrict__ segs,
+	double* __restrict__ part, const double* __restrict__ scal, int S, int check_stop)
+{
+	__shared__ double sh[LZ_THREADS];
+	const LzSeg g = segs[blockIdx.x];
+	if (check_stop && scal[g.prob * S + S - 1] != 0) return;
+	double v = 0;
+	for (int64_t i = g.begin + threadIdx.x; i < g.end; i += LZ_THREADS) v += x[i] * y[i];
+	const double s = block_sum(v, sh);
+	if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(LZ_THREADS) void k_blz_normalize(const double* __restrict__ vs, double* __restrict__ v0, const LzSeg* __restrict__ segs,
+	const LzProb* __restrict__ probs, const double* __restrict__ part)
+{
+	__shared__ double sh[LZ_THREADS];
+	const LzSeg g = segs[blockIdx.x];
+	const LzProb q = probs[g.prob];
+	const double inv = 1 / sqrt(sum_range(part, q.b0, q.b1, sh));
+	for (int64_t i = g.begin + threadIdx.x; i < g.end; i += LZ_THREADS) v0[i] = inv * vs[i];
+}
+
+__global__ __launch_bounds__(LZ_THREADS) void k_blz_alpha_update(double* __restrict__ w, const double* __restrict__ vj, const double* __restrict__ vjm1,
+	int j, int maxiter, const LzSeg* __restrict__ segs, const LzProb* __restrict__ probs, const double* __restrict__ part, double* __restrict__ part2,
+	double* __restrict__ scal)
+{
+	__shared__ double sh[LZ_THREADS];
+	const int S = 2 * maxiter + 2;
+	const LzSeg g = segs[blockIdx.x];
+	double* sc = scal + (int64_t)g.prob * S;
+	if (sc[S - 1] != 0) return;
+	const LzProb q = probs[g.prob];
+	const double alpha = sum_range(part, q.b0, q.b1, sh);
+	if ((int)blockIdx.x == q.b0 && threadIdx.x == 0) sc[j] = alpha;
+	if (j == maxiter - 1) return;
+	const double bprev = (j > 0) ? sc[maxiter + j - 1] : 0.0;
+	double v = 0;
+	for (int64_t i = g.begin + threadIdx.x; i < g.end; i += LZ_THREADS) {
+		double wi = w[i];
+		if (j > 0) wi -= alpha * vj[i] + bprev * vjm1[i];
+		else wi -= alpha * vj[i];
+		w[i] = wi;
+		v += wi * wi;
+	}
+	const double s = block_sum(v, sh);
+	if (threadIdx.x == 0) part2[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(LZ_THREADS) void k_blz_beta_scale(const double* __restrict__ w, double* __restrict__ vnext, int j, int maxiter,
+	const LzSeg* __restrict__ segs, const LzProb* __restrict__ probs, const double* __restrict__ part2, double* __restrict__ scal)
+{
+	__shared__ double sh[LZ_THREADS];
+	const int S = 2 * maxiter + 2;
+	const LzSeg g = segs[blockIdx.x];
+	double* sc = scal + (int64_t)g.prob * S;
+	if (sc[S - 1] != 0) return;
+	const LzProb q = probs[g.prob];
+	const double beta = sqrt(sum_range(part2, q.b0, q.b1, sh));
+	const double thresh = 100.0 * (double)q.n * 2.2204460492503131e-16;   // krylov.c:58
+	if (beta < thresh) {
+		// every block of the problem computes the same beta and returns here
+		if ((int)blockIdx.x == q.b0 && threadIdx.x == 0) { sc[maxiter + j] = beta; sc[2 * maxiter] = j + 1; sc[S - 1] = 1; }
+		return;
+	}
+	if ((int)blockIdx.x == q.b0 && threadIdx.x == 0) sc[maxiter + j] = beta;
+	const double inv = 1 / beta;
+	for (int64_t i = g.begin + threadIdx.x; i < g.end; i += LZ_THREADS) vnext[i] = inv * w[i];
+}
+
+// u = sum_{j < numiter_p} z_p[j] V_j on each problem's segment
+__global__ __launch_bounds__(LZ_THREADS) void k_blz_ritz(const double* __restrict__ V, int64_t ntot, const double* __restrict__ z, const int* __restrict__ nit,
+	int maxiter, const LzSeg* __restrict__ segs, double* __restrict__ u)
+{
+	const LzSeg g = segs[blockIdx.x];
+	const int m = nit[g.prob];
+	const double* zp = z + (int64_t)g.prob * maxiter;
+	for (int64_t i = g.begin + threadIdx.x; i < g.end; i += LZ_THREADS) {
+		double acc = 0;
+		for (int jj = 0; jj < m; jj++) acc += V[(int64_t)jj * ntot + i] * zp[jj];
+		u[i] = acc;
+	}
+}
+
+void BatchLanczosWork::drop_graph()
+{
+	if (graph) (void)hipGraphExecDestroy(graph);
+	graph = nullptr;
+	graph_maxiter = 0;
+}
+
+BatchLanczosWork::~BatchLanczosWork()
+{
+	drop_graph();
+	dev_free(v); dev_free(w); dev_free(vs); dev_free(scal); dev_free(part); dev_free(z); dev_free(nit);
+}
+
+static void enqueue_batch_recurrence(BatchPlan& b, const double* vstart, int maxiter)
+{
+	BatchLanczosWork& lw = b.lz;
+	hipStream_t s = b.stream;
+	const int64_t ntot = b.off.back();
+	const int G = b.nblocks;
+	const int S = 2 * maxiter + 2;
+	double* part = lw.part;
+	double* part2 = lw.part + G;
+	launch_fill(lw.scal, 0.0, (ct_long)b.nprob * S, s);
+	hipLaunchKernelGGL(k_blz_dot_partial, dim3(G), dim3(LZ_THREADS), 0, s, vstart, vstart, b.segs, part, (const double*)lw.scal, S, 0);
+	hipLaunchKernelGGL(k_blz_normalize, dim3(G), dim3(LZ_THREADS), 0, s, vstart, lw.v, b.segs, b.probs, (const double*)part);
+	CTG_HIP(hipGetLastError());
+	for (int j = 0; j < maxiter; j++) {
+		const double* vj = lw.v + (int64_t)j * ntot;
+		batch_apply(b, vj, lw.w);
+		hipLaunchKernelGGL(k_blz_dot_partial, dim3(G), dim3(LZ_THREADS), 0, s, (const double*)lw.w, vj, b.segs, part, (const double*)lw.scal, S, 1);
+		hipLaunchKernelGGL(k_blz_alpha_update, dim3(G), dim3(LZ_THREADS), 0, s, lw.w, vj, j > 0 ? vj - ntot : vj, j, maxiter, b.segs, b.probs,
+			(const double*)part, part2, lw.scal);
+		if (j < maxiter - 1)
+			hipLaunchKernelGGL(k_blz_beta_scale, dim3(G), dim3(LZ_THREADS), 0, s, (const double*)lw.w, lw.v + (int64_t)(j + 1) * ntot, j, maxiter,
+				b.segs, b.probs, (const double*)part2, lw.scal);
+		CTG_HIP(hipGetLastError());
+	}
+}
+
+int lanczos_ground_state_batch(BatchPlan& b, const double* vstart, int maxiter, double* lambda, int* status, double* u, bool use_graph)
+{
+	if (maxiter < 1 || b.nprob < 1) return CT_ERR_ARG;
+	BatchLanczosWork& lw = b.lz;
+	hipStream_t s = b.stream;
+	const int64_t ntot = b.off.back();
+	const int S = 2 * maxiter + 2;
+	if (lw.cap < maxiter) {
+		lw.drop_graph();
+		dev_free(lw.v); dev_free(lw.scal); dev_free(lw.z);
+		lw.v = (double*)dev_alloc((size_t)maxiter * (size_t)ntot * sizeof(double));
+		lw.scal = (double*)dev_alloc((size_t)b.nprob * S * sizeof(double));
+		lw.z = (double*)dev_alloc((size_t)b.nprob * maxiter * sizeof(double));
+		lw.cap = maxiter;
+	}
+	if (!lw.w) {
+		lw.w = (double*)dev_alloc((size_t)ntot * sizeof(double));
+		lw.vs = (double*)dev_alloc((size_t)ntot * sizeof(double));
+		lw.part = (double*)dev_alloc((size_t)2 * b.nblocks * sizeof(double));
+		lw.nit = (int*)dev_alloc((size_t)b.nprob * sizeof(int));
+	}
+	if (use_graph) {
+		launch_copy(vstart, lw.vs, ntot, s);
+		if (!lw.graph || lw.graph_maxiter != maxiter) {
+			lw.drop_graph();
+			hipGraph_t g = nullptr;
+			CTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+			try {
+				enqueue_batch_recurrence(b, lw.vs, maxiter);
+			}
+			catch (...) {
+				(void)hipStreamEndCapture(s, &g);
+				if (g) (void)hipGraphDestroy(g);
+				throw;
+			}
+			CTG_HIP(hipStreamEndCapture(s, &g));
+			const hipError_t e = hipGraphInstantiate(&lw.graph, g, nullptr, nullptr, 0);
+			(void)hipGraphDestroy(g);
+			CTG_HIP(e);
+			lw.graph_maxiter = maxiter;
+		}
+		CTG_HIP(hipGraphLaunch(lw.graph, s));
+	}
+	else enqueue_batch_recurrence(b, vstart, maxiter);
+	std::vector<double> hs((size_t)b.nprob * S);
+	CTG_HIP(hipMemcpyAsync(hs.data(), lw.scal, hs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+	CTG_HIP(hipStreamSynchronize(s));
+	std::vector<double> zall((size_t)b.nprob * maxiter, 0.0);
+	std::vector<int> nit(b.nprob, 0);
+	int worst = 0;
+	for (int p = 0; p < b.nprob; p++) {
+		const double* h = hs.data() + (size_t)p * S;
+		const int numiter = (h[S - 1] != 0) ? (int)h[2 * maxiter] : maxiter;
+		int st = 0;
+		lambda[p] = std::numeric_limits<double>::quiet_NaN();
+		if (numiter < 1) st = -1;
+		else {
+			std::vector<double> d(h, h + numiter), e((size_t)numiter + 1, 0.0), z((size_t)numiter * numiter);
+			for (int i = 0; i < numiter - 1; i++) e[i] = h[maxiter + i];
+			if (host_tridiag_eigh(numiter, d.data(), e.data(), z.data()) != 0) st = -2;
+			else {
+				lambda[p] = d[0];
+				for (int jj = 0; jj < numiter; jj++) zall[(size_t)p * maxiter + jj] = z[jj];
+				nit[p] = numiter;
+			}
+		}
+		if (status) status[p] = st;
+		if (st != 0 && worst == 0) worst = st;
+	}
+	CTG_HIP(hipMemcpyAsync(lw.z, zall.data(), zall.size() * sizeof(double), hipMemcpyHostToDevice, s));
+	CTG_HIP(hipMemcpyAsync(lw.nit, nit.data(), nit.size() * sizeof(int), hipMemcpyHostToDevice, s));
+	hipLaunchKernelGGL(k_blz_ritz, dim3(b.nblocks), dim3(LZ_THREADS), 0, s, (const double*)lw.v, ntot, (const double*)lw.z, (const int*)lw.nit, maxiter,
+		b.segs, u);
+	CTG_HIP(hipGetLastError());
+	CTG_HIP(hipStreamSynchronize(s));
+	return worst;
 }
 
 }  // namespace ctg

@@ -1,0 +1,348 @@
+// Serving loop for independent DMRG ground-state solves with continuous batching.
+//
+// A worker thread owns every device resource of the server (one HIP stream). Each tick it
+//   1. admits queued requests in arrival order while their estimated device footprint fits the
+//      HBM budget and a slot is free (construct the MPO and the seeded start MPS, upload them,
+//      right-normalise, build the environments: the state stays device-resident until done);
+//   2. collects the current two-site eigenproblem of every active request and solves all of them
+//      with one batched Lanczos (one grouped launch per stage for all matvecs of a step, one
+//      recurrence per request), grouped by Lanczos iteration count;
+//   3. splits the optimised two-site tensors: device reshapes, the block SVDs of all requests in
+//      parallel host threads, device uploads and environment updates;
+//   4. retires requests whose sweeps are done and frees their budget.
+// A request that arrives while others run joins at the next tick. The grouped launch uses one
+// fixed launch form and every output tile is summed by one wavefront in a fixed order, and each
+// request's Lanczos reductions cover only its own segment, so a request's results do not depend
+// on the other requests in the batch.
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+
+#include "ct_internal.h"
+
+namespace ctg {
+
+namespace {
+
+struct Ticket
+{
+	int64_t id = 0;
+	ct_serve_request req = {};
+	double t_submit = 0, t_admit = -1, t_done = -1;
+	int status = 1;
+	std::vector<double> en;
+	size_t reserved = 0;
+	std::unique_ptr<DmrgState> state;
+};
+
+size_t estimate_bytes(const ct_serve_request& r)
+{
+	// dense upper bounds: MPS sites L chi^2 d, environments 2 L chi^2 M, merged MPO pairs L M^2 d^4,
+	// Lanczos vectors (maxiter + 3) chi^2 d^2, grouped intermediates 2 M chi^2 d^2
+	const double d = (r.model == 0) ? 2 : 4, M = (r.model == 0) ? 5 : 6;
+	const double chi = (double)r.max_vdim, L = r.nsites;
+	const double el = L * chi * chi * d + 2 * L * chi * chi * M + L * M * M * d * d * d * d
+		+ (r.maxiter_lanczos + 3) * chi * chi * d * d + 2 * M * chi * chi * d * d;
+	return (size_t)(8.0 * el);
+}
+
+}  // namespace
+
+struct Server
+{
+	std::mutex mu;
+	std::condition_variable cv_work, cv_done;
+	std::deque<std::shared_ptr<Ticket>> pending;
+	std::map<int64_t, std::shared_ptr<Ticket>> tickets;
+	std::vector<std::shared_ptr<Ticket>> active;   // worker thread only
+	int64_t next_id = 1;
+	bool stop = false;
+	double budget = 0;
+	size_t used = 0;
+	int max_active = 0;
+	int host_threads = 8;
+	int64_t ticks = 0, lanczos_batches = 0, completed = 0, max_batch = 0;
+	double busy_s = 0;
+	hipStream_t st = nullptr;
+	std::chrono::steady_clock::time_point t0;
+	std::thread worker;
+
+	double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+
+	void finish(Ticket& t, int status)
+	{
+		std::lock_guard<std::mutex> g(mu);
+		t.status = status;
+		t.t_done = now();
+		used -= t.reserved;
+		t.reserved = 0;
+		t.state.reset();
+		completed++;
+		cv_done.notify_all();
+	}
+
+	int start_request(Ticket& t)
+	{
+		const ct_serve_request& r = t.req;
+		mpo H = {};
+		mps psi = {};
+		if (r.model == 0) ct_construct_heisenberg_xxz_1d_mpo(r.nsites, r.param[0], r.param[1], r.param[2], &H);
+		else ct_construct_fermi_hubbard_1d_mpo(r.nsites, r.param[0], r.param[1], r.param[2], &H);
+		rng_state rng;
+		seed_rng_state(r.seed, &rng);
+		construct_random_mps(CT_DOUBLE_REAL, r.nsites, H.d, H.qsite, r.sector, r.max_vdim, &rng, &psi);
+		t.state.reset(new DmrgState());
+		t.state->st = st;
+		int ret = CT_ERR_HIP;
+		try {
+			ret = t.state->init(&H, &psi, r.num_sweeps, r.maxiter_lanczos, r.tol_split, r.max_vdim, r.tol_energy);
+		}
+		catch (const HipError&) {
+			ret = CT_ERR_HIP;
+		}
+		delete_mps(&psi);
+		delete_mpo(&H);
+		return ret;
+	}
+
+	// one batched Lanczos over the active requests with this iteration count, then their splits
+	void step_group(std::vector<std::shared_ptr<Ticket>>& grp, int maxiter)
+	{
+		std::vector<BatchInput> in(grp.size());
+		for (size_t i = 0; i < grp.size(); i++) {
+			DmrgState& S = *grp[i]->state;
+			in[i].theta = S.theta.meta;
+			in[i].w = DevTensor::view(S.h2[S.bond], S.h2[S.bond].meta);
+			in[i].l = DevTensor::view(S.lb[S.bond], S.lb[S.bond].meta);
+			in[i].r = DevTensor::view(S.rb[S.bond + 1], S.rb[S.bond + 1].meta);
+		}
+		std::unique_ptr<BatchPlan> b(build_batch_plan(in, st, false));
+		const int64_t ntot = b->off.back();
+		double* v0 = (double*)dev_alloc((size_t)std::max<int64_t>(ntot, 1) * sizeof(double));
+		double* u = (double*)dev_alloc((size_t)std::max<int64_t>(ntot, 1) * sizeof(double));
+		for (size_t i = 0; i < grp.size(); i++) {
+			const DmrgState& S = *grp[i]->state;
+			if (S.theta.meta.nelem) launch_copy(S.theta.d, v0 + b->off[i], S.theta.meta.nelem, st);
+		}
+		std::vector<double> lam(grp.size());
+		std::vector<int> status(grp.size());
+		lanczos_ground_state_batch(*b, v0, maxiter, lam.data(), status.data(), u, false);
+		lanczos_batches++;
+		max_batch = std::max<int64_t>(max_batch, (int64_t)grp.size());
+		// device part of the splits
+		for (size_t i = 0; i < grp.size(); i++) {
+			if (status[i] != 0) continue;
+			DmrgState& S = *grp[i]->state;
+			DevTensor a_opt(S.theta.meta);
+			if (S.theta.meta.nelem) launch_copy(u + b->off[i], a_opt.d, S.theta.meta.nelem, st);
+			S.advance_begin(a_opt);
+		}
+		dev_free(v0);
+		dev_free(u);
+		b.reset();
+		// host block SVDs in parallel (no HIP calls in advance_host)
+		{
+			const int nt = std::max(1, std::min<int>(host_threads, (int)grp.size()));
+			std::vector<std::thread> th;
+			for (int k = 0; k < nt; k++)
+				th.emplace_back([&, k] {
+					for (size_t i = k; i < grp.size(); i += nt)
+						if (status[i] == 0) grp[i]->state->advance_host();
+				});
+			for (auto& x : th) x.join();
+		}
+		for (size_t i = 0; i < grp.size(); i++) {
+			Ticket& t = *grp[i];
+			if (status[i] != 0) { finish(t, status[i]); continue; }
+			int ret;
+			try {
+				ret = t.state->advance_end(lam[i]);
+			}
+			catch (const HipError&) {
+				ret = CT_ERR_HIP;
+			}
+			if (ret != 0) { finish(t, ret); continue; }
+			if (t.state->done) {
+				t.en = t.state->en_sweeps;
+				finish(t, 0);
+			}
+		}
+	}
+
+	void loop()
+	{
+		for (;;) {
+			std::vector<std::shared_ptr<Ticket>> admit;
+			{
+				std::unique_lock<std::mutex> g(mu);
+				cv_work.wait(g, [&] { return stop || !pending.empty() || !active.empty(); });
+				if (stop) return;
+				// arrival order; at most a few initialisations per tick so running requests keep moving
+				while (!pending.empty() && (int)(active.size() + admit.size()) < max_active && admit.size() < 8) {
+					const std::shared_ptr<Ticket>& t = pending.front();
+					const size_t need = estimate_bytes(t->req);
+					if (used + need > budget && !(active.empty() && admit.empty())) break;
+					t->reserved = need;
+					used += need;
+					t->t_admit = now();
+					admit.push_back(t);
+					pending.pop_front();
+				}
+			}
+			const double tick0 = now();
+			for (auto& t : admit) {
+				const int ret = start_request(*t);
+				if (ret != 0) finish(*t, ret);
+				else active.push_back(t);
+			}
+			if (!active.empty()) {
+				std::map<int, std::vector<std::shared_ptr<Ticket>>> groups;
+				for (auto& t : active) groups[t->req.maxiter_lanczos].push_back(t);
+				for (auto& kv : groups) {
+					try {
+						step_group(kv.second, kv.first);
+					}
+					catch (const HipError&) {
+						for (auto& t : kv.second) if (t->status == 1) finish(*t, CT_ERR_HIP);
+					}
+				}
+				active.erase(std::remove_if(active.begin(), active.end(), [](const std::shared_ptr<Ticket>& t) { return t->status != 1; }), active.end());
+				ticks++;
+			}
+			busy_s += now() - tick0;
+		}
+	}
+};
+
+}  // namespace ctg
+
+using namespace ctg;
+
+struct ct_serve
+{
+	std::unique_ptr<Server> s;
+};
+
+extern "C" {
+
+int ct_serve_create(const double hbm_budget_bytes, const int max_active, ct_serve** server)
+{
+	if (!server || hbm_budget_bytes <= 0 || max_active < 1) return CT_ERR_ARG;
+	try {
+		std::unique_ptr<ct_serve> h(new ct_serve());
+		h->s.reset(new Server());
+		Server& S = *h->s;
+		S.budget = hbm_budget_bytes;
+		S.max_active = max_active;
+		const unsigned hc = std::thread::hardware_concurrency();
+		S.host_threads = (int)std::max(1u, std::min(16u, hc ? hc : 8u));
+		S.t0 = std::chrono::steady_clock::now();
+		CTG_HIP(hipStreamCreateWithFlags(&S.st, hipStreamNonBlocking));
+		S.worker = std::thread([&S] { S.loop(); });
+		*server = h.release();
+		return 0;
+	}
+	catch (const HipError&) {
+		return CT_ERR_HIP;
+	}
+}
+
+int ct_serve_submit(ct_serve* server, const struct ct_serve_request* req, int64_t* ticket)
+{
+	if (!server || !req || !ticket) return CT_ERR_ARG;
+	if (req->model < 0 || req->model > 1 || req->nsites < 2 || req->max_vdim < 1 || req->num_sweeps < 1 || req->maxiter_lanczos < 1) return CT_ERR_ARG;
+	Server& S = *server->s;
+	auto t = std::make_shared<Ticket>();
+	t->req = *req;
+	{
+		std::lock_guard<std::mutex> g(S.mu);
+		t->id = S.next_id++;
+		t->t_submit = S.now();
+		S.pending.push_back(t);
+		S.tickets[t->id] = t;
+	}
+	S.cv_work.notify_one();
+	*ticket = t->id;
+	return 0;
+}
+
+static void fill_result(const Ticket& t, struct ct_serve_result* res)
+{
+	res->status = t.status;
+	res->sweeps = (int)t.en.size();
+	res->energy = t.en.empty() ? std::nan("") : t.en.back();
+	res->energy_change = t.en.size() >= 2 ? std::fabs(t.en[t.en.size() - 1] - t.en[t.en.size() - 2]) : std::nan("");
+	res->t_submit = t.t_submit;
+	res->t_admit = t.t_admit;
+	res->t_done = t.t_done;
+}
+
+int ct_serve_poll(ct_serve* server, const int64_t ticket, const int wait, struct ct_serve_result* res)
+{
+	if (!server || !res) return CT_ERR_ARG;
+	Server& S = *server->s;
+	std::unique_lock<std::mutex> g(S.mu);
+	auto it = S.tickets.find(ticket);
+	if (it == S.tickets.end()) return CT_ERR_ARG;
+	const std::shared_ptr<Ticket> t = it->second;
+	if (wait) S.cv_done.wait(g, [&] { return t->status != 1 || S.stop; });
+	fill_result(*t, res);
+	return t->status == 1 ? 1 : 0;
+}
+
+int ct_serve_energies(ct_serve* server, const int64_t ticket, double* en_sweeps, const int capacity)
+{
+	if (!server) return CT_ERR_ARG;
+	Server& S = *server->s;
+	std::lock_guard<std::mutex> g(S.mu);
+	auto it = S.tickets.find(ticket);
+	if (it == S.tickets.end()) return CT_ERR_ARG;
+	const Ticket& t = *it->second;
+	const int n = (int)t.en.size();
+	for (int i = 0; i < std::min(n, capacity); i++) en_sweeps[i] = t.en[i];
+	return n;
+}
+
+int ct_serve_stats(ct_serve* server, double* stats8)
+{
+	if (!server || !stats8) return CT_ERR_ARG;
+	Server& S = *server->s;
+	std::lock_guard<std::mutex> g(S.mu);
+	stats8[0] = (double)S.ticks;
+	stats8[1] = (double)S.lanczos_batches;
+	stats8[2] = (double)S.completed;
+	stats8[3] = (double)S.pending.size();
+	stats8[4] = (double)S.used;
+	stats8[5] = (double)S.max_batch;
+	stats8[6] = S.busy_s;
+	stats8[7] = S.now();
+	return 0;
+}
+
+int ct_serve_destroy(ct_serve* server)
+{
+	if (!server) return CT_ERR_ARG;
+	Server& S = *server->s;
+	{
+		std::lock_guard<std::mutex> g(S.mu);
+		S.stop = true;
+	}
+	S.cv_work.notify_all();
+	S.cv_done.notify_all();
+	if (S.worker.joinable()) S.worker.join();
+	for (auto& t : S.active) t->state.reset();
+	S.active.clear();
+	(void)hipStreamSynchronize(S.st);
+	(void)hipStreamDestroy(S.st);
+	delete server;
+	return 0;
+}
+
+}  // extern "C"

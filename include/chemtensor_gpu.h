@@ -180,6 +180,84 @@ int ct_matvec_apply_timed(ct_matvec_plan* plan, const double* d_x, double* d_y, 
 /// the plan's HIP stream (hipStream_t) as an opaque pointer
 void* ct_matvec_plan_stream(ct_matvec_plan* plan);
 
+//________________________________________________________________________________________________________________________
+//
+// batches of independent H_eff problems (no counterpart in the reference, which applies one H_eff
+// at a time): one fused launch applies all of them to the concatenation of their packed vectors,
+// and the batched Lanczos runs one recurrence per problem with the matvecs of a step in that launch
+
+typedef struct ct_matvec_batch ct_matvec_batch;
+
+/// problems p = 0 .. nprob-1 as in ct_matvec_plan_create (fused FP64 form); operands are uploaded
+int ct_matvec_batch_create(const int nprob, const struct block_sparse_tensor* const* a, const struct block_sparse_tensor* const* w,
+	const struct block_sparse_tensor* const* l, const struct block_sparse_tensor* const* r, ct_matvec_batch** batch);
+int ct_matvec_batch_destroy(ct_matvec_batch* batch);
+/// total length of the concatenated vectors
+ct_long ct_matvec_batch_dim(const ct_matvec_batch* batch);
+/// offsets[0 .. nprob]: problem p occupies [offsets[p], offsets[p+1]) of the concatenated vectors
+int ct_matvec_batch_offsets(const ct_matvec_batch* batch, ct_long* offsets);
+/// algorithmic flops (reference GEMM count) and bytes of one batched matvec, summed over problems
+double ct_matvec_batch_flops(const ct_matvec_batch* batch);
+double ct_matvec_batch_bytes(const ct_matvec_batch* batch);
+/// form chosen by timing both at creation (1: one fused launch, 0: three grouped launches) and
+/// the measured ms per batched matvec of each ([0] grouped, [1] fused; 0 if not available)
+int ct_matvec_batch_form(const ct_matvec_batch* batch, int* fused, float* ms2);
+void* ct_matvec_batch_stream(ct_matvec_batch* batch);
+/// y_p = H_eff,p x_p for every problem, one launch
+int ct_matvec_batch_apply(ct_matvec_batch* batch, const double* d_x, double* d_y);
+/// Lanczos ground state of every problem (krylov.c:172-251, numeig = 1): lambda[nprob],
+/// status[nprob] (0, -1 breakdown before one iteration, -2 eigensolver failure; may be NULL),
+/// d_u the concatenated Ritz vectors. Returns 0 or the first nonzero status.
+int ct_lanczos_ground_state_batch(ct_matvec_batch* batch, const double* d_vstart, const int maxiter, double* lambda, int* status, double* d_u);
+
+//________________________________________________________________________________________________________________________
+//
+// serving loop (no counterpart in the reference): independent ground-state solves are admitted by
+// an HBM budget, stay device-resident, and the matvecs of all running solves of a Lanczos step are
+// applied in one batched launch (continuous batching)
+
+typedef struct ct_serve ct_serve;
+
+struct ct_serve_request
+{
+	int model;              ///< 0: spin-1/2 XXZ chain (param J, D, h), 1: Fermi-Hubbard chain (param t, U, mu)
+	int nsites;
+	double param[3];
+	qnumber sector;         ///< quantum number sector of the state (XXZ: 2 Sz; Hubbard: N * 65536 + 2 Sz, as encode_quantum_number_pair)
+	int pad;
+	ct_long max_vdim;       ///< maximum virtual bond dimension
+	int num_sweeps;         ///< maximum number of sweeps
+	int maxiter_lanczos;
+	double tol_split;       ///< SVD truncation tolerance
+	double tol_energy;      ///< stop after a sweep whose energy changed by less than this (0: always num_sweeps)
+	uint64_t seed;          ///< seed of the start state (construct_random_mps)
+};
+
+struct ct_serve_result
+{
+	int status;             ///< 0 done, 1 queued or running, < 0 failed
+	int sweeps;             ///< sweeps performed
+	double energy;          ///< energy of the last sweep
+	double energy_change;   ///< |E(last sweep) - E(previous sweep)| (NaN after one sweep)
+	double t_submit, t_admit, t_done;  ///< seconds since the server was created (-1: not yet)
+};
+
+/// start a server with a device-memory budget (bytes) for admitted requests and at most
+/// max_active requests in flight; the server owns a worker thread and a HIP stream
+int ct_serve_create(const double hbm_budget_bytes, const int max_active, ct_serve** server);
+/// queue a request; returns its ticket
+int ct_serve_submit(ct_serve* server, const struct ct_serve_request* req, int64_t* ticket);
+/// state of a request: returns 0 when it has finished (res->status 0 or < 0), 1 while queued or
+/// running; with wait != 0 blocks until it has finished
+int ct_serve_poll(ct_serve* server, const int64_t ticket, const int wait, struct ct_serve_result* res);
+/// energies of the finished sweeps of a request (up to capacity); returns their number
+int ct_serve_energies(ct_serve* server, const int64_t ticket, double* en_sweeps, const int capacity);
+/// [0] ticks, [1] batched Lanczos solves, [2] finished requests, [3] queued requests, [4] reserved
+/// bytes, [5] largest batch, [6] busy seconds of the worker, [7] seconds since creation
+int ct_serve_stats(ct_serve* server, double* stats8);
+/// stop the worker (queued and running requests are dropped) and free everything
+int ct_serve_destroy(ct_serve* server);
+
 /// include/util/krylov.h:10
 typedef void lanczos_linear_func_d(const ct_long n, const void* data, const double* v, double* ret);
 

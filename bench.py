@@ -100,7 +100,7 @@ def bench_batch(ct, nprob, seed, steps, warmup, lanczos_iter):
     log(f"batch of {nprob}: {ms:.4f} ms per batched matvec ({batch.flops / (ms * 1e-3) / 1e12:.3f} TFLOP/s), "
         f"Lanczos {lanczos_iter} steps {lz_ms:.3f} ms; generate {t_gen:.1f}s plan {t_plan:.2f}s")
     return {
-        "workload": f"{nprob} independent two-site H_eff matvecs (XXZ / Fermi-Hubbard, L 16-64, chi from {{64,128,256,512,1024}}) in one fused launch",
+        "workload": f"{nprob} independent two-site H_eff matvecs (XXZ / Fermi-Hubbard, L 16-64, chi from {{64,128,256,512,1024}}) in one batched launch (fused, or one grouped launch per stage)",
         "problems": [f"{m} L={L} chi={c}" for (m, L, c, _) in specs],
         "elements": int(batch.dim),
         "flops_per_batched_matvec": batch.flops,

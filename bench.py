@@ -315,7 +315,7 @@ def main():
         "plan_stats": plan.stats(),
         "autotune_grouped": {"forms": forms["grouped_fp64"]["plan"].forms()[0],
                              "ms_per_form": [[round(x, 5) for x in r] for r in forms["grouped_fp64"]["plan"].forms()[1]],
-                     "form_names": ["wave_simple", "wave_stream", "wg_chunk", "wg_stream"]},
+                     "form_names": ["wave_simple", "wave_stream", "wg_chunk", "wg_stream", "column_stream"]},
     }
     ys = {k: v["y"].numpy() for k, v in forms.items()}
     for k in ys:

@@ -226,6 +226,40 @@ DevTensor dev_split_axis(const DevTensor& t, int i_ax, const ct_long* new_dims, 
 //
 // matvec plan
 
+// Stage 2 as a column stream: for a block pair (A,C) the contraction T2(A,M,Q,C) = sum W T1(A,P,N,C)
+// applies one small matrix W_D (rows (M,Q,m,q) with qM + qQ = D, columns (P,N,p,n) with
+// qP + qN = D, D = qC + qo - qA) to every column (a,c) of the pair. A wavefront takes 64 columns:
+// it loads their K inputs (T1 read once), and writes their R outputs (T2 written once); W_D comes
+// from a dense, zero-padded copy through scalar loads.
+constexpr int WA_KMAX = 32;
+struct WaUnit { int64_t kb, rb, wrow; int dA, dC, K, R; };   // kb / rb index kbase / rbase; wrow indexes wmat rows
+struct WaItem { int unit, col0; };
+struct WApplyHost
+{
+	std::vector<WaUnit> units;
+	std::vector<WaItem> items;
+	std::vector<int64_t> kbase, rbase;   // element offset of column k / row r at a = 0, c = 0 (-1: absent block)
+	std::vector<int> kstride, rstride;   // element stride of a
+	std::vector<int64_t> widx;           // W_D entries as W element indices (-1: zero), WA_KMAX per row
+	int kmax = 0;                        // largest K of any class
+};
+struct WApply
+{
+	WaUnit* units = nullptr;
+	WaItem* items = nullptr;
+	int64_t* kbase = nullptr;
+	int64_t* rbase = nullptr;
+	int* kstride = nullptr;
+	int* rstride = nullptr;
+	double* wmat = nullptr;
+	int nitems = 0;
+	int kmax = 0;
+	bool ok = false;                     // every class fits WA_KMAX columns
+	void upload(const WApplyHost& h, const double* w, hipStream_t s);
+	void release();
+};
+void launch_wapply(const WApply& wa, const double* t1, double* t2, hipStream_t s);
+
 // Fused single-launch FP64 matvec. One workgroup owns a "unit": k consecutive columns c of one
 // right-bond sector C of the output. It stages the R columns it needs in LDS, computes
 // T1(A,P,N,C)[:,:,:,c] = theta(A,P,B) R(B,N,C)[:,:,c] and T2(A,M,Q,C)[:,:,:,c] = W T1 into LDS and
@@ -306,6 +340,8 @@ struct MatvecPlan
 	// grouped forms: three launches; stage 1 reads x, stage 3 writes y
 	DevGemm stage[3];
 	GemmBases stage_bases[3] = {};
+	WApply wapply;                              // column-stream form of stage 2
+	bool use_wapply = false;
 	// fused form: one launch
 	FusedPlan fused;
 	// plain form: the reference's operation sequence replayed as individual launches
@@ -328,6 +364,7 @@ struct MatvecPlan
 struct GroupedTables
 {
 	GemmBatch s1, s2, s3;
+	WApplyHost wa;
 	int64_t t1_elems = 0, t2_elems = 0;
 	double stage_bytes[3] = { 0, 0, 0 };
 };
@@ -336,7 +373,13 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 
 // contract checks of apply_local_hamiltonian (degrees, unit outer bonds, matching axes); throws HipError
 void validate_matvec_operands(const Meta& theta, const Meta& w, const Meta& l, const Meta& r);
-void autotune_grouped(DevGemm* stage, const GemmBases* bases, int64_t nx, hipStream_t s, float (*tuned_ms)[GEMM_NUM_FORMS]);
+// time every launch form of each grouped stage (and the column-stream stage 2 when available) and
+// keep the fastest; sets *use_wapply
+void autotune_grouped(DevGemm* stage, const GemmBases* bases, const WApply* wa, bool* use_wapply, int64_t nx, hipStream_t s,
+	float (*tuned_ms)[GEMM_NUM_FORMS]);
+// one grouped matvec: stage 1, stage 2 (MFMA tiles or column stream), stage 3
+void launch_grouped_stages(const DevGemm* stage, const GemmBases* bases, const WApply* wa, bool use_wapply, const double* x, double* y,
+	hipStream_t s, hipEvent_t* ev);
 
 // FP64 flops of the reference's unfused operation sequence: sum of 2 m n k over its GEMM calls
 double reference_flops(const Meta& th, const Meta& w, const Meta& l, const Meta& r);
@@ -378,6 +421,8 @@ struct BatchPlan
 	// grouped three-stage form over the concatenated operands
 	DevGemm stage[3];
 	GemmBases stage_bases[3] = {};
+	WApply wapply;
+	bool use_wapply = false;
 	double* t1 = nullptr;
 	double* t2 = nullptr;
 	bool use_fused = false;                     // the faster of the two forms, timed at plan creation

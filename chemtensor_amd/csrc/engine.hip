@@ -5,6 +5,7 @@
 // transposes folded into operand addressing) and plain (per-block launches) forms.
 
 #include <algorithm>
+#include <map>
 #include <cstring>
 #include <numeric>
 #include <unordered_map>
@@ -443,6 +444,7 @@ DevTensor dev_split_axis(const DevTensor& t, int i_ax, const ct_long* nd2, const
 MatvecPlan::~MatvecPlan()
 {
 	for (int i = 0; i < 3; i++) stage[i].release();
+	wapply.release();
 	fused.release();
 	for (void* p : owned) dev_free(p);
 	for (int i = 0; i < 4; i++) if (ev[i]) (void)hipEventDestroy(ev[i]);
@@ -546,6 +548,77 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 			s2.add_term(off_ptr(w_off + W.offset[W.flat_index(iw)]), off_ptr(t1_base + t1_off[t1_key(A, P, N, C)]), (int)(dP * dN), (int)(dP * dN), 1, (int)dC, 1, dP * dN * dC);
 		}
 	}
+	// column-stream form of stage 2: classes D = qM + qQ = qP + qN, one unit per block pair (A,C)
+	{
+		WApplyHost& wa = g.wa;
+		struct Cls { int64_t wrow = 0; int K = 0, R = 0; std::vector<std::pair<ct_long, ct_long>> rows, cols; };
+		std::map<qnumber, Cls> classes;
+		auto get_class = [&](qnumber D) -> const Cls& {
+			auto it = classes.find(D);
+			if (it != classes.end()) return it->second;
+			Cls c;
+			for (ct_long M = 0; M < nM; M++) for (ct_long Q = 0; Q < nQ; Q++)
+				if (qb(W, 0, M) + qb(W, 1, Q) == D) { c.rows.emplace_back(M, Q); c.R += (int)(bc(W, 0, M) * bc(W, 1, Q)); }
+			for (ct_long P = 0; P < nP; P++) for (ct_long N = 0; N < nN; N++)
+				if (qb(W, 2, P) + qb(W, 3, N) == D) { c.cols.emplace_back(P, N); c.K += (int)(bc(W, 2, P) * bc(W, 3, N)); }
+			c.wrow = (int64_t)(wa.widx.size() / WA_KMAX);
+			wa.kmax = std::max(wa.kmax, c.K);
+			if (c.K <= WA_KMAX) {
+				for (auto& mq : c.rows) {
+					const ct_long M = mq.first, Q = mq.second, dM = bc(W, 0, M), dQ = bc(W, 1, Q);
+					for (ct_long m = 0; m < dM; m++) for (ct_long q = 0; q < dQ; q++) {
+						std::vector<int64_t> row(WA_KMAX, -1);
+						int k = 0;
+						for (auto& pn : c.cols) {
+							const ct_long P = pn.first, N = pn.second, dP = bc(W, 2, P), dN = bc(W, 3, N);
+							const ct_long iw[4] = { M, Q, P, N };
+							const ct_long kw = W.flat_index(iw);
+							const bool pres = W.present(kw);
+							for (ct_long pp = 0; pp < dP; pp++) for (ct_long n = 0; n < dN; n++, k++)
+								if (pres) row[k] = w_off + W.offset[kw] + ((m * dQ + q) * dP + pp) * dN + n;
+						}
+						wa.widx.insert(wa.widx.end(), row.begin(), row.end());
+					}
+				}
+			}
+			return classes.emplace(D, std::move(c)).first->second;
+		};
+		for (ct_long A = 0; A < nA; A++) for (ct_long C = 0; C < nC; C++) {
+			const qnumber D = qb(R, 2, C) + qo - qb(th, 0, A);
+			const Cls& c = get_class(D);
+			bool any = false;
+			for (auto& mq : c.rows) if (t2_off[t2_key(A, mq.first, mq.second, C)] >= 0) { any = true; break; }
+			if (!any) continue;
+			const ct_long dA = bc(th, 0, A), dC = bc(R, 2, C);
+			WaUnit u;
+			u.kb = (int64_t)wa.kbase.size();
+			u.rb = (int64_t)wa.rbase.size();
+			u.wrow = c.wrow;
+			u.dA = (int)dA;
+			u.dC = (int)dC;
+			u.K = c.K;
+			u.R = c.R;
+			for (auto& pn : c.cols) {
+				const ct_long P = pn.first, N = pn.second, dP = bc(W, 2, P), dN = bc(W, 3, N);
+				const ct_long o = t1_off[t1_key(A, P, N, C)];
+				for (ct_long pp = 0; pp < dP; pp++) for (ct_long n = 0; n < dN; n++) {
+					wa.kbase.push_back(o >= 0 ? t1_base + o + (pp * dN + n) * dC : -1);
+					wa.kstride.push_back((int)(dP * dN * dC));
+				}
+			}
+			for (auto& mq : c.rows) {
+				const ct_long M = mq.first, Q = mq.second, dM = bc(W, 0, M), dQ = bc(W, 1, Q);
+				const ct_long o = t2_off[t2_key(A, M, Q, C)];
+				for (ct_long m = 0; m < dM; m++) for (ct_long q = 0; q < dQ; q++) {
+					wa.rbase.push_back(o >= 0 ? t2_base + o + (m * dQ + q) * dC : -1);
+					wa.rstride.push_back((int)(dM * dQ * dC));
+				}
+			}
+			const int unit = (int)wa.units.size();
+			wa.units.push_back(u);
+			for (int col0 = 0; col0 < (int)(dA * dC); col0 += 64) wa.items.push_back(WaItem{ unit, col0 });
+		}
+	}
 	// stage 3: y(X,Q,C)[x q c] = sum_(A,M) L(O,A,M,X)^T T2(A,M,Q,C)
 	ct_long iy[3];
 	for (ct_long k = 0; k < th.nblocks; k++) {
@@ -591,6 +664,7 @@ static void build_grouped(MatvecPlan& p)
 	p.stage_bases[0] = GemmBases{ nullptr, p.r.d, p.interm[0].d };
 	p.stage_bases[1] = GemmBases{ p.w.d, p.interm[0].d, p.interm[1].d };
 	p.stage_bases[2] = GemmBases{ p.l.d, p.interm[1].d, nullptr };
+	if (p.mode == CT_MATVEC_GROUPED_FP64) p.wapply.upload(g.wa, p.w.d, p.stream);
 	p.nterms = g.s1.terms.size() + g.s2.terms.size() + g.s3.terms.size();
 	p.ntasks = g.s1.tasks.size() + g.s2.tasks.size() + g.s3.tasks.size();
 	p.nitems = p.stage[0].nitems + p.stage[1].nitems + p.stage[2].nitems;
@@ -692,8 +766,9 @@ static void build_plain(MatvecPlan& p)
 // Plan-time autotuner: every launch form of every stage runs on the plan's own operands (one
 // warm-up and three timed launches, HIP events on the plan stream); the fastest form is kept.
 // time every launch form of each grouped stage on the plan's own descriptors (x = 1, y scratch)
-// and keep the fastest
-void autotune_grouped(DevGemm* stage, const GemmBases* bases, int64_t nx, hipStream_t s, float (*tuned_ms)[GEMM_NUM_FORMS])
+// and keep the fastest; stage 2 also tries the column-stream form
+void autotune_grouped(DevGemm* stage, const GemmBases* bases, const WApply* wa, bool* use_wapply, int64_t nx, hipStream_t s,
+	float (*tuned_ms)[GEMM_NUM_FORMS])
 {
 	double* x = (double*)dev_alloc((size_t)std::max<int64_t>(nx, 1) * sizeof(double));
 	double* y = (double*)dev_alloc((size_t)std::max<int64_t>(nx, 1) * sizeof(double));
@@ -701,6 +776,17 @@ void autotune_grouped(DevGemm* stage, const GemmBases* bases, int64_t nx, hipStr
 	hipEvent_t e0, e1;
 	CTG_HIP(hipEventCreate(&e0));
 	CTG_HIP(hipEventCreate(&e1));
+	auto time3 = [&](const std::function<void()>& f) {
+		f();
+		CTG_HIP(hipEventRecord(e0, s));
+		for (int rep = 0; rep < 3; rep++) f();
+		CTG_HIP(hipEventRecord(e1, s));
+		CTG_HIP(hipEventSynchronize(e1));
+		float ms = 0;
+		CTG_HIP(hipEventElapsedTime(&ms, e0, e1));
+		return ms / 3;
+	};
+	float best2 = 1e30f;
 	for (int st = 0; st < 3; st++) {
 		DevGemm& g = stage[st];
 		GemmBases b = bases[st];
@@ -710,17 +796,23 @@ void autotune_grouped(DevGemm* stage, const GemmBases* bases, int64_t nx, hipStr
 		int best_form = g.form;
 		for (int f = 0; f < GEMM_NUM_FORMS; f++) {
 			g.form = f;
-			launch_grouped_gemm_f64(g, b, s);
-			CTG_HIP(hipEventRecord(e0, s));
-			for (int rep = 0; rep < 3; rep++) launch_grouped_gemm_f64(g, b, s);
-			CTG_HIP(hipEventRecord(e1, s));
-			CTG_HIP(hipEventSynchronize(e1));
-			float ms = 0;
-			CTG_HIP(hipEventElapsedTime(&ms, e0, e1));
-			if (tuned_ms) tuned_ms[st][f] = ms / 3;
+			const float ms = time3([&] { launch_grouped_gemm_f64(g, b, s); });
+			if (tuned_ms) tuned_ms[st][f] = ms;
 			if (ms < best) { best = ms; best_form = f; }
 		}
 		g.form = best_form;
+		if (st == 1) best2 = best;
+	}
+	if (use_wapply) {
+		*use_wapply = false;
+		if (wa && wa->ok) {
+			const float ms = time3([&] { launch_wapply(*wa, bases[1].B, bases[1].C, s); });
+			*use_wapply = ms < best2;
+			// CT_STAGE2=mfma|stream overrides the timing-based choice
+			const char* force = getenv("CT_STAGE2");
+			if (force && strcmp(force, "mfma") == 0) *use_wapply = false;
+			if (force && strcmp(force, "stream") == 0) *use_wapply = true;
+		}
 	}
 	CTG_HIP(hipEventDestroy(e0));
 	CTG_HIP(hipEventDestroy(e1));
@@ -729,9 +821,22 @@ void autotune_grouped(DevGemm* stage, const GemmBases* bases, int64_t nx, hipStr
 	dev_free(y);
 }
 
+void launch_grouped_stages(const DevGemm* stage, const GemmBases* bases, const WApply* wa, bool use_wapply, const double* x, double* y,
+	hipStream_t s, hipEvent_t* ev)
+{
+	for (int st = 0; st < 3; st++) {
+		GemmBases b = bases[st];
+		if (st == 0) b.A = x;
+		if (st == 2) b.C = y;
+		if (st == 1 && use_wapply && wa && wa->ok) launch_wapply(*wa, b.B, b.C, s);
+		else launch_grouped_gemm_f64(stage[st], b, s);
+		if (ev) CTG_HIP(hipEventRecord(ev[st + 1], s));
+	}
+}
+
 static void autotune_stages(MatvecPlan& p)
 {
-	autotune_grouped(p.stage, p.stage_bases, p.theta_meta.nelem, p.stream, p.tuned_ms);
+	autotune_grouped(p.stage, p.stage_bases, &p.wapply, &p.use_wapply, p.theta_meta.nelem, p.stream, p.tuned_ms);
 }
 
 void validate_matvec_operands(const Meta& theta, const Meta& w, const Meta& l, const Meta& r)
@@ -801,16 +906,16 @@ void matvec_apply(MatvecPlan& p, const double* x, double* y, float* stage_ms)
 			if (stage_ms) CTG_HIP(hipEventRecord(p.ev[st + 1], s));
 		}
 	}
-	else {
+	else if (p.mode == CT_MATVEC_GROUPED_SPLIT) {
 		for (int st = 0; st < 3; st++) {
 			GemmBases b = p.stage_bases[st];
 			if (st == 0) b.A = x;
 			if (st == 2) b.C = y;
-			if (p.mode == CT_MATVEC_GROUPED_SPLIT) launch_grouped_gemm_split(p.stage[st], b, s);
-			else launch_grouped_gemm_f64(p.stage[st], b, s);
+			launch_grouped_gemm_split(p.stage[st], b, s);
 			if (stage_ms) CTG_HIP(hipEventRecord(p.ev[st + 1], s));
 		}
 	}
+	else launch_grouped_stages(p.stage, p.stage_bases, &p.wapply, p.use_wapply, x, y, s, stage_ms ? p.ev : nullptr);
 	if (stage_ms) {
 		CTG_HIP(hipEventSynchronize(p.ev[3]));
 		for (int st = 0; st < 3; st++) CTG_HIP(hipEventElapsedTime(&stage_ms[st], p.ev[st], p.ev[st + 1]));

@@ -522,6 +522,7 @@ BatchPlan::~BatchPlan()
 {
 	fused.release();
 	for (int i = 0; i < 3; i++) stage[i].release();
+	wapply.release();
 	dev_free(t1);
 	dev_free(t2);
 	dev_free(w);
@@ -571,13 +572,15 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 	b->stage_bases[0] = GemmBases{ nullptr, b->r, b->t1 };
 	b->stage_bases[1] = GemmBases{ b->w, b->t1, b->t2 };
 	b->stage_bases[2] = GemmBases{ b->l, b->t2, nullptr };
+	b->wapply.upload(g.wa, b->w, s);
 	if (tune) {
-		autotune_grouped(b->stage, b->stage_bases, nx, s, nullptr);
+		autotune_grouped(b->stage, b->stage_bases, &b->wapply, &b->use_wapply, nx, s, nullptr);
 		b->fused_ok = build_fused_batch(b->fused, probs, s);
 	}
 	else {
 		// one fixed launch form: the summation order of a problem does not depend on the batch
 		for (int i = 0; i < 3; i++) b->stage[i].form = GEMM_WAVE_STREAM;
+		b->use_wapply = b->wapply.ok;
 	}
 	// reduction blocks of the batched Lanczos: up to 64 blocks of >= 4096 elements per problem
 	std::vector<LzSeg> segs;
@@ -633,12 +636,7 @@ void batch_apply(BatchPlan& b, const double* x, double* y)
 		launch_fused_matvec(b.fused, x, b.w, b.l, b.r, y, b.stream);
 		return;
 	}
-	for (int st = 0; st < 3; st++) {
-		GemmBases g = b.stage_bases[st];
-		if (st == 0) g.A = x;
-		if (st == 2) g.C = y;
-		launch_grouped_gemm_f64(b.stage[st], g, b.stream);
-	}
+	launch_grouped_stages(b.stage, b.stage_bases, &b.wapply, b.use_wapply, x, y, b.stream, nullptr);
 }
 
 }  // namespace ctg

@@ -616,4 +616,102 @@ void launch_fill(double* p, double v, ct_long n, hipStream_t s)
 	CTG_HIP(hipGetLastError());
 }
 
+//________________________________________________________________________________________________________________________
+//
+// column-stream stage 2: T2 = W_D T1 on the columns (a, c) of a block pair
+
+template <int KM>
+__global__ __launch_bounds__(256) void k_wapply(const WaUnit* __restrict__ units, const WaItem* __restrict__ items, int nitems,
+	const int64_t* __restrict__ kbase, const int* __restrict__ kstride, const int64_t* __restrict__ rbase, const int* __restrict__ rstride,
+	const double* __restrict__ wmat, const double* __restrict__ t1, double* __restrict__ t2)
+{
+	// the item index is the same for the 64 lanes of a wavefront: descriptor loads go through the scalar cache
+	const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+	if (w >= nitems) return;
+	const int lane = threadIdx.x & 63;
+	const WaItem it = items[w];
+	const WaUnit u = units[it.unit];
+	const int j = it.col0 + lane;
+	const bool act = j < u.dA * u.dC;
+	const int a = act ? j / u.dC : 0, c = act ? j - (j / u.dC) * u.dC : 0;
+	double x[KM];
+	#pragma unroll
+	for (int k = 0; k < KM; k++) {
+		double v = 0.0;
+		if (k < u.K) {
+			const int64_t b = kbase[u.kb + k];
+			if (act && b >= 0) v = t1[b + (int64_t)a * kstride[u.kb + k] + c];
+		}
+		x[k] = v;
+	}
+	const double* __restrict__ wm = wmat + u.wrow * WA_KMAX;
+	for (int r = 0; r < u.R; r++) {
+		double y = 0.0;
+		#pragma unroll
+		for (int k = 0; k < KM; k++) y = fma(wm[(int64_t)r * WA_KMAX + k], x[k], y);
+		const int64_t b = rbase[u.rb + r];
+		if (act && b >= 0) t2[b + (int64_t)a * rstride[u.rb + r] + c] = y;
+	}
+}
+
+// dense W_D copies: wmat[i] = W[widx[i]] (0 where widx < 0)
+__global__ __launch_bounds__(256) void k_gather_w(const double* __restrict__ w, const int64_t* __restrict__ widx, int64_t n, double* __restrict__ wmat)
+{
+	for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+		const int64_t q = widx[i];
+		wmat[i] = q >= 0 ? w[q] : 0.0;
+	}
+}
+
+template <typename T>
+static T* upload_array(const std::vector<T>& v, hipStream_t s)
+{
+	T* d = (T*)dev_alloc(std::max<size_t>(1, v.size()) * sizeof(T));
+	if (!v.empty()) CTG_HIP(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+	return d;
+}
+
+void WApply::upload(const WApplyHost& h, const double* w, hipStream_t s)
+{
+	release();
+	ok = h.kmax <= WA_KMAX;
+	if (!ok) return;
+	units = upload_array(h.units, s);
+	items = upload_array(h.items, s);
+	kbase = upload_array(h.kbase, s);
+	rbase = upload_array(h.rbase, s);
+	kstride = upload_array(h.kstride, s);
+	rstride = upload_array(h.rstride, s);
+	int64_t* widx = upload_array(h.widx, s);
+	const int64_t n = (int64_t)h.widx.size();
+	wmat = (double*)dev_alloc((size_t)std::max<int64_t>(n, 1) * sizeof(double));
+	if (n > 0) {
+		hipLaunchKernelGGL(k_gather_w, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0, s, w, (const int64_t*)widx, n, wmat);
+		CTG_HIP(hipGetLastError());
+	}
+	CTG_HIP(hipStreamSynchronize(s));
+	dev_free(widx);
+	nitems = (int)h.items.size();
+	kmax = h.kmax;
+}
+
+void WApply::release()
+{
+	dev_free(units); dev_free(items); dev_free(kbase); dev_free(rbase); dev_free(kstride); dev_free(rstride); dev_free(wmat);
+	units = nullptr; items = nullptr; kbase = nullptr; rbase = nullptr; kstride = nullptr; rstride = nullptr; wmat = nullptr;
+	nitems = 0;
+	ok = false;
+}
+
+void launch_wapply(const WApply& wa, const double* t1, double* t2, hipStream_t s)
+{
+	if (wa.nitems == 0) return;
+	const dim3 grid((unsigned)((wa.nitems + 3) / 4));
+	if (wa.kmax <= 16)
+		hipLaunchKernelGGL(k_wapply<16>, grid, dim3(256), 0, s, wa.units, wa.items, wa.nitems, wa.kbase, wa.kstride, wa.rbase, wa.rstride, wa.wmat, t1, t2);
+	else
+		hipLaunchKernelGGL(k_wapply<32>, grid, dim3(256), 0, s, wa.units, wa.items, wa.nitems, wa.kbase, wa.kstride, wa.rbase, wa.rstride, wa.wmat, t1, t2);
+	CTG_HIP(hipGetLastError());
+}
+
 }  // namespace ctg

@@ -170,7 +170,8 @@ int ct_matvec_plan_stats(const ct_matvec_plan* plan, double* stats4);
 int ct_matvec_plan_stage_flops(const ct_matvec_plan* plan, double* flops3);
 /// per-stage algorithmic bytes: 8 (|inputs| + |output|) of each stage's launch
 int ct_matvec_plan_stage_bytes(const ct_matvec_plan* plan, double* bytes3);
-/// launch form chosen per stage (0 wave/simple, 1 wave/stream, 2 workgroup/chunk, 3 workgroup/stream)
+/// launch form chosen per stage (0 wave/simple, 1 wave/stream, 2 workgroup/chunk, 3 workgroup/stream,
+/// 4 column stream of stage 2)
 /// and the autotuner's time (ms) of every form, [stage * 4 + form]; 0 where not timed
 int ct_matvec_plan_forms(const ct_matvec_plan* plan, int* forms3, float* tuned_ms12);
 /// y = H_eff x on device vectors (packed in serialize order) on the plan's stream

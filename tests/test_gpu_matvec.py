@@ -113,6 +113,24 @@ def test_matvec_all_forms_match_oracle(model, L, chi):
         assert np.array_equal(y.numpy(), got)
 
 
+@pytest.mark.parametrize("stage2", ["mfma", "stream"])
+def test_grouped_stage2_forms_match_oracle(stage2, monkeypatch):
+    """Both stage-2 forms of the grouped matvec (MFMA tiles, column stream) against the oracle."""
+    monkeypatch.setenv("CT_STAGE2", stage2)
+    for model, L, chi in [("hubbard", 32, 512), ("xxz", 32, 256), ("hubbard", 8, 24)]:
+        theta, h2, left, right, _ = orc.matvec_problem(model, L, chi, 4242)
+        want = orc.apply_local_hamiltonian(theta, h2, left, right).serialize()
+        plan = ct.MatvecPlan(theta, h2, left, right, ct.GROUPED_FP64)
+        assert (plan.forms()[0][1] == 4) == (stage2 == "stream")
+        x = ct.DeviceArray.from_numpy(theta.serialize())
+        y = ct.DeviceArray(plan.dim)
+        plan.apply(x, y)
+        plan.synchronize()
+        err = rel(y.numpy(), want)
+        say("  stage2", stage2, model, L, chi, "rel err", err)
+        assert err < TOL
+
+
 def test_apply_local_hamiltonian_dropin_with_reference_structs():
     """The oracle's structs (reference layout) go straight into the product's drop-in entry point."""
     theta, h2, left, right, _ = orc.matvec_problem("hubbard", 16, 48, 77)

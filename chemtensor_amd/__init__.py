@@ -84,6 +84,7 @@ def lib():
             "ct_matvec_plan_stage_bytes": (C.c_int, [P, DP]),
             "ct_matvec_plan_forms": (C.c_int, [P, IP, FP]),
             "ct_matvec_apply": (C.c_int, [P, P, P]),
+            "ct_matvec_plan_stage_time": (C.c_int, [P, C.c_int, P, P, C.c_int, FP]),
             "ct_matvec_apply_timed": (C.c_int, [P, P, P, FP]),
             "ct_matvec_plan_stream": (P, [P]),
             "ct_lanczos_ground_state": (C.c_int, [P, P, C.c_int, DP, P]),
@@ -351,6 +352,12 @@ class MatvecPlan:
     def synchronize(self):
         check(lib().ct_stream_synchronize(self.stream), "ct_stream_synchronize")
 
+    def stage_time(self, stage, x, y, reps=50):
+        """average ms of one stage's launch over reps back-to-back launches"""
+        ms = C.c_float(0)
+        check(lib().ct_matvec_plan_stage_time(self.h, stage, x.ptr, y.ptr, reps, C.byref(ms)), "ct_matvec_plan_stage_time")
+        return ms.value
+
     def lanczos(self, vstart, maxiter):
         lam = C.c_double(0)
         u = DeviceArray(self.dim)
@@ -454,9 +461,10 @@ class Server:
         return buf[:max(n, 0)].copy()
 
     def stats(self):
-        st = np.zeros(8)
+        st = np.zeros(13)
         lib().ct_serve_stats(self.h, st.ctypes.data_as(DP))
-        keys = ["ticks", "lanczos_batches", "completed", "queued", "reserved_bytes", "max_batch", "busy_s", "uptime_s"]
+        keys = ["ticks", "lanczos_batches", "completed", "queued", "reserved_bytes", "max_batch", "busy_s", "uptime_s",
+                "start_s", "plan_s", "lanczos_s", "split_device_s", "split_host_s"]
         return dict(zip(keys, st.tolist()))
 
     def close(self):

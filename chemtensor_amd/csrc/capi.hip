@@ -140,6 +140,18 @@ int ct_matvec_plan_forms(const ct_matvec_plan* plan, int* forms3, float* tuned_m
 	return 0;
 }
 
+int ct_matvec_plan_stage_time(ct_matvec_plan* plan, const int stage, const double* d_x, double* d_y, const int reps, float* ms)
+{
+	if (!plan || stage < 0 || stage > 2 || reps < 1 || !ms) return CT_ERR_ARG;
+	try {
+		*ms = matvec_stage_time(*plan->p, stage, d_x, d_y, reps);
+		return 0;
+	}
+	catch (const HipError& e) {
+		return map_error(e);
+	}
+}
+
 int ct_matvec_plan_stats(const ct_matvec_plan* plan, double* stats4)
 {
 	const MatvecPlan& p = *plan->p;

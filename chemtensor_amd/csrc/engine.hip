@@ -891,6 +891,36 @@ MatvecPlan* build_matvec_plan(const block_sparse_tensor* a, const block_sparse_t
 	return build_matvec_plan_dev(th, std::move(dw), std::move(dl), std::move(dr), mode, s, autotune);
 }
 
+float matvec_stage_time(MatvecPlan& p, int stage, const double* x, double* y, int reps)
+{
+	hipStream_t s = p.stream;
+	auto launch = [&] {
+		if (p.mode == CT_MATVEC_FUSED_FP64) {
+			if (stage == 0) launch_fused_matvec(p.fused, x, p.w.d, p.l.d, p.r.d, y, s);
+			return;
+		}
+		if (p.mode == CT_MATVEC_PLAIN_FP64) {
+			const int i0 = stage == 0 ? 0 : p.plain_stage_end[stage - 1];
+			for (int i = i0; i < p.plain_stage_end[stage]; i++) p.plain_ops[i](x, y);
+			return;
+		}
+		GemmBases b = p.stage_bases[stage];
+		if (stage == 0) b.A = x;
+		if (stage == 2) b.C = y;
+		if (p.mode == CT_MATVEC_GROUPED_SPLIT) launch_grouped_gemm_split(p.stage[stage], b, s);
+		else if (stage == 1 && p.use_wapply && p.wapply.ok) launch_wapply(p.wapply, b.B, b.C, s);
+		else launch_grouped_gemm_f64(p.stage[stage], b, s);
+	};
+	launch();
+	CTG_HIP(hipEventRecord(p.ev[0], s));
+	for (int r = 0; r < reps; r++) launch();
+	CTG_HIP(hipEventRecord(p.ev[1], s));
+	CTG_HIP(hipEventSynchronize(p.ev[1]));
+	float ms = 0;
+	CTG_HIP(hipEventElapsedTime(&ms, p.ev[0], p.ev[1]));
+	return ms / reps;
+}
+
 void matvec_apply(MatvecPlan& p, const double* x, double* y, float* stage_ms)
 {
 	hipStream_t s = p.stream;

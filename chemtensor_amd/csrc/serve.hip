@@ -70,6 +70,9 @@ struct Server
 	int host_threads = 8;
 	int64_t ticks = 0, lanczos_batches = 0, completed = 0, max_batch = 0;
 	double busy_s = 0;
+	// seconds spent in each part of the ticks: start of requests, batch plans, batched Lanczos,
+	// device part of the splits before / after the host SVDs, host SVDs
+	double t_start = 0, t_plan = 0, t_lanczos = 0, t_split_dev = 0, t_split_host = 0;
 	hipStream_t st = nullptr;
 	std::chrono::steady_clock::time_point t0;
 	std::thread worker;
@@ -123,7 +126,10 @@ struct Server
 			in[i].l = DevTensor::view(S.lb[S.bond], S.lb[S.bond].meta);
 			in[i].r = DevTensor::view(S.rb[S.bond + 1], S.rb[S.bond + 1].meta);
 		}
+		double c0 = now();
 		std::unique_ptr<BatchPlan> b(build_batch_plan(in, st, false));
+		double c1 = now();
+		t_plan += c1 - c0;
 		const int64_t ntot = b->off.back();
 		double* v0 = (double*)dev_alloc((size_t)std::max<int64_t>(ntot, 1) * sizeof(double));
 		double* u = (double*)dev_alloc((size_t)std::max<int64_t>(ntot, 1) * sizeof(double));
@@ -135,6 +141,8 @@ struct Server
 		std::vector<int> status(grp.size());
 		lanczos_ground_state_batch(*b, v0, maxiter, lam.data(), status.data(), u, false);
 		lanczos_batches++;
+		c0 = now();
+		t_lanczos += c0 - c1;
 		max_batch = std::max<int64_t>(max_batch, (int64_t)grp.size());
 		// device part of the splits
 		for (size_t i = 0; i < grp.size(); i++) {
@@ -147,6 +155,8 @@ struct Server
 		dev_free(v0);
 		dev_free(u);
 		b.reset();
+		c1 = now();
+		t_split_dev += c1 - c0;
 		// host block SVDs in parallel (no HIP calls in advance_host)
 		{
 			const int nt = std::max(1, std::min<int>(host_threads, (int)grp.size()));
@@ -158,6 +168,8 @@ struct Server
 				});
 			for (auto& x : th) x.join();
 		}
+		c0 = now();
+		t_split_host += c0 - c1;
 		for (size_t i = 0; i < grp.size(); i++) {
 			Ticket& t = *grp[i];
 			if (status[i] != 0) { finish(t, status[i]); continue; }
@@ -174,6 +186,7 @@ struct Server
 				finish(t, 0);
 			}
 		}
+		t_split_dev += now() - c0;
 	}
 
 	void loop()
@@ -202,6 +215,7 @@ struct Server
 				if (ret != 0) finish(*t, ret);
 				else active.push_back(t);
 			}
+			t_start += now() - tick0;
 			if (!active.empty()) {
 				std::map<int, std::vector<std::shared_ptr<Ticket>>> groups;
 				for (auto& t : active) groups[t->req.maxiter_lanczos].push_back(t);
@@ -310,9 +324,10 @@ int ct_serve_energies(ct_serve* server, const int64_t ticket, double* en_sweeps,
 	return n;
 }
 
-int ct_serve_stats(ct_serve* server, double* stats8)
+int ct_serve_stats(ct_serve* server, double* stats13)
 {
-	if (!server || !stats8) return CT_ERR_ARG;
+	if (!server || !stats13) return CT_ERR_ARG;
+	double* stats8 = stats13;
 	Server& S = *server->s;
 	std::lock_guard<std::mutex> g(S.mu);
 	stats8[0] = (double)S.ticks;
@@ -323,6 +338,11 @@ int ct_serve_stats(ct_serve* server, double* stats8)
 	stats8[5] = (double)S.max_batch;
 	stats8[6] = S.busy_s;
 	stats8[7] = S.now();
+	stats13[8] = S.t_start;
+	stats13[9] = S.t_plan;
+	stats13[10] = S.t_lanczos;
+	stats13[11] = S.t_split_dev;
+	stats13[12] = S.t_split_host;
 	return 0;
 }
 

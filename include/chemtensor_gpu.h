@@ -174,6 +174,9 @@ int ct_matvec_plan_stage_bytes(const ct_matvec_plan* plan, double* bytes3);
 /// 4 column stream of stage 2)
 /// and the autotuner's time (ms) of every form, [stage * 4 + form]; 0 where not timed
 int ct_matvec_plan_forms(const ct_matvec_plan* plan, int* forms3, float* tuned_ms12);
+/// average device time (ms) of one stage (0, 1, 2; the fused form has only stage 0) over 'reps'
+/// back-to-back launches on the given vectors: the per-launch duration of that stage's kernel
+int ct_matvec_plan_stage_time(ct_matvec_plan* plan, const int stage, const double* d_x, double* d_y, const int reps, float* ms);
 /// y = H_eff x on device vectors (packed in serialize order) on the plan's stream
 int ct_matvec_apply(ct_matvec_plan* plan, const double* d_x, double* d_y);
 /// as ct_matvec_apply, recording the device time (ms) of each of the three stages
@@ -254,8 +257,10 @@ int ct_serve_poll(ct_serve* server, const int64_t ticket, const int wait, struct
 /// energies of the finished sweeps of a request (up to capacity); returns their number
 int ct_serve_energies(ct_serve* server, const int64_t ticket, double* en_sweeps, const int capacity);
 /// [0] ticks, [1] batched Lanczos solves, [2] finished requests, [3] queued requests, [4] reserved
-/// bytes, [5] largest batch, [6] busy seconds of the worker, [7] seconds since creation
-int ct_serve_stats(ct_serve* server, double* stats8);
+/// bytes, [5] largest batch, [6] busy seconds of the worker, [7] seconds since creation; seconds
+/// spent in [8] starting requests, [9] batch plans, [10] batched Lanczos, [11] device part of the
+/// splits and environment updates, [12] host block SVDs
+int ct_serve_stats(ct_serve* server, double* stats13);
 /// stop the worker (queued and running requests are dropped) and free everything
 int ct_serve_destroy(ct_serve* server);
 

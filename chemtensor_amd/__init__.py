@@ -461,10 +461,10 @@ class Server:
         return buf[:max(n, 0)].copy()
 
     def stats(self):
-        st = np.zeros(13)
+        st = np.zeros(14)
         lib().ct_serve_stats(self.h, st.ctypes.data_as(DP))
         keys = ["ticks", "lanczos_batches", "completed", "queued", "reserved_bytes", "max_batch", "busy_s", "uptime_s",
-                "start_s", "plan_s", "lanczos_s", "split_device_s", "split_host_s"]
+                "start_s", "plan_s", "lanczos_s", "split_device_sum_s", "split_host_sum_s", "splits_wall_s"]
         return dict(zip(keys, st.tolist()))
 
     def close(self):

@@ -5,6 +5,7 @@
 
 #include <map>
 #include <mutex>
+#include <thread>
 #include <cstring>
 
 #include "ct_internal.h"
@@ -32,10 +33,14 @@ hipStream_t default_stream()
 
 namespace {
 
+// Freed blocks are cached per host thread and handed out again only to the thread that freed
+// them. A thread's device work is ordered on the stream it uses, and callers that work on several
+// threads synchronise their stream at the end of each task, so a block never returns to use while
+// another stream may still read it.
 struct Pool
 {
 	std::mutex mu;
-	std::multimap<size_t, void*> free_blocks;
+	std::map<std::thread::id, std::multimap<size_t, void*>> free_blocks;
 	std::map<void*, size_t> live;
 	size_t bytes_live = 0, bytes_cached = 0;
 };
@@ -63,13 +68,14 @@ void* dev_alloc(size_t nbytes)
 	const size_t sz = round_size(nbytes);
 	{
 		std::lock_guard<std::mutex> g(p.mu);
-		auto it = p.free_blocks.lower_bound(sz);
-		if (it != p.free_blocks.end() && it->first <= sz + sz / 4) {
+		auto& fb = p.free_blocks[std::this_thread::get_id()];
+		auto it = fb.lower_bound(sz);
+		if (it != fb.end() && it->first <= sz + sz / 4) {
 			void* ptr = it->second;
 			p.live[ptr] = it->first;
 			p.bytes_live += it->first;
 			p.bytes_cached -= it->first;
-			p.free_blocks.erase(it);
+			fb.erase(it);
 			return ptr;
 		}
 	}
@@ -94,7 +100,7 @@ void dev_free(void* ptr)
 	std::lock_guard<std::mutex> g(p.mu);
 	auto it = p.live.find(ptr);
 	if (it == p.live.end()) return;
-	p.free_blocks.emplace(it->second, ptr);
+	p.free_blocks[std::this_thread::get_id()].emplace(it->second, ptr);
 	p.bytes_cached += it->second;
 	p.bytes_live -= it->second;
 	p.live.erase(it);
@@ -104,9 +110,13 @@ void dev_trim()
 {
 	Pool& p = pool();
 	std::lock_guard<std::mutex> g(p.mu);
-	if (!p.free_blocks.empty()) (void)hipDeviceSynchronize();
-	for (auto& kv : p.free_blocks) (void)hipFree(kv.second);
-	p.free_blocks.clear();
+	bool any = false;
+	for (auto& kv : p.free_blocks) any = any || !kv.second.empty();
+	if (any) (void)hipDeviceSynchronize();
+	for (auto& kv : p.free_blocks) {
+		for (auto& b : kv.second) (void)hipFree(b.second);
+		kv.second.clear();
+	}
 	p.bytes_cached = 0;
 }
 

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -53,10 +54,73 @@ size_t estimate_bytes(const ct_serve_request& r)
 	return (size_t)(8.0 * el);
 }
 
+// persistent host threads, each with its own HIP stream; run() hands out task indices and
+// returns when all tasks have finished (every task ends with its stream synchronised)
+struct HostPool
+{
+	std::vector<std::thread> th;
+	std::mutex mu;
+	std::condition_variable cv, cv_done;
+	std::function<void(int, hipStream_t)> fn;
+	int ntasks = 0, next = 0, done = 0;
+	int64_t gen = 0;
+	bool stop = false;
+
+	void start(int n)
+	{
+		for (int i = 0; i < n; i++)
+			th.emplace_back([this] {
+				hipStream_t s = nullptr;
+				if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+				int64_t seen = 0;
+				for (;;) {
+					std::unique_lock<std::mutex> g(mu);
+					cv.wait(g, [&] { return stop || gen != seen; });
+					if (stop) break;
+					seen = gen;
+					while (next < ntasks) {
+						const int i = next++;
+						g.unlock();
+						fn(i, s);
+						if (s) (void)hipStreamSynchronize(s);
+						g.lock();
+						if (++done == ntasks) cv_done.notify_all();
+					}
+				}
+				if (s) (void)hipStreamDestroy(s);
+			});
+	}
+
+	void run(int n, const std::function<void(int, hipStream_t)>& f)
+	{
+		if (n <= 0) return;
+		std::unique_lock<std::mutex> g(mu);
+		fn = f;
+		ntasks = n;
+		next = 0;
+		done = 0;
+		gen++;
+		cv.notify_all();
+		cv_done.wait(g, [&] { return done == ntasks; });
+	}
+
+	void shutdown()
+	{
+		{
+			std::lock_guard<std::mutex> g(mu);
+			stop = true;
+		}
+		cv.notify_all();
+		for (auto& t : th) t.join();
+		th.clear();
+	}
+};
+
 }  // namespace
 
 struct Server
 {
+	HostPool hosts;
 	std::mutex mu;
 	std::condition_variable cv_work, cv_done;
 	std::deque<std::shared_ptr<Ticket>> pending;
@@ -72,7 +136,7 @@ struct Server
 	double busy_s = 0;
 	// seconds spent in each part of the ticks: start of requests, batch plans, batched Lanczos,
 	// device part of the splits before / after the host SVDs, host SVDs
-	double t_start = 0, t_plan = 0, t_lanczos = 0, t_split_dev = 0, t_split_host = 0;
+	double t_start = 0, t_plan = 0, t_lanczos = 0, t_split_dev = 0, t_split_host = 0, t_splits = 0;
 	hipStream_t st = nullptr;
 	std::chrono::steady_clock::time_point t0;
 	std::thread worker;
@@ -91,7 +155,7 @@ struct Server
 		cv_done.notify_all();
 	}
 
-	int start_request(Ticket& t)
+	int start_request(Ticket& t, hipStream_t s)
 	{
 		const ct_serve_request& r = t.req;
 		mpo H = {};
@@ -102,7 +166,7 @@ struct Server
 		seed_rng_state(r.seed, &rng);
 		construct_random_mps(CT_DOUBLE_REAL, r.nsites, H.d, H.qsite, r.sector, r.max_vdim, &rng, &psi);
 		t.state.reset(new DmrgState());
-		t.state->st = st;
+		t.state->st = s;
 		int ret = CT_ERR_HIP;
 		try {
 			ret = t.state->init(&H, &psi, r.num_sweeps, r.maxiter_lanczos, r.tol_split, r.max_vdim, r.tol_energy);
@@ -144,49 +208,46 @@ struct Server
 		c0 = now();
 		t_lanczos += c0 - c1;
 		max_batch = std::max<int64_t>(max_batch, (int64_t)grp.size());
-		// device part of the splits
-		for (size_t i = 0; i < grp.size(); i++) {
-			if (status[i] != 0) continue;
+		// splits of all requests in parallel host threads, each on its own stream: device reshape
+		// and download, host block SVD, upload and environment update
+		std::vector<int> ret(grp.size(), 0);
+		std::vector<double> t_dev(grp.size(), 0.0), t_host(grp.size(), 0.0);
+		hosts.run((int)grp.size(), [&](int i, hipStream_t s) {
+			if (status[i] != 0) { ret[i] = status[i]; return; }
 			DmrgState& S = *grp[i]->state;
-			DevTensor a_opt(S.theta.meta);
-			if (S.theta.meta.nelem) launch_copy(u + b->off[i], a_opt.d, S.theta.meta.nelem, st);
-			S.advance_begin(a_opt);
-		}
+			try {
+				S.st = s;
+				const double a0 = now();
+				{
+					DevTensor a_opt(S.theta.meta);
+					if (S.theta.meta.nelem) launch_copy(u + b->off[i], a_opt.d, S.theta.meta.nelem, s);
+					S.advance_begin(a_opt);
+				}
+				const double a1 = now();
+				S.advance_host();
+				const double a2 = now();
+				ret[i] = S.advance_end(lam[i]);
+				t_dev[i] = (a1 - a0) + (now() - a2);
+				t_host[i] = a2 - a1;
+			}
+			catch (const HipError&) {
+				ret[i] = CT_ERR_HIP;
+			}
+		});
 		dev_free(v0);
 		dev_free(u);
 		b.reset();
-		c1 = now();
-		t_split_dev += c1 - c0;
-		// host block SVDs in parallel (no HIP calls in advance_host)
-		{
-			const int nt = std::max(1, std::min<int>(host_threads, (int)grp.size()));
-			std::vector<std::thread> th;
-			for (int k = 0; k < nt; k++)
-				th.emplace_back([&, k] {
-					for (size_t i = k; i < grp.size(); i += nt)
-						if (status[i] == 0) grp[i]->state->advance_host();
-				});
-			for (auto& x : th) x.join();
-		}
-		c0 = now();
-		t_split_host += c0 - c1;
 		for (size_t i = 0; i < grp.size(); i++) {
+			t_split_dev += t_dev[i];
+			t_split_host += t_host[i];
 			Ticket& t = *grp[i];
-			if (status[i] != 0) { finish(t, status[i]); continue; }
-			int ret;
-			try {
-				ret = t.state->advance_end(lam[i]);
-			}
-			catch (const HipError&) {
-				ret = CT_ERR_HIP;
-			}
-			if (ret != 0) { finish(t, ret); continue; }
+			if (ret[i] != 0) { finish(t, ret[i]); continue; }
 			if (t.state->done) {
 				t.en = t.state->en_sweeps;
 				finish(t, 0);
 			}
 		}
-		t_split_dev += now() - c0;
+		t_splits += now() - c0;
 	}
 
 	void loop()
@@ -210,10 +271,13 @@ struct Server
 				}
 			}
 			const double tick0 = now();
-			for (auto& t : admit) {
-				const int ret = start_request(*t);
-				if (ret != 0) finish(*t, ret);
-				else active.push_back(t);
+			{
+				std::vector<int> ret(admit.size(), 0);
+				hosts.run((int)admit.size(), [&](int i, hipStream_t s) { ret[i] = start_request(*admit[i], s); });
+				for (size_t i = 0; i < admit.size(); i++) {
+					if (ret[i] != 0) finish(*admit[i], ret[i]);
+					else active.push_back(admit[i]);
+				}
 			}
 			t_start += now() - tick0;
 			if (!active.empty()) {
@@ -259,6 +323,7 @@ int ct_serve_create(const double hbm_budget_bytes, const int max_active, ct_serv
 		S.host_threads = (int)std::max(1u, std::min(16u, hc ? hc : 8u));
 		S.t0 = std::chrono::steady_clock::now();
 		CTG_HIP(hipStreamCreateWithFlags(&S.st, hipStreamNonBlocking));
+		S.hosts.start(S.host_threads);
 		S.worker = std::thread([&S] { S.loop(); });
 		*server = h.release();
 		return 0;
@@ -324,9 +389,10 @@ int ct_serve_energies(ct_serve* server, const int64_t ticket, double* en_sweeps,
 	return n;
 }
 
-int ct_serve_stats(ct_serve* server, double* stats13)
+int ct_serve_stats(ct_serve* server, double* stats14)
 {
-	if (!server || !stats13) return CT_ERR_ARG;
+	if (!server || !stats14) return CT_ERR_ARG;
+	double* stats13 = stats14;
 	double* stats8 = stats13;
 	Server& S = *server->s;
 	std::lock_guard<std::mutex> g(S.mu);
@@ -343,6 +409,7 @@ int ct_serve_stats(ct_serve* server, double* stats13)
 	stats13[10] = S.t_lanczos;
 	stats13[11] = S.t_split_dev;
 	stats13[12] = S.t_split_host;
+	stats13[13] = S.t_splits;
 	return 0;
 }
 
@@ -357,6 +424,7 @@ int ct_serve_destroy(ct_serve* server)
 	S.cv_work.notify_all();
 	S.cv_done.notify_all();
 	if (S.worker.joinable()) S.worker.join();
+	S.hosts.shutdown();
 	for (auto& t : S.active) t->state.reset();
 	S.active.clear();
 	(void)hipStreamSynchronize(S.st);

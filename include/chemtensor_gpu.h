@@ -258,9 +258,10 @@ int ct_serve_poll(ct_serve* server, const int64_t ticket, const int wait, struct
 int ct_serve_energies(ct_serve* server, const int64_t ticket, double* en_sweeps, const int capacity);
 /// [0] ticks, [1] batched Lanczos solves, [2] finished requests, [3] queued requests, [4] reserved
 /// bytes, [5] largest batch, [6] busy seconds of the worker, [7] seconds since creation; seconds
-/// spent in [8] starting requests, [9] batch plans, [10] batched Lanczos, [11] device part of the
-/// splits and environment updates, [12] host block SVDs
-int ct_serve_stats(ct_serve* server, double* stats13);
+/// spent in [8] starting requests, [9] batch plans, [10] batched Lanczos, [13] the splits and
+/// environment updates (wall time; they run in parallel host threads), of which per request summed
+/// [11] device work and [12] host block SVDs
+int ct_serve_stats(ct_serve* server, double* stats14);
 /// stop the worker (queued and running requests are dropped) and free everything
 int ct_serve_destroy(ct_serve* server);
 

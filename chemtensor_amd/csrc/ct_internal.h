@@ -370,6 +370,8 @@ struct GroupedTables
 };
 void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const Meta& L, const Meta& R,
 	int64_t x_off, int64_t w_off, int64_t l_off, int64_t r_off);
+// append src, planned on its own (intermediates from offset 0), behind the intermediates of dst
+void merge_grouped_tables(GroupedTables& dst, const GroupedTables& src);
 
 // contract checks of apply_local_hamiltonian (degrees, unit outer bonds, matching axes); throws HipError
 void validate_matvec_operands(const Meta& theta, const Meta& w, const Meta& l, const Meta& r);
@@ -438,7 +440,9 @@ struct BatchPlan
 struct BatchInput { Meta theta; DevTensor w, l, r; };
 // tune: time the grouped launch forms and the fused form and keep the fastest; otherwise the
 // grouped form with heuristic launch forms (no timing launches, for plans rebuilt often)
-BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tune = true);
+// par(n, f): run f(0 .. n-1), possibly in parallel host threads (host planning only, no HIP calls)
+using ParallelFor = std::function<void(int, const std::function<void(int)>&)>;
+BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tune = true, const ParallelFor* par = nullptr);
 void batch_apply(BatchPlan& b, const double* x, double* y);
 // per-problem ground states: lambda[nprob], status[nprob] (0, -1 breakdown, -2 eigensolver), u concatenated
 int lanczos_ground_state_batch(BatchPlan& b, const double* vstart, int maxiter, double* lambda, int* status, double* u, bool use_graph);

@@ -524,28 +524,36 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 	std::vector<ct_long> t2_off((size_t)(nA * nM * nQ * nC), -1);
 	auto t2_key = [&](ct_long A, ct_long M, ct_long Q, ct_long C) { return ((A * nM + M) * nQ + Q) * nC + C; };
 	ct_long t2_elems = 0;
-	for (ct_long A = 0; A < nA; A++) for (ct_long C = 0; C < nC; C++) for (ct_long M = 0; M < nM; M++) for (ct_long Q = 0; Q < nQ; Q++) {
-		if (qb(W, 0, M) + qb(W, 1, Q) + qb(th, 0, A) - qb(R, 2, C) - qo != 0) continue;
-		std::vector<std::pair<ct_long, ct_long>> tl;  // (P, N)
-		for (ct_long P = 0; P < nP; P++) {
-			const ct_long N = W.find_qnum(3, qb(W, 0, M) + qb(W, 1, Q) - qb(W, 2, P));
-			if (N < 0) continue;
-			const ct_long iw[4] = { M, Q, P, N };
-			if (!W.present(W.flat_index(iw))) continue;
-			if (t1_off[t1_key(A, P, N, C)] < 0) continue;
-			tl.emplace_back(P, N);
-		}
-		if (tl.empty()) continue;
-		const ct_long dA = bc(th, 0, A), dM = bc(W, 0, M), dQ = bc(W, 1, Q), dC = bc(R, 2, C);
-		const ct_long off = t2_elems;
-		t2_off[t2_key(A, M, Q, C)] = off;
-		t2_elems += dA * dM * dQ * dC;
-		s2.add_task(off_ptr_mut(t2_base + off), (int)(dM * dQ), (int)(dA * dC), (int)dC, (int)dC, dM * dQ * dC, false);
-		for (auto& pn : tl) {
-			const ct_long P = pn.first, N = pn.second;
-			const ct_long dP = bc(W, 2, P), dN = bc(W, 3, N);
-			const ct_long iw[4] = { M, Q, P, N };
-			s2.add_term(off_ptr(w_off + W.offset[W.flat_index(iw)]), off_ptr(t1_base + t1_off[t1_key(A, P, N, C)]), (int)(dP * dN), (int)(dP * dN), 1, (int)dC, 1, dP * dN * dC);
+	// (M,Q) pairs by their quantum number sum
+	std::map<qnumber, std::vector<std::pair<ct_long, ct_long>>> mq_by_sum;
+	for (ct_long M = 0; M < nM; M++) for (ct_long Q = 0; Q < nQ; Q++) mq_by_sum[qb(W, 0, M) + qb(W, 1, Q)].emplace_back(M, Q);
+	std::vector<std::pair<ct_long, ct_long>> tl;  // (P, N)
+	for (ct_long A = 0; A < nA; A++) for (ct_long C = 0; C < nC; C++) {
+		auto grp = mq_by_sum.find(qb(R, 2, C) + qo - qb(th, 0, A));
+		if (grp == mq_by_sum.end()) continue;
+		for (const auto& mq : grp->second) {
+			const ct_long M = mq.first, Q = mq.second;
+			tl.clear();
+			for (ct_long P = 0; P < nP; P++) {
+				const ct_long N = W.find_qnum(3, qb(W, 0, M) + qb(W, 1, Q) - qb(W, 2, P));
+				if (N < 0) continue;
+				const ct_long iw[4] = { M, Q, P, N };
+				if (!W.present(W.flat_index(iw))) continue;
+				if (t1_off[t1_key(A, P, N, C)] < 0) continue;
+				tl.emplace_back(P, N);
+			}
+			if (tl.empty()) continue;
+			const ct_long dA = bc(th, 0, A), dM = bc(W, 0, M), dQ = bc(W, 1, Q), dC = bc(R, 2, C);
+			const ct_long off = t2_elems;
+			t2_off[t2_key(A, M, Q, C)] = off;
+			t2_elems += dA * dM * dQ * dC;
+			s2.add_task(off_ptr_mut(t2_base + off), (int)(dM * dQ), (int)(dA * dC), (int)dC, (int)dC, dM * dQ * dC, false);
+			for (auto& pn : tl) {
+				const ct_long P = pn.first, N = pn.second;
+				const ct_long dP = bc(W, 2, P), dN = bc(W, 3, N);
+				const ct_long iw[4] = { M, Q, P, N };
+				s2.add_term(off_ptr(w_off + W.offset[W.flat_index(iw)]), off_ptr(t1_base + t1_off[t1_key(A, P, N, C)]), (int)(dP * dN), (int)(dP * dN), 1, (int)dC, 1, dP * dN * dC);
+			}
 		}
 	}
 	// column-stream form of stage 2: classes D = qM + qQ = qP + qN, one unit per block pair (A,C)
@@ -643,6 +651,38 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 	g.stage_bytes[0] += 8.0 * (double)(th.nelem + R.nelem + t1_elems);
 	g.stage_bytes[1] += 8.0 * (double)(t1_elems + W.nelem + t2_elems);
 	g.stage_bytes[2] += 8.0 * (double)(t2_elems + L.nelem + th.nelem);
+}
+
+// Append src (planned with zero intermediate bases) to dst: its T1 / T2 offsets move behind dst's
+static const double* shift_ptr(const double* p, int64_t elems) { return (const double*)((intptr_t)p + (intptr_t)(elems * (int64_t)sizeof(double))); }
+
+void merge_grouped_tables(GroupedTables& dst, const GroupedTables& src)
+{
+	const int64_t t1s = dst.t1_elems, t2s = dst.t2_elems;
+	auto append = [](GemmBatch& d, const GemmBatch& b, int64_t c_shift, int64_t b_shift) {
+		const int32_t q0 = (int32_t)d.terms.size();
+		for (GemmTask t : b.tasks) { t.C = (double*)shift_ptr(t.C, c_shift); t.t0 += q0; t.t1 += q0; d.tasks.push_back(t); }
+		for (GemmTerm m : b.terms) { m.B = shift_ptr(m.B, b_shift); d.terms.push_back(m); }
+		d.flops += b.flops;
+	};
+	append(dst.s1, src.s1, t1s, 0);
+	append(dst.s2, src.s2, t2s, t1s);
+	append(dst.s3, src.s3, 0, t2s);
+	WApplyHost& a = dst.wa;
+	const WApplyHost& b = src.wa;
+	const int64_t kb0 = (int64_t)a.kbase.size(), rb0 = (int64_t)a.rbase.size(), wr0 = (int64_t)(a.widx.size() / WA_KMAX);
+	const int u0 = (int)a.units.size();
+	for (WaUnit u : b.units) { u.kb += kb0; u.rb += rb0; u.wrow += wr0; a.units.push_back(u); }
+	for (WaItem it : b.items) { it.unit += u0; a.items.push_back(it); }
+	for (int64_t v : b.kbase) a.kbase.push_back(v >= 0 ? v + t1s : v);
+	for (int64_t v : b.rbase) a.rbase.push_back(v >= 0 ? v + t2s : v);
+	a.kstride.insert(a.kstride.end(), b.kstride.begin(), b.kstride.end());
+	a.rstride.insert(a.rstride.end(), b.rstride.begin(), b.rstride.end());
+	a.widx.insert(a.widx.end(), b.widx.begin(), b.widx.end());
+	a.kmax = std::max(a.kmax, b.kmax);
+	dst.t1_elems += src.t1_elems;
+	dst.t2_elems += src.t2_elems;
+	for (int i = 0; i < 3; i++) dst.stage_bytes[i] += src.stage_bytes[i];
 }
 
 static void build_grouped(MatvecPlan& p)

@@ -532,7 +532,7 @@ BatchPlan::~BatchPlan()
 	dev_free(probs);
 }
 
-BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tune)
+BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tune, const ParallelFor* par)
 {
 	std::unique_ptr<BatchPlan> b(new BatchPlan());
 	b->stream = s;
@@ -562,8 +562,18 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 	}
 	// grouped three-stage tables over the concatenated operands
 	GroupedTables g;
-	for (size_t i = 0; i < in.size(); i++)
-		plan_grouped_tables(g, in[i].theta, in[i].w.meta, in[i].l.meta, in[i].r.meta, probs[i].x_off, probs[i].w_off, probs[i].l_off, probs[i].r_off);
+	if (par && in.size() > 1) {
+		// plan every problem on its own (in parallel), then concatenate
+		std::vector<GroupedTables> part(in.size());
+		(*par)((int)in.size(), [&](int i) {
+			plan_grouped_tables(part[i], in[i].theta, in[i].w.meta, in[i].l.meta, in[i].r.meta, probs[i].x_off, probs[i].w_off, probs[i].l_off, probs[i].r_off);
+		});
+		for (const GroupedTables& t : part) merge_grouped_tables(g, t);
+	}
+	else {
+		for (size_t i = 0; i < in.size(); i++)
+			plan_grouped_tables(g, in[i].theta, in[i].w.meta, in[i].l.meta, in[i].r.meta, probs[i].x_off, probs[i].w_off, probs[i].l_off, probs[i].r_off);
+	}
 	b->t1 = (double*)dev_alloc((size_t)std::max<int64_t>(g.t1_elems, 1) * sizeof(double));
 	b->t2 = (double*)dev_alloc((size_t)std::max<int64_t>(g.t2_elems, 1) * sizeof(double));
 	b->stage[0].upload(g.s1, s);

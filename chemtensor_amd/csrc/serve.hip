@@ -191,7 +191,8 @@ struct Server
 			in[i].r = DevTensor::view(S.rb[S.bond + 1], S.rb[S.bond + 1].meta);
 		}
 		double c0 = now();
-		std::unique_ptr<BatchPlan> b(build_batch_plan(in, st, false));
+		const ParallelFor par = [this](int n, const std::function<void(int)>& f) { hosts.run(n, [&f](int i, hipStream_t) { f(i); }); };
+		std::unique_ptr<BatchPlan> b(build_batch_plan(in, st, false, &par));
 		double c1 = now();
 		t_plan += c1 - c0;
 		const int64_t ntot = b->off.back();

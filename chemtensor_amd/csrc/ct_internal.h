@@ -233,7 +233,7 @@ DevTensor dev_split_axis(const DevTensor& t, int i_ax, const ct_long* new_dims, 
 // from a dense, zero-padded copy through scalar loads.
 constexpr int WA_KMAX = 32;
 struct WaUnit { int64_t kb, rb, wrow; int dA, dC, K, R; };   // kb / rb index kbase / rbase; wrow indexes wmat rows
-struct WaItem { int unit, col0; };
+struct WaItem { WaUnit u; int unit, col0; };                 // the unit travels with the item (one scalar load)
 struct WApplyHost
 {
 	std::vector<WaUnit> units;

@@ -624,7 +624,7 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 			}
 			const int unit = (int)wa.units.size();
 			wa.units.push_back(u);
-			for (int col0 = 0; col0 < (int)(dA * dC); col0 += 64) wa.items.push_back(WaItem{ unit, col0 });
+			for (int col0 = 0; col0 < (int)(dA * dC); col0 += 64) wa.items.push_back(WaItem{ u, unit, col0 });
 		}
 	}
 	// stage 3: y(X,Q,C)[x q c] = sum_(A,M) L(O,A,M,X)^T T2(A,M,Q,C)
@@ -673,7 +673,7 @@ void merge_grouped_tables(GroupedTables& dst, const GroupedTables& src)
 	const int64_t kb0 = (int64_t)a.kbase.size(), rb0 = (int64_t)a.rbase.size(), wr0 = (int64_t)(a.widx.size() / WA_KMAX);
 	const int u0 = (int)a.units.size();
 	for (WaUnit u : b.units) { u.kb += kb0; u.rb += rb0; u.wrow += wr0; a.units.push_back(u); }
-	for (WaItem it : b.items) { it.unit += u0; a.items.push_back(it); }
+	for (WaItem it : b.items) { it.unit += u0; it.u.kb += kb0; it.u.rb += rb0; it.u.wrow += wr0; a.items.push_back(it); }
 	for (int64_t v : b.kbase) a.kbase.push_back(v >= 0 ? v + t1s : v);
 	for (int64_t v : b.rbase) a.rbase.push_back(v >= 0 ? v + t2s : v);
 	a.kstride.insert(a.kstride.end(), b.kstride.begin(), b.kstride.end());

@@ -630,7 +630,7 @@ __global__ __launch_bounds__(256) void k_wapply(const WaUnit* __restrict__ units
 	if (w >= nitems) return;
 	const int lane = threadIdx.x & 63;
 	const WaItem it = items[w];
-	const WaUnit u = units[it.unit];
+	const WaUnit& u = it.u;
 	const int j = it.col0 + lane;
 	const bool act = j < u.dA * u.dC;
 	const int a = act ? j / u.dC : 0, c = act ? j - (j / u.dC) * u.dC : 0;

@@ -480,7 +480,8 @@ struct DmrgState
 };
 
 MatvecPlan* build_matvec_plan(const block_sparse_tensor* a, const block_sparse_tensor* w, const block_sparse_tensor* l, const block_sparse_tensor* r, int mode, bool autotune);
-MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& l, DevTensor&& r, int mode, hipStream_t s, bool autotune);
+MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& l, DevTensor&& r, int mode, hipStream_t s, bool autotune,
+	bool reference_count = true);
 void matvec_apply(MatvecPlan& p, const double* x, double* y, float* stage_ms);
 // average duration (ms) of one stage's launches, 'reps' launched back to back (stage 0 only for the fused form)
 float matvec_stage_time(MatvecPlan& p, int stage, const double* x, double* y, int reps);

@@ -140,7 +140,7 @@ static int minimize_local_energy(const DevTensor& w, const DevTensor& l, const D
 	double* en, DevTensor& a_opt, hipStream_t st)
 {
 	std::unique_ptr<MatvecPlan> p(build_matvec_plan_dev(a_start.meta, DevTensor::view(w, w.meta), DevTensor::view(l, l.meta), DevTensor::view(r, r.meta),
-		CT_MATVEC_GROUPED_FP64, st, false));
+		CT_MATVEC_GROUPED_FP64, st, false, false));
 	a_opt = DevTensor(a_start.meta);
 	return lanczos_ground_state(*p, a_start.d, maxiter, en, a_opt.d);
 }

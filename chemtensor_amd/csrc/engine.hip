@@ -891,7 +891,7 @@ void validate_matvec_operands(const Meta& theta, const Meta& w, const Meta& l, c
 		throw HipError{ hipErrorInvalidValue, "incompatible dimensions or quantum numbers of apply_local_hamiltonian operands" };
 }
 
-MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& l, DevTensor&& r, int mode, hipStream_t s, bool autotune)
+MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& l, DevTensor&& r, int mode, hipStream_t s, bool autotune, bool reference_count)
 {
 	validate_matvec_operands(theta, w.meta, l.meta, r.meta);
 	MatvecPlan* p = new MatvecPlan();
@@ -912,13 +912,18 @@ MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& 
 		}
 		else build_grouped(*p);
 		if (mode == CT_MATVEC_GROUPED_FP64 && autotune) autotune_stages(*p);
+		// without timing, stage 2 takes the column stream (faster at every size measured)
+		if (mode == CT_MATVEC_GROUPED_FP64 && !autotune) p->use_wapply = p->wapply.ok;
 		for (int i = 0; i < 4; i++) CTG_HIP(hipEventCreate(&p->ev[i]));
 	}
 	catch (...) {
 		delete p;
 		throw;
 	}
-	p->flops = reference_flops(theta, p->w.meta, p->l.meta, p->r.meta);
+	// the reference's flop count replans its unfused sequence; plans rebuilt at every bond use the
+	// executed count instead
+	p->flops = reference_count ? reference_flops(theta, p->w.meta, p->l.meta, p->r.meta)
+		: (double)p->stage_flops[0] + p->stage_flops[1] + p->stage_flops[2];
 	p->bytes = 8.0 * (2.0 * (double)theta.nelem + (double)p->w.meta.nelem + (double)p->l.meta.nelem + (double)p->r.meta.nelem);
 	return p;
 }

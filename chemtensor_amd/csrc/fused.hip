@@ -549,7 +549,7 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 		nl += p.l.meta.nelem;
 		nr += p.r.meta.nelem;
 		b->off.push_back(nx);
-		b->flops += reference_flops(th, p.w.meta, p.l.meta, p.r.meta);
+		if (tune) b->flops += reference_flops(th, p.w.meta, p.l.meta, p.r.meta);
 		b->bytes += 8.0 * (2.0 * (double)th.nelem + (double)(p.w.meta.nelem + p.l.meta.nelem + p.r.meta.nelem));
 	}
 	b->w = (double*)dev_alloc((size_t)std::max<int64_t>(nw, 1) * sizeof(double));
@@ -576,6 +576,7 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 	}
 	b->t1 = (double*)dev_alloc((size_t)std::max<int64_t>(g.t1_elems, 1) * sizeof(double));
 	b->t2 = (double*)dev_alloc((size_t)std::max<int64_t>(g.t2_elems, 1) * sizeof(double));
+	if (!tune) b->flops = g.s1.flops + g.s2.flops + g.s3.flops;   // executed count (no replanning of the reference sequence)
 	b->stage[0].upload(g.s1, s);
 	b->stage[1].upload(g.s2, s);
 	b->stage[2].upload(g.s3, s);

@@ -259,17 +259,32 @@ def main():
     # every stage is below the FP64 ridge point (about 10 flop/byte), so HBM is its roofline
     intensity = sflops[dom] / sbytes[dom]
     achieved = sbytes[dom] / (kernel_ms[dom] * 1e-3) / 1e9
-    traffic = None
+    traffic, traffic_source = None, None
     if args.hbm_traffic_kib:
         fetch_kib, write_kib = (float(v) for v in args.hbm_traffic_kib.split(","))
         traffic = round((fetch_kib * args.fetch_scale + write_kib) * 1024.0)
+        traffic_source = "--hbm-traffic-kib"
     y_g = forms["grouped_fp64"]["y"].numpy()
+    names = ["k_grouped_gemm_f64", "k_stream_gemm_f64_wave", "k_grouped_gemm_f64_wg", "k_stream_gemm_f64_wg", "k_wapply"]
+    kernel_name = "k_fused_matvec" if head == "fused_fp64" else names[plan.forms()[0][dom]]
+    workload = (f"single two-site H_eff matvec, Fermi-Hubbard L={args.nsites} t=1 U=4 half filling, centre bond, chi={args.chi}"
+                if args.model == "hubbard" else f"single two-site H_eff matvec, XXZ L={args.nsites}, centre bond, chi={args.chi}")
+    if traffic is None:
+        # HBM bytes per launch of this kernel on this workload from the committed rocprofv3 --pmc
+        # FETCH_SIZE / WRITE_SIZE passes (scaled by the copy calibration), when they cover it
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
+        if os.path.exists(path):
+            with open(path) as f:
+                tab = json.load(f)
+            kib = tab["kib_per_launch"].get(f"{kernel_name}:{dom + 1}")
+            if tab.get("workload") == workload and kib is not None:
+                traffic = round((kib[0] * tab["fetch_scale"] + kib[1] * tab["write_scale"]) * 1024.0)
+                traffic_source = "profiles/hbm_traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)"
     if head == "fused_fp64":
         kernel_desc = "k_fused_matvec (one launch per matvec: theta.R, W.T1, L.T2 with T1/T2 in LDS)"
         form_desc = "fused FP64 (one launch per matvec, intermediates in LDS)"
     else:
-        names = ["k_grouped_gemm_f64", "k_stream_gemm_f64_wave", "k_grouped_gemm_f64_wg", "k_stream_gemm_f64_wg", "k_wapply"]
-        kernel_desc = f"{names[plan.forms()[0][dom]]}: grouped stage {dom + 1} ({['theta.R', 'W.T1', 'L.T2'][dom]})"
+        kernel_desc = f"{kernel_name}: grouped stage {dom + 1} ({['theta.R', 'W.T1', 'L.T2'][dom]})"
         form_desc = "batched grouped FP64 MFMA (3 launches per matvec)"
 
     result = {
@@ -286,8 +301,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded construct_random_mps bond quantum numbers, N(0,1) blocks)",
         "config": {
-            "workload": f"single two-site H_eff matvec, Fermi-Hubbard L={args.nsites} t=1 U=4 half filling, centre bond, chi={args.chi}"
-                        if args.model == "hubbard" else f"single two-site H_eff matvec, XXZ L={args.nsites}, centre bond, chi={args.chi}",
+            "workload": workload,
             "chi": args.chi, "nsites": args.nsites, "model": args.model, "theta_elements": int(plan.dim),
             "form": form_desc,
         },
@@ -298,6 +312,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": traffic,
+            "traffic_source": traffic_source,
             "kernel": kernel_desc,
             "kernel_algorithmic_bytes_per_launch": sbytes[dom],
             "kernel_flops_per_launch": sflops[dom],

@@ -64,6 +64,8 @@ std::vector<ct_long> fanout(const Meta& m, int ax, qnumber q);
 // device memory: caching allocator and device tensors
 
 void* dev_alloc(size_t nbytes);
+// enqueue a host-to-device copy of host data on s; the host data may be freed on return
+void copy_h2d_async(void* dst, const void* src, size_t nbytes, hipStream_t s);
 void dev_free(void* p);
 void dev_trim();
 size_t dev_bytes_in_use();

@@ -66,10 +66,9 @@ void DevGemm::upload(const GemmBatch& b, hipStream_t s)
 	tasks = (GemmTask*)dev_alloc(std::max<size_t>(1, b.tasks.size()) * sizeof(GemmTask));
 	terms = (GemmTerm*)dev_alloc(std::max<size_t>(1, b.terms.size()) * sizeof(GemmTerm));
 	items = (WorkItem*)dev_alloc(std::max<size_t>(1, it.size()) * sizeof(WorkItem));
-	if (ntasks) CTG_HIP(hipMemcpyAsync(tasks, b.tasks.data(), b.tasks.size() * sizeof(GemmTask), hipMemcpyHostToDevice, s));
-	if (nterms) CTG_HIP(hipMemcpyAsync(terms, b.terms.data(), b.terms.size() * sizeof(GemmTerm), hipMemcpyHostToDevice, s));
-	if (nitems) CTG_HIP(hipMemcpyAsync(items, it.data(), it.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s));
-	CTG_HIP(hipStreamSynchronize(s));
+	copy_h2d_async(tasks, b.tasks.data(), b.tasks.size() * sizeof(GemmTask), s);
+	copy_h2d_async(terms, b.terms.data(), b.terms.size() * sizeof(GemmTerm), s);
+	copy_h2d_async(items, it.data(), it.size() * sizeof(WorkItem), s);
 	const int cap = num_compute_units() * 8;
 	grid = std::max(1, std::min((nitems + 3) / 4, cap));
 	// workgroup-level split of the k-step stream only pays for long streams
@@ -187,10 +186,10 @@ DevTensor dev_dot(const DevTensor& s, int axs, const DevTensor& t, int axt, int 
 		run_batch_plain(b, bases, st);
 	}
 	else {
+		// descriptor buffers return to this thread's cache; later uses are ordered behind the launch on st
 		DevGemm g;
 		g.upload(b, st);
 		launch_grouped_gemm_f64(g, bases, st);
-		CTG_HIP(hipStreamSynchronize(st));
 		g.release();
 	}
 	return r;
@@ -239,9 +238,8 @@ DevTensor dev_transpose(const int* perm, const DevTensor& t, hipStream_t st)
 	DevTensor r(transpose_meta(perm, t.meta));
 	const std::vector<PermBlock> pbs = transpose_blocks(perm, t.meta, r.meta);
 	PermBlock* d = (PermBlock*)dev_alloc(std::max<size_t>(1, pbs.size()) * sizeof(PermBlock));
-	if (!pbs.empty()) CTG_HIP(hipMemcpyAsync(d, pbs.data(), pbs.size() * sizeof(PermBlock), hipMemcpyHostToDevice, st));
+	copy_h2d_async(d, pbs.data(), pbs.size() * sizeof(PermBlock), st);
 	launch_permute_blocks(t.d, r.d, d, (int)pbs.size(), st);
-	CTG_HIP(hipStreamSynchronize(st));
 	dev_free(d);
 	return r;
 }
@@ -304,13 +302,12 @@ DevTensor dev_cyclic_partial_trace(const DevTensor& t, int nt, hipStream_t st)
 	const int ng = (int)gs.size() - 1;
 	TraceItem* di = (TraceItem*)dev_alloc(std::max<size_t>(1, items.size()) * sizeof(TraceItem));
 	int* dg = (int*)dev_alloc(gs.size() * sizeof(int));
-	CTG_HIP(hipMemcpyAsync(di, items.data(), items.size() * sizeof(TraceItem), hipMemcpyHostToDevice, st));
-	CTG_HIP(hipMemcpyAsync(dg, gs.data(), gs.size() * sizeof(int), hipMemcpyHostToDevice, st));
+	copy_h2d_async(di, items.data(), items.size() * sizeof(TraceItem), st);
+	copy_h2d_async(dg, gs.data(), gs.size() * sizeof(int), st);
 	if (ng > 0) {
 		hipLaunchKernelGGL(k_partial_trace, dim3(std::min(ng, 8192)), dim3(256), 0, st, t.d, r.d, di, dg, ng);
 		CTG_HIP(hipGetLastError());
 	}
-	CTG_HIP(hipStreamSynchronize(st));
 	dev_free(di);
 	dev_free(dg);
 	return r;
@@ -324,9 +321,8 @@ static void run_segments(const double* src, double* dst, const std::vector<CopyS
 {
 	if (segs.empty()) return;
 	CopySeg* d = (CopySeg*)dev_alloc(segs.size() * sizeof(CopySeg));
-	CTG_HIP(hipMemcpyAsync(d, segs.data(), segs.size() * sizeof(CopySeg), hipMemcpyHostToDevice, st));
+	copy_h2d_async(d, segs.data(), segs.size() * sizeof(CopySeg), st);
 	launch_copy_segments(src, dst, d, (int)segs.size(), st);
-	CTG_HIP(hipStreamSynchronize(st));
 	dev_free(d);
 }
 
@@ -731,8 +727,7 @@ static void build_plain(MatvecPlan& p)
 	auto perm_desc = [&](const int* perm, const Meta& from, const Meta& to) {
 		std::vector<PermBlock> pbs = transpose_blocks(perm, from, to);
 		PermBlock* d = (PermBlock*)dev_alloc(std::max<size_t>(1, pbs.size()) * sizeof(PermBlock));
-		if (!pbs.empty()) CTG_HIP(hipMemcpyAsync(d, pbs.data(), pbs.size() * sizeof(PermBlock), hipMemcpyHostToDevice, st));
-		CTG_HIP(hipStreamSynchronize(st));
+		copy_h2d_async(d, pbs.data(), pbs.size() * sizeof(PermBlock), st);
 		p.owned.push_back(d);
 		return std::make_pair(d, (int)pbs.size());
 	};
@@ -752,8 +747,7 @@ static void build_plain(MatvecPlan& p)
 		}
 	}
 	CopySeg* dseg = (CopySeg*)dev_alloc(std::max<size_t>(1, segs.size()) * sizeof(CopySeg));
-	if (!segs.empty()) CTG_HIP(hipMemcpyAsync(dseg, segs.data(), segs.size() * sizeof(CopySeg), hipMemcpyHostToDevice, st));
-	CTG_HIP(hipStreamSynchronize(st));
+	copy_h2d_async(dseg, segs.data(), segs.size() * sizeof(CopySeg), st);
 	p.owned.push_back(dseg);
 	const int nseg = (int)segs.size();
 	// buffers

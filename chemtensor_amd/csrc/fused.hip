@@ -446,7 +446,7 @@ template <typename T>
 static T* upload_vec(const std::vector<T>& v, hipStream_t s)
 {
 	T* d = (T*)dev_alloc(std::max<size_t>(1, v.size()) * sizeof(T));
-	if (!v.empty()) CTG_HIP(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+	copy_h2d_async(d, v.data(), v.size() * sizeof(T), s);
 	return d;
 }
 

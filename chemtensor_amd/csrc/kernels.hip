@@ -2,7 +2,9 @@
 //
 //  grouped_gemm_f64   persistent grouped FP64 GEMM: every wavefront owns one 32 x 32 output
 //                     tile at a time (2 x 2 v_mfma_f64_16x16x4 sub-tiles), sums all terms of
-//                     its task in registers and writes the tile once (no atomics, deterministic)
+//                     its task in registers and writes the tile once (no atomics, deterministic);
+//                     the term descriptors of a task are loaded once, one per lane
+//  wapply             stage 2 of the matvec as a column stream (small dense W_D per class)
 //  grouped_gemm_split the same schedule with each FP64 operand split into FP32 terms on FP32
 //                     MFMA; every MFMA result is lifted into FP64 accumulators
 //  plain_gemm_f64     one output block x one contracted block per launch (the per-block form)
@@ -32,6 +34,46 @@ __device__ __forceinline__ const double* rebase(const double* base, const double
 	return (const double*)((const char*)base + (intptr_t)off);
 }
 
+// Term descriptors of a task are held one per lane (64 at a time, one vector load) and read with
+// v_readlane: walking the terms costs no dependent memory round trip.
+struct TermCache
+{
+	uint32_t w[12];   // the GemmTerm of term base + lane (12 dwords)
+	int base;
+};
+
+static_assert(sizeof(GemmTerm) == 48, "GemmTerm layout");
+
+__device__ __forceinline__ void term_cache_load(TermCache& c, const GemmTerm* __restrict__ terms, int base, int t1, int lane)
+{
+	c.base = base;
+	const int idx = base + lane;
+	if (idx < t1) {
+		const uint4* src = (const uint4*)(terms + idx);
+		const uint4 a = src[0], b = src[1], d = src[2];
+		c.w[0] = a.x; c.w[1] = a.y; c.w[2] = a.z; c.w[3] = a.w;
+		c.w[4] = b.x; c.w[5] = b.y; c.w[6] = b.z; c.w[7] = b.w;
+		c.w[8] = d.x; c.w[9] = d.y; c.w[10] = d.z; c.w[11] = d.w;
+	}
+	else {
+		#pragma unroll
+		for (int i = 0; i < 12; i++) c.w[i] = 0;
+	}
+}
+
+__device__ __forceinline__ uint32_t term_word(const TermCache& c, int t, int i)
+{
+	return (uint32_t)__builtin_amdgcn_readlane((int)c.w[i], t - c.base);
+}
+
+__device__ __forceinline__ int term_K(const TermCache& c, int t) { return (int)term_word(c, t, 6); }
+
+__device__ __forceinline__ const double* term_ptr(const TermCache& c, int t, int i)
+{
+	const uint64_t lo = term_word(c, t, i), hi = term_word(c, t, i + 1);
+	return (const double*)(intptr_t)(lo | (hi << 32));
+}
+
 __device__ __forceinline__ void wave_tile_f64(const GemmTask& tk, const GemmTerm* __restrict__ terms, int m0, int n0, int lane, GemmBases bs)
 {
 	const int i16 = lane & 15, kq = lane >> 4;
@@ -41,17 +83,23 @@ __device__ __forceinline__ void wave_tile_f64(const GemmTask& tk, const GemmTerm
 	const int c0 = n0 + i16, c1 = n0 + 16 + i16;
 	const bool vr0 = r0 < tk.M, vr1 = r1 < tk.M, vc0 = c0 < tk.N, vc1 = c1 < tk.N;
 	double4_t acc00 = { 0, 0, 0, 0 }, acc01 = { 0, 0, 0, 0 }, acc10 = { 0, 0, 0, 0 }, acc11 = { 0, 0, 0, 0 };
+	TermCache tc;
+	term_cache_load(tc, terms, tk.t0, tk.t1, lane);
 	for (int t = tk.t0; t < tk.t1; t++) {
-		const GemmTerm tm = terms[t];
-		const double* __restrict__ A = rebase(bs.A, tm.A);
-		const double* __restrict__ B = rebase(bs.B, tm.B);
-		const int64_t ao0 = (int64_t)(vr0 ? r0 : 0) * tm.sa_i, ao1 = (int64_t)(vr1 ? r1 : 0) * tm.sa_i;
-		const int64_t bo0 = b_col_offset(vc0 ? c0 : 0, tk.nci, tm.sb_batch, tm.sb_j);
-		const int64_t bo1 = b_col_offset(vc1 ? c1 : 0, tk.nci, tm.sb_batch, tm.sb_j);
-		for (int k0 = 0; k0 < tm.K; k0 += 4) {
+		if (t - tc.base >= 64) term_cache_load(tc, terms, t, tk.t1, lane);
+		const double* __restrict__ A = rebase(bs.A, term_ptr(tc, t, 0));
+		const double* __restrict__ B = rebase(bs.B, term_ptr(tc, t, 2));
+		const int64_t sb_batch = (int64_t)term_ptr(tc, t, 4);
+		const int K = term_K(tc, t);
+		const int sa_i = (int)term_word(tc, t, 7), sa_k = (int)term_word(tc, t, 8);
+		const int sb_k = (int)term_word(tc, t, 9), sb_j = (int)term_word(tc, t, 10);
+		const int64_t ao0 = (int64_t)(vr0 ? r0 : 0) * sa_i, ao1 = (int64_t)(vr1 ? r1 : 0) * sa_i;
+		const int64_t bo0 = b_col_offset(vc0 ? c0 : 0, tk.nci, sb_batch, sb_j);
+		const int64_t bo1 = b_col_offset(vc1 ? c1 : 0, tk.nci, sb_batch, sb_j);
+		for (int k0 = 0; k0 < K; k0 += 4) {
 			const int k = k0 + kq;
-			const bool vk = k < tm.K;
-			const int64_t ak = (int64_t)(vk ? k : 0) * tm.sa_k, bk = (int64_t)(vk ? k : 0) * tm.sb_k;
+			const bool vk = k < K;
+			const int64_t ak = (int64_t)(vk ? k : 0) * sa_k, bk = (int64_t)(vk ? k : 0) * sb_k;
 			const double a0 = (vk && vr0) ? A[ao0 + ak] : 0.0;
 			const double b0 = (vk && vc0) ? B[bo0 + bk] : 0.0;
 			acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
@@ -96,7 +144,8 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(const GemmTask* __rest
 {
 	const int lane = threadIdx.x & 63;
 	const int nwaves = gridDim.x * 4;
-	for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < nitems; w += nwaves) {
+	// the item index is uniform across the wavefront: its descriptors are scalar loads
+	for (int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6))); w < nitems; w += nwaves) {
 		const WorkItem it = items[w];
 		const GemmTask& tk = it.task;
 		wave_tile_f64(tk, terms, it.m0, it.n0, lane, bs);
@@ -120,25 +169,31 @@ __device__ __forceinline__ void wg_tile_f64(const GemmTask& tk, const GemmTerm* 
 	const bool vr0 = r0 < tk.M, vr1 = r1 < tk.M, vc0 = c0 < tk.N, vc1 = c1 < tk.N;
 	double4_t acc00 = { 0, 0, 0, 0 }, acc01 = { 0, 0, 0, 0 }, acc10 = { 0, 0, 0, 0 }, acc11 = { 0, 0, 0, 0 };
 	int g = 0;
+	TermCache tc;
+	term_cache_load(tc, terms, tk.t0, tk.t1, lane);
 	for (int t = tk.t0; t < tk.t1; t++) {
-		const GemmTerm tm = terms[t];
-		const int nsteps = (tm.K + 3) >> 2;
+		if (t - tc.base >= 64) term_cache_load(tc, terms, t, tk.t1, lane);
+		const int K = term_K(tc, t);
+		const int nsteps = (K + 3) >> 2;
 		const int nchunks = (nsteps + CHUNK - 1) / CHUNK;
 		const int first = (wave - g % 4 + 4) % 4;
 		g += nchunks;
 		if (first >= nchunks) continue;
-		const double* __restrict__ A = rebase(bs.A, tm.A);
-		const double* __restrict__ B = rebase(bs.B, tm.B);
-		const int64_t ao0 = (int64_t)(vr0 ? r0 : 0) * tm.sa_i, ao1 = (int64_t)(vr1 ? r1 : 0) * tm.sa_i;
-		const int64_t bo0 = b_col_offset(vc0 ? c0 : 0, tk.nci, tm.sb_batch, tm.sb_j);
-		const int64_t bo1 = b_col_offset(vc1 ? c1 : 0, tk.nci, tm.sb_batch, tm.sb_j);
+		const double* __restrict__ A = rebase(bs.A, term_ptr(tc, t, 0));
+		const double* __restrict__ B = rebase(bs.B, term_ptr(tc, t, 2));
+		const int64_t sb_batch = (int64_t)term_ptr(tc, t, 4);
+		const int sa_i = (int)term_word(tc, t, 7), sa_k = (int)term_word(tc, t, 8);
+		const int sb_k = (int)term_word(tc, t, 9), sb_j = (int)term_word(tc, t, 10);
+		const int64_t ao0 = (int64_t)(vr0 ? r0 : 0) * sa_i, ao1 = (int64_t)(vr1 ? r1 : 0) * sa_i;
+		const int64_t bo0 = b_col_offset(vc0 ? c0 : 0, tk.nci, sb_batch, sb_j);
+		const int64_t bo1 = b_col_offset(vc1 ? c1 : 0, tk.nci, sb_batch, sb_j);
 		for (int c = first; c < nchunks; c += 4) {
 			double a0[CHUNK], a1[CHUNK], b0[CHUNK], b1[CHUNK];
 			#pragma unroll
 			for (int u = 0; u < CHUNK; u++) {
 				const int k = (c * CHUNK + u) * 4 + kq;
-				const bool vk = k < tm.K;
-				const int64_t ak = (int64_t)(vk ? k : 0) * tm.sa_k, bk = (int64_t)(vk ? k : 0) * tm.sb_k;
+				const bool vk = k < K;
+				const int64_t ak = (int64_t)(vk ? k : 0) * sa_k, bk = (int64_t)(vk ? k : 0) * sb_k;
 				a0[u] = (vk && vr0) ? A[ao0 + ak] : 0.0;
 				a1[u] = (vk && vr1) ? A[ao1 + ak] : 0.0;
 				b0[u] = (vk && vc0) ? B[bo0 + bk] : 0.0;
@@ -200,7 +255,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64_wg(const GemmTask* __r
 // consumed in batches of SLOTS k-steps whose loads are all issued before their MFMAs, so a
 // batch costs one memory round trip even when it spans several short terms. With nparts > 1,
 // batch b of the stream belongs to part b % nparts (the wavefronts of a workgroup).
-constexpr int SLOTS = 8;
+constexpr int SLOTS = 4;
 
 struct Acc4 { double4_t a00, a01, a10, a11; };
 
@@ -214,11 +269,17 @@ __device__ __forceinline__ void stream_tile_f64(const GemmTask& tk, const GemmTe
 	const int r0 = m0 + i16, r1 = m0 + 16 + i16;
 	const int c0 = n0 + i16, c1 = n0 + 16 + i16;
 	const bool vr0 = r0 < tk.M, vr1 = r1 < tk.M, vc0 = c0 < tk.N, vc1 = c1 < tk.N;
+	TermCache tc;
+	term_cache_load(tc, terms, tk.t0, tk.t1, lane);
+	auto ensure = [&](int t) {
+		if (t - tc.base >= 64) term_cache_load(tc, terms, tk.t0 + ((t - tk.t0) & ~63), tk.t1, lane);
+	};
 	int t = tk.t0, step = 0;
 	// advance the cursor by 'skip' k-steps
 	auto advance = [&](int skip) {
 		while (t < tk.t1 && skip > 0) {
-			const int ns = (terms[t].K + 3) >> 2;
+			ensure(t);
+			const int ns = (term_K(tc, t) + 3) >> 2;
 			const int rem = ns - step;
 			if (skip >= rem) { skip -= rem; t++; step = 0; }
 			else { step += skip; skip = 0; }
@@ -231,19 +292,23 @@ __device__ __forceinline__ void stream_tile_f64(const GemmTask& tk, const GemmTe
 		#pragma unroll
 		for (int u = 0; u < SLOTS; u++) {
 			if (t < tk.t1) {
-				const GemmTerm tm = terms[t];
-				const double* __restrict__ A = rebase(bs.A, tm.A);
-				const double* __restrict__ B = rebase(bs.B, tm.B);
+				ensure(t);
+				const double* __restrict__ A = rebase(bs.A, term_ptr(tc, t, 0));
+				const double* __restrict__ B = rebase(bs.B, term_ptr(tc, t, 2));
+				const int64_t sb_batch = (int64_t)term_ptr(tc, t, 4);
+				const int K = term_K(tc, t);
+				const int sa_i = (int)term_word(tc, t, 7), sa_k = (int)term_word(tc, t, 8);
+				const int sb_k = (int)term_word(tc, t, 9), sb_j = (int)term_word(tc, t, 10);
 				const int k = step * 4 + kq;
-				const bool vk = k < tm.K;
-				const int64_t ak = (int64_t)(vk ? k : 0) * tm.sa_k, bk = (int64_t)(vk ? k : 0) * tm.sb_k;
-				a0[u] = (vk && vr0) ? A[(int64_t)r0 * tm.sa_i + ak] : 0.0;
-				a1[u] = (vk && vr1) ? A[(int64_t)r1 * tm.sa_i + ak] : 0.0;
-				b0[u] = (vk && vc0) ? B[b_col_offset(c0, tk.nci, tm.sb_batch, tm.sb_j) + bk] : 0.0;
-				b1[u] = (vk && vc1) ? B[b_col_offset(c1, tk.nci, tm.sb_batch, tm.sb_j) + bk] : 0.0;
+				const bool vk = k < K;
+				const int64_t ak = (int64_t)(vk ? k : 0) * sa_k, bk = (int64_t)(vk ? k : 0) * sb_k;
+				a0[u] = (vk && vr0) ? A[(int64_t)r0 * sa_i + ak] : 0.0;
+				a1[u] = (vk && vr1) ? A[(int64_t)r1 * sa_i + ak] : 0.0;
+				b0[u] = (vk && vc0) ? B[b_col_offset(c0, tk.nci, sb_batch, sb_j) + bk] : 0.0;
+				b1[u] = (vk && vc1) ? B[b_col_offset(c1, tk.nci, sb_batch, sb_j) + bk] : 0.0;
 				filled = u + 1;
 				step++;
-				if (step * 4 >= tm.K) { t++; step = 0; }
+				if (step * 4 >= K) { t++; step = 0; }
 			}
 			else { a0[u] = 0; a1[u] = 0; b0[u] = 0; b1[u] = 0; }
 		}
@@ -283,7 +348,7 @@ __global__ __launch_bounds__(256) void k_stream_gemm_f64_wave(const GemmTask* __
 {
 	const int lane = threadIdx.x & 63;
 	const int nwaves = gridDim.x * 4;
-	for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < nitems; w += nwaves) {
+	for (int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6))); w < nitems; w += nwaves) {
 		const WorkItem it = items[w];
 		const GemmTask& tk = it.task;
 		Acc4 acc;
@@ -667,7 +732,7 @@ template <typename T>
 static T* upload_array(const std::vector<T>& v, hipStream_t s)
 {
 	T* d = (T*)dev_alloc(std::max<size_t>(1, v.size()) * sizeof(T));
-	if (!v.empty()) CTG_HIP(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+	copy_h2d_async(d, v.data(), v.size() * sizeof(T), s);
 	return d;
 }
 
@@ -689,8 +754,7 @@ void WApply::upload(const WApplyHost& h, const double* w, hipStream_t s)
 		hipLaunchKernelGGL(k_gather_w, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0, s, w, (const int64_t*)widx, n, wmat);
 		CTG_HIP(hipGetLastError());
 	}
-	CTG_HIP(hipStreamSynchronize(s));
-	dev_free(widx);
+	dev_free(widx);   // reused only behind the gather kernel on s
 	nitems = (int)h.items.size();
 	kmax = h.kmax;
 }

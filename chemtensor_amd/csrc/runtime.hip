@@ -3,6 +3,7 @@
 // bond, which must not turn into hipMalloc/hipFree calls) and host <-> device transfers of
 // block-sparse tensors in their packed (serialize-order) layout.
 
+#include <deque>
 #include <map>
 #include <mutex>
 #include <thread>
@@ -120,6 +121,62 @@ void dev_trim()
 	p.bytes_cached = 0;
 }
 
+//________________________________________________________________________________________________________________________
+//
+// host-to-device copies of host-side tables without host synchronisation: the data is copied into
+// a pinned staging block owned by the calling thread and the DMA is enqueued on the stream; a block
+// is reused only after the event recorded behind its copy has completed. Staging blocks live until
+// the process ends.
+
+namespace {
+
+struct StageBlock { char* p; size_t cap; hipEvent_t ev; };
+
+struct Staging
+{
+	std::vector<StageBlock> idle;
+	std::deque<StageBlock> busy;
+};
+
+Staging& staging()
+{
+	thread_local Staging* st = new Staging();
+	return *st;
+}
+
+}  // namespace
+
+void copy_h2d_async(void* dst, const void* src, size_t nbytes, hipStream_t s)
+{
+	if (nbytes == 0) return;
+	Staging& S = staging();
+	while (!S.busy.empty() && hipEventQuery(S.busy.front().ev) == hipSuccess) {
+		S.idle.push_back(S.busy.front());
+		S.busy.pop_front();
+	}
+	int best = -1;
+	for (int i = 0; i < (int)S.idle.size(); i++)
+		if (S.idle[i].cap >= nbytes && (best < 0 || S.idle[i].cap < S.idle[best].cap)) best = i;
+	StageBlock b;
+	if (best >= 0) {
+		b = S.idle[best];
+		S.idle.erase(S.idle.begin() + best);
+	}
+	else {
+		size_t cap = 64 * 1024;
+		while (cap < nbytes) cap *= 2;
+		void* p = nullptr;
+		CTG_HIP(hipHostMalloc(&p, cap, hipHostMallocDefault));
+		b.p = (char*)p;
+		b.cap = cap;
+		CTG_HIP(hipEventCreateWithFlags(&b.ev, hipEventDisableTiming));
+	}
+	memcpy(b.p, src, nbytes);
+	CTG_HIP(hipMemcpyAsync(dst, b.p, nbytes, hipMemcpyHostToDevice, s));
+	CTG_HIP(hipEventRecord(b.ev, s));
+	S.busy.push_back(b);
+}
+
 size_t dev_bytes_in_use()
 {
 	Pool& p = pool();
@@ -169,8 +226,7 @@ DevTensor upload(const block_sparse_tensor* t, hipStream_t s)
 	DevTensor dt(meta_of(t));
 	std::vector<double> host((size_t)dt.meta.nelem);
 	block_sparse_tensor_serialize_entries(t, host.data());
-	CTG_HIP(hipMemcpyAsync(dt.d, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, s));
-	CTG_HIP(hipStreamSynchronize(s));
+	copy_h2d_async(dt.d, host.data(), host.size() * sizeof(double), s);
 	return dt;
 }
 
@@ -200,8 +256,7 @@ HostPacked download_packed(const DevTensor& t, hipStream_t s)
 DevTensor upload_packed(const HostPacked& h, hipStream_t s)
 {
 	DevTensor t(h.meta);
-	if (!h.v.empty()) CTG_HIP(hipMemcpyAsync(t.d, h.v.data(), h.v.size() * sizeof(double), hipMemcpyHostToDevice, s));
-	CTG_HIP(hipStreamSynchronize(s));
+	copy_h2d_async(t.d, h.v.data(), h.v.size() * sizeof(double), s);
 	return t;
 }
 

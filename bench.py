@@ -265,7 +265,7 @@ def main():
         traffic = round((fetch_kib * args.fetch_scale + write_kib) * 1024.0)
         traffic_source = "--hbm-traffic-kib"
     y_g = forms["grouped_fp64"]["y"].numpy()
-    names = ["k_grouped_gemm_f64", "k_stream_gemm_f64_wave", "k_grouped_gemm_f64_wg", "k_stream_gemm_f64_wg", "k_wapply"]
+    names = ["k_grouped_gemm_f64", "k_stream_gemm_f64_wave", "k_grouped_gemm_f64_wg", "k_stream_gemm_f64_wg", "k_wapply", "k_f12"]
     kernel_name = "k_fused_matvec" if head == "fused_fp64" else names[plan.forms()[0][dom]]
     workload = (f"single two-site H_eff matvec, Fermi-Hubbard L={args.nsites} t=1 U=4 half filling, centre bond, chi={args.chi}"
                 if args.model == "hubbard" else f"single two-site H_eff matvec, XXZ L={args.nsites}, centre bond, chi={args.chi}")
@@ -284,7 +284,7 @@ def main():
         kernel_desc = "k_fused_matvec (one launch per matvec: theta.R, W.T1, L.T2 with T1/T2 in LDS)"
         form_desc = "fused FP64 (one launch per matvec, intermediates in LDS)"
     else:
-        kernel_desc = f"{kernel_name}: grouped stage {dom + 1} ({['theta.R', 'W.T1', 'L.T2'][dom]})"
+        kernel_desc = f"{kernel_name}: grouped stage {dom + 1} ({['theta.R' if kernel_name != 'k_f12' else 'theta.R and W.T1 fused', 'W.T1', 'L.T2'][dom]})"
         form_desc = "batched grouped FP64 MFMA (3 launches per matvec)"
 
     result = {
@@ -335,7 +335,7 @@ def main():
         "plan_stats": plan.stats(),
         "autotune_grouped": {"forms": forms["grouped_fp64"]["plan"].forms()[0],
                              "ms_per_form": [[round(x, 5) for x in r] for r in forms["grouped_fp64"]["plan"].forms()[1]],
-                     "form_names": ["wave_simple", "wave_stream", "wg_chunk", "wg_stream", "column_stream"]},
+                     "form_names": ["wave_simple", "wave_stream", "wg_chunk", "wg_stream", "column_stream", "fused_stages_1_2"]},
     }
     ys = {k: v["y"].numpy() for k, v in forms.items()}
     for k in ys:

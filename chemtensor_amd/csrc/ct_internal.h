@@ -236,10 +236,21 @@ DevTensor dev_split_axis(const DevTensor& t, int i_ax, const ct_long* new_dims, 
 constexpr int WA_KMAX = 32;
 struct WaUnit { int64_t kb, rb, wrow; int dA, dC, K, R; };   // kb / rb index kbase / rbase; wrow indexes wmat rows
 struct WaItem { WaUnit u; int unit, col0; };                 // the unit travels with the item (one scalar load)
+// Stages 1 and 2 fused (no T1 in HBM): a wavefront takes a tile (a0..a0+ta, c0..c0+tc) of a block
+// pair, stages the rows of theta(A,P,B) and the columns of R(B,N,C) of every column group (P,N) of
+// the pair's class in LDS, forms each T1 value on the fly and applies W_D as in the column stream.
+struct F12Group { int64_t th_off, r_off; int dP, dN, dB, pad; };   // th_off / r_off -1: zero T1 block
+struct F12Item { WaUnit u; int g0, ng, a0, ta, c0, tc, lds, pad; };
+constexpr int F12_LDS = 1024;    // doubles of LDS staging per wavefront (8 KB: 20 wavefronts per CU)
+constexpr int F12_RMAX = 16;     // largest number of W_D rows
+
 struct WApplyHost
 {
 	std::vector<WaUnit> units;
 	std::vector<WaItem> items;
+	std::vector<F12Group> groups;
+	std::vector<F12Item> items12;
+	bool f12_ok = true;                  // every tile fits F12_LDS and every class has at most F12_RMAX rows
 	std::vector<int64_t> kbase, rbase;   // element offset of column k / row r at a = 0, c = 0 (-1: absent block)
 	std::vector<int> kstride, rstride;   // element stride of a
 	std::vector<int64_t> widx;           // W_D entries as W element indices (-1: zero), WA_KMAX per row
@@ -249,6 +260,10 @@ struct WApply
 {
 	WaUnit* units = nullptr;
 	WaItem* items = nullptr;
+	F12Group* groups = nullptr;
+	F12Item* items12 = nullptr;
+	int nitems12 = 0;
+	bool f12_ok = false;
 	int64_t* kbase = nullptr;
 	int64_t* rbase = nullptr;
 	int* kstride = nullptr;
@@ -261,6 +276,8 @@ struct WApply
 	void release();
 };
 void launch_wapply(const WApply& wa, const double* t1, double* t2, hipStream_t s);
+// stages 1 and 2 fused: T2 from theta (x) and R directly
+void launch_f12(const WApply& wa, const double* x, const double* r, double* t2, hipStream_t s);
 
 // Fused single-launch FP64 matvec. One workgroup owns a "unit": k consecutive columns c of one
 // right-bond sector C of the output. It stages the R columns it needs in LDS, computes
@@ -344,6 +361,7 @@ struct MatvecPlan
 	GemmBases stage_bases[3] = {};
 	WApply wapply;                              // column-stream form of stage 2
 	bool use_wapply = false;
+	bool use_f12 = false;                       // stages 1 and 2 fused (no T1 in HBM)
 	// fused form: one launch
 	FusedPlan fused;
 	// plain form: the reference's operation sequence replayed as individual launches
@@ -379,10 +397,10 @@ void merge_grouped_tables(GroupedTables& dst, const GroupedTables& src);
 void validate_matvec_operands(const Meta& theta, const Meta& w, const Meta& l, const Meta& r);
 // time every launch form of each grouped stage (and the column-stream stage 2 when available) and
 // keep the fastest; sets *use_wapply
-void autotune_grouped(DevGemm* stage, const GemmBases* bases, const WApply* wa, bool* use_wapply, int64_t nx, hipStream_t s,
+void autotune_grouped(DevGemm* stage, const GemmBases* bases, const WApply* wa, bool* use_wapply, bool* use_f12, int64_t nx, hipStream_t s,
 	float (*tuned_ms)[GEMM_NUM_FORMS]);
 // one grouped matvec: stage 1, stage 2 (MFMA tiles or column stream), stage 3
-void launch_grouped_stages(const DevGemm* stage, const GemmBases* bases, const WApply* wa, bool use_wapply, const double* x, double* y,
+void launch_grouped_stages(const DevGemm* stage, const GemmBases* bases, const WApply* wa, bool use_wapply, bool use_f12, const double* x, double* y,
 	hipStream_t s, hipEvent_t* ev);
 
 // FP64 flops of the reference's unfused operation sequence: sum of 2 m n k over its GEMM calls
@@ -427,6 +445,7 @@ struct BatchPlan
 	GemmBases stage_bases[3] = {};
 	WApply wapply;
 	bool use_wapply = false;
+	bool use_f12 = false;
 	double* t1 = nullptr;
 	double* t2 = nullptr;
 	bool use_fused = false;                     // the faster of the two forms, timed at plan creation

@@ -621,6 +621,44 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 			const int unit = (int)wa.units.size();
 			wa.units.push_back(u);
 			for (int col0 = 0; col0 < (int)(dA * dC); col0 += 64) wa.items.push_back(WaItem{ u, unit, col0 });
+			// fused stages 1 + 2: one group per column group (P,N) of the class
+			if (c.R > F12_RMAX) wa.f12_ok = false;
+			const int g0 = (int)wa.groups.size();
+			for (auto& pn : c.cols) {
+				const ct_long P = pn.first, N = pn.second, dP = bc(W, 2, P), dN = bc(W, 3, N);
+				F12Group gr = { -1, -1, (int)dP, (int)dN, 0, 0 };
+				const ct_long B = th.find_qnum(2, qb(th, 0, A) + qb(th, 1, P));
+				if (B >= 0 && t1_off[t1_key(A, P, N, C)] >= 0) {
+					const ct_long ith[3] = { A, P, B };
+					const ct_long irr[4] = { B, N, C, 0 };
+					gr.th_off = x_off + th.offset[th.flat_index(ith)];
+					gr.r_off = r_off + R.offset[R.flat_index(irr)];
+					gr.dB = (int)bc(th, 2, B);
+				}
+				wa.groups.push_back(gr);
+			}
+			const int ng = (int)wa.groups.size() - g0;
+			auto lds_need = [&](int ta, int tc) {
+				int n = 0;
+				for (int gi = g0; gi < g0 + ng; gi++) {
+					const F12Group& gr = wa.groups[gi];
+					if (gr.th_off >= 0) n += ta * gr.dP * gr.dB + gr.dB * gr.dN * tc;
+				}
+				return n;
+			};
+			int tc = (int)std::min<ct_long>(dC, 64);
+			int ta = (int)std::min<ct_long>(dA, std::max(1, 64 / tc));
+			while (lds_need(ta, tc) > F12_LDS) {
+				if (ta > 1) ta = std::max(1, ta / 2);
+				else if (tc > 1) { tc = std::max(1, tc / 2); ta = (int)std::min<ct_long>(dA, std::max(1, 64 / tc)); if (lds_need(ta, tc) > F12_LDS) ta = 1; }
+				else break;
+			}
+			if (lds_need(ta, tc) > F12_LDS) wa.f12_ok = false;
+			for (int a0 = 0; a0 < (int)dA; a0 += ta)
+				for (int c0 = 0; c0 < (int)dC; c0 += tc) {
+					const int ta_i = std::min(ta, (int)dA - a0), tc_i = std::min(tc, (int)dC - c0);
+					wa.items12.push_back(F12Item{ u, g0, ng, a0, ta_i, c0, tc_i, lds_need(ta_i, tc_i), 0 });
+				}
 		}
 	}
 	// stage 3: y(X,Q,C)[x q c] = sum_(A,M) L(O,A,M,X)^T T2(A,M,Q,C)
@@ -670,6 +708,10 @@ void merge_grouped_tables(GroupedTables& dst, const GroupedTables& src)
 	const int u0 = (int)a.units.size();
 	for (WaUnit u : b.units) { u.kb += kb0; u.rb += rb0; u.wrow += wr0; a.units.push_back(u); }
 	for (WaItem it : b.items) { it.unit += u0; it.u.kb += kb0; it.u.rb += rb0; it.u.wrow += wr0; a.items.push_back(it); }
+	const int g0 = (int)a.groups.size();
+	a.groups.insert(a.groups.end(), b.groups.begin(), b.groups.end());
+	for (F12Item it : b.items12) { it.g0 += g0; it.u.kb += kb0; it.u.rb += rb0; it.u.wrow += wr0; a.items12.push_back(it); }
+	a.f12_ok = a.f12_ok && b.f12_ok;
 	for (int64_t v : b.kbase) a.kbase.push_back(v >= 0 ? v + t1s : v);
 	for (int64_t v : b.rbase) a.rbase.push_back(v >= 0 ? v + t2s : v);
 	a.kstride.insert(a.kstride.end(), b.kstride.begin(), b.kstride.end());
@@ -801,7 +843,7 @@ static void build_plain(MatvecPlan& p)
 // warm-up and three timed launches, HIP events on the plan stream); the fastest form is kept.
 // time every launch form of each grouped stage on the plan's own descriptors (x = 1, y scratch)
 // and keep the fastest; stage 2 also tries the column-stream form
-void autotune_grouped(DevGemm* stage, const GemmBases* bases, const WApply* wa, bool* use_wapply, int64_t nx, hipStream_t s,
+void autotune_grouped(DevGemm* stage, const GemmBases* bases, const WApply* wa, bool* use_wapply, bool* use_f12, int64_t nx, hipStream_t s,
 	float (*tuned_ms)[GEMM_NUM_FORMS])
 {
 	double* x = (double*)dev_alloc((size_t)std::max<int64_t>(nx, 1) * sizeof(double));
@@ -820,7 +862,7 @@ void autotune_grouped(DevGemm* stage, const GemmBases* bases, const WApply* wa, 
 		CTG_HIP(hipEventElapsedTime(&ms, e0, e1));
 		return ms / 3;
 	};
-	float best2 = 1e30f;
+	float best1 = 1e30f, best2 = 1e30f;
 	for (int st = 0; st < 3; st++) {
 		DevGemm& g = stage[st];
 		GemmBases b = bases[st];
@@ -835,6 +877,7 @@ void autotune_grouped(DevGemm* stage, const GemmBases* bases, const WApply* wa, 
 			if (ms < best) { best = ms; best_form = f; }
 		}
 		g.form = best_form;
+		if (st == 0) best1 = best;
 		if (st == 1) best2 = best;
 	}
 	if (use_wapply) {
@@ -842,10 +885,22 @@ void autotune_grouped(DevGemm* stage, const GemmBases* bases, const WApply* wa, 
 		if (wa && wa->ok) {
 			const float ms = time3([&] { launch_wapply(*wa, bases[1].B, bases[1].C, s); });
 			*use_wapply = ms < best2;
+			if (*use_wapply) best2 = ms;
 			// CT_STAGE2=mfma|stream overrides the timing-based choice
 			const char* force = getenv("CT_STAGE2");
 			if (force && strcmp(force, "mfma") == 0) *use_wapply = false;
 			if (force && strcmp(force, "stream") == 0) *use_wapply = true;
+		}
+	}
+	if (use_f12) {
+		*use_f12 = false;
+		if (wa && wa->ok && wa->f12_ok) {
+			const float ms = time3([&] { launch_f12(*wa, x, bases[0].B, bases[1].C, s); });
+			*use_f12 = ms < best1 + best2;
+			// CT_STAGE12=fused|separate overrides the timing-based choice
+			const char* force = getenv("CT_STAGE12");
+			if (force && strcmp(force, "separate") == 0) *use_f12 = false;
+			if (force && strcmp(force, "fused") == 0) *use_f12 = true;
 		}
 	}
 	CTG_HIP(hipEventDestroy(e0));
@@ -855,14 +910,17 @@ void autotune_grouped(DevGemm* stage, const GemmBases* bases, const WApply* wa, 
 	dev_free(y);
 }
 
-void launch_grouped_stages(const DevGemm* stage, const GemmBases* bases, const WApply* wa, bool use_wapply, const double* x, double* y,
+void launch_grouped_stages(const DevGemm* stage, const GemmBases* bases, const WApply* wa, bool use_wapply, bool use_f12, const double* x, double* y,
 	hipStream_t s, hipEvent_t* ev)
 {
+	const bool f12 = use_f12 && wa && wa->f12_ok;
 	for (int st = 0; st < 3; st++) {
 		GemmBases b = bases[st];
 		if (st == 0) b.A = x;
 		if (st == 2) b.C = y;
-		if (st == 1 && use_wapply && wa && wa->ok) launch_wapply(*wa, b.B, b.C, s);
+		if (st == 0 && f12) launch_f12(*wa, x, b.B, bases[1].C, s);
+		else if (st == 1 && f12) { /* done by the fused launch */ }
+		else if (st == 1 && use_wapply && wa && wa->ok) launch_wapply(*wa, b.B, b.C, s);
 		else launch_grouped_gemm_f64(stage[st], b, s);
 		if (ev) CTG_HIP(hipEventRecord(ev[st + 1], s));
 	}
@@ -870,7 +928,15 @@ void launch_grouped_stages(const DevGemm* stage, const GemmBases* bases, const W
 
 static void autotune_stages(MatvecPlan& p)
 {
-	autotune_grouped(p.stage, p.stage_bases, &p.wapply, &p.use_wapply, p.theta_meta.nelem, p.stream, p.tuned_ms);
+	autotune_grouped(p.stage, p.stage_bases, &p.wapply, &p.use_wapply, &p.use_f12, p.theta_meta.nelem, p.stream, p.tuned_ms);
+	if (p.use_f12 && p.wapply.f12_ok) {
+		// one launch for stages 1 and 2: theta, R and W in, T2 out (T1 never written)
+		const double t1 = (double)p.interm[0].meta.nelem;
+		p.stage_bytes[0] = p.stage_bytes[0] + p.stage_bytes[1] - 16.0 * t1;
+		p.stage_bytes[1] = 0;
+		p.stage_flops[0] += p.stage_flops[1];
+		p.stage_flops[1] = 0;
+	}
 }
 
 void validate_matvec_operands(const Meta& theta, const Meta& w, const Meta& l, const Meta& r)
@@ -947,6 +1013,7 @@ float matvec_stage_time(MatvecPlan& p, int stage, const double* x, double* y, in
 		if (stage == 0) b.A = x;
 		if (stage == 2) b.C = y;
 		if (p.mode == CT_MATVEC_GROUPED_SPLIT) launch_grouped_gemm_split(p.stage[stage], b, s);
+		else if (p.use_f12 && p.wapply.f12_ok) { if (stage == 0) launch_f12(p.wapply, x, b.B, p.stage_bases[1].C, s); }
 		else if (stage == 1 && p.use_wapply && p.wapply.ok) launch_wapply(p.wapply, b.B, b.C, s);
 		else launch_grouped_gemm_f64(p.stage[stage], b, s);
 	};
@@ -984,7 +1051,7 @@ void matvec_apply(MatvecPlan& p, const double* x, double* y, float* stage_ms)
 			if (stage_ms) CTG_HIP(hipEventRecord(p.ev[st + 1], s));
 		}
 	}
-	else launch_grouped_stages(p.stage, p.stage_bases, &p.wapply, p.use_wapply, x, y, s, stage_ms ? p.ev : nullptr);
+	else launch_grouped_stages(p.stage, p.stage_bases, &p.wapply, p.use_wapply, p.use_f12, x, y, s, stage_ms ? p.ev : nullptr);
 	if (stage_ms) {
 		CTG_HIP(hipEventSynchronize(p.ev[3]));
 		for (int st = 0; st < 3; st++) CTG_HIP(hipEventElapsedTime(&stage_ms[st], p.ev[st], p.ev[st + 1]));

@@ -585,7 +585,7 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 	b->stage_bases[2] = GemmBases{ b->l, b->t2, nullptr };
 	b->wapply.upload(g.wa, b->w, s);
 	if (tune) {
-		autotune_grouped(b->stage, b->stage_bases, &b->wapply, &b->use_wapply, nx, s, nullptr);
+		autotune_grouped(b->stage, b->stage_bases, &b->wapply, &b->use_wapply, &b->use_f12, nx, s, nullptr);
 		b->fused_ok = build_fused_batch(b->fused, probs, s);
 	}
 	else {
@@ -647,7 +647,7 @@ void batch_apply(BatchPlan& b, const double* x, double* y)
 		launch_fused_matvec(b.fused, x, b.w, b.l, b.r, y, b.stream);
 		return;
 	}
-	launch_grouped_stages(b.stage, b.stage_bases, &b.wapply, b.use_wapply, x, y, b.stream, nullptr);
+	launch_grouped_stages(b.stage, b.stage_bases, &b.wapply, b.use_wapply, b.use_f12, x, y, b.stream, nullptr);
 }
 
 }  // namespace ctg

@@ -719,6 +719,78 @@ __global__ __launch_bounds__(256) void k_wapply(const WaUnit* __restrict__ units
 	}
 }
 
+// stages 1 and 2 fused: one wavefront (workgroup of 64) per tile of a block pair
+__global__ __launch_bounds__(64) void k_f12(const F12Item* __restrict__ items, int nitems, const F12Group* __restrict__ groups,
+	const int64_t* __restrict__ rbase, const int* __restrict__ rstride, const double* __restrict__ wmat,
+	const double* __restrict__ x, const double* __restrict__ r, double* __restrict__ t2)
+{
+	__shared__ double sm[F12_LDS];
+	const int w = blockIdx.x;
+	if (w >= nitems) return;
+	const int lane = threadIdx.x;
+	const F12Item it = items[w];
+	const WaUnit& u = it.u;
+	// stage the theta rows a0 .. a0+ta and the R columns c0 .. c0+tc of every present column group
+	int off = 0;
+	for (int g = 0; g < it.ng; g++) {
+		const F12Group gr = groups[it.g0 + g];
+		if (gr.th_off < 0) continue;
+		const int nth = it.ta * gr.dP * gr.dB;
+		const double* __restrict__ ths = x + gr.th_off + (int64_t)it.a0 * gr.dP * gr.dB;
+		for (int e = lane; e < nth; e += 64) sm[off + e] = ths[e];
+		off += nth;
+		const int nr = gr.dB * gr.dN * it.tc;
+		const double* __restrict__ rs = r + gr.r_off + it.c0;
+		for (int e = lane; e < nr; e += 64) {
+			const int row = e / it.tc, ci = e - row * it.tc;
+			sm[off + e] = rs[(int64_t)row * u.dC + ci];
+		}
+		off += nr;
+	}
+	__syncthreads();
+	const bool act = lane < it.ta * it.tc;
+	const int ai = act ? lane / it.tc : 0, ci = act ? lane - (lane / it.tc) * it.tc : 0;
+	double y[F12_RMAX];
+	#pragma unroll
+	for (int q = 0; q < F12_RMAX; q++) y[q] = 0.0;
+	const double* __restrict__ wm = wmat + u.wrow * WA_KMAX;
+	int k = 0;
+	off = 0;
+	for (int g = 0; g < it.ng; g++) {
+		const F12Group gr = groups[it.g0 + g];
+		if (gr.th_off < 0) { k += gr.dP * gr.dN; continue; }
+		const double* ths = sm + off + ai * gr.dP * gr.dB;
+		const double* rs = sm + off + it.ta * gr.dP * gr.dB + ci;
+		for (int p = 0; p < gr.dP; p++) {
+			for (int n = 0; n < gr.dN; n++, k++) {
+				// T1(A,P,N,C)[a p n c] for this lane's column
+				double xv = 0.0;
+				for (int b = 0; b < gr.dB; b++) xv = fma(ths[p * gr.dB + b], rs[(b * gr.dN + n) * it.tc], xv);
+				#pragma unroll
+				for (int q = 0; q < F12_RMAX; q++)
+					if (q < u.R) y[q] = fma(wm[q * WA_KMAX + k], xv, y[q]);
+			}
+		}
+		off += it.ta * gr.dP * gr.dB + gr.dB * gr.dN * it.tc;
+	}
+	if (!act) return;
+	const int a = it.a0 + ai, c = it.c0 + ci;
+	#pragma unroll
+	for (int q = 0; q < F12_RMAX; q++) {
+		if (q < u.R) {
+			const int64_t b = rbase[u.rb + q];
+			if (b >= 0) t2[b + (int64_t)a * rstride[u.rb + q] + c] = y[q];
+		}
+	}
+}
+
+void launch_f12(const WApply& wa, const double* x, const double* r, double* t2, hipStream_t s)
+{
+	if (wa.nitems12 == 0) return;
+	hipLaunchKernelGGL(k_f12, dim3((unsigned)wa.nitems12), dim3(64), 0, s, wa.items12, wa.nitems12, wa.groups, wa.rbase, wa.rstride, wa.wmat, x, r, t2);
+	CTG_HIP(hipGetLastError());
+}
+
 // dense W_D copies: wmat[i] = W[widx[i]] (0 where widx < 0)
 __global__ __launch_bounds__(256) void k_gather_w(const double* __restrict__ w, const int64_t* __restrict__ widx, int64_t n, double* __restrict__ wmat)
 {
@@ -756,15 +828,25 @@ void WApply::upload(const WApplyHost& h, const double* w, hipStream_t s)
 	}
 	dev_free(widx);   // reused only behind the gather kernel on s
 	nitems = (int)h.items.size();
+	f12_ok = h.f12_ok && !h.items12.empty();
+	if (f12_ok) {
+		groups = upload_array(h.groups, s);
+		items12 = upload_array(h.items12, s);
+		nitems12 = (int)h.items12.size();
+	}
 	kmax = h.kmax;
 }
 
 void WApply::release()
 {
 	dev_free(units); dev_free(items); dev_free(kbase); dev_free(rbase); dev_free(kstride); dev_free(rstride); dev_free(wmat);
+	dev_free(groups); dev_free(items12);
 	units = nullptr; items = nullptr; kbase = nullptr; rbase = nullptr; kstride = nullptr; rstride = nullptr; wmat = nullptr;
+	groups = nullptr; items12 = nullptr;
 	nitems = 0;
+	nitems12 = 0;
 	ok = false;
+	f12_ok = false;
 }
 
 void launch_wapply(const WApply& wa, const double* t1, double* t2, hipStream_t s)

@@ -113,15 +113,21 @@ def test_matvec_all_forms_match_oracle(model, L, chi):
         assert np.array_equal(y.numpy(), got)
 
 
-@pytest.mark.parametrize("stage2", ["mfma", "stream"])
+@pytest.mark.parametrize("stage2", ["mfma", "stream", "fused12"])
 def test_grouped_stage2_forms_match_oracle(stage2, monkeypatch):
-    """Both stage-2 forms of the grouped matvec (MFMA tiles, column stream) against the oracle."""
-    monkeypatch.setenv("CT_STAGE2", stage2)
+    """The stage-2 forms of the grouped matvec (MFMA tiles, column stream, stages 1 and 2 fused)
+    against the oracle."""
+    monkeypatch.setenv("CT_STAGE2", "mfma" if stage2 == "mfma" else "stream")
+    monkeypatch.setenv("CT_STAGE12", "fused" if stage2 == "fused12" else "separate")
     for model, L, chi in [("hubbard", 32, 512), ("xxz", 32, 256), ("hubbard", 8, 24)]:
         theta, h2, left, right, _ = orc.matvec_problem(model, L, chi, 4242)
         want = orc.apply_local_hamiltonian(theta, h2, left, right).serialize()
         plan = ct.MatvecPlan(theta, h2, left, right, ct.GROUPED_FP64)
-        assert (plan.forms()[0][1] == 4) == (stage2 == "stream")
+        expect = {"mfma": 0, "stream": 4, "fused12": 5}[stage2]
+        if stage2 == "mfma":
+            assert plan.forms()[0][1] not in (4, 5)
+        else:
+            assert plan.forms()[0][1] == expect
         x = ct.DeviceArray.from_numpy(theta.serialize())
         y = ct.DeviceArray(plan.dim)
         plan.apply(x, y)

@@ -234,7 +234,7 @@ DevTensor dev_split_axis(const DevTensor& t, int i_ax, const ct_long* new_dims, 
 // it loads their K inputs (T1 read once), and writes their R outputs (T2 written once); W_D comes
 // from a dense, zero-padded copy through scalar loads.
 constexpr int WA_KMAX = 32;
-struct WaUnit { int64_t kb, rb, wrow; int dA, dC, K, R; };   // kb / rb index kbase / rbase; wrow indexes wmat rows
+struct WaUnit { int64_t kb, rb, wrow, wcol; int dA, dC, K, R; };   // kb / rb index kbase / rbase; wrow indexes wmat rows, wcol wmatT columns
 struct WaItem { WaUnit u; int unit, col0; };                 // the unit travels with the item (one scalar load)
 // Stages 1 and 2 fused (no T1 in HBM): a wavefront takes a tile (a0..a0+ta, c0..c0+tc) of a block
 // pair, stages the rows of theta(A,P,B) and the columns of R(B,N,C) of every column group (P,N) of
@@ -254,6 +254,7 @@ struct WApplyHost
 	std::vector<int64_t> kbase, rbase;   // element offset of column k / row r at a = 0, c = 0 (-1: absent block)
 	std::vector<int> kstride, rstride;   // element stride of a
 	std::vector<int64_t> widx;           // W_D entries as W element indices (-1: zero), WA_KMAX per row
+	std::vector<int64_t> widxT;          // the same, transposed: F12_RMAX per column (k), WA_KMAX columns per class
 	int kmax = 0;                        // largest K of any class
 };
 struct WApply
@@ -269,6 +270,7 @@ struct WApply
 	int* kstride = nullptr;
 	int* rstride = nullptr;
 	double* wmat = nullptr;
+	double* wmatT = nullptr;             // W_D columns (F12_RMAX entries each) for the fused stages
 	int nitems = 0;
 	int kmax = 0;
 	bool ok = false;                     // every class fits WA_KMAX columns

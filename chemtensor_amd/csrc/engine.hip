@@ -555,7 +555,7 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 	// column-stream form of stage 2: classes D = qM + qQ = qP + qN, one unit per block pair (A,C)
 	{
 		WApplyHost& wa = g.wa;
-		struct Cls { int64_t wrow = 0; int K = 0, R = 0; std::vector<std::pair<ct_long, ct_long>> rows, cols; };
+		struct Cls { int64_t wrow = 0, wcol = 0; int K = 0, R = 0; std::vector<std::pair<ct_long, ct_long>> rows, cols; };
 		std::map<qnumber, Cls> classes;
 		auto get_class = [&](qnumber D) -> const Cls& {
 			auto it = classes.find(D);
@@ -566,6 +566,7 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 			for (ct_long P = 0; P < nP; P++) for (ct_long N = 0; N < nN; N++)
 				if (qb(W, 2, P) + qb(W, 3, N) == D) { c.cols.emplace_back(P, N); c.K += (int)(bc(W, 2, P) * bc(W, 3, N)); }
 			c.wrow = (int64_t)(wa.widx.size() / WA_KMAX);
+			c.wcol = (int64_t)(wa.widxT.size() / F12_RMAX);
 			wa.kmax = std::max(wa.kmax, c.K);
 			if (c.K <= WA_KMAX) {
 				for (auto& mq : c.rows) {
@@ -584,6 +585,11 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 						wa.widx.insert(wa.widx.end(), row.begin(), row.end());
 					}
 				}
+				// transposed copy for the fused stages: column k holds rows 0 .. min(R, F12_RMAX)
+				std::vector<int64_t> colT((size_t)WA_KMAX * F12_RMAX, -1);
+				for (int r = 0; r < std::min(c.R, F12_RMAX); r++)
+					for (int k = 0; k < WA_KMAX; k++) colT[(size_t)k * F12_RMAX + r] = wa.widx[(size_t)(c.wrow + r) * WA_KMAX + k];
+				wa.widxT.insert(wa.widxT.end(), colT.begin(), colT.end());
 			}
 			return classes.emplace(D, std::move(c)).first->second;
 		};
@@ -598,6 +604,7 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 			u.kb = (int64_t)wa.kbase.size();
 			u.rb = (int64_t)wa.rbase.size();
 			u.wrow = c.wrow;
+			u.wcol = c.wcol;
 			u.dA = (int)dA;
 			u.dC = (int)dC;
 			u.K = c.K;
@@ -705,18 +712,20 @@ void merge_grouped_tables(GroupedTables& dst, const GroupedTables& src)
 	WApplyHost& a = dst.wa;
 	const WApplyHost& b = src.wa;
 	const int64_t kb0 = (int64_t)a.kbase.size(), rb0 = (int64_t)a.rbase.size(), wr0 = (int64_t)(a.widx.size() / WA_KMAX);
+	const int64_t wc0 = (int64_t)(a.widxT.size() / F12_RMAX);
 	const int u0 = (int)a.units.size();
-	for (WaUnit u : b.units) { u.kb += kb0; u.rb += rb0; u.wrow += wr0; a.units.push_back(u); }
-	for (WaItem it : b.items) { it.unit += u0; it.u.kb += kb0; it.u.rb += rb0; it.u.wrow += wr0; a.items.push_back(it); }
+	for (WaUnit u : b.units) { u.kb += kb0; u.rb += rb0; u.wrow += wr0; u.wcol += wc0; a.units.push_back(u); }
+	for (WaItem it : b.items) { it.unit += u0; it.u.kb += kb0; it.u.rb += rb0; it.u.wrow += wr0; it.u.wcol += wc0; a.items.push_back(it); }
 	const int g0 = (int)a.groups.size();
 	a.groups.insert(a.groups.end(), b.groups.begin(), b.groups.end());
-	for (F12Item it : b.items12) { it.g0 += g0; it.u.kb += kb0; it.u.rb += rb0; it.u.wrow += wr0; a.items12.push_back(it); }
+	for (F12Item it : b.items12) { it.g0 += g0; it.u.kb += kb0; it.u.rb += rb0; it.u.wrow += wr0; it.u.wcol += wc0; a.items12.push_back(it); }
 	a.f12_ok = a.f12_ok && b.f12_ok;
 	for (int64_t v : b.kbase) a.kbase.push_back(v >= 0 ? v + t1s : v);
 	for (int64_t v : b.rbase) a.rbase.push_back(v >= 0 ? v + t2s : v);
 	a.kstride.insert(a.kstride.end(), b.kstride.begin(), b.kstride.end());
 	a.rstride.insert(a.rstride.end(), b.rstride.begin(), b.rstride.end());
 	a.widx.insert(a.widx.end(), b.widx.begin(), b.widx.end());
+	a.widxT.insert(a.widxT.end(), b.widxT.begin(), b.widxT.end());
 	a.kmax = std::max(a.kmax, b.kmax);
 	dst.t1_elems += src.t1_elems;
 	dst.t2_elems += src.t2_elems;

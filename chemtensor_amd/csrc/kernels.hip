@@ -721,7 +721,7 @@ __global__ __launch_bounds__(256) void k_wapply(const WaUnit* __restrict__ units
 
 // stages 1 and 2 fused: one wavefront (workgroup of 64) per tile of a block pair
 __global__ __launch_bounds__(64) void k_f12(const F12Item* __restrict__ items, int nitems, const F12Group* __restrict__ groups,
-	const int64_t* __restrict__ rbase, const int* __restrict__ rstride, const double* __restrict__ wmat,
+	const int64_t* __restrict__ rbase, const int* __restrict__ rstride, const double* __restrict__ wmatT,
 	const double* __restrict__ x, const double* __restrict__ r, double* __restrict__ t2)
 {
 	__shared__ double sm[F12_LDS];
@@ -753,7 +753,7 @@ __global__ __launch_bounds__(64) void k_f12(const F12Item* __restrict__ items, i
 	double y[F12_RMAX];
 	#pragma unroll
 	for (int q = 0; q < F12_RMAX; q++) y[q] = 0.0;
-	const double* __restrict__ wm = wmat + u.wrow * WA_KMAX;
+	const double* __restrict__ wm = wmatT + u.wcol * F12_RMAX;   // column k at wm[k * F12_RMAX]
 	int k = 0;
 	off = 0;
 	for (int g = 0; g < it.ng; g++) {
@@ -766,9 +766,9 @@ __global__ __launch_bounds__(64) void k_f12(const F12Item* __restrict__ items, i
 				// T1(A,P,N,C)[a p n c] for this lane's column
 				double xv = 0.0;
 				for (int b = 0; b < gr.dB; b++) xv = fma(ths[p * gr.dB + b], rs[(b * gr.dN + n) * it.tc], xv);
+				const double* __restrict__ wk = wm + (int64_t)k * F12_RMAX;
 				#pragma unroll
-				for (int q = 0; q < F12_RMAX; q++)
-					if (q < u.R) y[q] = fma(wm[q * WA_KMAX + k], xv, y[q]);
+				for (int q = 0; q < F12_RMAX; q++) y[q] = fma(wk[q], xv, y[q]);   // zero beyond R
 			}
 		}
 		off += it.ta * gr.dP * gr.dB + gr.dB * gr.dN * it.tc;
@@ -787,7 +787,7 @@ __global__ __launch_bounds__(64) void k_f12(const F12Item* __restrict__ items, i
 void launch_f12(const WApply& wa, const double* x, const double* r, double* t2, hipStream_t s)
 {
 	if (wa.nitems12 == 0) return;
-	hipLaunchKernelGGL(k_f12, dim3((unsigned)wa.nitems12), dim3(64), 0, s, wa.items12, wa.nitems12, wa.groups, wa.rbase, wa.rstride, wa.wmat, x, r, t2);
+	hipLaunchKernelGGL(k_f12, dim3((unsigned)wa.nitems12), dim3(64), 0, s, wa.items12, wa.nitems12, wa.groups, wa.rbase, wa.rstride, wa.wmatT, x, r, t2);
 	CTG_HIP(hipGetLastError());
 }
 
@@ -833,6 +833,14 @@ void WApply::upload(const WApplyHost& h, const double* w, hipStream_t s)
 		groups = upload_array(h.groups, s);
 		items12 = upload_array(h.items12, s);
 		nitems12 = (int)h.items12.size();
+		int64_t* widxT = upload_array(h.widxT, s);
+		const int64_t nt = (int64_t)h.widxT.size();
+		wmatT = (double*)dev_alloc((size_t)std::max<int64_t>(nt, 1) * sizeof(double));
+		if (nt > 0) {
+			hipLaunchKernelGGL(k_gather_w, dim3((unsigned)std::min<int64_t>((nt + 255) / 256, 4096)), dim3(256), 0, s, w, (const int64_t*)widxT, nt, wmatT);
+			CTG_HIP(hipGetLastError());
+		}
+		dev_free(widxT);
 	}
 	kmax = h.kmax;
 }
@@ -840,7 +848,8 @@ void WApply::upload(const WApplyHost& h, const double* w, hipStream_t s)
 void WApply::release()
 {
 	dev_free(units); dev_free(items); dev_free(kbase); dev_free(rbase); dev_free(kstride); dev_free(rstride); dev_free(wmat);
-	dev_free(groups); dev_free(items12);
+	dev_free(groups); dev_free(items12); dev_free(wmatT);
+	wmatT = nullptr;
 	units = nullptr; items = nullptr; kbase = nullptr; rbase = nullptr; kstride = nullptr; rstride = nullptr; wmat = nullptr;
 	groups = nullptr; items12 = nullptr;
 	nitems = 0;

@@ -27,6 +27,7 @@ LEADING, TRAILING = 0, 1
 AXIS_IN, AXIS_OUT = -1, 1
 PLAIN_FP64, GROUPED_FP64, GROUPED_SPLIT, FUSED_FP64 = 0, 1, 2, 3
 
+CT_ERR_HIP, CT_ERR_UNSUPPORTED, CT_ERR_NOMEM, CT_ERR_ARG = -100, -101, -102, -103
 ERRORS = {-1: "Krylov breakdown / factorisation failure", -2: "eigensolver failure", -100: "HIP runtime error",
           -101: "unsupported input", -102: "out of device memory / budget", -103: "invalid argument"}
 
@@ -76,6 +77,7 @@ def lib():
             "mpo_merge_tensor_pair": (None, [P, P, bst]),
             "ct_matvec_plan_create": (C.c_int, [P, P, P, P, C.c_int, C.POINTER(P)]),
             "ct_matvec_plan_destroy": (C.c_int, [P]),
+            "ct_matvec_plan_check": (C.c_int, [P, P, P, P]),
             "ct_matvec_plan_dim": (ct_long, [P]),
             "ct_matvec_plan_flops": (C.c_double, [P]),
             "ct_matvec_plan_bytes": (C.c_double, [P]),
@@ -233,6 +235,11 @@ def mpo_merge_tensor_pair(a0, a1):
     lib().mpo_merge_tensor_pair(_ptr(a0), _ptr(a1), r.sptr)
     r.live = True
     return r
+
+
+def matvec_plan_check(a, w, l, r):
+    """Host-only validation of the grouped matvec tables (ct_matvec_plan_check); 0 when valid."""
+    return lib().ct_matvec_plan_check(_ptr(a), _ptr(w), _ptr(l), _ptr(r))
 
 
 def generate_matvec_problem(model, nsites, chi, seed):

@@ -228,6 +228,20 @@ int eigensystem_krylov_symmetric(const ct_long n, lanczos_linear_func_d afunc, c
 	}
 }
 
+int ct_matvec_plan_check(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l,
+	const struct block_sparse_tensor* r)
+{
+	if (!a || !w || !l || !r) return CT_ERR_ARG;
+	try {
+		const Meta ma = meta_of(a), mw = meta_of(w), ml = meta_of(l), mr = meta_of(r);
+		validate_matvec_operands(ma, mw, ml, mr);
+		return check_grouped_tables(ma, mw, ml, mr);
+	}
+	catch (const HipError& e) {
+		return map_error(e);
+	}
+}
+
 int ct_matvec_batch_create(const int nprob, const struct block_sparse_tensor* const* a, const struct block_sparse_tensor* const* w,
 	const struct block_sparse_tensor* const* l, const struct block_sparse_tensor* const* r, ct_matvec_batch** batch)
 {

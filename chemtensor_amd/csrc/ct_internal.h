@@ -392,6 +392,8 @@ struct GroupedTables
 };
 void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const Meta& L, const Meta& R,
 	int64_t x_off, int64_t w_off, int64_t l_off, int64_t r_off);
+// host check of the tables (bounds, each intermediate / output entry written once); 0 when valid
+int check_grouped_tables(const Meta& th, const Meta& W, const Meta& L, const Meta& R);
 // append src, planned on its own (intermediates from offset 0), behind the intermediates of dst
 void merge_grouped_tables(GroupedTables& dst, const GroupedTables& src);
 

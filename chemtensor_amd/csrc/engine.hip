@@ -732,6 +732,91 @@ void merge_grouped_tables(GroupedTables& dst, const GroupedTables& src)
 	for (int i = 0; i < 3; i++) dst.stage_bytes[i] += src.stage_bytes[i];
 }
 
+// Host check of the grouped tables of one problem: every operand index inside its buffer and every
+// entry of T1, T2 and y written by exactly one output of each form (grouped stages, column-stream
+// stage 2, fused stages 1 and 2). Returns 0 or the number of the first failed check.
+int check_grouped_tables(const Meta& th, const Meta& W, const Meta& L, const Meta& R)
+{
+	GroupedTables g;
+	plan_grouped_tables(g, th, W, L, R, 0, 0, 0, 0);
+	auto elem = [](const void* p) { return (int64_t)((intptr_t)p / (intptr_t)sizeof(double)); };
+	auto check_batch = [&](const GemmBatch& b, int64_t na, int64_t nb, int64_t nc, std::vector<int>* writes) -> bool {
+		for (const GemmTask& t : b.tasks) {
+			const int64_t c0 = elem(t.C);
+			for (int i = 0; i < t.M; i++) for (int j = 0; j < t.N; j++) {
+				const int jb = j / t.nci, jc = j % t.nci;
+				const int64_t ci = c0 + (int64_t)jb * t.sc_batch + jc + (int64_t)i * t.sc_i;
+				if (ci < 0 || ci >= nc) return false;
+				if (writes) (*writes)[ci]++;
+			}
+			for (int q = t.t0; q < t.t1; q++) {
+				const GemmTerm& m = b.terms[q];
+				const int64_t a0 = elem(m.A), b0 = elem(m.B);
+				const int jb = (t.N - 1) / t.nci, jc = std::min(t.N - 1, t.nci - 1);
+				const int64_t amax = a0 + (int64_t)(t.M - 1) * m.sa_i + (int64_t)(m.K - 1) * m.sa_k;
+				const int64_t bmax = b0 + (int64_t)jb * m.sb_batch + (int64_t)jc * m.sb_j + (int64_t)(m.K - 1) * m.sb_k;
+				if (a0 < 0 || amax >= na || b0 < 0 || bmax >= nb) return false;
+			}
+		}
+		return true;
+	};
+	auto once = [](const std::vector<int>& w) { for (int v : w) if (v != 1) return false; return true; };
+	std::vector<int> w1((size_t)g.t1_elems, 0), w2((size_t)g.t2_elems, 0), wy((size_t)th.nelem, 0);
+	if (!check_batch(g.s1, th.nelem, R.nelem, g.t1_elems, &w1) || !once(w1)) return 1;
+	if (!check_batch(g.s2, W.nelem, g.t1_elems, g.t2_elems, &w2) || !once(w2)) return 2;
+	if (!check_batch(g.s3, L.nelem, g.t2_elems, th.nelem, &wy) || !once(wy)) return 3;
+	const WApplyHost& wa = g.wa;
+	if (wa.kmax <= WA_KMAX) {
+		std::vector<int> ws((size_t)g.t2_elems, 0);
+		for (const WaItem& it : wa.items) {
+			const WaUnit& u = it.u;
+			for (int lane = 0; lane < 64; lane++) {
+				const int j = it.col0 + lane;
+				if (j >= u.dA * u.dC) continue;
+				const int a = j / u.dC, c = j % u.dC;
+				for (int k = 0; k < u.K; k++) {
+					const int64_t b = wa.kbase[u.kb + k];
+					if (b >= 0 && (b + (int64_t)a * wa.kstride[u.kb + k] + c >= g.t1_elems)) return 4;
+				}
+				for (int r = 0; r < u.R; r++) {
+					const int64_t b = wa.rbase[u.rb + r];
+					if (b < 0) continue;
+					const int64_t i = b + (int64_t)a * wa.rstride[u.rb + r] + c;
+					if (i < 0 || i >= g.t2_elems) return 4;
+					ws[i]++;
+				}
+			}
+		}
+		if (!once(ws)) return 5;
+	}
+	if (wa.kmax <= WA_KMAX && wa.f12_ok) {
+		std::vector<int> wf((size_t)g.t2_elems, 0);
+		for (const F12Item& it : wa.items12) {
+			const WaUnit& u = it.u;
+			if (it.ta * it.tc > 64 || it.lds > F12_LDS || u.R > F12_RMAX) return 6;
+			int lds = 0;
+			for (int gi = it.g0; gi < it.g0 + it.ng; gi++) {
+				const F12Group& gr = wa.groups[gi];
+				if (gr.th_off < 0) continue;
+				if (gr.th_off + (int64_t)(it.a0 + it.ta) * gr.dP * gr.dB > th.nelem) return 6;
+				if (gr.r_off + (int64_t)(gr.dB * gr.dN - 1) * u.dC + it.c0 + it.tc > R.nelem) return 6;
+				lds += it.ta * gr.dP * gr.dB + gr.dB * gr.dN * it.tc;
+			}
+			if (lds != it.lds) return 6;
+			for (int ai = 0; ai < it.ta; ai++) for (int ci = 0; ci < it.tc; ci++)
+				for (int r = 0; r < u.R; r++) {
+					const int64_t b = wa.rbase[u.rb + r];
+					if (b < 0) continue;
+					const int64_t i = b + (int64_t)(it.a0 + ai) * wa.rstride[u.rb + r] + it.c0 + ci;
+					if (i < 0 || i >= g.t2_elems) return 6;
+					wf[i]++;
+				}
+		}
+		if (!once(wf)) return 7;
+	}
+	return 0;
+}
+
 static void build_grouped(MatvecPlan& p)
 {
 	GroupedTables g;

@@ -158,6 +158,12 @@ enum ct_matvec_mode
 int ct_matvec_plan_create(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l, const struct block_sparse_tensor* r,
 	const int mode, ct_matvec_plan** plan);
 int ct_matvec_plan_destroy(ct_matvec_plan* plan);
+/// Host-only check of the tables a grouped plan would launch (no device needed): every operand index
+/// inside its buffer and every entry of the intermediates and of the result written by exactly one
+/// output, for the grouped stages, the column-stream stage 2 and the fused stages 1 and 2.
+/// Returns 0 when valid, a positive check number otherwise, or a CT_ERR_* code.
+int ct_matvec_plan_check(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l,
+	const struct block_sparse_tensor* r);
 /// length of the packed vectors (= block_sparse_tensor_num_elements_blocks(a))
 ct_long ct_matvec_plan_dim(const ct_matvec_plan* plan);
 /// algorithmic FP64 flops of one matvec: sum over the reference's GEMM calls of 2 m n k

@@ -1107,7 +1107,7 @@ float matvec_stage_time(MatvecPlan& p, int stage, const double* x, double* y, in
 		if (stage == 0) b.A = x;
 		if (stage == 2) b.C = y;
 		if (p.mode == CT_MATVEC_GROUPED_SPLIT) launch_grouped_gemm_split(p.stage[stage], b, s);
-		else if (p.use_f12 && p.wapply.f12_ok) { if (stage == 0) launch_f12(p.wapply, x, b.B, p.stage_bases[1].C, s); }
+		else if (stage < 2 && p.use_f12 && p.wapply.f12_ok) { if (stage == 0) launch_f12(p.wapply, x, b.B, p.stage_bases[1].C, s); }
 		else if (stage == 1 && p.use_wapply && p.wapply.ok) launch_wapply(p.wapply, b.B, b.C, s);
 		else launch_grouped_gemm_f64(p.stage[stage], b, s);
 	};

@@ -11,6 +11,31 @@
 
 #include "ct_internal.h"
 
+#include <atomic>
+#include <numeric>
+#include <thread>
+
+namespace ctg {
+
+// host threads lent to block SVDs across all concurrent callers (at most the core count in use)
+static std::atomic<int> svd_threads_busy(0);
+
+static int host_svd_threads(int want)
+{
+	const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+	const int cap = std::min(16, hw);
+	int busy = svd_threads_busy.load();
+	for (;;) {
+		const int n = std::min(want, cap - busy);
+		if (n <= 1) return 1;   // nothing taken: the caller runs the blocks itself
+		if (svd_threads_busy.compare_exchange_weak(busy, busy + n)) return n;
+	}
+}
+
+static void host_svd_threads_release(int n) { svd_threads_busy -= n; }
+
+}  // namespace ctg
+
 namespace ctg {
 
 static Meta matrix_meta(ct_long d0, const qnumber* q0, ct_long d1, const qnumber* q1, int ad0, int ad1)
@@ -43,7 +68,10 @@ int host_block_svd(const HostPacked& a, HostPacked& u, std::vector<double>& s, H
 		u.v[(size_t)u.meta.offset[u.meta.flat_index(ib)]] = 1;
 		return 0;
 	}
-	int ret = 0;
+	// the blocks are independent: large splits run them in parallel host threads
+	struct Job { const double* a; double* ub; double* vb; double* sb; ct_long r, c; double work; };
+	std::vector<Job> jobs;
+	double total = 0;
 	for (ct_long i = 0; i < m.dim_blocks[0]; i++) for (ct_long j = 0; j < m.dim_blocks[1]; j++) {
 		const ct_long ib[2] = { i, j };
 		const ct_long ka = m.flat_index(ib);
@@ -51,13 +79,30 @@ int host_block_svd(const HostPacked& a, HostPacked& u, std::vector<double>& s, H
 		const ct_long r = m.block_count[0][i], c = m.block_count[1][j], k = std::min(r, c);
 		const ct_long kq = u.meta.find_qnum(1, m.qnums_blocks[1][j]);
 		const ct_long iu[2] = { i, kq }, iv[2] = { kq, j };
-		double* ub = u.v.data() + u.meta.offset[u.meta.flat_index(iu)];
-		double* vb = vh.v.data() + vh.meta.offset[vh.meta.flat_index(iv)];
-		std::vector<double> sb((size_t)k);
-		if (host_svd(r, c, a.v.data() + m.offset[ka], ub, sb.data(), vb) != 0) ret = -1;
 		ct_long l0 = 0;
 		while (l0 < ni && qi[l0] != m.qnums_blocks[1][j]) l0++;
-		std::copy(sb.begin(), sb.end(), s.begin() + l0);
+		const double w = (double)std::max(r, c) * (double)k * (double)k;
+		jobs.push_back(Job{ a.v.data() + m.offset[ka], u.v.data() + u.meta.offset[u.meta.flat_index(iu)],
+			vh.v.data() + vh.meta.offset[vh.meta.flat_index(iv)], s.data() + l0, r, c, w });
+		total += w;
+	}
+	std::atomic<int> ret(0);
+	auto run = [&](size_t i) { if (host_svd(jobs[i].r, jobs[i].c, jobs[i].a, jobs[i].ub, jobs[i].sb, jobs[i].vb) != 0) ret = -1; };
+	const int nt = (total > 4e6) ? host_svd_threads((int)jobs.size()) : 1;
+	if (nt <= 1) {
+		for (size_t i = 0; i < jobs.size(); i++) run(i);
+	}
+	else {
+		// largest blocks first, dealt dynamically
+		std::vector<size_t> order(jobs.size());
+		std::iota(order.begin(), order.end(), 0);
+		std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return jobs[x].work > jobs[y].work; });
+		std::atomic<size_t> next(0);
+		std::vector<std::thread> th;
+		for (int t = 0; t < nt; t++)
+			th.emplace_back([&] { for (size_t q; (q = next++) < order.size();) run(order[q]); });
+		for (auto& x : th) x.join();
+		host_svd_threads_release(nt);
 	}
 	return ret;
 }

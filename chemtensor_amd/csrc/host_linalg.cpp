@@ -186,25 +186,39 @@ int host_rq(ct_long m, ct_long n, const double* a, double* r, double* q)
 }
 
 // one-sided Jacobi on the columns of g (M x N column-major, M >= N); v accumulates rotations
+// One-sided (Hestenes) Jacobi on the columns of g (M x N, column-major), accumulating the rotations
+// in v (N x N). Column norms are computed once per sweep and updated with each rotation
+// (alpha' = alpha - t gamma, beta' = beta + t gamma), so a pair costs one dot product and two
+// column updates.
 static int jacobi(ct_long M, ct_long N, double* g, double* v)
 {
 	std::fill(v, v + (size_t)(N * N), 0.0);
 	for (ct_long i = 0; i < N; i++) CM(v, N, i, i) = 1;
 	const double tol = 2.2204460492503131e-16 * std::max(std::sqrt((double)M), 4.0);
+	std::vector<double> nrm((size_t)N);
 	for (int sweep = 0; sweep < 80; sweep++) {
 		bool rotated = false;
+		for (ct_long p = 0; p < N; p++) {
+			const double* gp = &CM(g, M, 0, p);
+			double d = 0;
+			for (ct_long i = 0; i < M; i++) d += gp[i] * gp[i];
+			nrm[p] = d;
+		}
 		for (ct_long p = 0; p < N - 1; p++) {
 			for (ct_long q = p + 1; q < N; q++) {
 				double* gp = &CM(g, M, 0, p);
 				double* gq = &CM(g, M, 0, q);
-				double al = 0, be = 0, ga = 0;
-				for (ct_long i = 0; i < M; i++) { al += gp[i] * gp[i]; be += gq[i] * gq[i]; ga += gp[i] * gq[i]; }
+				const double al = nrm[p], be = nrm[q];
+				double ga = 0;
+				for (ct_long i = 0; i < M; i++) ga += gp[i] * gq[i];
 				if (ga == 0 || std::fabs(ga) <= tol * std::sqrt(al * be)) continue;
 				rotated = true;
 				const double zeta = (be - al) / (2 * ga);
 				const double t = std::copysign(1.0, zeta) / (std::fabs(zeta) + std::sqrt(1 + zeta * zeta));
 				const double c = 1 / std::sqrt(1 + t * t), s = c * t;
 				for (ct_long i = 0; i < M; i++) { const double x = gp[i], y = gq[i]; gp[i] = c * x - s * y; gq[i] = s * x + c * y; }
+				nrm[p] = al - t * ga;
+				nrm[q] = be + t * ga;
 				double* vp = &CM(v, N, 0, p);
 				double* vq = &CM(v, N, 0, q);
 				for (ct_long i = 0; i < N; i++) { const double x = vp[i], y = vq[i]; vp[i] = c * x - s * y; vq[i] = s * x + c * y; }

@@ -20,6 +20,8 @@ CPP_SOURCES = ["host_tensor.cpp", "host_linalg.cpp", "host_models.cpp", "host_bo
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
 # gfx950 device code; packed FP32 instructions are disabled for all HIP code
 HIP_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# host-only sources (block bookkeeping, host linear algebra): AVX2 + FMA servers
+HOST_FLAGS = ["-march=x86-64-v3"]
 
 
 def hipcc():
@@ -35,6 +37,8 @@ def _compile(src):
     cmd = [hipcc(), "-c", path, "-o", obj] + COMMON
     if src.endswith(".hip"):
         cmd += HIP_FLAGS
+    else:
+        cmd += HOST_FLAGS
     subprocess.run(cmd, check=True, capture_output=True, text=True)
     return obj
 

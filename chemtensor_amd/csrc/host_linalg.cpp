@@ -200,17 +200,33 @@ static int jacobi(ct_long M, ct_long N, double* g, double* v)
 		bool rotated = false;
 		for (ct_long p = 0; p < N; p++) {
 			const double* gp = &CM(g, M, 0, p);
-			double d = 0;
-			for (ct_long i = 0; i < M; i++) d += gp[i] * gp[i];
-			nrm[p] = d;
+			double d4[4] = { 0, 0, 0, 0 };
+			ct_long i = 0;
+			for (; i + 4 <= M; i += 4) {
+				d4[0] += gp[i] * gp[i];
+				d4[1] += gp[i + 1] * gp[i + 1];
+				d4[2] += gp[i + 2] * gp[i + 2];
+				d4[3] += gp[i + 3] * gp[i + 3];
+			}
+			for (; i < M; i++) d4[0] += gp[i] * gp[i];
+			nrm[p] = (d4[0] + d4[1]) + (d4[2] + d4[3]);
 		}
 		for (ct_long p = 0; p < N - 1; p++) {
 			for (ct_long q = p + 1; q < N; q++) {
 				double* gp = &CM(g, M, 0, p);
 				double* gq = &CM(g, M, 0, q);
 				const double al = nrm[p], be = nrm[q];
-				double ga = 0;
-				for (ct_long i = 0; i < M; i++) ga += gp[i] * gq[i];
+				// four independent partial sums (vectorisable, fixed order)
+				double g4[4] = { 0, 0, 0, 0 };
+				ct_long i = 0;
+				for (; i + 4 <= M; i += 4) {
+					g4[0] += gp[i] * gq[i];
+					g4[1] += gp[i + 1] * gq[i + 1];
+					g4[2] += gp[i + 2] * gq[i + 2];
+					g4[3] += gp[i + 3] * gq[i + 3];
+				}
+				for (; i < M; i++) g4[0] += gp[i] * gq[i];
+				const double ga = (g4[0] + g4[1]) + (g4[2] + g4[3]);
 				if (ga == 0 || std::fabs(ga) <= tol * std::sqrt(al * be)) continue;
 				rotated = true;
 				const double zeta = (be - al) / (2 * ga);

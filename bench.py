@@ -169,6 +169,7 @@ def bench_serve(ct, nreq, rate, L_range, chi_range, seed, budget, max_active, ti
         "workload": f"{nreq} DMRG ground-state solves, Poisson arrivals at {rate}/s, XXZ (Delta~U[0,2]) or Hubbard (U~U[0,8]), "
                     f"L~U{{{L_range[0]}..{L_range[1]}}}, chi~U{{{chi_range[0]}..{chi_range[1]}}}, stop at |dE_sweep| < 1e-10",
         "requests": nreq, "solved": len(done), "failed": int(sum(1 for r in res if r.status < 0)),
+        "failures": [{"request": desc[i], "status": int(r.status)} for i, r in enumerate(res) if r.status != 0],
         "arrival_rate_per_s": rate,
         "solves_per_s": round(len(done) / makespan, 4),
         "latency_p50_s": round(float(np.percentile(lat, 50)), 4),

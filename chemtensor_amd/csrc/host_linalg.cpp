@@ -187,57 +187,43 @@ int host_rq(ct_long m, ct_long n, const double* a, double* r, double* q)
 
 // one-sided Jacobi on the columns of g (M x N column-major, M >= N); v accumulates rotations
 // One-sided (Hestenes) Jacobi on the columns of g (M x N, column-major), accumulating the rotations
-// in v (N x N). Column norms are computed once per sweep and updated with each rotation
-// (alpha' = alpha - t gamma, beta' = beta + t gamma), so a pair costs one dot product and two
-// column updates.
+// in v (N x N). The two column norms and the inner product of a pair are formed exactly in one
+// pass (four independent partial sums each, vectorisable, fixed order): norms carried over from
+// earlier rotations would cancel to tiny or negative values for rank-deficient columns.
 static int jacobi(ct_long M, ct_long N, double* g, double* v)
 {
 	std::fill(v, v + (size_t)(N * N), 0.0);
 	for (ct_long i = 0; i < N; i++) CM(v, N, i, i) = 1;
 	const double tol = 2.2204460492503131e-16 * std::max(std::sqrt((double)M), 4.0);
-	std::vector<double> nrm((size_t)N);
 	for (int sweep = 0; sweep < 80; sweep++) {
 		bool rotated = false;
-		for (ct_long p = 0; p < N; p++) {
-			const double* gp = &CM(g, M, 0, p);
-			double d4[4] = { 0, 0, 0, 0 };
-			ct_long i = 0;
-			for (; i + 4 <= M; i += 4) {
-				d4[0] += gp[i] * gp[i];
-				d4[1] += gp[i + 1] * gp[i + 1];
-				d4[2] += gp[i + 2] * gp[i + 2];
-				d4[3] += gp[i + 3] * gp[i + 3];
-			}
-			for (; i < M; i++) d4[0] += gp[i] * gp[i];
-			nrm[p] = (d4[0] + d4[1]) + (d4[2] + d4[3]);
-		}
 		for (ct_long p = 0; p < N - 1; p++) {
 			for (ct_long q = p + 1; q < N; q++) {
 				double* gp = &CM(g, M, 0, p);
 				double* gq = &CM(g, M, 0, q);
-				const double al = nrm[p], be = nrm[q];
-				// four independent partial sums (vectorisable, fixed order)
-				double g4[4] = { 0, 0, 0, 0 };
+				double a4[4] = { 0, 0, 0, 0 }, b4[4] = { 0, 0, 0, 0 }, g4[4] = { 0, 0, 0, 0 };
 				ct_long i = 0;
 				for (; i + 4 <= M; i += 4) {
-					g4[0] += gp[i] * gq[i];
-					g4[1] += gp[i + 1] * gq[i + 1];
-					g4[2] += gp[i + 2] * gq[i + 2];
-					g4[3] += gp[i + 3] * gq[i + 3];
+					for (int l = 0; l < 4; l++) {
+						const double x = gp[i + l], y = gq[i + l];
+						a4[l] += x * x;
+						b4[l] += y * y;
+						g4[l] += x * y;
+					}
 				}
-				for (; i < M; i++) g4[0] += gp[i] * gq[i];
+				for (; i < M; i++) { a4[0] += gp[i] * gp[i]; b4[0] += gq[i] * gq[i]; g4[0] += gp[i] * gq[i]; }
+				const double al = (a4[0] + a4[1]) + (a4[2] + a4[3]);
+				const double be = (b4[0] + b4[1]) + (b4[2] + b4[3]);
 				const double ga = (g4[0] + g4[1]) + (g4[2] + g4[3]);
 				if (ga == 0 || std::fabs(ga) <= tol * std::sqrt(al * be)) continue;
 				rotated = true;
 				const double zeta = (be - al) / (2 * ga);
 				const double t = std::copysign(1.0, zeta) / (std::fabs(zeta) + std::sqrt(1 + zeta * zeta));
 				const double c = 1 / std::sqrt(1 + t * t), s = c * t;
-				for (ct_long i = 0; i < M; i++) { const double x = gp[i], y = gq[i]; gp[i] = c * x - s * y; gq[i] = s * x + c * y; }
-				nrm[p] = al - t * ga;
-				nrm[q] = be + t * ga;
+				for (ct_long k = 0; k < M; k++) { const double x = gp[k], y = gq[k]; gp[k] = c * x - s * y; gq[k] = s * x + c * y; }
 				double* vp = &CM(v, N, 0, p);
 				double* vq = &CM(v, N, 0, q);
-				for (ct_long i = 0; i < N; i++) { const double x = vp[i], y = vq[i]; vp[i] = c * x - s * y; vq[i] = s * x + c * y; }
+				for (ct_long k = 0; k < N; k++) { const double x = vp[k], y = vq[k]; vp[k] = c * x - s * y; vq[k] = s * x + c * y; }
 			}
 		}
 		if (!rotated) return 0;

@@ -1,19 +1,17 @@
 // Serving loop for independent DMRG ground-state solves with continuous batching.
 //
-// A worker thread owns every device resource of the server (one HIP stream). Each tick it
-//   1. admits queued requests in arrival order while their estimated device footprint fits the
-//      HBM budget and a slot is free (construct the MPO and the seeded start MPS, upload them,
-//      right-normalise, build the environments: the state stays device-resident until done);
-//   2. collects the current two-site eigenproblem of every active request and solves all of them
-//      with one batched Lanczos (one grouped launch per stage for all matvecs of a step, one
-//      recurrence per request), grouped by Lanczos iteration count;
-//   3. splits the optimised two-site tensors: device reshapes, the block SVDs of all requests in
-//      parallel host threads, device uploads and environment updates;
-//   4. retires requests whose sweeps are done and frees their budget.
-// A request that arrives while others run joins at the next tick. The grouped launch uses one
-// fixed launch form and every output tile is summed by one wavefront in a fixed order, and each
-// request's Lanczos reductions cover only its own segment, so a request's results do not depend
-// on the other requests in the batch.
+// Each admitted request cycles on its own: [start] -> ready -> batched Lanczos -> split -> ready ...
+// A worker thread owns the batched Lanczos (one HIP stream): whenever requests are ready it plans
+// one batch over all of them (one grouped launch per stage for all their matvecs of a Lanczos step,
+// one recurrence per request, grouped by Lanczos iteration count), solves it and hands every
+// request to a pool of host threads (each with its own stream) for its split: device reshape and
+// download, host block SVD with truncation, upload, environment update, next two-site tensor. A
+// request rejoins the ready set when its split is done, so slow splits of large requests do not
+// hold back small ones, and a request that arrives while others run is admitted (in arrival
+// order, while the HBM budget and the slots allow) and starts on the pool at once. Requests that
+// finish free their budget. The grouped launch uses one fixed launch form and every output tile is
+// summed by one wavefront in a fixed order, and each request's Lanczos reductions cover only its
+// own segment, so a request's results do not depend on the other requests in the batch.
 
 #include <algorithm>
 #include <chrono>
@@ -54,16 +52,14 @@ size_t estimate_bytes(const ct_serve_request& r)
 	return (size_t)(8.0 * el);
 }
 
-// persistent host threads, each with its own HIP stream; run() hands out task indices and
-// returns when all tasks have finished (every task ends with its stream synchronised)
+// Persistent host threads, each with its own HIP stream, serving a task queue. Every task ends with
+// its stream synchronised, so buffers it freed are reused (by its thread) only in stream order.
 struct HostPool
 {
 	std::vector<std::thread> th;
 	std::mutex mu;
-	std::condition_variable cv, cv_done;
-	std::function<void(int, hipStream_t)> fn;
-	int ntasks = 0, next = 0, done = 0;
-	int64_t gen = 0;
+	std::condition_variable cv;
+	std::deque<std::function<void(hipStream_t)>> q;
 	bool stop = false;
 
 	void start(int n)
@@ -72,36 +68,47 @@ struct HostPool
 			th.emplace_back([this] {
 				hipStream_t s = nullptr;
 				if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
-				int64_t seen = 0;
 				for (;;) {
-					std::unique_lock<std::mutex> g(mu);
-					cv.wait(g, [&] { return stop || gen != seen; });
-					if (stop) break;
-					seen = gen;
-					while (next < ntasks) {
-						const int i = next++;
-						g.unlock();
-						fn(i, s);
-						if (s) (void)hipStreamSynchronize(s);
-						g.lock();
-						if (++done == ntasks) cv_done.notify_all();
+					std::function<void(hipStream_t)> f;
+					{
+						std::unique_lock<std::mutex> g(mu);
+						cv.wait(g, [&] { return stop || !q.empty(); });
+						if (stop) break;
+						f = std::move(q.front());
+						q.pop_front();
 					}
+					f(s);
+					if (s) (void)hipStreamSynchronize(s);
 				}
 				if (s) (void)hipStreamDestroy(s);
 			});
 	}
 
+	void submit(std::function<void(hipStream_t)> f)
+	{
+		{
+			std::lock_guard<std::mutex> g(mu);
+			q.push_back(std::move(f));
+		}
+		cv.notify_one();
+	}
+
+	// f(0 .. n-1) on the pool; returns when all have finished
 	void run(int n, const std::function<void(int, hipStream_t)>& f)
 	{
 		if (n <= 0) return;
-		std::unique_lock<std::mutex> g(mu);
-		fn = f;
-		ntasks = n;
-		next = 0;
-		done = 0;
-		gen++;
-		cv.notify_all();
-		cv_done.wait(g, [&] { return done == ntasks; });
+		std::mutex m;
+		std::condition_variable done;
+		int left = n;
+		for (int i = 0; i < n; i++)
+			submit([&, i](hipStream_t s) {
+				f(i, s);
+				if (s) (void)hipStreamSynchronize(s);
+				std::lock_guard<std::mutex> g(m);
+				if (--left == 0) done.notify_all();
+			});
+		std::unique_lock<std::mutex> g(m);
+		done.wait(g, [&] { return left == 0; });
 	}
 
 	void shutdown()
@@ -109,6 +116,7 @@ struct HostPool
 		{
 			std::lock_guard<std::mutex> g(mu);
 			stop = true;
+			q.clear();
 		}
 		cv.notify_all();
 		for (auto& t : th) t.join();
@@ -125,7 +133,8 @@ struct Server
 	std::condition_variable cv_work, cv_done;
 	std::deque<std::shared_ptr<Ticket>> pending;
 	std::map<int64_t, std::shared_ptr<Ticket>> tickets;
-	std::vector<std::shared_ptr<Ticket>> active;   // worker thread only
+	std::deque<std::shared_ptr<Ticket>> ready;     // started requests waiting for their next Lanczos solve
+	int64_t running = 0;                            // admitted requests not yet finished
 	int64_t next_id = 1;
 	bool stop = false;
 	double budget = 0;
@@ -145,14 +154,28 @@ struct Server
 
 	void finish(Ticket& t, int status)
 	{
+		if (status < 0)
+			fprintf(stderr, "chemtensor_gpu: serve request %lld (model %d, L %d, chi %lld) failed with status %d\n",
+				(long long)t.id, t.req.model, t.req.nsites, (long long)t.req.max_vdim, status);
+		t.state.reset();
 		std::lock_guard<std::mutex> g(mu);
 		t.status = status;
 		t.t_done = now();
 		used -= t.reserved;
 		t.reserved = 0;
-		t.state.reset();
 		completed++;
+		running--;
 		cv_done.notify_all();
+		cv_work.notify_one();   // budget and a slot freed
+	}
+
+	void make_ready(const std::shared_ptr<Ticket>& t)
+	{
+		{
+			std::lock_guard<std::mutex> g(mu);
+			ready.push_back(t);
+		}
+		cv_work.notify_one();
 	}
 
 	int start_request(Ticket& t, hipStream_t s)
@@ -209,44 +232,50 @@ struct Server
 		c0 = now();
 		t_lanczos += c0 - c1;
 		max_batch = std::max<int64_t>(max_batch, (int64_t)grp.size());
-		// splits of all requests in parallel host threads, each on its own stream: device reshape
-		// and download, host block SVD, upload and environment update
-		std::vector<int> ret(grp.size(), 0);
-		std::vector<double> t_dev(grp.size(), 0.0), t_host(grp.size(), 0.0);
-		hosts.run((int)grp.size(), [&](int i, hipStream_t s) {
-			if (status[i] != 0) { ret[i] = status[i]; return; }
-			DmrgState& S = *grp[i]->state;
-			try {
-				S.st = s;
-				const double a0 = now();
-				{
-					DevTensor a_opt(S.theta.meta);
-					if (S.theta.meta.nelem) launch_copy(u + b->off[i], a_opt.d, S.theta.meta.nelem, s);
-					S.advance_begin(a_opt);
-				}
-				const double a1 = now();
-				S.advance_host();
-				const double a2 = now();
-				ret[i] = S.advance_end(lam[i]);
-				t_dev[i] = (a1 - a0) + (now() - a2);
-				t_host[i] = a2 - a1;
-			}
-			catch (const HipError&) {
-				ret[i] = CT_ERR_HIP;
-			}
-		});
+		// every request's ground state into its own buffer, then its split as an asynchronous host task
+		std::vector<std::shared_ptr<DevTensor>> aopt(grp.size());
+		for (size_t i = 0; i < grp.size(); i++) {
+			if (status[i] != 0) continue;
+			const DmrgState& S = *grp[i]->state;
+			aopt[i] = std::make_shared<DevTensor>(S.theta.meta);
+			if (S.theta.meta.nelem) launch_copy(u + b->off[i], aopt[i]->d, S.theta.meta.nelem, st);
+		}
+		CTG_HIP(hipStreamSynchronize(st));
 		dev_free(v0);
 		dev_free(u);
 		b.reset();
 		for (size_t i = 0; i < grp.size(); i++) {
-			t_split_dev += t_dev[i];
-			t_split_host += t_host[i];
-			Ticket& t = *grp[i];
-			if (ret[i] != 0) { finish(t, ret[i]); continue; }
-			if (t.state->done) {
-				t.en = t.state->en_sweeps;
-				finish(t, 0);
-			}
+			std::shared_ptr<Ticket> t = grp[i];
+			if (status[i] != 0) { finish(*t, status[i]); continue; }
+			const double en = lam[i];
+			std::shared_ptr<DevTensor> a = aopt[i];
+			hosts.submit([this, t, a, en](hipStream_t s) {
+				DmrgState& S = *t->state;
+				int ret;
+				double td = 0, th = 0;
+				try {
+					S.st = s;
+					const double a0 = now();
+					S.advance_begin(*a);
+					const double a1 = now();
+					S.advance_host();
+					const double a2 = now();
+					ret = S.advance_end(en);
+					td = (a1 - a0) + (now() - a2);
+					th = a2 - a1;
+				}
+				catch (const HipError&) {
+					ret = CT_ERR_HIP;
+				}
+				{
+					std::lock_guard<std::mutex> g(mu);
+					t_split_dev += td;
+					t_split_host += th;
+				}
+				if (ret != 0) finish(*t, ret);
+				else if (S.done) { t->en = S.en_sweeps; finish(*t, 0); }
+				else make_ready(t);
+			});
 		}
 		t_splits += now() - c0;
 	}
@@ -254,47 +283,52 @@ struct Server
 	void loop()
 	{
 		for (;;) {
-			std::vector<std::shared_ptr<Ticket>> admit;
+			std::vector<std::shared_ptr<Ticket>> batch;
 			{
 				std::unique_lock<std::mutex> g(mu);
-				cv_work.wait(g, [&] { return stop || !pending.empty() || !active.empty(); });
+				auto admissible = [&] {
+					if (pending.empty() || running >= max_active) return false;
+					const size_t need = estimate_bytes(pending.front()->req);
+					return used + need <= budget || running == 0;
+				};
+				cv_work.wait(g, [&] { return stop || !ready.empty() || admissible(); });
 				if (stop) return;
-				// arrival order; at most a few initialisations per tick so running requests keep moving
-				while (!pending.empty() && (int)(active.size() + admit.size()) < max_active && admit.size() < 8) {
-					const std::shared_ptr<Ticket>& t = pending.front();
-					const size_t need = estimate_bytes(t->req);
-					if (used + need > budget && !(active.empty() && admit.empty())) break;
-					t->reserved = need;
-					used += need;
-					t->t_admit = now();
-					admit.push_back(t);
+				// admission in arrival order while the budget and the slots allow; the start of a
+				// request (MPO, seeded MPS, upload, normalisation, environments) is a host task
+				while (admissible()) {
+					std::shared_ptr<Ticket> t = pending.front();
 					pending.pop_front();
+					t->reserved = estimate_bytes(t->req);
+					used += t->reserved;
+					running++;
+					t->t_admit = now();
+					hosts.submit([this, t](hipStream_t s) {
+						const double c = now();
+						const int ret = start_request(*t, s);
+						{
+							std::lock_guard<std::mutex> g2(mu);
+							t_start += now() - c;
+						}
+						if (ret != 0) finish(*t, ret);
+						else make_ready(t);
+					});
 				}
+				batch.assign(ready.begin(), ready.end());
+				ready.clear();
 			}
+			if (batch.empty()) continue;
 			const double tick0 = now();
-			{
-				std::vector<int> ret(admit.size(), 0);
-				hosts.run((int)admit.size(), [&](int i, hipStream_t s) { ret[i] = start_request(*admit[i], s); });
-				for (size_t i = 0; i < admit.size(); i++) {
-					if (ret[i] != 0) finish(*admit[i], ret[i]);
-					else active.push_back(admit[i]);
+			std::map<int, std::vector<std::shared_ptr<Ticket>>> groups;
+			for (auto& t : batch) groups[t->req.maxiter_lanczos].push_back(t);
+			for (auto& kv : groups) {
+				try {
+					step_group(kv.second, kv.first);
+				}
+				catch (const HipError&) {
+					for (auto& t : kv.second) if (t->status == 1) finish(*t, CT_ERR_HIP);
 				}
 			}
-			t_start += now() - tick0;
-			if (!active.empty()) {
-				std::map<int, std::vector<std::shared_ptr<Ticket>>> groups;
-				for (auto& t : active) groups[t->req.maxiter_lanczos].push_back(t);
-				for (auto& kv : groups) {
-					try {
-						step_group(kv.second, kv.first);
-					}
-					catch (const HipError&) {
-						for (auto& t : kv.second) if (t->status == 1) finish(*t, CT_ERR_HIP);
-					}
-				}
-				active.erase(std::remove_if(active.begin(), active.end(), [](const std::shared_ptr<Ticket>& t) { return t->status != 1; }), active.end());
-				ticks++;
-			}
+			ticks++;
 			busy_s += now() - tick0;
 		}
 	}
@@ -426,8 +460,7 @@ int ct_serve_destroy(ct_serve* server)
 	S.cv_done.notify_all();
 	if (S.worker.joinable()) S.worker.join();
 	S.hosts.shutdown();
-	for (auto& t : S.active) t->state.reset();
-	S.active.clear();
+	for (auto& kv : S.tickets) kv.second->state.reset();
 	(void)hipStreamSynchronize(S.st);
 	(void)hipStreamDestroy(S.st);
 	delete server;

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -185,18 +186,28 @@ int host_rq(ct_long m, ct_long n, const double* a, double* r, double* q)
 	return 0;
 }
 
-// one-sided Jacobi on the columns of g (M x N column-major, M >= N); v accumulates rotations
 // One-sided (Hestenes) Jacobi on the columns of g (M x N, column-major), accumulating the rotations
 // in v (N x N). The two column norms and the inner product of a pair are formed exactly in one
 // pass (four independent partial sums each, vectorisable, fixed order): norms carried over from
 // earlier rotations would cancel to tiny or negative values for rank-deficient columns.
+// A column whose norm falls below eps * ||A||_F is numerically zero and is set to zero: a column in
+// the span of the others would otherwise be projected out again and again, each rotation leaving a
+// residual of rounding noise eps times smaller that is not orthogonal to the rest, until it
+// underflows (the limit is within the backward error of the decomposition; zero columns are
+// completed to an orthonormal set by the caller).
 static int jacobi(ct_long M, ct_long N, double* g, double* v)
 {
+	constexpr double eps = 2.2204460492503131e-16;
 	std::fill(v, v + (size_t)(N * N), 0.0);
 	for (ct_long i = 0; i < N; i++) CM(v, N, i, i) = 1;
-	const double tol = 2.2204460492503131e-16 * std::max(std::sqrt((double)M), 4.0);
+	const double tol = eps * std::max(std::sqrt((double)M), 4.0);
+	double fro2 = 0;
+	for (ct_long i = 0; i < M * N; i++) fro2 += g[i] * g[i];
+	const double small2 = fro2 * eps * eps;
+	double worst = 0;   // largest |gamma| / sqrt(alpha beta) of the last sweep
 	for (int sweep = 0; sweep < 80; sweep++) {
 		bool rotated = false;
+		worst = 0;
 		for (ct_long p = 0; p < N - 1; p++) {
 			for (ct_long q = p + 1; q < N; q++) {
 				double* gp = &CM(g, M, 0, p);
@@ -215,7 +226,14 @@ static int jacobi(ct_long M, ct_long N, double* g, double* v)
 				const double al = (a4[0] + a4[1]) + (a4[2] + a4[3]);
 				const double be = (b4[0] + b4[1]) + (b4[2] + b4[3]);
 				const double ga = (g4[0] + g4[1]) + (g4[2] + g4[3]);
-				if (ga == 0 || std::fabs(ga) <= tol * std::sqrt(al * be)) continue;
+				if (al <= small2 || be <= small2) {
+					if (al != 0 && al <= small2) std::fill(gp, gp + M, 0.0);
+					if (be != 0 && be <= small2) std::fill(gq, gq + M, 0.0);
+					continue;
+				}
+				const double rel = std::fabs(ga) / (std::sqrt(al) * std::sqrt(be));
+				if (rel <= tol) continue;
+				worst = std::max(worst, rel);
 				rotated = true;
 				const double zeta = (be - al) / (2 * ga);
 				const double t = std::copysign(1.0, zeta) / (std::fabs(zeta) + std::sqrt(1 + zeta * zeta));
@@ -228,6 +246,7 @@ static int jacobi(ct_long M, ct_long N, double* g, double* v)
 		}
 		if (!rotated) return 0;
 	}
+	fprintf(stderr, "chemtensor_gpu: Jacobi SVD (%lld x %lld) did not converge (off-orthogonality %.3e)\n", (long long)M, (long long)N, worst);
 	return -1;
 }
 
@@ -255,23 +274,29 @@ int host_svd(ct_long m, ct_long n, const double* a, double* u, double* s, double
 		valid[jj] = sv[j] > 0;
 		if (valid[jj]) for (ct_long i = 0; i < M; i++) CM(lu.data(), M, i, jj) = CM(g.data(), M, i, j) / sv[j];
 	}
-	// complete columns of zero singular values to an orthonormal set
+	// complete columns of zero singular values to an orthonormal set: each takes the unit vector e_i
+	// with the largest component outside the current span (1 - sum of squares of row i over the set
+	// columns, at least (M - set columns) / M on average), orthogonalised twice and normalised
+	std::vector<double> rowsq((size_t)M, 0.0);
+	for (ct_long l = 0; l < N; l++) if (valid[l]) {
+		const double* y = &CM(lu.data(), M, 0, l);
+		for (ct_long i = 0; i < M; i++) rowsq[i] += y[i] * y[i];
+	}
 	for (ct_long j = 0; j < N; j++) {
 		if (valid[j]) continue;
+		const ct_long e = std::min_element(rowsq.begin(), rowsq.end()) - rowsq.begin();
 		double* x = &CM(lu.data(), M, 0, j);
-		for (ct_long e = 0; e < M; e++) {
-			for (ct_long i = 0; i < M; i++) x[i] = (i == e ? 1 : 0);
-			for (int pass = 0; pass < 2; pass++)
-				for (ct_long l = 0; l < N; l++) {
-					if (l == j || !valid[l]) continue;
-					const double* y = &CM(lu.data(), M, 0, l);
-					double d = 0;
-					for (ct_long i = 0; i < M; i++) d += x[i] * y[i];
-					for (ct_long i = 0; i < M; i++) x[i] -= d * y[i];
-				}
-			const double nx = nrm2s(M, x, 1);
-			if (nx > 0.5) { for (ct_long i = 0; i < M; i++) x[i] /= nx; break; }
-		}
+		for (ct_long i = 0; i < M; i++) x[i] = (i == e ? 1 : 0);
+		for (int pass = 0; pass < 2; pass++)
+			for (ct_long l = 0; l < N; l++) {
+				if (l == j || !valid[l]) continue;
+				const double* y = &CM(lu.data(), M, 0, l);
+				double d = 0;
+				for (ct_long i = 0; i < M; i++) d += x[i] * y[i];
+				for (ct_long i = 0; i < M; i++) x[i] -= d * y[i];
+			}
+		const double nx = nrm2s(M, x, 1);
+		for (ct_long i = 0; i < M; i++) { x[i] /= nx; rowsq[i] += x[i] * x[i]; }
 		valid[j] = 1;
 	}
 	if (tall) {

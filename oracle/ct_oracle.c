@@ -387,6 +387,12 @@ static int jacobi_svd_tall(ct_long m, ct_long n, double* g, double* v)
 	for (ct_long i = 0; i < n; i++) CM(v, n, i, i) = 1;
 	// rotation threshold sqrt(m) eps (as in LAPACK dgesvj); exact eps can cycle on rounding
 	const double eps = 2.2204460492503131e-16 * fmax(sqrt((double)m), 4.0);
+	// columns with norm below eps ||a||_F are numerically zero and set to zero: a column in the span
+	// of the others is otherwise projected out again and again, its rounding-noise residual never
+	// orthogonal to the rest, until it underflows
+	double fro2 = 0;
+	for (ct_long i = 0; i < m * n; i++) fro2 += g[i] * g[i];
+	const double small2 = fro2 * 2.2204460492503131e-16 * 2.2204460492503131e-16;
 	for (int sweep = 0; sweep < 80; sweep++) {
 		bool rotated = false;
 		for (ct_long p = 0; p < n - 1; p++) {
@@ -395,7 +401,12 @@ static int jacobi_svd_tall(ct_long m, ct_long n, double* g, double* v)
 				double* gp = &CM(g, m, 0, p);
 				double* gq = &CM(g, m, 0, q);
 				for (ct_long i = 0; i < m; i++) { alpha += gp[i] * gp[i]; beta += gq[i] * gq[i]; gamma += gp[i] * gq[i]; }
-				if (gamma == 0 || fabs(gamma) <= eps * sqrt(alpha * beta)) continue;
+				if (alpha <= small2 || beta <= small2) {
+					if (alpha != 0 && alpha <= small2) for (ct_long i = 0; i < m; i++) gp[i] = 0;
+					if (beta != 0 && beta <= small2) for (ct_long i = 0; i < m; i++) gq[i] = 0;
+					continue;
+				}
+				if (gamma == 0 || fabs(gamma) <= eps * sqrt(alpha) * sqrt(beta)) continue;
 				rotated = true;
 				const double zeta = (beta - alpha) / (2 * gamma);
 				const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1 + zeta * zeta));
@@ -411,31 +422,36 @@ static int jacobi_svd_tall(ct_long m, ct_long n, double* g, double* v)
 	return -1;
 }
 
-// complete the columns of u (m x k, column-major) whose norm is zero to an orthonormal set
+// complete the columns of u (m x k, column-major) whose norm is zero to an orthonormal set: each
+// takes the unit vector e_i with the largest component outside the span of the columns set so far,
+// orthogonalised twice (classical Gram-Schmidt) and normalised
 static void complete_orthonormal_columns(ct_long m, ct_long k, double* u, const bool* valid)
 {
+	double* rowsq = xcalloc((size_t)m, sizeof(double));
+	for (ct_long l = 0; l < k; l++) {
+		if (!valid[l]) continue;
+		for (ct_long i = 0; i < m; i++) rowsq[i] += CM(u, m, i, l) * CM(u, m, i, l);
+	}
 	for (ct_long j = 0; j < k; j++) {
 		if (valid[j]) continue;
-		for (ct_long e = 0; e < m; e++) {
-			double* x = &CM(u, m, 0, j);
-			for (ct_long i = 0; i < m; i++) x[i] = (i == e ? 1 : 0);
-			for (int pass = 0; pass < 2; pass++) {
-				for (ct_long l = 0; l < k; l++) {
-					if (l == j || !valid[l]) continue;
-					const double* y = &CM(u, m, 0, l);
-					double d = 0;
-					for (ct_long i = 0; i < m; i++) d += x[i] * y[i];
-					for (ct_long i = 0; i < m; i++) x[i] -= d * y[i];
-				}
-			}
-			const double nx = nrm2(m, x, 1);
-			if (nx > 0.5) {
-				for (ct_long i = 0; i < m; i++) x[i] /= nx;
-				break;
+		ct_long e = 0;
+		for (ct_long i = 1; i < m; i++) if (rowsq[i] < rowsq[e]) e = i;
+		double* x = &CM(u, m, 0, j);
+		for (ct_long i = 0; i < m; i++) x[i] = (i == e ? 1 : 0);
+		for (int pass = 0; pass < 2; pass++) {
+			for (ct_long l = 0; l < k; l++) {
+				if (l == j || !valid[l]) continue;
+				const double* y = &CM(u, m, 0, l);
+				double d = 0;
+				for (ct_long i = 0; i < m; i++) d += x[i] * y[i];
+				for (ct_long i = 0; i < m; i++) x[i] -= d * y[i];
 			}
 		}
+		const double nx = nrm2(m, x, 1);
+		for (ct_long i = 0; i < m; i++) { x[i] /= nx; rowsq[i] += x[i] * x[i]; }
 		((bool*)valid)[j] = true;
 	}
+	free(rowsq);
 }
 
 // dense_tensor.c:3538-3700: a = u diag(s) vh with s descending (reference: LAPACK dgesvd)

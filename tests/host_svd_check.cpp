@@ -1,0 +1,33 @@
+// Test driver for the product's host SVD (chemtensor_amd/csrc/host_linalg.cpp): reads matrices
+// from a binary file (int64 count, then per matrix int64 m, int64 n, m*n doubles row-major) and
+// writes, per matrix, int32 status followed by u (m x k), s (k), vh (k x n) row-major.
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "ct_internal.h"
+
+int main(int argc, char** argv)
+{
+	if (argc != 3) return 2;
+	FILE* in = fopen(argv[1], "rb");
+	FILE* out = fopen(argv[2], "wb");
+	if (!in || !out) return 2;
+	int64_t count = 0;
+	if (fread(&count, 8, 1, in) != 1) return 2;
+	for (int64_t c = 0; c < count; c++) {
+		int64_t mn[2];
+		if (fread(mn, 8, 2, in) != 2) return 2;
+		const int64_t m = mn[0], n = mn[1], k = m < n ? m : n;
+		std::vector<double> a((size_t)(m * n)), u((size_t)(m * k)), s((size_t)k), vh((size_t)(k * n));
+		if (fread(a.data(), 8, a.size(), in) != a.size()) return 2;
+		const int32_t rc = ctg::host_svd(m, n, a.data(), u.data(), s.data(), vh.data());
+		fwrite(&rc, 4, 1, out);
+		fwrite(u.data(), 8, u.size(), out);
+		fwrite(s.data(), 8, s.size(), out);
+		fwrite(vh.data(), 8, vh.size(), out);
+	}
+	fclose(out);
+	fclose(in);
+	return 0;
+}

@@ -336,3 +336,17 @@ def test_fermi_hubbard_mpo_fixture(golden):
     mpo = orc.MPO.hubbard(4, 11. / 9, 13. / 4, 3. / 7)
     got = mpo.to_matrix()
     assert np.max(np.abs(got - golden("test_fermi_hubbard_1d_mpo")["D_fermi_hubbard_1d_mat"])) < 1e-13
+
+
+def test_svd_stress_matrices_oracle():
+    """the oracle's Jacobi SVD on inputs with columns in the span of the others, triplicated
+    columns and graded columns, against numpy's LAPACK SVD"""
+    from test_product_host import _check_svd, _svd_stress_matrices
+    L = orc.lib()
+    for a in _svd_stress_matrices(np.random.default_rng(7)):
+        m, n = a.shape
+        k = min(m, n)
+        a = np.ascontiguousarray(a)
+        u = np.zeros((m, k)); s = np.zeros(k); vh = np.zeros((k, n))
+        assert L.orc_h_svd_dense(m, n, orc.dptr(a), orc.dptr(u), orc.dptr(s), orc.dptr(vh)) == 0, a.shape
+        _check_svd(a, u, s, vh)

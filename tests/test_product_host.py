@@ -73,3 +73,70 @@ def test_product_mpo_matches_reference_fixture(golden):
     assert n == 256
     ref = golden("test_fermi_hubbard_1d_mpo")["D_fermi_hubbard_1d_mat"]
     assert np.max(np.abs(mat.reshape(256, 256) - ref)) < 1e-13
+
+
+def _svd_stress_matrices(rng):
+    """matrices that are hard for one-sided Jacobi: columns in the span of the others (random
+    low rank, exact triplicates), graded columns down to 1e-15, zero and one-element cases"""
+    mats = []
+    for (m, n, r) in [(60, 40, 7), (40, 60, 5), (97, 95, 30), (33, 33, 1)]:
+        mats.append(rng.standard_normal((m, r)) @ rng.standard_normal((r, n)))
+    # triplicated rows: one-sided Jacobi without a zero-column cutoff never converges on these
+    for (m, r, n) in [(40, 30, 60), (30, 29, 30), (60, 60, 62)]:
+        x = np.repeat(rng.standard_normal((m, r)), 3, axis=1)[:, :n]
+        mats.append(x.T.copy())
+        mats.append(x)
+    mats.append(rng.standard_normal((80, 75)) * 10.0 ** (-15.0 * np.arange(75) / 75))
+    mats.append(np.zeros((5, 3)))
+    mats.append(np.array([[2.5]]))
+    mats.append(rng.standard_normal((50, 30)))
+    return mats
+
+
+def _check_svd(a, u, s, vh):
+    k = min(a.shape)
+    scale = max(1.0, np.linalg.norm(a))
+    assert np.max(np.abs((u * s) @ vh - a)) < 1e-13 * scale
+    assert np.max(np.abs(u.T @ u - np.eye(k))) < 1e-12
+    assert np.max(np.abs(vh @ vh.T - np.eye(k))) < 1e-12
+    assert np.all(np.diff(s) <= 0) and np.all(s >= 0)
+    assert np.max(np.abs(s - np.linalg.svd(a, compute_uv=False))) < 1e-13 * scale
+
+
+def test_host_svd_rank_deficient_and_fixtures(golden, tmp_path):
+    """the product's host Jacobi SVD (compiled here from its source with a small driver) against
+    numpy's LAPACK SVD on rank-deficient inputs and the dense SVD fixture"""
+    import os
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        import pytest
+        pytest.skip("g++ not available")
+    here = os.path.dirname(os.path.abspath(__file__))
+    csrc = os.path.join(os.path.dirname(here), "chemtensor_amd", "csrc")
+    exe = str(tmp_path / "svd_check")
+    subprocess.run([gxx, "-O2", "-std=c++17", "-march=x86-64-v3", "-I", csrc, "-I", "/opt/rocm/include",
+                    "-D__HIP_PLATFORM_AMD__", "-x", "c++", os.path.join(here, "host_svd_check.cpp"),
+                    "-x", "c++", os.path.join(csrc, "host_linalg.cpp"), "-o", exe], check=True)
+    mats = _svd_stress_matrices(np.random.default_rng(7))
+    z = golden("test_dense_tensor_svd")
+    mats += [z[k] for k in sorted(z.files) if z[k].dtype == np.float64 and z[k].ndim == 2]
+    with open(tmp_path / "in.bin", "wb") as f:
+        np.array([len(mats)], dtype=np.int64).tofile(f)
+        for a in mats:
+            np.array(a.shape, dtype=np.int64).tofile(f)
+            np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+    subprocess.run([exe, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")], check=True, timeout=120)
+    raw = open(tmp_path / "out.bin", "rb").read()
+    pos = 0
+    for a in mats:
+        m, n = a.shape
+        k = min(m, n)
+        rc = int(np.frombuffer(raw, np.int32, 1, pos)[0]); pos += 4
+        u = np.frombuffer(raw, np.float64, m * k, pos).reshape(m, k); pos += 8 * m * k
+        s = np.frombuffer(raw, np.float64, k, pos); pos += 8 * k
+        vh = np.frombuffer(raw, np.float64, k * n, pos).reshape(k, n); pos += 8 * k * n
+        assert rc == 0, a.shape
+        _check_svd(a, u, s, vh)
+    assert pos == len(raw)

@@ -102,7 +102,9 @@ HostPacked download_packed(const DevTensor& t, hipStream_t s);
 DevTensor upload_packed(const HostPacked& h, hipStream_t s);
 
 // host block factorisations (host_bond.cpp)
-int host_block_svd(const HostPacked& a, HostPacked& u, std::vector<double>& s, HostPacked& vh);
+// block SVD of a packed block-sparse matrix; iso < 0: one-sided Jacobi (u, vh isometric), iso 0 / 1:
+// host_svd_factor per block (vh = diag(s) Vh, resp. u = U diag(s))
+int host_block_svd(const HostPacked& a, HostPacked& u, std::vector<double>& s, HostPacked& vh, int iso = -1);
 ct_long host_retained_bond_indices(const std::vector<double>& sigma, double tol, bool relative, ct_long max_vdim, std::vector<ct_long>& ind, trunc_info& info);
 HostPacked host_slice(const HostPacked& t, int i_ax, const std::vector<ct_long>& ind);
 void host_scale_axis(HostPacked& t, bool leading, const std::vector<double>& vec);
@@ -514,6 +516,11 @@ int lanczos_ground_state(MatvecPlan& p, const double* vstart, int maxiter, doubl
 
 // host FP64 linear algebra for the truncating split (host_linalg.cpp)
 int host_svd(ct_long m, ct_long n, const double* a, double* u, double* s, double* vh);
+// a (m x n, row-major) = left (m x k) * right (k x n), k = min(m, n), with s the singular values in
+// descending order; iso 0: left has orthonormal columns and right = diag(s) Vh; iso 1: right has
+// orthonormal rows and left = U diag(s). Eigenvectors of the Gram matrix of the shorter side
+// (after a QR of a tall matrix), the other factor by projection.
+int host_svd_factor(ct_long m, ct_long n, const double* a, int iso, double* left, double* s, double* right);
 int host_qr(ct_long m, ct_long n, const double* a, double* q, double* r);
 int host_rq(ct_long m, ct_long n, const double* a, double* r, double* q);
 int host_tridiag_eigh(int n, double* d, double* e, double* z);

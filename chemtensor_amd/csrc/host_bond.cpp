@@ -46,7 +46,7 @@ static Meta matrix_meta(ct_long d0, const qnumber* q0, ct_long d1, const qnumber
 	return make_meta(2, dim, ad, qn);
 }
 
-int host_block_svd(const HostPacked& a, HostPacked& u, std::vector<double>& s, HostPacked& vh)
+int host_block_svd(const HostPacked& a, HostPacked& u, std::vector<double>& s, HostPacked& vh, int iso)
 {
 	const Meta& m = a.meta;
 	std::vector<qnumber> qi;
@@ -87,7 +87,11 @@ int host_block_svd(const HostPacked& a, HostPacked& u, std::vector<double>& s, H
 		total += w;
 	}
 	std::atomic<int> ret(0);
-	auto run = [&](size_t i) { if (host_svd(jobs[i].r, jobs[i].c, jobs[i].a, jobs[i].ub, jobs[i].sb, jobs[i].vb) != 0) ret = -1; };
+	auto run = [&](size_t i) {
+		const Job& j = jobs[i];
+		const int r = (iso < 0) ? host_svd(j.r, j.c, j.a, j.ub, j.sb, j.vb) : host_svd_factor(j.r, j.c, j.a, iso, j.ub, j.sb, j.vb);
+		if (r != 0) ret = -1;
+	};
 	const int nt = (total > 4e6) ? host_svd_threads((int)jobs.size()) : 1;
 	if (nt <= 1) {
 		for (size_t i = 0; i < jobs.size(); i++) run(i);
@@ -205,9 +209,12 @@ void host_scale_axis(HostPacked& t, bool leading, const std::vector<double>& vec
 
 int host_split_svd(const HostPacked& a, double tol, ct_long max_vdim, bool renormalize, int distr, HostPacked& a0, HostPacked& a1, trunc_info& info)
 {
+	// the factor that carries the singular values comes out of the factorisation already scaled
+	// (projection onto the isometric factor), so no division by small singular values is needed
+	const int iso = (distr == SVD_DISTR_LEFT) ? 1 : 0;
 	HostPacked u, vh;
 	std::vector<double> s;
-	int ret = host_block_svd(a, u, s, vh);
+	int ret = host_block_svd(a, u, s, vh, iso);
 	if (ret < 0) return ret;
 	std::vector<ct_long> ind;
 	const ct_long num = host_retained_bond_indices(s, tol, true, max_vdim, ind, info);
@@ -222,16 +229,13 @@ int host_split_svd(const HostPacked& a, double tol, ct_long max_vdim, bool renor
 	}
 	a0 = host_slice(u, 1, ind);
 	a1 = host_slice(vh, 0, ind);
-	std::vector<double> sr((size_t)num);
-	for (ct_long i = 0; i < num; i++) sr[i] = s[ind[i]];
 	if (renormalize) {
 		double all = 0;
 		for (double x : s) all += x * x;
 		const double scale = std::sqrt(all) / info.norm_sigma;
-		for (double& x : sr) x *= scale;
+		HostPacked& t = (distr == SVD_DISTR_LEFT) ? a0 : a1;
+		for (double& x : t.v) x *= scale;
 	}
-	if (distr == SVD_DISTR_LEFT) host_scale_axis(a0, false, sr);
-	else host_scale_axis(a1, true, sr);
 	return 0;
 }
 

@@ -31,18 +31,21 @@ static double nrm2s(ct_long n, const double* x, ct_long inc)
 	return scale * std::sqrt(ssq);
 }
 
-int host_tridiag_eigh(int n, double* d, double* offd, double* z)
+// Implicit-shift QL iteration on the symmetric tridiagonal (d, offd) (the algorithm of EISPACK
+// tql2), applying the rotations to the columns of z (n x n, column-major) as it goes: with z = I on
+// entry the columns become the eigenvectors of the tridiagonal, with z = Q those of Q T Q^T.
+// Eigenvalues (and columns) are returned in ascending order. An off-diagonal element deflates when
+// it is below eps times its two diagonal neighbours or, with anorm > 0, below eps * anorm (absolute
+// accuracy relative to the norm: clusters of eigenvalues at the rounding level then converge).
+static int ql_vectors(int n, double* d, const double* offd, double* z, double anorm = 0)
 {
-	// implicit-shift QL iteration with eigenvector accumulation; eigenvalues ascending
 	std::vector<double> e((size_t)n + 1, 0.0);
 	for (int i = 0; i < n - 1; i++) e[i] = offd[i];
-	for (int i = 0; i < n * n; i++) z[i] = 0;
-	for (int i = 0; i < n; i++) z[i + n * i] = 1;
 	const double eps = 2.2204460492503131e-16;
 	for (int l = 0; l < n; l++) {
 		for (int iter = 0;; iter++) {
 			int m = l;
-			for (; m < n - 1; m++) if (std::fabs(e[m]) <= eps * (std::fabs(d[m]) + std::fabs(d[m + 1]))) break;
+			for (; m < n - 1; m++) if (std::fabs(e[m]) <= eps * (std::fabs(d[m]) + std::fabs(d[m + 1])) || std::fabs(e[m]) <= eps * anorm) break;
 			if (m == l) break;
 			if (iter >= 60) return -1;
 			double g = (d[l + 1] - d[l]) / (2 * e[l]);
@@ -63,10 +66,12 @@ int host_tridiag_eigh(int n, double* d, double* offd, double* z)
 				p = s * r;
 				d[i + 1] = g + p;
 				g = c * r - b;
+				double* zi = z + (size_t)n * i;
+				double* zj = z + (size_t)n * (i + 1);
 				for (int k = 0; k < n; k++) {
-					f = z[k + n * (i + 1)];
-					z[k + n * (i + 1)] = s * z[k + n * i] + c * f;
-					z[k + n * i] = c * z[k + n * i] - s * f;
+					const double x = zi[k], y = zj[k];
+					zj[k] = s * x + c * y;
+					zi[k] = c * x - s * y;
 				}
 			}
 			if (deflated) continue;
@@ -86,6 +91,13 @@ int host_tridiag_eigh(int n, double* d, double* offd, double* z)
 	std::copy(d2.begin(), d2.end(), d);
 	std::copy(z2.begin(), z2.end(), z);
 	return 0;
+}
+
+int host_tridiag_eigh(int n, double* d, double* offd, double* z)
+{
+	for (int i = 0; i < n * n; i++) z[i] = 0;
+	for (int i = 0; i < n; i++) z[i + n * i] = 1;
+	return ql_vectors(n, d, offd, z);
 }
 
 // Householder reflector (LAPACK dlarfg convention): H (alpha; x) = (beta; 0)
@@ -306,6 +318,170 @@ int host_svd(ct_long m, ct_long n, const double* a, double* u, double* s, double
 	else {
 		for (ct_long i = 0; i < m; i++) for (ct_long jj = 0; jj < k; jj++) u[i * k + jj] = CM(v.data(), N, i, ord[jj]);
 		for (ct_long jj = 0; jj < k; jj++) for (ct_long i = 0; i < n; i++) vh[jj * n + i] = CM(lu.data(), M, i, jj);
+	}
+	return 0;
+}
+
+// Gram matrix of the rows of b (k x n, row-major): c (k x k, column-major, lower triangle and
+// diagonal) = b b^T. 4 x 4 row tiles with four partial sums per entry (fixed order).
+static void gram_rows(ct_long k, ct_long n, const double* b, double* c)
+{
+	const ct_long n4 = n & ~(ct_long)3;
+	for (ct_long i0 = 0; i0 < k; i0 += 4) {
+		const ct_long ni = std::min<ct_long>(4, k - i0);
+		for (ct_long j0 = 0; j0 <= i0; j0 += 4) {
+			const ct_long nj = std::min<ct_long>(4, k - j0);
+			double acc[4][4][4] = {};
+			const double* bi[4];
+			const double* bj[4];
+			for (int a = 0; a < 4; a++) bi[a] = b + (size_t)(i0 + std::min<ct_long>(a, ni - 1)) * n;
+			for (int a = 0; a < 4; a++) bj[a] = b + (size_t)(j0 + std::min<ct_long>(a, nj - 1)) * n;
+			for (ct_long t = 0; t < n4; t += 4)
+				for (int a = 0; a < 4; a++)
+					for (int q = 0; q < 4; q++)
+						for (int l = 0; l < 4; l++) acc[a][q][l] += bi[a][t + l] * bj[q][t + l];
+			for (int a = 0; a < ni; a++)
+				for (int q = 0; q < nj; q++) {
+					double x = (acc[a][q][0] + acc[a][q][1]) + (acc[a][q][2] + acc[a][q][3]);
+					for (ct_long t = n4; t < n; t++) x += bi[a][t] * bj[q][t];
+					if (j0 + q <= i0 + a) CM(c, k, i0 + a, j0 + q) = x;
+				}
+		}
+	}
+}
+
+// Householder reduction of the symmetric c (k x k, column-major, lower triangle used) to
+// tridiagonal form T = Q^T c Q (d: diagonal, e: subdiagonal); q receives Q explicitly.
+static void tridiagonalize(ct_long k, double* c, double* d, double* e, double* q)
+{
+	std::vector<double> tau((size_t)std::max<ct_long>(k, 1), 0.0), p((size_t)k), w((size_t)k);
+	for (ct_long j = 0; j + 2 < k; j++) {
+		const ct_long r = k - j - 1;          // length of the reflector
+		double* v = &CM(c, k, j + 1, j);
+		larfg(r, v[0], v + 1, 1, tau[j]);
+		e[j] = v[0];
+		d[j] = CM(c, k, j, j);
+		const double t = tau[j];
+		if (t == 0) continue;
+		v[0] = 1;
+		// p = t * A22 v with A22 = c[j+1:, j+1:] (lower triangle)
+		double* A = &CM(c, k, j + 1, j + 1);
+		std::fill(p.begin(), p.begin() + r, 0.0);
+		for (ct_long cj = 0; cj < r; cj++) {
+			const double* col = A + (size_t)cj * k;
+			const double vj = v[cj];
+			double s4[4] = { 0, 0, 0, 0 };
+			ct_long i = cj + 1;
+			for (; i + 4 <= r; i += 4)
+				for (int l = 0; l < 4; l++) { p[i + l] += col[i + l] * vj; s4[l] += col[i + l] * v[i + l]; }
+			double sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+			for (; i < r; i++) { p[i] += col[i] * vj; sum += col[i] * v[i]; }
+			p[cj] += col[cj] * vj + sum;
+		}
+		double pv = 0;
+		for (ct_long i = 0; i < r; i++) { p[i] *= t; pv += p[i] * v[i]; }
+		const double alpha = -0.5 * t * pv;
+		for (ct_long i = 0; i < r; i++) w[i] = p[i] + alpha * v[i];
+		// A22 -= v w^T + w v^T (lower triangle)
+		for (ct_long cj = 0; cj < r; cj++) {
+			double* col = A + (size_t)cj * k;
+			const double vj = v[cj], wj = w[cj];
+			for (ct_long i = cj; i < r; i++) col[i] -= v[i] * wj + w[i] * vj;
+		}
+	}
+	if (k >= 2) { d[k - 2] = CM(c, k, k - 2, k - 2); e[k - 2] = CM(c, k, k - 1, k - 2); }
+	if (k >= 1) d[k - 1] = CM(c, k, k - 1, k - 1);
+	// Q = H_0 H_1 ... H_{k-3}, accumulated backwards; H_j acts on rows and columns j+1 .. k-1
+	std::fill(q, q + (size_t)(k * k), 0.0);
+	for (ct_long i = 0; i < k; i++) CM(q, k, i, i) = 1;
+	for (ct_long j = k - 3; j >= 0; j--) {
+		if (tau[j] == 0) continue;
+		double* v = &CM(c, k, j + 1, j);
+		v[0] = 1;
+		apply_left(k - j - 1, k - j - 1, v, 1, tau[j], &CM(q, k, j + 1, j + 1), k);
+	}
+}
+
+// Factorisation of b (k x n, row-major, k <= n) as b = z * right with z (k x k, column-major)
+// orthogonal: the eigenvectors of b b^T in descending order of their eigenvalue, right = z^T b.
+static int gram_factor(ct_long k, ct_long n, const double* b, std::vector<double>& z, std::vector<double>& right)
+{
+	std::vector<double> c((size_t)(k * k), 0.0), d((size_t)k), e((size_t)std::max<ct_long>(k, 1), 0.0), q((size_t)(k * k));
+	gram_rows(k, n, b, c.data());
+	tridiagonalize(k, c.data(), d.data(), e.data(), q.data());
+	double anorm = 0;
+	for (ct_long i = 0; i < k; i++) anorm = std::max(anorm, std::fabs(d[i]) + (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < k ? std::fabs(e[i]) : 0.0));
+	if (ql_vectors((int)k, d.data(), e.data(), q.data(), anorm) != 0) return -1;
+	// descending order
+	z.assign((size_t)(k * k), 0.0);
+	for (ct_long l = 0; l < k; l++) std::copy(&CM(q.data(), k, 0, k - 1 - l), &CM(q.data(), k, 0, k - 1 - l) + k, &CM(z.data(), k, 0, l));
+	// right = z^T b, four output rows per pass over b
+	right.assign((size_t)(k * n), 0.0);
+	for (ct_long l0 = 0; l0 < k; l0 += 4) {
+		const ct_long nl = std::min<ct_long>(4, k - l0);
+		for (ct_long i = 0; i < k; i++) {
+			const double* bi = b + (size_t)i * n;
+			for (ct_long l = 0; l < nl; l++) {
+				const double zl = CM(z.data(), k, i, l0 + l);
+				double* r = right.data() + (size_t)(l0 + l) * n;
+				for (ct_long t = 0; t < n; t++) r[t] += zl * bi[t];
+			}
+		}
+	}
+	return 0;
+}
+
+int host_svd_factor(ct_long m, ct_long n, const double* a, int iso, double* left, double* s, double* right)
+{
+	const ct_long k = std::min(m, n);
+	if (k == 0) return 0;
+	if (iso == 1) {
+		// a^T = left' right' with left' isometric: a = right'^T left'^T
+		std::vector<double> at((size_t)(m * n)), l2((size_t)(n * k)), r2((size_t)(k * m));
+		for (ct_long i = 0; i < m; i++) for (ct_long j = 0; j < n; j++) at[(size_t)j * m + i] = a[(size_t)i * n + j];
+		const int ret = host_svd_factor(n, m, at.data(), 0, l2.data(), s, r2.data());
+		if (ret != 0) return ret;
+		for (ct_long i = 0; i < m; i++) for (ct_long l = 0; l < k; l++) left[(size_t)i * k + l] = r2[(size_t)l * m + i];
+		for (ct_long l = 0; l < k; l++) for (ct_long j = 0; j < n; j++) right[(size_t)l * n + j] = l2[(size_t)j * k + l];
+		return 0;
+	}
+	std::vector<double> z, rt, qm, rm;
+	const double* b = a;
+	if (m > n) {
+		// a = Q R: factor R (n x n) and left = Q z
+		qm.resize((size_t)(m * n));
+		rm.resize((size_t)(n * n));
+		if (host_qr(m, n, a, qm.data(), rm.data()) != 0) return -1;
+		b = rm.data();
+	}
+	if (gram_factor(k, n, b, z, rt) != 0) return -1;
+	// singular values as the norms of the rows of right (sorted descending; ties keep their order)
+	std::vector<double> sv((size_t)k);
+	for (ct_long l = 0; l < k; l++) sv[l] = nrm2s(n, rt.data() + (size_t)l * n, 1);
+	std::vector<ct_long> ord((size_t)k);
+	std::iota(ord.begin(), ord.end(), 0);
+	std::stable_sort(ord.begin(), ord.end(), [&](ct_long x, ct_long y) { return sv[x] > sv[y]; });
+	for (ct_long l = 0; l < k; l++) {
+		s[l] = sv[ord[l]];
+		std::copy(rt.data() + (size_t)ord[l] * n, rt.data() + (size_t)(ord[l] + 1) * n, right + (size_t)l * n);
+	}
+	if (m <= n) {
+		for (ct_long i = 0; i < m; i++) for (ct_long l = 0; l < k; l++) left[(size_t)i * k + l] = CM(z.data(), k, i, ord[l]);
+	}
+	else {
+		// left = Q z (m x n times n x n): rows of Q against the reordered rows of z^T
+		std::vector<double> zr((size_t)(k * k));
+		for (ct_long i = 0; i < k; i++) for (ct_long l = 0; l < k; l++) zr[(size_t)i * k + l] = CM(z.data(), k, i, ord[l]);
+		for (ct_long r = 0; r < m; r++) {
+			double* out = left + (size_t)r * k;
+			std::fill(out, out + k, 0.0);
+			const double* qr = qm.data() + (size_t)r * n;
+			for (ct_long i = 0; i < k; i++) {
+				const double x = qr[i];
+				const double* zi = zr.data() + (size_t)i * k;
+				for (ct_long l = 0; l < k; l++) out[l] += x * zi[l];
+			}
+		}
 	}
 	return 0;
 }

@@ -103,9 +103,8 @@ def _check_svd(a, u, s, vh):
     assert np.max(np.abs(s - np.linalg.svd(a, compute_uv=False))) < 1e-13 * scale
 
 
-def test_host_svd_rank_deficient_and_fixtures(golden, tmp_path):
-    """the product's host Jacobi SVD (compiled here from its source with a small driver) against
-    numpy's LAPACK SVD on rank-deficient inputs and the dense SVD fixture"""
+def _run_host_svd(tmp_path, mats, mode):
+    """run the product's host SVD routines (compiled here from their source with a small driver)"""
     import os
     import shutil
     import subprocess
@@ -115,21 +114,20 @@ def test_host_svd_rank_deficient_and_fixtures(golden, tmp_path):
         pytest.skip("g++ not available")
     here = os.path.dirname(os.path.abspath(__file__))
     csrc = os.path.join(os.path.dirname(here), "chemtensor_amd", "csrc")
-    exe = str(tmp_path / "svd_check")
-    subprocess.run([gxx, "-O2", "-std=c++17", "-march=x86-64-v3", "-I", csrc, "-I", "/opt/rocm/include",
-                    "-D__HIP_PLATFORM_AMD__", "-x", "c++", os.path.join(here, "host_svd_check.cpp"),
-                    "-x", "c++", os.path.join(csrc, "host_linalg.cpp"), "-o", exe], check=True)
-    mats = _svd_stress_matrices(np.random.default_rng(7))
-    z = golden("test_dense_tensor_svd")
-    mats += [z[k] for k in sorted(z.files) if z[k].dtype == np.float64 and z[k].ndim == 2]
+    exe = tmp_path / "svd_check"
+    if not exe.exists():
+        subprocess.run([gxx, "-O2", "-std=c++17", "-march=x86-64-v3", "-I", csrc, "-I", "/opt/rocm/include",
+                        "-D__HIP_PLATFORM_AMD__", "-x", "c++", os.path.join(here, "host_svd_check.cpp"),
+                        "-x", "c++", os.path.join(csrc, "host_linalg.cpp"), "-o", str(exe)], check=True)
     with open(tmp_path / "in.bin", "wb") as f:
         np.array([len(mats)], dtype=np.int64).tofile(f)
         for a in mats:
             np.array(a.shape, dtype=np.int64).tofile(f)
             np.ascontiguousarray(a, dtype=np.float64).tofile(f)
-    subprocess.run([exe, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")], check=True, timeout=120)
+    subprocess.run([str(exe), str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), mode], check=True, timeout=120)
     raw = open(tmp_path / "out.bin", "rb").read()
     pos = 0
+    out = []
     for a in mats:
         m, n = a.shape
         k = min(m, n)
@@ -137,6 +135,43 @@ def test_host_svd_rank_deficient_and_fixtures(golden, tmp_path):
         u = np.frombuffer(raw, np.float64, m * k, pos).reshape(m, k); pos += 8 * m * k
         s = np.frombuffer(raw, np.float64, k, pos); pos += 8 * k
         vh = np.frombuffer(raw, np.float64, k * n, pos).reshape(k, n); pos += 8 * k * n
+        out.append((rc, u, s, vh))
+    assert pos == len(raw)
+    return out
+
+
+def _svd_inputs(golden):
+    mats = _svd_stress_matrices(np.random.default_rng(7))
+    z = golden("test_dense_tensor_svd")
+    return mats + [z[k] for k in sorted(z.files) if z[k].dtype == np.float64 and z[k].ndim == 2]
+
+
+def test_host_svd_rank_deficient_and_fixtures(golden, tmp_path):
+    """the product's host Jacobi SVD against numpy's LAPACK SVD on rank-deficient inputs and the
+    dense SVD fixture"""
+    mats = _svd_inputs(golden)
+    for a, (rc, u, s, vh) in zip(mats, _run_host_svd(tmp_path, mats, "jacobi")):
         assert rc == 0, a.shape
         _check_svd(a, u, s, vh)
-    assert pos == len(raw)
+
+
+def test_host_split_factor_rank_deficient_and_fixtures(golden, tmp_path):
+    """the split factorisation (Gram eigenvectors of the shorter side, the other factor by
+    projection): left @ right reproduces a, the isometric factor is orthonormal, and the squared
+    singular values agree with numpy's LAPACK SVD to eps * ||a||^2 (the eigenvalue accuracy of the
+    Gram matrix)"""
+    mats = _svd_inputs(golden)
+    for iso in (0, 1):
+        for a, (rc, left, s, right) in zip(mats, _run_host_svd(tmp_path, mats, "factor%d" % iso)):
+            assert rc == 0, a.shape
+            k = min(a.shape)
+            scale = max(1.0, np.linalg.norm(a))
+            assert np.max(np.abs(left @ right - a)) < 1e-13 * scale
+            iso_f = left.T @ left if iso == 0 else right @ right.T
+            assert np.max(np.abs(iso_f - np.eye(k))) < 1e-12
+            # the other factor carries the singular values: its columns / rows have norms s
+            other = np.linalg.norm(right, axis=1) if iso == 0 else np.linalg.norm(left, axis=0)
+            assert np.max(np.abs(other - s)) < 1e-13 * scale
+            assert np.all(np.diff(s) <= 0) and np.all(s >= 0)
+            s_np = np.linalg.svd(a, compute_uv=False)
+            assert np.max(np.abs(s ** 2 - s_np ** 2)) < 1e-13 * scale ** 2

@@ -497,6 +497,15 @@ struct DmrgState
 	HostPacked split_in, split_m0, split_m1;
 	trunc_info split_info = {};
 	int split_ret = 0;
+	// phase timing (environment variable CT_DMRG_PHASES=1): the stream is synchronised at every
+	// phase boundary and the wall time since the previous boundary is added to the phase
+	enum { PH_PLAN, PH_LANCZOS, PH_SPLIT_BEGIN, PH_SPLIT_HOST, PH_SPLIT_UPLOAD, PH_ENVIRONMENT, PH_MERGE, PH_COUNT };
+	bool trace = false;
+	double phase_s[PH_COUNT] = {};
+	double phase_t0 = 0;
+	void phase_mark();
+	void phase_end(int ph);
+	void phase_report() const;
 	int init(const mpo* H, const mps* psi, int num_sweeps, int maxiter, double tol_split, ct_long max_vdim, double tol_energy);
 	void advance_begin(const DevTensor& a_opt);
 	void advance_host();

@@ -9,6 +9,9 @@
 #include <cstring>
 #include <memory>
 
+#include <chrono>
+#include <cstdlib>
+
 #include "ct_internal.h"
 
 namespace ctg {
@@ -134,17 +137,6 @@ static int local_orthonormalize_rq(DevTensor& a, DevTensor& a_prev, hipStream_t 
 	return 0;
 }
 
-// mps.c:1119-1160
-// dmrg.c:86-147 with the Lanczos iteration on the device
-static int minimize_local_energy(const DevTensor& w, const DevTensor& l, const DevTensor& r, const DevTensor& a_start, int maxiter,
-	double* en, DevTensor& a_opt, hipStream_t st)
-{
-	std::unique_ptr<MatvecPlan> p(build_matvec_plan_dev(a_start.meta, DevTensor::view(w, w.meta), DevTensor::view(l, l.meta), DevTensor::view(r, r.meta),
-		CT_MATVEC_GROUPED_FP64, st, false, false));
-	a_opt = DevTensor(a_start.meta);
-	return lanczos_ground_state(*p, a_start.d, maxiter, en, a_opt.d);
-}
-
 // the dummy tensor of dimensions (D, 1, D) with the outer bond quantum numbers and entry 1
 static DevTensor dummy_site(const Meta& a, int ax, hipStream_t st)
 {
@@ -210,8 +202,39 @@ int DmrgState::init(const mpo* H, const mps* psi, int num_sweeps_, int maxiter_,
 
 // split of the optimised two-site tensor (mps_split_tensor_svd, mps.c:1119-1160) in three parts:
 // device reshape and download, host block SVD with truncation, upload and device reshape
+static double wall_now()
+{
+	return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void DmrgState::phase_mark()
+{
+	if (!trace) return;
+	CTG_HIP(hipStreamSynchronize(st));
+	phase_t0 = wall_now();
+}
+
+void DmrgState::phase_end(int ph)
+{
+	if (!trace) return;
+	CTG_HIP(hipStreamSynchronize(st));
+	const double t = wall_now();
+	phase_s[ph] += t - phase_t0;
+	phase_t0 = t;
+}
+
+void DmrgState::phase_report() const
+{
+	if (!trace) return;
+	static const char* names[PH_COUNT] = { "plan", "lanczos", "split_begin", "split_host", "split_upload", "environment", "merge" };
+	fprintf(stderr, "chemtensor_gpu: dmrg phases (L %d, max_vdim %lld, %d sweeps):", L, (long long)max_vdim, sweep);
+	for (int i = 0; i < PH_COUNT; i++) fprintf(stderr, " %s %.4f s", names[i], phase_s[i]);
+	fprintf(stderr, "\n");
+}
+
 void DmrgState::advance_begin(const DevTensor& a_opt)
 {
+	phase_mark();
 	const ct_long dd[2] = { d, d };
 	const int ad[2] = { TENSOR_AXIS_OUT, TENSOR_AXIS_OUT };
 	const qnumber* qs[2] = { qsite.data(), qsite.data() };
@@ -220,17 +243,21 @@ void DmrgState::advance_begin(const DevTensor& a_opt)
 	DevTensor am = dev_flatten_axes(tmp, 1, TENSOR_AXIS_IN, st);
 	split_meta = a2.meta;
 	split_in = download_packed(am, st);
+	phase_end(PH_SPLIT_BEGIN);
 }
 
 void DmrgState::advance_host()
 {
+	const double t0 = trace ? wall_now() : 0;
 	split_ret = host_split_svd(split_in, tol_split, max_vdim, false, dir == 0 ? SVD_DISTR_RIGHT : SVD_DISTR_LEFT, split_m0, split_m1, split_info);
 	split_in = HostPacked();
+	if (trace) phase_s[PH_SPLIT_HOST] += wall_now() - t0;
 }
 
 int DmrgState::advance_end(double en)
 {
 	if (split_ret < 0) return split_ret;
+	phase_mark();
 	{
 		DevTensor d0 = upload_packed(split_m0, st), d1 = upload_packed(split_m1, st);
 		const Meta& m = split_meta;
@@ -243,6 +270,7 @@ int DmrgState::advance_end(double en)
 		split_m0 = HostPacked();
 		split_m1 = HostPacked();
 	}
+	phase_end(PH_SPLIT_UPLOAD);
 	const int i = bond;
 	if (dir == 0) {
 		lb[i + 1] = step_left(A[i], A[i], W[i], lb[i], st);
@@ -265,9 +293,11 @@ int DmrgState::advance_end(double en)
 			bond = (L > 2) ? 0 : L - 2;
 		}
 	}
+	phase_end(PH_ENVIRONMENT);
 	if (!done) theta = merge_mps(A[bond], A[bond + 1], st);
 	else theta = DevTensor();
 	CTG_HIP(hipStreamSynchronize(st));
+	phase_end(PH_MERGE);
 	return 0;
 }
 
@@ -298,16 +328,26 @@ int dmrg_twosite_device(const mpo* H, int num_sweeps, int maxiter, double tol_sp
 {
 	DmrgState S;
 	S.st = default_stream();
+	const char* tr = getenv("CT_DMRG_PHASES");
+	S.trace = tr && tr[0] == '1';
 	int ret = S.init(H, psi, num_sweeps, maxiter, tol_split, max_vdim, 0.0);
 	if (ret != 0) return ret;
 	while (!S.done) {
 		DevTensor ao;
 		double en = 0;
-		ret = minimize_local_energy(S.h2[S.bond], S.lb[S.bond], S.rb[S.bond + 1], S.theta, maxiter, &en, ao, S.st);
+		// minimize_local_energy (dmrg.c:86-147) with the Lanczos iteration on the device
+		S.phase_mark();
+		std::unique_ptr<MatvecPlan> p(build_matvec_plan_dev(S.theta.meta, DevTensor::view(S.h2[S.bond], S.h2[S.bond].meta),
+			DevTensor::view(S.lb[S.bond], S.lb[S.bond].meta), DevTensor::view(S.rb[S.bond + 1], S.rb[S.bond + 1].meta), CT_MATVEC_GROUPED_FP64, S.st, false, false));
+		S.phase_end(DmrgState::PH_PLAN);
+		ao = DevTensor(S.theta.meta);
+		ret = lanczos_ground_state(*p, S.theta.d, maxiter, &en, ao.d);
+		S.phase_end(DmrgState::PH_LANCZOS);
 		if (ret < 0) return ret;
 		ret = S.advance(en, ao);
 		if (ret < 0) return ret;
 	}
+	S.phase_report();
 	for (int n = 0; n < num_sweeps; n++) en_sweeps[n] = S.en_sweeps[n];
 	for (int i = 0; i < S.L - 1; i++) entropy[i] = S.entropy[i];
 	S.download_mps(psi);

@@ -155,6 +155,7 @@ struct WorkItem
 	int32_t n0;
 	int32_t cost;
 	int32_t task_index;
+	GemmTerm first;        // the task's first term: its operand loads need no descriptor round trip
 };
 
 // tile edge of one wavefront's output tile (2 x 2 MFMA 16x16 sub-tiles)

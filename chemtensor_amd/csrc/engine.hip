@@ -49,7 +49,7 @@ std::vector<WorkItem> GemmBatch::work_items() const
 		const int cost = (int)std::min<int64_t>(ksum + 4, (int64_t)1 << 30);
 		for (int m0 = 0; m0 < t.M; m0 += TILE)
 			for (int n0 = 0; n0 < t.N; n0 += TILE)
-				items.push_back(WorkItem{ t, m0, n0, cost, (int32_t)ti });
+				items.push_back(WorkItem{ t, m0, n0, cost, (int32_t)ti, t.t1 > t.t0 ? terms[t.t0] : GemmTerm{} });
 	}
 	std::stable_sort(items.begin(), items.end(), [](const WorkItem& a, const WorkItem& b) { return a.cost > b.cost; });
 	return items;

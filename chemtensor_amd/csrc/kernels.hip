@@ -74,7 +74,45 @@ __device__ __forceinline__ const double* term_ptr(const TermCache& c, int t, int
 	return (const double*)(intptr_t)(lo | (hi << 32));
 }
 
-__device__ __forceinline__ void wave_tile_f64(const GemmTask& tk, const GemmTerm* __restrict__ terms, int m0, int n0, int lane, GemmBases bs)
+// The fields of term t: the task's first term comes with the work item (scalar registers), later
+// ones from the lane-held cache. Uniform across the wavefront.
+struct TermF
+{
+	const double* A;
+	const double* B;
+	int64_t sb_batch;
+	int K, sa_i, sa_k, sb_k, sb_j;
+};
+
+__device__ __forceinline__ TermF term_fields(const TermCache& c, int t, const GemmTerm& first, int t0)
+{
+	TermF f;
+	if (t == t0) {
+		f.A = first.A; f.B = first.B; f.sb_batch = first.sb_batch;
+		f.K = first.K; f.sa_i = first.sa_i; f.sa_k = first.sa_k; f.sb_k = first.sb_k; f.sb_j = first.sb_j;
+	}
+	else {
+		f.A = term_ptr(c, t, 0); f.B = term_ptr(c, t, 2); f.sb_batch = (int64_t)term_ptr(c, t, 4);
+		f.K = term_K(c, t); f.sa_i = (int)term_word(c, t, 7); f.sa_k = (int)term_word(c, t, 8);
+		f.sb_k = (int)term_word(c, t, 9); f.sb_j = (int)term_word(c, t, 10);
+	}
+	return f;
+}
+
+__device__ __forceinline__ int term_K_at(const TermCache& c, int t, const GemmTerm& first, int t0)
+{
+	return (t == t0) ? first.K : term_K(c, t);
+}
+
+// the cache of a task's terms: loaded (one vector load) only when the task has more than one term,
+// in flight while the first term's operands are loaded
+__device__ __forceinline__ void term_cache_start(TermCache& c, const GemmTerm* __restrict__ terms, const GemmTask& tk, int lane)
+{
+	c.base = tk.t0;
+	if (tk.t1 - tk.t0 > 1) term_cache_load(c, terms, tk.t0, tk.t1, lane);
+}
+
+__device__ __forceinline__ void wave_tile_f64(const GemmTask& tk, const GemmTerm* __restrict__ terms, const GemmTerm& first, int m0, int n0, int lane, GemmBases bs)
 {
 	const int i16 = lane & 15, kq = lane >> 4;
 	const bool two_m = (m0 + 16 < tk.M);
@@ -84,15 +122,16 @@ __device__ __forceinline__ void wave_tile_f64(const GemmTask& tk, const GemmTerm
 	const bool vr0 = r0 < tk.M, vr1 = r1 < tk.M, vc0 = c0 < tk.N, vc1 = c1 < tk.N;
 	double4_t acc00 = { 0, 0, 0, 0 }, acc01 = { 0, 0, 0, 0 }, acc10 = { 0, 0, 0, 0 }, acc11 = { 0, 0, 0, 0 };
 	TermCache tc;
-	term_cache_load(tc, terms, tk.t0, tk.t1, lane);
+	term_cache_start(tc, terms, tk, lane);
 	for (int t = tk.t0; t < tk.t1; t++) {
 		if (t - tc.base >= 64) term_cache_load(tc, terms, t, tk.t1, lane);
-		const double* __restrict__ A = rebase(bs.A, term_ptr(tc, t, 0));
-		const double* __restrict__ B = rebase(bs.B, term_ptr(tc, t, 2));
-		const int64_t sb_batch = (int64_t)term_ptr(tc, t, 4);
-		const int K = term_K(tc, t);
-		const int sa_i = (int)term_word(tc, t, 7), sa_k = (int)term_word(tc, t, 8);
-		const int sb_k = (int)term_word(tc, t, 9), sb_j = (int)term_word(tc, t, 10);
+		const TermF f = term_fields(tc, t, first, tk.t0);
+		const double* __restrict__ A = rebase(bs.A, f.A);
+		const double* __restrict__ B = rebase(bs.B, f.B);
+		const int64_t sb_batch = f.sb_batch;
+		const int K = f.K;
+		const int sa_i = f.sa_i, sa_k = f.sa_k;
+		const int sb_k = f.sb_k, sb_j = f.sb_j;
 		const int64_t ao0 = (int64_t)(vr0 ? r0 : 0) * sa_i, ao1 = (int64_t)(vr1 ? r1 : 0) * sa_i;
 		const int64_t bo0 = b_col_offset(vc0 ? c0 : 0, tk.nci, sb_batch, sb_j);
 		const int64_t bo1 = b_col_offset(vc1 ? c1 : 0, tk.nci, sb_batch, sb_j);
@@ -148,7 +187,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(const GemmTask* __rest
 	for (int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6))); w < nitems; w += nwaves) {
 		const WorkItem it = items[w];
 		const GemmTask& tk = it.task;
-		wave_tile_f64(tk, terms, it.m0, it.n0, lane, bs);
+		wave_tile_f64(tk, terms, it.first, it.m0, it.n0, lane, bs);
 	}
 }
 
@@ -158,7 +197,8 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64(const GemmTask* __rest
 // parallelism). The four partial tiles are summed through LDS in a fixed order.
 constexpr int CHUNK = 4;
 
-__device__ __forceinline__ void wg_tile_f64(const GemmTask& tk, const GemmTerm* __restrict__ terms, int m0, int n0, GemmBases bs, double* __restrict__ red)
+__device__ __forceinline__ void wg_tile_f64(const GemmTask& tk, const GemmTerm* __restrict__ terms, const GemmTerm& first, int m0, int n0, GemmBases bs,
+	double* __restrict__ red)
 {
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int i16 = lane & 15, kq = lane >> 4;
@@ -170,24 +210,25 @@ __device__ __forceinline__ void wg_tile_f64(const GemmTask& tk, const GemmTerm* 
 	double4_t acc00 = { 0, 0, 0, 0 }, acc01 = { 0, 0, 0, 0 }, acc10 = { 0, 0, 0, 0 }, acc11 = { 0, 0, 0, 0 };
 	int g = 0;
 	TermCache tc;
-	term_cache_load(tc, terms, tk.t0, tk.t1, lane);
+	term_cache_start(tc, terms, tk, lane);
 	for (int t = tk.t0; t < tk.t1; t++) {
 		if (t - tc.base >= 64) term_cache_load(tc, terms, t, tk.t1, lane);
-		const int K = term_K(tc, t);
+		const int K = term_K_at(tc, t, first, tk.t0);
 		const int nsteps = (K + 3) >> 2;
 		const int nchunks = (nsteps + CHUNK - 1) / CHUNK;
-		const int first = (wave - g % 4 + 4) % 4;
+		const int c_first = (wave - g % 4 + 4) % 4;
 		g += nchunks;
-		if (first >= nchunks) continue;
-		const double* __restrict__ A = rebase(bs.A, term_ptr(tc, t, 0));
-		const double* __restrict__ B = rebase(bs.B, term_ptr(tc, t, 2));
-		const int64_t sb_batch = (int64_t)term_ptr(tc, t, 4);
-		const int sa_i = (int)term_word(tc, t, 7), sa_k = (int)term_word(tc, t, 8);
-		const int sb_k = (int)term_word(tc, t, 9), sb_j = (int)term_word(tc, t, 10);
+		if (c_first >= nchunks) continue;
+		const TermF f = term_fields(tc, t, first, tk.t0);
+		const double* __restrict__ A = rebase(bs.A, f.A);
+		const double* __restrict__ B = rebase(bs.B, f.B);
+		const int64_t sb_batch = f.sb_batch;
+		const int sa_i = f.sa_i, sa_k = f.sa_k;
+		const int sb_k = f.sb_k, sb_j = f.sb_j;
 		const int64_t ao0 = (int64_t)(vr0 ? r0 : 0) * sa_i, ao1 = (int64_t)(vr1 ? r1 : 0) * sa_i;
 		const int64_t bo0 = b_col_offset(vc0 ? c0 : 0, tk.nci, sb_batch, sb_j);
 		const int64_t bo1 = b_col_offset(vc1 ? c1 : 0, tk.nci, sb_batch, sb_j);
-		for (int c = first; c < nchunks; c += 4) {
+		for (int c = c_first; c < nchunks; c += 4) {
 			double a0[CHUNK], a1[CHUNK], b0[CHUNK], b1[CHUNK];
 			#pragma unroll
 			for (int u = 0; u < CHUNK; u++) {
@@ -247,7 +288,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64_wg(const GemmTask* __r
 	for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
 		const WorkItem it = items[w];
 		const GemmTask& tk = it.task;
-		wg_tile_f64(tk, terms, it.m0, it.n0, bs, red);
+		wg_tile_f64(tk, terms, it.first, it.m0, it.n0, bs, red);
 	}
 }
 
@@ -259,7 +300,7 @@ constexpr int SLOTS = 4;
 
 struct Acc4 { double4_t a00, a01, a10, a11; };
 
-__device__ __forceinline__ void stream_tile_f64(const GemmTask& tk, const GemmTerm* __restrict__ terms, int m0, int n0, GemmBases bs,
+__device__ __forceinline__ void stream_tile_f64(const GemmTask& tk, const GemmTerm* __restrict__ terms, const GemmTerm& first, int m0, int n0, GemmBases bs,
 	int part, int nparts, Acc4& acc)
 {
 	const int lane = threadIdx.x & 63;
@@ -270,7 +311,7 @@ __device__ __forceinline__ void stream_tile_f64(const GemmTask& tk, const GemmTe
 	const int c0 = n0 + i16, c1 = n0 + 16 + i16;
 	const bool vr0 = r0 < tk.M, vr1 = r1 < tk.M, vc0 = c0 < tk.N, vc1 = c1 < tk.N;
 	TermCache tc;
-	term_cache_load(tc, terms, tk.t0, tk.t1, lane);
+	term_cache_start(tc, terms, tk, lane);
 	auto ensure = [&](int t) {
 		if (t - tc.base >= 64) term_cache_load(tc, terms, tk.t0 + ((t - tk.t0) & ~63), tk.t1, lane);
 	};
@@ -279,7 +320,7 @@ __device__ __forceinline__ void stream_tile_f64(const GemmTask& tk, const GemmTe
 	auto advance = [&](int skip) {
 		while (t < tk.t1 && skip > 0) {
 			ensure(t);
-			const int ns = (term_K(tc, t) + 3) >> 2;
+			const int ns = (term_K_at(tc, t, first, tk.t0) + 3) >> 2;
 			const int rem = ns - step;
 			if (skip >= rem) { skip -= rem; t++; step = 0; }
 			else { step += skip; skip = 0; }
@@ -293,12 +334,13 @@ __device__ __forceinline__ void stream_tile_f64(const GemmTask& tk, const GemmTe
 		for (int u = 0; u < SLOTS; u++) {
 			if (t < tk.t1) {
 				ensure(t);
-				const double* __restrict__ A = rebase(bs.A, term_ptr(tc, t, 0));
-				const double* __restrict__ B = rebase(bs.B, term_ptr(tc, t, 2));
-				const int64_t sb_batch = (int64_t)term_ptr(tc, t, 4);
-				const int K = term_K(tc, t);
-				const int sa_i = (int)term_word(tc, t, 7), sa_k = (int)term_word(tc, t, 8);
-				const int sb_k = (int)term_word(tc, t, 9), sb_j = (int)term_word(tc, t, 10);
+				const TermF f = term_fields(tc, t, first, tk.t0);
+				const double* __restrict__ A = rebase(bs.A, f.A);
+				const double* __restrict__ B = rebase(bs.B, f.B);
+				const int64_t sb_batch = f.sb_batch;
+				const int K = f.K;
+				const int sa_i = f.sa_i, sa_k = f.sa_k;
+				const int sb_k = f.sb_k, sb_j = f.sb_j;
 				const int k = step * 4 + kq;
 				const bool vk = k < K;
 				const int64_t ak = (int64_t)(vk ? k : 0) * sa_k, bk = (int64_t)(vk ? k : 0) * sb_k;
@@ -353,7 +395,7 @@ __global__ __launch_bounds__(256) void k_stream_gemm_f64_wave(const GemmTask* __
 		const GemmTask& tk = it.task;
 		Acc4 acc;
 		acc.a00 = acc.a01 = acc.a10 = acc.a11 = double4_t{ 0, 0, 0, 0 };
-		stream_tile_f64(tk, terms, it.m0, it.n0, bs, 0, 1, acc);
+		stream_tile_f64(tk, terms, it.first, it.m0, it.n0, bs, 0, 1, acc);
 		store_sub(tk, it.m0, it.n0, bs, 0, 0, acc.a00, lane);
 		store_sub(tk, it.m0, it.n0, bs, 0, 1, acc.a01, lane);
 		store_sub(tk, it.m0, it.n0, bs, 1, 0, acc.a10, lane);
@@ -373,7 +415,7 @@ __global__ __launch_bounds__(256) void k_stream_gemm_f64_wg(const GemmTask* __re
 		const GemmTask& tk = it.task;
 		Acc4 acc;
 		acc.a00 = acc.a01 = acc.a10 = acc.a11 = double4_t{ 0, 0, 0, 0 };
-		stream_tile_f64(tk, terms, it.m0, it.n0, bs, wave, 4, acc);
+		stream_tile_f64(tk, terms, it.first, it.m0, it.n0, bs, wave, 4, acc);
 		#pragma unroll
 		for (int r = 0; r < 4; r++) {
 			red[((wave * 16) + r) * 64 + lane] = acc.a00[r];
@@ -402,7 +444,7 @@ __global__ __launch_bounds__(256) void k_plain_gemm_f64(GemmTask tk, GemmTerm tm
 	if (w >= tiles) return;
 	tk.t0 = 0;
 	tk.t1 = 1;
-	wave_tile_f64(tk, &tm, (w / tiles_n) * TILE, (w % tiles_n) * TILE, lane, bs);
+	wave_tile_f64(tk, &tm, tm, (w / tiles_n) * TILE, (w % tiles_n) * TILE, lane, bs);
 }
 
 //________________________________________________________________________________________________________________________

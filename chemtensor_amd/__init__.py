@@ -123,6 +123,7 @@ def lib():
             "ct_construct_heisenberg_xxz_1d_mpo": (None, [C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(MPOStruct)]),
             "ct_construct_fermi_hubbard_1d_mpo": (None, [C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(MPOStruct)]),
             "delete_mpo": (None, [C.POINTER(MPOStruct)]),
+            "ct_dense_svd_device": (C.c_int, [ct_long, ct_long, DP, DP, DP, DP]),
             "delete_mps": (None, [C.POINTER(MPSStruct)]),
             "seed_rng_state": (None, [C.c_uint64, P]),
             "construct_random_mps": (None, [C.c_int, C.c_int, ct_long, IP, C.c_int32, ct_long, P, C.POINTER(MPSStruct)]),
@@ -228,6 +229,17 @@ def apply_local_hamiltonian(a, w, l, r):
     lib().apply_local_hamiltonian(_ptr(a), _ptr(w), _ptr(l), _ptr(r), b.sptr)
     b.live = True
     return b
+
+
+def dense_svd_device(a):
+    """u, s, vh of a dense matrix through the device block-SVD kernel of the DMRG split"""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    m, n = a.shape
+    k = min(m, n)
+    u = np.zeros((m, k)); s = np.zeros(k); vh = np.zeros((k, n))
+    ret = lib().ct_dense_svd_device(m, n, a.ctypes.data_as(DP), u.ctypes.data_as(DP), s.ctypes.data_as(DP), vh.ctypes.data_as(DP))
+    check(ret, "ct_dense_svd_device")
+    return u, s, vh
 
 
 def mpo_merge_tensor_pair(a0, a1):

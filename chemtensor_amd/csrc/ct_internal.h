@@ -101,6 +101,38 @@ struct HostPacked
 HostPacked download_packed(const DevTensor& t, hipStream_t s);
 DevTensor upload_packed(const HostPacked& h, hipStream_t s);
 
+// layout of the block SVD of a packed block-sparse matrix (block_sparse_tensor.c:2686-2848): the new
+// bond's quantum numbers (one per singular value, blocks in row-major order), the metas of u (m x ni)
+// and vh (ni x n) and one job per present block (offsets in doubles; s_off: the block's first
+// singular value); 'dummy' when no block is present
+struct SvdLayout
+{
+	std::vector<qnumber> qi;
+	bool dummy = false;
+	Meta um, vhm;
+	struct Job { ct_long a_off, u_off, vh_off, s_off, r, c; };
+	std::vector<Job> jobs;
+};
+SvdLayout plan_block_svd(const Meta& m);
+
+// device block SVD (split.hip): one workgroup per block, one-sided Jacobi in LDS
+struct DevSvdJob { int64_t a, u, vh, s; int32_t r, c; };
+// per-block selection of columns (axis 1) or rows (axis 0) of a packed matrix, optionally scaled
+struct SliceBlock { int64_t src, dst; int32_t rows, cols, src_ld, axis, idx0, pad; };
+size_t block_svd_lds_bytes(const SvdLayout& lay);
+size_t block_svd_lds_limit();
+// u, vh: packed as lay.um / lay.vhm; s: one value per new-bond index, in device column order within
+// each block (not sorted); status: one int per block (0 converged, -1 not). False (nothing launched)
+// when a block does not fit the LDS of one workgroup.
+bool launch_block_svd(const SvdLayout& lay, const double* a, double* u, double* vh, double* s, int* status, hipStream_t st);
+// the slice of a packed matrix keeping logical indices 'ind' of axis i_ax (as host_slice), reading
+// logical index x from within-block position srcpos[x]; scale (indexed like ind) multiplies the kept
+// rows / columns when given
+void plan_slice_2d(const Meta& m, int i_ax, const std::vector<ct_long>& ind, const std::vector<ct_long>& srcpos, const std::vector<double>* scale,
+	Meta& out, std::vector<SliceBlock>& blocks, std::vector<int>& idx, std::vector<double>& sc);
+DevTensor device_slice_2d(const DevTensor& src, const Meta& out, const std::vector<SliceBlock>& blocks, const std::vector<int>& idx,
+	const std::vector<double>& sc, bool scaled, hipStream_t st);
+
 // host block factorisations (host_bond.cpp)
 // block SVD of a packed block-sparse matrix; iso < 0: one-sided Jacobi (u, vh isometric), iso 0 / 1:
 // host_svd_factor per block (vh = diag(s) Vh, resp. u = U diag(s))
@@ -498,6 +530,15 @@ struct DmrgState
 	HostPacked split_in, split_m0, split_m1;
 	trunc_info split_info = {};
 	int split_ret = 0;
+	// device split of the current bond (block SVD on the GPU, truncation on the host): the matrix,
+	// its factors, the singular values in the host layout (descending within each block) and the
+	// device column of each new-bond index
+	bool split_on_device = false;
+	SvdLayout split_lay;
+	DevTensor split_am, split_u, split_vh;
+	std::vector<double> split_s;
+	std::vector<ct_long> split_pos, split_ind;
+	ct_long split_num = 0;
 	// phase timing (environment variable CT_DMRG_PHASES=1): the stream is synchronised at every
 	// phase boundary and the wall time since the previous boundary is added to the phase
 	enum { PH_PLAN, PH_LANCZOS, PH_SPLIT_BEGIN, PH_SPLIT_HOST, PH_SPLIT_UPLOAD, PH_ENVIRONMENT, PH_MERGE, PH_COUNT };
@@ -512,6 +553,7 @@ struct DmrgState
 	void advance_host();
 	int advance_end(double en);
 	int advance(double en, DevTensor& a_opt);
+	static bool device_split_enabled();
 	void download_mps(mps* psi) const;
 	size_t device_bytes() const;
 };

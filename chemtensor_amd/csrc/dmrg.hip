@@ -11,6 +11,8 @@
 
 #include <chrono>
 #include <cstdlib>
+#include <numeric>
+#include <string>
 
 #include "ct_internal.h"
 
@@ -242,24 +244,111 @@ void DmrgState::advance_begin(const DevTensor& a_opt)
 	DevTensor tmp = dev_flatten_axes(a2, 0, TENSOR_AXIS_OUT, st);
 	DevTensor am = dev_flatten_axes(tmp, 1, TENSOR_AXIS_IN, st);
 	split_meta = a2.meta;
-	split_in = download_packed(am, st);
+	split_on_device = false;
+	if (device_split_enabled()) {
+		SvdLayout lay = plan_block_svd(am.meta);
+		if (!lay.dummy && block_svd_lds_bytes(lay) <= block_svd_lds_limit()) {
+			const size_t ni = lay.qi.size(), nj = lay.jobs.size();
+			DevTensor u(lay.um), vh(lay.vhm);
+			double* sd = (double*)dev_alloc(ni * sizeof(double));
+			int* stat = (int*)dev_alloc(nj * sizeof(int));
+			std::vector<double> sv(ni);
+			std::vector<int> status(nj);
+			bool ok = launch_block_svd(lay, am.d, u.d, vh.d, sd, stat, st);
+			if (ok) {
+				CTG_HIP(hipMemcpyAsync(sv.data(), sd, ni * sizeof(double), hipMemcpyDeviceToHost, st));
+				CTG_HIP(hipMemcpyAsync(status.data(), stat, nj * sizeof(int), hipMemcpyDeviceToHost, st));
+				CTG_HIP(hipStreamSynchronize(st));
+				for (int x : status) ok = ok && (x == 0);
+			}
+			dev_free(sd);
+			dev_free(stat);
+			if (ok) {
+				// descending order within each block (stable, as the host SVD returns them)
+				split_s.assign(ni, 0.0);
+				split_pos.assign(ni, 0);
+				for (const SvdLayout::Job& j : lay.jobs) {
+					const ct_long k = std::min(j.r, j.c);
+					std::vector<ct_long> ord((size_t)k);
+					std::iota(ord.begin(), ord.end(), 0);
+					std::stable_sort(ord.begin(), ord.end(), [&](ct_long x, ct_long y) { return sv[(size_t)(j.s_off + x)] > sv[(size_t)(j.s_off + y)]; });
+					for (ct_long p = 0; p < k; p++) {
+						split_s[(size_t)(j.s_off + p)] = sv[(size_t)(j.s_off + ord[p])];
+						split_pos[(size_t)(j.s_off + p)] = ord[p];
+					}
+				}
+				split_lay = std::move(lay);
+				split_u = std::move(u);
+				split_vh = std::move(vh);
+				split_am = std::move(am);
+				split_on_device = true;
+			}
+		}
+	}
+	if (!split_on_device) split_in = download_packed(am, st);
 	phase_end(PH_SPLIT_BEGIN);
 }
 
 void DmrgState::advance_host()
 {
 	const double t0 = trace ? wall_now() : 0;
-	split_ret = host_split_svd(split_in, tol_split, max_vdim, false, dir == 0 ? SVD_DISTR_RIGHT : SVD_DISTR_LEFT, split_m0, split_m1, split_info);
-	split_in = HostPacked();
+	if (split_on_device) {
+		split_num = host_retained_bond_indices(split_s, tol_split, true, max_vdim, split_ind, split_info);
+		split_ret = 0;
+	}
+	else {
+		split_ret = host_split_svd(split_in, tol_split, max_vdim, false, dir == 0 ? SVD_DISTR_RIGHT : SVD_DISTR_LEFT, split_m0, split_m1, split_info);
+		split_in = HostPacked();
+	}
 	if (trace) phase_s[PH_SPLIT_HOST] += wall_now() - t0;
+}
+
+// the device split with CT_SPLIT=device (read at every split). It is not the default: at the block
+// sizes of these sweeps the LDS Jacobi (latency-bound steps, ~10 sweeps) takes longer than the host
+// factorisation, and the blocks of large bonds exceed the LDS of a workgroup.
+bool DmrgState::device_split_enabled()
+{
+	const char* e = getenv("CT_SPLIT");
+	return e && std::string(e) == "device";
 }
 
 int DmrgState::advance_end(double en)
 {
 	if (split_ret < 0) return split_ret;
 	phase_mark();
+	if (split_on_device && split_num == 0) {
+		// every singular value zero: the reference's path (a unit vector on the isometric side)
+		split_in = download_packed(split_am, st);
+		split_ret = host_split_svd(split_in, tol_split, max_vdim, false, dir == 0 ? SVD_DISTR_RIGHT : SVD_DISTR_LEFT, split_m0, split_m1, split_info);
+		split_in = HostPacked();
+		split_on_device = false;
+		if (split_ret < 0) return split_ret;
+	}
 	{
-		DevTensor d0 = upload_packed(split_m0, st), d1 = upload_packed(split_m1, st);
+		DevTensor d0, d1;
+		if (split_on_device) {
+			// retained columns of U and rows of Vh; the singular values go to the left factor when
+			// sweeping right-to-left, to the right one otherwise (mps_split_tensor_svd)
+			const bool left = (dir != 0);
+			std::vector<double> sr(split_ind.size());
+			for (size_t i = 0; i < sr.size(); i++) sr[i] = split_s[(size_t)split_ind[i]];
+			Meta m0, m1;
+			std::vector<SliceBlock> b;
+			std::vector<int> ix;
+			std::vector<double> sc;
+			plan_slice_2d(split_lay.um, 1, split_ind, split_pos, left ? &sr : nullptr, m0, b, ix, sc);
+			d0 = device_slice_2d(split_u, m0, b, ix, sc, left, st);
+			plan_slice_2d(split_lay.vhm, 0, split_ind, split_pos, left ? nullptr : &sr, m1, b, ix, sc);
+			d1 = device_slice_2d(split_vh, m1, b, ix, sc, !left, st);
+			split_u = DevTensor();
+			split_vh = DevTensor();
+			split_am = DevTensor();
+			split_on_device = false;
+		}
+		else {
+			d0 = upload_packed(split_m0, st);
+			d1 = upload_packed(split_m1, st);
+		}
 		const Meta& m = split_meta;
 		const ct_long dl0[2] = { m.dim_logical[0], m.dim_logical[1] }, dl1[2] = { m.dim_logical[2], m.dim_logical[3] };
 		const int ad0[2] = { m.axis_dir[0], m.axis_dir[1] }, ad1[2] = { m.axis_dir[2], m.axis_dir[3] };

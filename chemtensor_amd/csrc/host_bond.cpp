@@ -46,25 +46,45 @@ static Meta matrix_meta(ct_long d0, const qnumber* q0, ct_long d1, const qnumber
 	return make_meta(2, dim, ad, qn);
 }
 
-int host_block_svd(const HostPacked& a, HostPacked& u, std::vector<double>& s, HostPacked& vh, int iso)
+SvdLayout plan_block_svd(const Meta& m)
 {
-	const Meta& m = a.meta;
-	std::vector<qnumber> qi;
+	SvdLayout L;
 	for (ct_long j = 0; j < m.dim_blocks[1]; j++) for (ct_long i = 0; i < m.dim_blocks[0]; i++) {
 		if (m.axis_dir[0] * m.qnums_blocks[0][i] + m.axis_dir[1] * m.qnums_blocks[1][j] != 0) continue;
 		const ct_long k = std::min(m.block_count[0][i], m.block_count[1][j]);
-		for (ct_long l = 0; l < k; l++) qi.push_back(m.qnums_blocks[1][j]);
+		for (ct_long l = 0; l < k; l++) L.qi.push_back(m.qnums_blocks[1][j]);
 	}
-	const bool dummy = qi.empty();
-	if (dummy) qi.push_back(-m.axis_dir[0] * m.axis_dir[1] * m.qnums_blocks[0][0]);
-	const ct_long ni = (ct_long)qi.size();
-	u.meta = matrix_meta(m.dim_logical[0], m.qnums_logical[0].data(), ni, qi.data(), m.axis_dir[0], m.axis_dir[1]);
+	L.dummy = L.qi.empty();
+	if (L.dummy) L.qi.push_back(-m.axis_dir[0] * m.axis_dir[1] * m.qnums_blocks[0][0]);
+	const ct_long ni = (ct_long)L.qi.size();
+	L.um = matrix_meta(m.dim_logical[0], m.qnums_logical[0].data(), ni, L.qi.data(), m.axis_dir[0], m.axis_dir[1]);
+	L.vhm = matrix_meta(ni, L.qi.data(), m.dim_logical[1], m.qnums_logical[1].data(), -m.axis_dir[1], m.axis_dir[1]);
+	if (L.dummy) return L;
+	for (ct_long i = 0; i < m.dim_blocks[0]; i++) for (ct_long j = 0; j < m.dim_blocks[1]; j++) {
+		const ct_long ib[2] = { i, j };
+		const ct_long ka = m.flat_index(ib);
+		if (!m.present(ka)) continue;
+		const ct_long r = m.block_count[0][i], c = m.block_count[1][j];
+		const ct_long kq = L.um.find_qnum(1, m.qnums_blocks[1][j]);
+		const ct_long iu[2] = { i, kq }, iv[2] = { kq, j };
+		ct_long l0 = 0;
+		while (l0 < ni && L.qi[l0] != m.qnums_blocks[1][j]) l0++;
+		L.jobs.push_back(SvdLayout::Job{ m.offset[ka], L.um.offset[L.um.flat_index(iu)], L.vhm.offset[L.vhm.flat_index(iv)], l0, r, c });
+	}
+	return L;
+}
+
+int host_block_svd(const HostPacked& a, HostPacked& u, std::vector<double>& s, HostPacked& vh, int iso)
+{
+	const SvdLayout lay = plan_block_svd(a.meta);
+	const ct_long ni = (ct_long)lay.qi.size();
+	u.meta = lay.um;
 	u.v.assign((size_t)u.meta.nelem, 0.0);
-	vh.meta = matrix_meta(ni, qi.data(), m.dim_logical[1], m.qnums_logical[1].data(), -m.axis_dir[1], m.axis_dir[1]);
+	vh.meta = lay.vhm;
 	vh.v.assign((size_t)vh.meta.nelem, 0.0);
 	s.assign((size_t)ni, 0.0);
-	if (dummy) {
-		const ct_long ib[2] = { 0, u.meta.find_qnum(1, qi[0]) };
+	if (lay.dummy) {
+		const ct_long ib[2] = { 0, u.meta.find_qnum(1, lay.qi[0]) };
 		u.v[(size_t)u.meta.offset[u.meta.flat_index(ib)]] = 1;
 		return 0;
 	}
@@ -72,18 +92,10 @@ int host_block_svd(const HostPacked& a, HostPacked& u, std::vector<double>& s, H
 	struct Job { const double* a; double* ub; double* vb; double* sb; ct_long r, c; double work; };
 	std::vector<Job> jobs;
 	double total = 0;
-	for (ct_long i = 0; i < m.dim_blocks[0]; i++) for (ct_long j = 0; j < m.dim_blocks[1]; j++) {
-		const ct_long ib[2] = { i, j };
-		const ct_long ka = m.flat_index(ib);
-		if (!m.present(ka)) continue;
-		const ct_long r = m.block_count[0][i], c = m.block_count[1][j], k = std::min(r, c);
-		const ct_long kq = u.meta.find_qnum(1, m.qnums_blocks[1][j]);
-		const ct_long iu[2] = { i, kq }, iv[2] = { kq, j };
-		ct_long l0 = 0;
-		while (l0 < ni && qi[l0] != m.qnums_blocks[1][j]) l0++;
-		const double w = (double)std::max(r, c) * (double)k * (double)k;
-		jobs.push_back(Job{ a.v.data() + m.offset[ka], u.v.data() + u.meta.offset[u.meta.flat_index(iu)],
-			vh.v.data() + vh.meta.offset[vh.meta.flat_index(iv)], s.data() + l0, r, c, w });
+	for (const SvdLayout::Job& j : lay.jobs) {
+		const ct_long k = std::min(j.r, j.c);
+		const double w = (double)std::max(j.r, j.c) * (double)k * (double)k;
+		jobs.push_back(Job{ a.v.data() + j.a_off, u.v.data() + j.u_off, vh.v.data() + j.vh_off, s.data() + j.s_off, j.r, j.c, w });
 		total += w;
 	}
 	std::atomic<int> ret(0);

@@ -164,6 +164,13 @@ int ct_matvec_plan_destroy(ct_matvec_plan* plan);
 /// Returns 0 when valid, a positive check number otherwise, or a CT_ERR_* code.
 int ct_matvec_plan_check(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l,
 	const struct block_sparse_tensor* r);
+/// SVD of one dense matrix a (m x n, row-major, host memory) on the device, with the kernel that
+/// factorises the blocks of the DMRG split (one-sided Jacobi in the LDS of one workgroup, the
+/// device counterpart of the block loop of block_sparse_tensor_svd, block_sparse_tensor.c:2686-2848):
+/// u (m x k), s (k, descending), vh (k x n), k = min(m, n), row-major host arrays.
+/// Returns 0, -1 when the sweeps did not converge, CT_ERR_UNSUPPORTED when the matrix does not fit
+/// the LDS of one workgroup ((max(m,n) + min(m,n)) min(m,n) doubles).
+int ct_dense_svd_device(const ct_long m, const ct_long n, const double* a, double* u, double* s, double* vh);
 /// length of the packed vectors (= block_sparse_tensor_num_elements_blocks(a))
 ct_long ct_matvec_plan_dim(const ct_matvec_plan* plan);
 /// algorithmic FP64 flops of one matvec: sum over the reference's GEMM calls of 2 m n k

@@ -194,10 +194,10 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=64, help="problems of the batched-launch measurement (0: skip)")
-    ap.add_argument("--serve", type=int, default=24, help="requests of the serving measurement (0: skip)")
+    ap.add_argument("--serve", type=int, default=16, help="requests of the serving measurement (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate (requests/s)")
-    ap.add_argument("--serve-L", default="16,32", help="chain length range")
-    ap.add_argument("--serve-chi", default="32,128", help="bond dimension range")
+    ap.add_argument("--serve-L", default="16,64", help="chain length range")
+    ap.add_argument("--serve-chi", default="32,512", help="bond dimension range")
     ap.add_argument("--serve-budget", type=float, default=200e9, help="HBM admission budget (bytes)")
     ap.add_argument("--lanczos-iter", type=int, default=20, help="Lanczos steps of the timed ground-state solve")
     ap.add_argument("--hbm-traffic-kib", default="",

@@ -415,16 +415,36 @@ static int gram_factor(ct_long k, ct_long n, const double* b, std::vector<double
 	// descending order
 	z.assign((size_t)(k * k), 0.0);
 	for (ct_long l = 0; l < k; l++) std::copy(&CM(q.data(), k, 0, k - 1 - l), &CM(q.data(), k, 0, k - 1 - l) + k, &CM(z.data(), k, 0, l));
-	// right = z^T b, four output rows per pass over b
+	// right = z^T b: tiles of 4 rows x 16 columns of 'right' accumulated in registers over the k
+	// rows of b (each entry sums i = 0 .. k-1 in order)
 	right.assign((size_t)(k * n), 0.0);
 	for (ct_long l0 = 0; l0 < k; l0 += 4) {
 		const ct_long nl = std::min<ct_long>(4, k - l0);
+		ct_long t0 = 0;
+		if (nl == 4) {
+			for (; t0 + 16 <= n; t0 += 16) {
+				double acc[4][16] = {};
+				for (ct_long i = 0; i < k; i++) {
+					const double* bi = b + (size_t)i * n + t0;
+					const double z0 = CM(z.data(), k, i, l0), z1 = CM(z.data(), k, i, l0 + 1);
+					const double z2 = CM(z.data(), k, i, l0 + 2), z3 = CM(z.data(), k, i, l0 + 3);
+					for (int c = 0; c < 16; c++) {
+						const double bv = bi[c];
+						acc[0][c] += z0 * bv;
+						acc[1][c] += z1 * bv;
+						acc[2][c] += z2 * bv;
+						acc[3][c] += z3 * bv;
+					}
+				}
+				for (int l = 0; l < 4; l++) std::copy(acc[l], acc[l] + 16, right.data() + (size_t)(l0 + l) * n + t0);
+			}
+		}
 		for (ct_long i = 0; i < k; i++) {
 			const double* bi = b + (size_t)i * n;
 			for (ct_long l = 0; l < nl; l++) {
 				const double zl = CM(z.data(), k, i, l0 + l);
 				double* r = right.data() + (size_t)(l0 + l) * n;
-				for (ct_long t = 0; t < n; t++) r[t] += zl * bi[t];
+				for (ct_long t = t0; t < n; t++) r[t] += zl * bi[t];
 			}
 		}
 	}

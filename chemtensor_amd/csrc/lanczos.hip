@@ -15,15 +15,16 @@ namespace ctg {
 
 constexpr int LZ_THREADS = 256;
 
+// sum over the workgroup: butterfly within each wavefront, then the four wavefront sums in order
+// (the same value in every thread, the same order in every launch)
 __device__ __forceinline__ double block_sum(double v, double* sh)
 {
-	sh[threadIdx.x] = v;
+	static_assert(LZ_THREADS == 256, "four wavefronts");
+	#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+	if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
 	__syncthreads();
-	for (int s = LZ_THREADS / 2; s > 0; s >>= 1) {
-		if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-		__syncthreads();
-	}
-	const double r = sh[0];
+	const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 	__syncthreads();
 	return r;
 }
@@ -152,7 +153,7 @@ LanczosWork::~LanczosWork()
 // v0 = vstart / |vstart|, then maxiter steps of the recurrence; only kernel launches on s
 static void enqueue_recurrence(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart, int maxiter)
 {
-	const int G = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 4095) / 4096));
+	const int G = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 1023) / 1024));
 	double* V = lw.v;
 	double* w = lw.w;
 	double* scal = lw.scal;

@@ -153,7 +153,7 @@ LanczosWork::~LanczosWork()
 // v0 = vstart / |vstart|, then maxiter steps of the recurrence; only kernel launches on s
 static void enqueue_recurrence(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart, int maxiter)
 {
-	const int G = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 1023) / 1024));
+	const int G = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 511) / 512));
 	double* V = lw.v;
 	double* w = lw.w;
 	double* scal = lw.scal;

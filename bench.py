@@ -117,6 +117,36 @@ def bench_batch(ct, nprob, seed, steps, warmup, lanczos_iter):
     }
 
 
+def bench_dmrg_precision(ct, seed):
+    """the precision option over whole sweeps: dmrg_twosite with FP64 matvecs and with split-FP32
+    matvecs (CT_DMRG_MATVEC=split; Lanczos scalars, eigensolver and split stay FP64) from the same
+    seeded start state; the energy difference per sweep"""
+    out = []
+    for model, L, chi, sweeps in (("hubbard", 16, 64, 4), ("xxz", 32, 64, 4)):
+        runs = {}
+        for form in ("fp64", "split"):
+            if form == "split":
+                os.environ["CT_DMRG_MATVEC"] = "split"
+            try:
+                if model == "hubbard":
+                    mpo = ct.MPO.hubbard(L, 1.0, 4.0, 0.0)
+                    psi = ct.MPS.random(L, 4, mpo.qsite, ct.encode_pair(L, 0), chi, seed)
+                else:
+                    mpo = ct.MPO.xxz(L, 1.0, 0.5, 0.0)
+                    psi = ct.MPS.random(L, 2, mpo.qsite, 0, chi, seed)
+                t0 = time.perf_counter()
+                en, _ent = ct.dmrg(mpo, psi, num_sweeps=sweeps, maxiter_lanczos=10, tol_split=1e-10, max_vdim=chi)
+                runs[form] = (en, time.perf_counter() - t0)
+            finally:
+                os.environ.pop("CT_DMRG_MATVEC", None)
+        err = [abs(a - b) for a, b in zip(runs["split"][0], runs["fp64"][0])]
+        out.append({"model": model, "nsites": L, "max_vdim": chi, "sweeps": sweeps,
+                    "energies_fp64": [float(x) for x in runs["fp64"][0]], "energy_error_split_fp32_per_sweep": err,
+                    "seconds_fp64": round(runs["fp64"][1], 3), "seconds_split_fp32": round(runs["split"][1], 3)})
+        log(f"dmrg {model} L={L} chi={chi}: split-FP32 energy error per sweep {['%.1e' % e for e in err]}")
+    return out
+
+
 def bench_serve(ct, nreq, rate, L_range, chi_range, seed, budget, max_active, timeout_s=240.0):
     """Serving loop: nreq ground-state solve requests with Poisson arrivals at `rate` per second,
     each an XXZ chain (Delta ~ U[0,2]) or a Fermi-Hubbard chain (U ~ U[0,8]) with L and chi drawn
@@ -200,6 +230,7 @@ def main():
     ap.add_argument("--serve-chi", default="32,512", help="bond dimension range")
     ap.add_argument("--serve-budget", type=float, default=200e9, help="HBM admission budget (bytes)")
     ap.add_argument("--lanczos-iter", type=int, default=20, help="Lanczos steps of the timed ground-state solve")
+    ap.add_argument("--dmrg-precision", type=int, default=1, help="energy error of whole sweeps with split-FP32 matvecs (0: skip)")
     ap.add_argument("--hbm-traffic-kib", default="",
                     help="FETCH_SIZE,WRITE_SIZE (KiB per launch) of the dominant kernel from rocprofv3 --pmc runs")
     ap.add_argument("--fetch-scale", type=float, default=2.0,
@@ -359,6 +390,9 @@ def main():
                          "ms_per_solve_eager": round(lz["eager"], 4), "ms_per_step_graph": round(lz["graph"] / args.lanczos_iter, 5),
                          "lambda0": lam, "form": head}
     log(f"lanczos {args.lanczos_iter} steps: graph {lz['graph']:.3f} ms, eager {lz['eager']:.3f} ms")
+
+    if args.dmrg_precision:
+        result["dmrg_precision"] = bench_dmrg_precision(ct, args.seed)
 
     if args.batch > 0:
         result["batched"] = bench_batch(ct, args.batch, args.seed, max(10, args.steps // 5), 3, args.lanczos_iter)

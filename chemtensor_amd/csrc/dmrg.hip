@@ -413,6 +413,15 @@ size_t DmrgState::device_bytes() const
 	return n * sizeof(double);
 }
 
+// the matvec form of the sweep's Lanczos: grouped FP64, or with CT_DMRG_MATVEC=split the grouped
+// form on FP32 MFMA with split operands (the Lanczos scalars, the tridiagonal eigensolver and the
+// split stay FP64), to measure the precision option's energy error over whole sweeps
+static int dmrg_matvec_mode()
+{
+	const char* e = getenv("CT_DMRG_MATVEC");
+	return (e && std::string(e) == "split") ? CT_MATVEC_GROUPED_SPLIT : CT_MATVEC_GROUPED_FP64;
+}
+
 int dmrg_twosite_device(const mpo* H, int num_sweeps, int maxiter, double tol_split, ct_long max_vdim, mps* psi, double* en_sweeps, double* entropy)
 {
 	DmrgState S;
@@ -427,7 +436,7 @@ int dmrg_twosite_device(const mpo* H, int num_sweeps, int maxiter, double tol_sp
 		// minimize_local_energy (dmrg.c:86-147) with the Lanczos iteration on the device
 		S.phase_mark();
 		std::unique_ptr<MatvecPlan> p(build_matvec_plan_dev(S.theta.meta, DevTensor::view(S.h2[S.bond], S.h2[S.bond].meta),
-			DevTensor::view(S.lb[S.bond], S.lb[S.bond].meta), DevTensor::view(S.rb[S.bond + 1], S.rb[S.bond + 1].meta), CT_MATVEC_GROUPED_FP64, S.st, false, false));
+			DevTensor::view(S.lb[S.bond], S.lb[S.bond].meta), DevTensor::view(S.rb[S.bond + 1], S.rb[S.bond + 1].meta), dmrg_matvec_mode(), S.st, false, false));
 		S.phase_end(DmrgState::PH_PLAN);
 		ao = DevTensor(S.theta.meta);
 		ret = lanczos_ground_state(*p, S.theta.d, maxiter, &en, ao.d);

@@ -58,3 +58,22 @@ def test_dmrg_matches_oracle(model, L, chi, sweeps, maxiter, tol, seed):
     assert np.max(np.abs(en_g - en_o)) < 1e-10
     assert pp.bond_dims() == op.bond_dims()
     assert np.max(np.abs(ent_g - ent_o)) < 1e-8
+
+
+def test_dmrg_split_fp32_matvec_energies_within_1e10():
+    """the precision option over whole sweeps (CT_DMRG_MATVEC=split: split-FP32 MFMA matvecs, FP64
+    Lanczos scalars, eigensolver and split): energies within 1e-10 of the FP64 sweep"""
+    import os
+    def run():
+        mpo = ct.MPO.hubbard(12, 1.0, 4.0, 0.0)
+        psi = ct.MPS.random(12, 4, mpo.qsite, ct.encode_pair(12, 0), 48, 11)
+        en, _ = ct.dmrg(mpo, psi, num_sweeps=3, maxiter_lanczos=10, tol_split=1e-10, max_vdim=48)
+        return en
+    en64 = run()
+    os.environ["CT_DMRG_MATVEC"] = "split"
+    try:
+        en32 = run()
+    finally:
+        os.environ.pop("CT_DMRG_MATVEC", None)
+    say("fp64", en64.tolist(), "split", en32.tolist())
+    assert np.max(np.abs(en32 - en64)) < 1e-10

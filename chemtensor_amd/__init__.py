@@ -122,6 +122,7 @@ def lib():
             "ct_generate_matvec_problem": (C.c_int, [C.c_int, C.c_int, ct_long, C.c_uint64, bst, bst, bst, bst]),
             "ct_construct_heisenberg_xxz_1d_mpo": (None, [C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(MPOStruct)]),
             "ct_construct_fermi_hubbard_1d_mpo": (None, [C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(MPOStruct)]),
+            "ct_construct_spin_molecular_hamiltonian_mpo": (None, [C.c_int, DP, DP, C.POINTER(MPOStruct)]),
             "delete_mpo": (None, [C.POINTER(MPOStruct)]),
             "ct_dense_svd_device": (C.c_int, [ct_long, ct_long, DP, DP, DP, DP]),
             "delete_mps": (None, [C.POINTER(MPSStruct)]),
@@ -559,6 +560,35 @@ class MPO:
         lib().ct_construct_fermi_hubbard_1d_mpo(nsites, t, u, mu, C.pointer(m.s))
         m.live = True
         return m
+
+    @staticmethod
+    def spin_molecular(tkin, vint):
+        """spin molecular Hamiltonian, tkin (n, n), vint (n, n, n, n) in physicists' convention"""
+        t = np.ascontiguousarray(tkin, dtype=np.float64)
+        v = np.ascontiguousarray(vint, dtype=np.float64)
+        n = t.shape[0]
+        assert t.shape == (n, n) and v.shape == (n, n, n, n)
+        m = MPO()
+        lib().ct_construct_spin_molecular_hamiltonian_mpo(n, t.ctypes.data_as(DP), v.ctypes.data_as(DP), C.pointer(m.s))
+        m.live = True
+        return m
+
+
+def molecular_coefficients(n, seed):
+    """Synthetic molecular integrals (seeded): t_ij symmetric with entries N(0,1)/n; chemists'
+    (ij|kl) with 8-fold symmetry and entries N(0, 0.1^2), made positive semidefinite as the pair
+    matrix [(ij), (kl)] (negative eigenvalues set to zero, which keeps the symmetries); returned as
+    (tkin, vint) with vint in physicists' convention, v_ijkl = (ik|jl)."""
+    rng = np.random.default_rng(seed)
+    t = rng.standard_normal((n, n)) / n
+    t = (t + t.T) / 2
+    g = rng.standard_normal((n, n, n, n)) * 0.1
+    g = (g + g.transpose(1, 0, 2, 3) + g.transpose(0, 1, 3, 2) + g.transpose(1, 0, 3, 2)
+         + g.transpose(2, 3, 0, 1) + g.transpose(3, 2, 0, 1) + g.transpose(2, 3, 1, 0) + g.transpose(3, 2, 1, 0)) / 8
+    w, u = np.linalg.eigh(g.reshape(n * n, n * n))
+    g = ((u * np.maximum(w, 0)) @ u.T).reshape(n, n, n, n)
+    v = np.ascontiguousarray(g.transpose(0, 2, 1, 3))
+    return t, v
 
 
 class MPS:

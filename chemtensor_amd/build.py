@@ -15,7 +15,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libchemtensor_gpu.so")
 
 HIP_SOURCES = ["kernels.hip", "fused.hip", "runtime.hip", "engine.hip", "lanczos.hip", "split.hip", "dmrg.hip", "serve.hip", "capi.hip"]
-CPP_SOURCES = ["host_tensor.cpp", "host_linalg.cpp", "host_models.cpp", "host_bond.cpp"]
+CPP_SOURCES = ["host_tensor.cpp", "host_linalg.cpp", "host_models.cpp", "host_molecular.cpp", "host_bond.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
 # gfx950 device code; packed FP32 instructions are disabled for all HIP code

@@ -153,6 +153,20 @@ def construct_fermi_hubbard_1d_mpo(nsites, t, u, mu):
     return MPO(_ct.MPO.hubbard(nsites, t, u, mu))
 
 
+def construct_spin_molecular_hamiltonian_mpo(tkin, vint, optimize=False):
+    """hamiltonian.c:2755-2990: H = sum t_ij a+_{i s} a_{j s} + 1/2 sum v_ijkl a+_{i s} a+_{j u} a_{l u} a_{k s}
+    (physicists' convention); 'optimize' is accepted for the reference's signature (the bond basis
+    here is always the finite-state machine over the terms)"""
+    tkin = np.asarray(tkin, dtype=np.float64)
+    vint = np.asarray(vint, dtype=np.float64)
+    n = tkin.shape[0] if tkin.ndim == 2 else -1
+    if tkin.ndim != 2 or tkin.shape != (n, n):
+        raise ValueError("'tkin' must be a square matrix")
+    if vint.shape != (n, n, n, n):
+        raise ValueError("'vint' must be a degree-4 tensor with the dimensions of 'tkin'")
+    return MPO(_ct.MPO.spin_molecular(tkin, vint))
+
+
 def dmrg(mpo, num_sweeps=5, maxiter_lanczos=20, tol_split=1e-10, max_vdim=256, qnum_sector=0, rng_seed=42):
     """Two-site DMRG on the GPU: (psi, en_sweeps, entropy), as chemtensor.dmrg (pymodule.c:3238-3358).
 

@@ -134,6 +134,13 @@ void ct_construct_heisenberg_xxz_1d_mpo(const int nsites, const double J, const 
 /// The operator of construct_fermi_hubbard_1d_mpo_assembly + mpo_from_assembly
 /// (src/operator/hamiltonian.c:240-366, mpo.c:59-175) built directly as an MPO (bond dim 6).
 void ct_construct_fermi_hubbard_1d_mpo(const int nsites, const double t, const double u, const double mu, struct mpo* mpo);
+/// The operator of construct_spin_molecular_hamiltonian_mpo_assembly + mpo_from_assembly
+/// (src/operator/hamiltonian.c:2755-2990, mpo.c:59-175): H = sum t_ij a+_{i s} a_{j s}
+/// + 1/2 sum v_ijkl a+_{i s} a+_{j u} a_{l u} a_{k s} over nsites spatial orbitals, tkin (n x n) and
+/// vint (n x n x n x n, physicists' convention) row-major; built directly as a finite-state-machine
+/// MPO over the terms (bond states: left parts of one or two operators, three-operator sums indexed
+/// by the fourth operator). Local basis and quantum numbers as the Fermi-Hubbard MPO.
+void ct_construct_spin_molecular_hamiltonian_mpo(const int nsites, const double* tkin, const double* vint, struct mpo* mpo);
 /// mpo.c:255-282 (mpo.h:56), on the GPU
 void mpo_merge_tensor_pair(const struct block_sparse_tensor* a0, const struct block_sparse_tensor* a1, struct block_sparse_tensor* a);
 

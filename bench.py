@@ -147,6 +147,43 @@ def bench_dmrg_precision(ct, seed):
     return out
 
 
+def bench_molecular(ct, n, chi, sweeps, seed, split):
+    """configuration 3: two-site DMRG on a synthetic spin molecular Hamiltonian (n spatial orbitals,
+    sector N = n, 2 Sz = 0), FP64 matvecs and split-FP32 matvecs from the same seeded start state;
+    energy per sweep and the split form's energy error per sweep"""
+    t, v = ct.molecular_coefficients(n, seed)
+    t0 = time.perf_counter()
+    mpo = ct.MPO.spin_molecular(t, v)
+    t_mpo = time.perf_counter() - t0
+    s = mpo.s
+    mdims = [int(s.a[i].dim_logical[3]) for i in range(n - 1)]
+    runs = {}
+    for form in (("fp64", "split") if split else ("fp64",)):
+        if form == "split":
+            os.environ["CT_DMRG_MATVEC"] = "split"
+        try:
+            psi = ct.MPS.random(n, 4, mpo.qsite, ct.encode_pair(n, 0), chi, seed)
+            t0 = time.perf_counter()
+            en, _ent = ct.dmrg(mpo, psi, num_sweeps=sweeps, maxiter_lanczos=10, tol_split=1e-10, max_vdim=chi)
+            runs[form] = (en, time.perf_counter() - t0, max(psi.bond_dims()))
+        finally:
+            os.environ.pop("CT_DMRG_MATVEC", None)
+    out = {"workload": f"two-site DMRG, spin molecular Hamiltonian n={n} orbitals (seeded integrals: t N(0,1)/n symmetric, "
+                       f"(ij|kl) 8-fold symmetric N(0,0.1^2) made PSD as a pair matrix), sector N={n} 2Sz=0, chi={chi}, "
+                       f"{sweeps} sweeps, 10 Lanczos steps",
+           "mpo_max_bond": max(mdims), "mpo_seconds": round(t_mpo, 3),
+           "energies_fp64": [float(x) for x in runs["fp64"][0]], "seconds_fp64": round(runs["fp64"][1], 3),
+           "mps_max_bond": runs["fp64"][2]}
+    log(f"molecular n={n} chi={chi}: MPO bond <= {max(mdims)} ({t_mpo:.2f} s), FP64 {sweeps} sweeps {runs['fp64'][1]:.2f} s, "
+        f"E = {runs['fp64'][0][-1]:.12f}")
+    if split:
+        err = [abs(a - b) for a, b in zip(runs["split"][0], runs["fp64"][0])]
+        out["energy_error_split_fp32_per_sweep"] = err
+        out["seconds_split_fp32"] = round(runs["split"][1], 3)
+        log(f"molecular split-FP32 energy error per sweep {['%.1e' % e for e in err]} ({runs['split'][1]:.2f} s)")
+    return out
+
+
 def bench_serve(ct, nreq, rate, L_range, chi_range, seed, budget, max_active, timeout_s=240.0):
     """Serving loop: nreq ground-state solve requests with Poisson arrivals at `rate` per second,
     each an XXZ chain (Delta ~ U[0,2]) or a Fermi-Hubbard chain (U ~ U[0,8]) with L and chi drawn
@@ -231,6 +268,10 @@ def main():
     ap.add_argument("--serve-budget", type=float, default=200e9, help="HBM admission budget (bytes)")
     ap.add_argument("--lanczos-iter", type=int, default=20, help="Lanczos steps of the timed ground-state solve")
     ap.add_argument("--dmrg-precision", type=int, default=1, help="energy error of whole sweeps with split-FP32 matvecs (0: skip)")
+    ap.add_argument("--molecular", type=int, default=20, help="orbitals of the molecular DMRG measurement (0: skip)")
+    ap.add_argument("--molecular-chi", type=int, default=512)
+    ap.add_argument("--molecular-sweeps", type=int, default=6)
+    ap.add_argument("--molecular-split", type=int, default=1, help="also run the sweeps with split-FP32 matvecs")
     ap.add_argument("--hbm-traffic-kib", default="",
                     help="FETCH_SIZE,WRITE_SIZE (KiB per launch) of the dominant kernel from rocprofv3 --pmc runs")
     ap.add_argument("--fetch-scale", type=float, default=2.0,
@@ -393,6 +434,10 @@ def main():
 
     if args.dmrg_precision:
         result["dmrg_precision"] = bench_dmrg_precision(ct, args.seed)
+
+    if args.molecular > 0:
+        result["molecular"] = bench_molecular(ct, args.molecular, args.molecular_chi, args.molecular_sweeps, args.seed,
+                                              args.molecular_split)
 
     if args.batch > 0:
         result["batched"] = bench_batch(ct, args.batch, args.seed, max(10, args.steps // 5), 3, args.lanczos_iter)

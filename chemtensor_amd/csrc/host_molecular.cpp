@@ -10,12 +10,15 @@
 //     term are ordered by site (the sign of the permutation goes into the coefficient), then the
 //     local operator at site k is the product of the term's operators there times P_k to the power
 //     of the number of the term's operators right of k (P = local fermion parity);
-//   - the state of a bond after site b is the part of a term left of it: nothing (I), one or two
-//     operators (their product, coefficient 1), three operators (indexed by the fourth operator,
-//     which is right of b; the coefficient is applied when the third operator is placed), or the
-//     whole term (H). Terms with the same left part share the state: the transition that places a
-//     term's coefficient adds up over the terms, every other transition (into one- or two-operator
-//     states, along a bond, a three-operator sum completed by its fourth operator) is set once.
+//   - the state of a term at the bond after site b: nothing placed yet (I), the whole term placed
+//     (H), or, on the bonds of the left half, its left operators (one or two, coefficient 1) or,
+//     with three placed, its remaining right operator (the coefficient included); on the bonds of
+//     the right half its single left operator (coefficient 1) or its remaining right operators (one
+//     or two, the coefficient included). Pairs of operators thus index states only on the shorter
+//     side of a bond. Terms with the same key share the state; the transition of a term from a state
+//     without its coefficient (I, left operators) to one with it (right operators, H) places the
+//     coefficient and adds up over the terms, every other transition is fixed by its two states
+//     and set once.
 // The bond quantum number of a state is the charge of its left part.
 
 #include <algorithm>
@@ -132,6 +135,29 @@ extern "C" void ct_construct_spin_molecular_hamiltonian_mpo(const int nsites, co
 				coeff = -coeff;
 			}
 		int cn = 0, cs = 0;   // charge of the left part
+		// the state of the term after site b with 'cnt' of its operators left of the bond
+		auto key_at = [&](int bond, int cnt) -> Key {
+			if (cnt == 0) return kI;
+			if (cnt == m) return kH;
+			const bool left_keyed = (bond + 1) <= (n - 1 - bond);
+			Key k;
+			if (m == 2) {
+				if (left_keyed) k = Key{ 1, ops[0] };
+				else k = Key{ 2, ops[1] };
+			}
+			else if (left_keyed) {
+				if (cnt <= 2) { k = Key{ 1 }; k.insert(k.end(), ops.begin(), ops.begin() + cnt); }
+				else k = Key{ 2, ops[3] };
+			}
+			else {
+				if (cnt == 1) k = Key{ 1, ops[0] };
+				else { k = Key{ 2 }; k.insert(k.end(), ops.begin() + cnt, ops.end()); }
+			}
+			return k;
+		};
+		// kinds 0 (I) and 1 (left operators) carry no coefficient, 2 (right operators) and 3 (H) do
+		auto weighted_kind = [](const Key& k) { return k[0] >= 2; };
+		Key pk = kI;
 		int prev = state(-1, kI, 0, 0);
 		int count = 0;
 		int x = 0;
@@ -139,26 +165,6 @@ extern "C" void ct_construct_spin_molecular_hamiltonian_mpo(const int nsites, co
 			const int x0 = x;
 			while (x < m && (ops[x] >> 2) == k) x++;
 			const int mk = x - x0;
-			Key nk;
-			double c = 1;
-			bool weighted = false;   // this transition carries the term's coefficient
-			if (mk == 0) {
-				// passing through: the left part continues, or the term is complete / not started
-				if (count == 0) nk = kI;
-				else if (count == m) nk = kH;
-				else if (m == 4 && count == 3) nk = Key{ 2, ops[3] };
-				else { nk = Key{ 1 }; nk.insert(nk.end(), ops.begin(), ops.begin() + count); }
-				const int next = state(k, nk, cn, cs);
-				if (count > 0 && count < m) {
-					auto key = std::make_pair(prev, next);
-					if (!pass[(size_t)k].count(key)) {
-						pass[(size_t)k][key] = true;
-						trans[(size_t)k][key] = (count % 2) ? L.parity : Mat4::id();
-					}
-				}
-				prev = next;
-				continue;
-			}
 			Mat4 loc = Mat4::id();
 			for (int y = x0; y < x; y++) {
 				const Op o = op_of(ops[y]);
@@ -168,27 +174,22 @@ extern "C" void ct_construct_spin_molecular_hamiltonian_mpo(const int nsites, co
 				cn += dn;
 				cs += ds;
 			}
-			if ((count + mk) % 2) loc = loc * L.parity;
-			const int before = count;
 			count += mk;
-			if (count == m) { nk = kH; if (before < 3 || m == 2) { c = coeff; weighted = true; } }
-			else if (m == 4 && count == 3) { nk = Key{ 2, ops[3] }; c = coeff; weighted = true; }
-			else { nk = Key{ 1 }; nk.insert(nk.end(), ops.begin(), ops.begin() + count); }
+			if (count % 2) loc = loc * L.parity;
+			const Key nk = key_at(k, count);
 			const int next = state(k, nk, cn, cs);
 			const auto key = std::make_pair(prev, next);
-			if (!weighted) {
-				// into a left part of one or two operators, or a three-operator sum completed by its
-				// fourth operator: the same operator for every term that shares it (set once)
-				if (!pass[(size_t)k].count(key)) {
-					pass[(size_t)k][key] = true;
-					trans[(size_t)k][key] = loc;
-				}
-			}
-			else {
-				// the transition that places the term's coefficient: the terms add up
+			if (!weighted_kind(pk) && weighted_kind(nk)) {
+				// the one transition of the term that places its coefficient: the terms add up
 				Mat4& t = trans[(size_t)k][key];
-				for (int e = 0; e < 16; e++) t.a[e] += c * loc.a[e];
+				for (int e = 0; e < 16; e++) t.a[e] += coeff * loc.a[e];
 			}
+			else if (!pass[(size_t)k].count(key)) {
+				// every other transition is fixed by its two states: set once
+				pass[(size_t)k][key] = true;
+				trans[(size_t)k][key] = loc;
+			}
+			pk = nk;
 			prev = next;
 		}
 	};

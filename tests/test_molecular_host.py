@@ -67,6 +67,9 @@ def test_spin_molecular_mpo_bond_dimensions():
     assert dims[0] == 1 and dims[-1] == 1
     # inner bond after site b: I, H, one-operator and two-operator left parts, three-operator sums
     # indexed by the remaining operator
+    # pairs of operators index states only on the shorter side of a bond: the bond dimensions are
+    # symmetric about the centre and grow quadratically towards it
+    assert dims == dims[::-1]
     for b in range(n - 1):
-        ops_left = 4 * (b + 1)
-        assert dims[b + 1] > ops_left
+        short = min(b + 1, n - 1 - b)
+        assert dims[b + 1] > 4 * short

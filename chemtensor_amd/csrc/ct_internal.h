@@ -425,8 +425,11 @@ struct GroupedTables
 	int64_t t1_elems = 0, t2_elems = 0;
 	double stage_bytes[3] = { 0, 0, 0 };
 };
+// lean: only the tables of the fixed-form launches (stage 2 as the column stream): no MFMA-tile
+// stage-2 tables and no fused-stage tables (the caller plans again without it when the column
+// stream does not apply, wa.kmax > WA_KMAX)
 void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const Meta& L, const Meta& R,
-	int64_t x_off, int64_t w_off, int64_t l_off, int64_t r_off);
+	int64_t x_off, int64_t w_off, int64_t l_off, int64_t r_off, bool lean = false);
 // host check of the tables (bounds, each intermediate / output entry written once); 0 when valid
 int check_grouped_tables(const Meta& th, const Meta& W, const Meta& L, const Meta& R);
 // append src, planned on its own (intermediates from offset 0), behind the intermediates of dst

@@ -474,7 +474,7 @@ double reference_flops(const Meta& th, const Meta& w, const Meta& l, const Meta&
 // Append the three grouped stages of one problem to g. Element offsets: theta / y at x_off, W at
 // w_off, L at l_off, R at r_off, its T1 / T2 blocks from the current ends of g's intermediates.
 void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const Meta& L, const Meta& R,
-	int64_t x_off, int64_t w_off, int64_t l_off, int64_t r_off)
+	int64_t x_off, int64_t w_off, int64_t l_off, int64_t r_off, bool lean)
 {
 	GemmBatch& s1 = g.s1;
 	GemmBatch& s2 = g.s2;
@@ -543,6 +543,11 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 			const ct_long off = t2_elems;
 			t2_off[t2_key(A, M, Q, C)] = off;
 			t2_elems += dA * dM * dQ * dC;
+			if (lean) {
+				// T2 layout and the flop count only: stage 2 runs as the column stream
+				for (auto& pn : tl) s2.flops += 2.0 * (double)(dM * dQ) * (double)(dA * dC) * (double)(bc(W, 2, pn.first) * bc(W, 3, pn.second));
+				continue;
+			}
 			s2.add_task(off_ptr_mut(t2_base + off), (int)(dM * dQ), (int)(dA * dC), (int)dC, (int)dC, dM * dQ * dC, false);
 			for (auto& pn : tl) {
 				const ct_long P = pn.first, N = pn.second;
@@ -629,6 +634,7 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 			wa.units.push_back(u);
 			for (int col0 = 0; col0 < (int)(dA * dC); col0 += 64) wa.items.push_back(WaItem{ u, unit, col0 });
 			// fused stages 1 + 2: one group per column group (P,N) of the class
+			if (lean) { wa.f12_ok = false; continue; }
 			if (c.R > F12_RMAX) wa.f12_ok = false;
 			const int g0 = (int)wa.groups.size();
 			for (auto& pn : c.cols) {
@@ -817,10 +823,14 @@ int check_grouped_tables(const Meta& th, const Meta& W, const Meta& L, const Met
 	return 0;
 }
 
-static void build_grouped(MatvecPlan& p)
+static void build_grouped(MatvecPlan& p, bool lean)
 {
 	GroupedTables g;
-	plan_grouped_tables(g, p.theta_meta, p.w.meta, p.l.meta, p.r.meta, 0, 0, 0, 0);
+	plan_grouped_tables(g, p.theta_meta, p.w.meta, p.l.meta, p.r.meta, 0, 0, 0, 0, lean);
+	if (lean && g.wa.kmax > WA_KMAX) {
+		g = GroupedTables();
+		plan_grouped_tables(g, p.theta_meta, p.w.meta, p.l.meta, p.r.meta, 0, 0, 0, 0, false);
+	}
 	Meta m1, m2;
 	m1.nelem = g.t1_elems;
 	m2.nelem = g.t2_elems;
@@ -1064,7 +1074,7 @@ MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& 
 			p->stage_bytes[0] = 8.0 * (2.0 * (double)theta.nelem + (double)p->w.meta.nelem + (double)p->l.meta.nelem + (double)p->r.meta.nelem);
 			p->nunits = p->fused.nunits;
 		}
-		else build_grouped(*p);
+		else build_grouped(*p, mode == CT_MATVEC_GROUPED_FP64 && !autotune);
 		if (mode == CT_MATVEC_GROUPED_FP64 && autotune) autotune_stages(*p);
 		// without timing, stage 2 takes the column stream (faster at every size measured)
 		if (mode == CT_MATVEC_GROUPED_FP64 && !autotune) p->use_wapply = p->wapply.ok;

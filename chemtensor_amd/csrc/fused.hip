@@ -560,20 +560,28 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 		if (in[i].l.meta.nelem) launch_copy(in[i].l.d, b->l + probs[i].l_off, in[i].l.meta.nelem, s);
 		if (in[i].r.meta.nelem) launch_copy(in[i].r.d, b->r + probs[i].r_off, in[i].r.meta.nelem, s);
 	}
-	// grouped three-stage tables over the concatenated operands
+	// grouped three-stage tables over the concatenated operands; without tuning only the tables of
+	// the fixed forms (stage 2 as the column stream) unless the column stream does not apply
 	GroupedTables g;
-	if (par && in.size() > 1) {
-		// plan every problem on its own (in parallel), then concatenate
-		std::vector<GroupedTables> part(in.size());
-		(*par)((int)in.size(), [&](int i) {
-			plan_grouped_tables(part[i], in[i].theta, in[i].w.meta, in[i].l.meta, in[i].r.meta, probs[i].x_off, probs[i].w_off, probs[i].l_off, probs[i].r_off);
-		});
-		for (const GroupedTables& t : part) merge_grouped_tables(g, t);
-	}
-	else {
-		for (size_t i = 0; i < in.size(); i++)
-			plan_grouped_tables(g, in[i].theta, in[i].w.meta, in[i].l.meta, in[i].r.meta, probs[i].x_off, probs[i].w_off, probs[i].l_off, probs[i].r_off);
-	}
+	auto plan_all = [&](bool lean) {
+		g = GroupedTables();
+		if (par && in.size() > 1) {
+			// plan every problem on its own (in parallel), then concatenate
+			std::vector<GroupedTables> part(in.size());
+			(*par)((int)in.size(), [&](int i) {
+				plan_grouped_tables(part[i], in[i].theta, in[i].w.meta, in[i].l.meta, in[i].r.meta, probs[i].x_off, probs[i].w_off,
+					probs[i].l_off, probs[i].r_off, lean);
+			});
+			for (const GroupedTables& t : part) merge_grouped_tables(g, t);
+		}
+		else {
+			for (size_t i = 0; i < in.size(); i++)
+				plan_grouped_tables(g, in[i].theta, in[i].w.meta, in[i].l.meta, in[i].r.meta, probs[i].x_off, probs[i].w_off, probs[i].l_off,
+					probs[i].r_off, lean);
+		}
+	};
+	plan_all(!tune);
+	if (!tune && g.wa.kmax > WA_KMAX) plan_all(false);
 	b->t1 = (double*)dev_alloc((size_t)std::max<int64_t>(g.t1_elems, 1) * sizeof(double));
 	b->t2 = (double*)dev_alloc((size_t)std::max<int64_t>(g.t2_elems, 1) * sizeof(double));
 	if (!tune) b->flops = g.s1.flops + g.s2.flops + g.s3.flops;   // executed count (no replanning of the reference sequence)

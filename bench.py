@@ -245,6 +245,10 @@ def bench_serve(ct, nreq, rate, L_range, chi_range, seed, budget, max_active, ti
         "queue_wait_p50_s": round(float(np.percentile([r.t_admit - r.t_submit for r in done] or [0], 50)), 4),
         "sweeps_mean": round(float(np.mean([r.sweeps for r in done])) if done else 0, 2),
         "makespan_s": round(makespan, 3),
+        "slowest": ({"request": desc[int(np.argmax([r.t_done - r.t_submit if r.status == 0 else -1 for r in res]))],
+                     "latency_s": round(float(lat.max()), 4),
+                     "sweeps": int(res[int(np.argmax([r.t_done - r.t_submit if r.status == 0 else -1 for r in res]))].sweeps)}
+                    if done else None),
         "hbm_budget_bytes": budget, "max_active": max_active,
         "server_stats": st,
     }

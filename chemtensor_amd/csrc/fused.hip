@@ -617,9 +617,10 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 	b->nblocks = (int)segs.size();
 	b->segs = upload_vec(segs, s);
 	b->probs = upload_vec(pr, s);
-	CTG_HIP(hipStreamSynchronize(s));
-	// time both forms on the batch itself and keep the faster
+	// time both forms on the batch itself and keep the faster (without tuning, the caller's
+	// launches follow on the same stream: the staged uploads and copies need no wait)
 	if (tune) {
+		CTG_HIP(hipStreamSynchronize(s));
 		double* x = (double*)dev_alloc((size_t)std::max<int64_t>(nx, 1) * sizeof(double));
 		double* y = (double*)dev_alloc((size_t)std::max<int64_t>(nx, 1) * sizeof(double));
 		launch_fill(x, 1.0, nx, s);

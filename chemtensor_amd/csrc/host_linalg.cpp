@@ -5,6 +5,8 @@
 // dense_tensor.c:2391-2879). QR and RQ use LAPACK's Householder sign convention (dlarfg), so
 // the gauge of the factors matches the reference; the SVD is Hestenes one-sided Jacobi.
 
+#include <immintrin.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -113,25 +115,83 @@ static void larfg(ct_long n, double& alpha, double* x, ct_long inc, double& tau)
 	alpha = beta;
 }
 
+// horizontal sum of the four lanes, (l0 + l1) + (l2 + l3)
+static inline double hsum4(__m256d v)
+{
+	alignas(32) double l[4];
+	_mm256_store_pd(l, v);
+	return (l[0] + l[1]) + (l[2] + l[3]);
+}
+
+// c (m x n, column-major, leading dimension ldc) <- (I - tau v v^T) c. Four columns at a time:
+// their dot products with v in four-lane partial sums, then the rank-1 update.
 static void apply_left(ct_long m, ct_long n, const double* v, ct_long incv, double tau, double* c, ct_long ldc)
 {
 	if (tau == 0) return;
-	for (ct_long j = 0; j < n; j++) {
-		double w = 0;
-		for (ct_long i = 0; i < m; i++) w += v[i * incv] * CM(c, ldc, i, j);
+	if (incv != 1) {
+		for (ct_long j = 0; j < n; j++) {
+			double w = 0;
+			for (ct_long i = 0; i < m; i++) w += v[i * incv] * CM(c, ldc, i, j);
+			w *= tau;
+			for (ct_long i = 0; i < m; i++) CM(c, ldc, i, j) -= v[i * incv] * w;
+		}
+		return;
+	}
+	const ct_long m4 = m & ~(ct_long)3;
+	ct_long j = 0;
+	for (; j + 4 <= n; j += 4) {
+		double* c0 = &CM(c, ldc, 0, j);
+		double* c1 = c0 + ldc;
+		double* c2 = c1 + ldc;
+		double* c3 = c2 + ldc;
+		__m256d a0 = _mm256_setzero_pd(), a1 = a0, a2 = a0, a3 = a0;
+		for (ct_long i = 0; i < m4; i += 4) {
+			const __m256d vv = _mm256_loadu_pd(v + i);
+			a0 = _mm256_fmadd_pd(vv, _mm256_loadu_pd(c0 + i), a0);
+			a1 = _mm256_fmadd_pd(vv, _mm256_loadu_pd(c1 + i), a1);
+			a2 = _mm256_fmadd_pd(vv, _mm256_loadu_pd(c2 + i), a2);
+			a3 = _mm256_fmadd_pd(vv, _mm256_loadu_pd(c3 + i), a3);
+		}
+		double w0 = hsum4(a0), w1 = hsum4(a1), w2 = hsum4(a2), w3 = hsum4(a3);
+		for (ct_long i = m4; i < m; i++) { w0 += v[i] * c0[i]; w1 += v[i] * c1[i]; w2 += v[i] * c2[i]; w3 += v[i] * c3[i]; }
+		w0 *= tau; w1 *= tau; w2 *= tau; w3 *= tau;
+		const __m256d b0 = _mm256_set1_pd(w0), b1 = _mm256_set1_pd(w1), b2 = _mm256_set1_pd(w2), b3 = _mm256_set1_pd(w3);
+		for (ct_long i = 0; i < m4; i += 4) {
+			const __m256d vv = _mm256_loadu_pd(v + i);
+			_mm256_storeu_pd(c0 + i, _mm256_fnmadd_pd(vv, b0, _mm256_loadu_pd(c0 + i)));
+			_mm256_storeu_pd(c1 + i, _mm256_fnmadd_pd(vv, b1, _mm256_loadu_pd(c1 + i)));
+			_mm256_storeu_pd(c2 + i, _mm256_fnmadd_pd(vv, b2, _mm256_loadu_pd(c2 + i)));
+			_mm256_storeu_pd(c3 + i, _mm256_fnmadd_pd(vv, b3, _mm256_loadu_pd(c3 + i)));
+		}
+		for (ct_long i = m4; i < m; i++) { c0[i] -= v[i] * w0; c1[i] -= v[i] * w1; c2[i] -= v[i] * w2; c3[i] -= v[i] * w3; }
+	}
+	for (; j < n; j++) {
+		double* cj = &CM(c, ldc, 0, j);
+		__m256d a0 = _mm256_setzero_pd();
+		for (ct_long i = 0; i < m4; i += 4) a0 = _mm256_fmadd_pd(_mm256_loadu_pd(v + i), _mm256_loadu_pd(cj + i), a0);
+		double w = hsum4(a0);
+		for (ct_long i = m4; i < m; i++) w += v[i] * cj[i];
 		w *= tau;
-		for (ct_long i = 0; i < m; i++) CM(c, ldc, i, j) -= v[i * incv] * w;
+		for (ct_long i = 0; i < m; i++) cj[i] -= v[i] * w;
 	}
 }
 
+// c (m x n, column-major) <- c (I - tau v v^T), v with stride incv: w = c v accumulated column by
+// column (each w_i sums j = 0 .. n-1 in order), then c -= tau w v^T column by column.
 static void apply_right(ct_long m, ct_long n, const double* v, ct_long incv, double tau, double* c, ct_long ldc)
 {
-	if (tau == 0) return;
-	for (ct_long i = 0; i < m; i++) {
-		double w = 0;
-		for (ct_long j = 0; j < n; j++) w += CM(c, ldc, i, j) * v[j * incv];
-		w *= tau;
-		for (ct_long j = 0; j < n; j++) CM(c, ldc, i, j) -= w * v[j * incv];
+	if (tau == 0 || m == 0) return;
+	std::vector<double> w((size_t)m, 0.0);
+	for (ct_long j = 0; j < n; j++) {
+		const double vj = v[j * incv];
+		const double* cj = &CM(c, ldc, 0, j);
+		for (ct_long i = 0; i < m; i++) w[i] += cj[i] * vj;
+	}
+	for (ct_long i = 0; i < m; i++) w[i] *= tau;
+	for (ct_long j = 0; j < n; j++) {
+		const double vj = v[j * incv];
+		double* cj = &CM(c, ldc, 0, j);
+		for (ct_long i = 0; i < m; i++) cj[i] -= w[i] * vj;
 	}
 }
 
@@ -323,29 +383,73 @@ int host_svd(ct_long m, ct_long n, const double* a, double* u, double* s, double
 }
 
 // Gram matrix of the rows of b (k x n, row-major): c (k x k, column-major, lower triangle and
-// diagonal) = b b^T. 4 x 4 row tiles with four partial sums per entry (fixed order).
+// diagonal) = b b^T. Tiles of 2 x 4 rows, four-lane partial sums per entry (fixed order).
 static void gram_rows(ct_long k, ct_long n, const double* b, double* c)
 {
 	const ct_long n4 = n & ~(ct_long)3;
-	for (ct_long i0 = 0; i0 < k; i0 += 4) {
-		const ct_long ni = std::min<ct_long>(4, k - i0);
-		for (ct_long j0 = 0; j0 <= i0; j0 += 4) {
+	for (ct_long i0 = 0; i0 < k; i0 += 2) {
+		const ct_long ni = std::min<ct_long>(2, k - i0);
+		for (ct_long j0 = 0; j0 < i0 + ni; j0 += 4) {
 			const ct_long nj = std::min<ct_long>(4, k - j0);
-			double acc[4][4][4] = {};
-			const double* bi[4];
+			const double* bi[2];
 			const double* bj[4];
-			for (int a = 0; a < 4; a++) bi[a] = b + (size_t)(i0 + std::min<ct_long>(a, ni - 1)) * n;
+			for (int a = 0; a < 2; a++) bi[a] = b + (size_t)(i0 + std::min<ct_long>(a, ni - 1)) * n;
 			for (int a = 0; a < 4; a++) bj[a] = b + (size_t)(j0 + std::min<ct_long>(a, nj - 1)) * n;
-			for (ct_long t = 0; t < n4; t += 4)
-				for (int a = 0; a < 4; a++)
-					for (int q = 0; q < 4; q++)
-						for (int l = 0; l < 4; l++) acc[a][q][l] += bi[a][t + l] * bj[q][t + l];
+			__m256d acc[2][4];
+			for (int a = 0; a < 2; a++) for (int q = 0; q < 4; q++) acc[a][q] = _mm256_setzero_pd();
+			for (ct_long t = 0; t < n4; t += 4) {
+				const __m256d x0 = _mm256_loadu_pd(bi[0] + t), x1 = _mm256_loadu_pd(bi[1] + t);
+				for (int q = 0; q < 4; q++) {
+					const __m256d y = _mm256_loadu_pd(bj[q] + t);
+					acc[0][q] = _mm256_fmadd_pd(x0, y, acc[0][q]);
+					acc[1][q] = _mm256_fmadd_pd(x1, y, acc[1][q]);
+				}
+			}
 			for (int a = 0; a < ni; a++)
 				for (int q = 0; q < nj; q++) {
-					double x = (acc[a][q][0] + acc[a][q][1]) + (acc[a][q][2] + acc[a][q][3]);
+					if (j0 + q > i0 + a) continue;
+					double x = hsum4(acc[a][q]);
 					for (ct_long t = n4; t < n; t++) x += bi[a][t] * bj[q][t];
-					if (j0 + q <= i0 + a) CM(c, k, i0 + a, j0 + q) = x;
+					CM(c, k, i0 + a, j0 + q) = x;
 				}
+		}
+	}
+}
+
+// right (k x n, row-major) = z^T b with z (k x k, column-major) and b (k x n, row-major): tiles of
+// 6 rows x 8 columns of right accumulated in registers over the k rows of b (each entry sums
+// i = 0 .. k-1 in order)
+static void project_rows(ct_long k, ct_long n, const double* z, const double* b, double* right)
+{
+	const ct_long n8 = n & ~(ct_long)7;
+	for (ct_long l0 = 0; l0 < k; l0 += 6) {
+		const ct_long nl = std::min<ct_long>(6, k - l0);
+		const double* zc[6];
+		for (int l = 0; l < 6; l++) zc[l] = z + (size_t)k * (size_t)(l0 + std::min<ct_long>(l, nl - 1));
+		for (ct_long t0 = 0; t0 < n8; t0 += 8) {
+			__m256d acc[6][2];
+			for (int l = 0; l < 6; l++) acc[l][0] = acc[l][1] = _mm256_setzero_pd();
+			for (ct_long i = 0; i < k; i++) {
+				const double* bi = b + (size_t)i * n + t0;
+				const __m256d b0 = _mm256_loadu_pd(bi), b1 = _mm256_loadu_pd(bi + 4);
+				for (int l = 0; l < 6; l++) {
+					const __m256d zz = _mm256_broadcast_sd(zc[l] + i);
+					acc[l][0] = _mm256_fmadd_pd(zz, b0, acc[l][0]);
+					acc[l][1] = _mm256_fmadd_pd(zz, b1, acc[l][1]);
+				}
+			}
+			for (int l = 0; l < nl; l++) {
+				_mm256_storeu_pd(right + (size_t)(l0 + l) * n + t0, acc[l][0]);
+				_mm256_storeu_pd(right + (size_t)(l0 + l) * n + t0 + 4, acc[l][1]);
+			}
+		}
+		for (ct_long l = 0; l < nl; l++) {
+			double* r = right + (size_t)(l0 + l) * n;
+			for (ct_long t = n8; t < n; t++) {
+				double x = 0;
+				for (ct_long i = 0; i < k; i++) x += zc[l][i] * b[(size_t)i * n + t];
+				r[t] = x;
+			}
 		}
 	}
 }
@@ -415,39 +519,8 @@ static int gram_factor(ct_long k, ct_long n, const double* b, std::vector<double
 	// descending order
 	z.assign((size_t)(k * k), 0.0);
 	for (ct_long l = 0; l < k; l++) std::copy(&CM(q.data(), k, 0, k - 1 - l), &CM(q.data(), k, 0, k - 1 - l) + k, &CM(z.data(), k, 0, l));
-	// right = z^T b: tiles of 4 rows x 16 columns of 'right' accumulated in registers over the k
-	// rows of b (each entry sums i = 0 .. k-1 in order)
 	right.assign((size_t)(k * n), 0.0);
-	for (ct_long l0 = 0; l0 < k; l0 += 4) {
-		const ct_long nl = std::min<ct_long>(4, k - l0);
-		ct_long t0 = 0;
-		if (nl == 4) {
-			for (; t0 + 16 <= n; t0 += 16) {
-				double acc[4][16] = {};
-				for (ct_long i = 0; i < k; i++) {
-					const double* bi = b + (size_t)i * n + t0;
-					const double z0 = CM(z.data(), k, i, l0), z1 = CM(z.data(), k, i, l0 + 1);
-					const double z2 = CM(z.data(), k, i, l0 + 2), z3 = CM(z.data(), k, i, l0 + 3);
-					for (int c = 0; c < 16; c++) {
-						const double bv = bi[c];
-						acc[0][c] += z0 * bv;
-						acc[1][c] += z1 * bv;
-						acc[2][c] += z2 * bv;
-						acc[3][c] += z3 * bv;
-					}
-				}
-				for (int l = 0; l < 4; l++) std::copy(acc[l], acc[l] + 16, right.data() + (size_t)(l0 + l) * n + t0);
-			}
-		}
-		for (ct_long i = 0; i < k; i++) {
-			const double* bi = b + (size_t)i * n;
-			for (ct_long l = 0; l < nl; l++) {
-				const double zl = CM(z.data(), k, i, l0 + l);
-				double* r = right.data() + (size_t)(l0 + l) * n;
-				for (ct_long t = t0; t < n; t++) r[t] += zl * bi[t];
-			}
-		}
-	}
+	project_rows(k, n, z.data(), b, right.data());
 	return 0;
 }
 

@@ -129,6 +129,8 @@ struct HostPool
 struct Server
 {
 	HostPool hosts;
+	// threads of the worker's batch plans only: a plan never queues behind the splits in 'hosts'
+	HostPool planners;
 	std::mutex mu;
 	std::condition_variable cv_work, cv_done;
 	std::deque<std::shared_ptr<Ticket>> pending;
@@ -214,7 +216,7 @@ struct Server
 			in[i].r = DevTensor::view(S.rb[S.bond + 1], S.rb[S.bond + 1].meta);
 		}
 		double c0 = now();
-		const ParallelFor par = [this](int n, const std::function<void(int)>& f) { hosts.run(n, [&f](int i, hipStream_t) { f(i); }); };
+		const ParallelFor par = [this](int n, const std::function<void(int)>& f) { planners.run(n, [&f](int i, hipStream_t) { f(i); }); };
 		std::unique_ptr<BatchPlan> b(build_batch_plan(in, st, false, &par));
 		double c1 = now();
 		t_plan += c1 - c0;
@@ -359,6 +361,7 @@ int ct_serve_create(const double hbm_budget_bytes, const int max_active, ct_serv
 		S.t0 = std::chrono::steady_clock::now();
 		CTG_HIP(hipStreamCreateWithFlags(&S.st, hipStreamNonBlocking));
 		S.hosts.start(S.host_threads);
+		S.planners.start(std::min(4, S.host_threads));
 		S.worker = std::thread([&S] { S.loop(); });
 		*server = h.release();
 		return 0;
@@ -459,6 +462,7 @@ int ct_serve_destroy(ct_serve* server)
 	S.cv_work.notify_all();
 	S.cv_done.notify_all();
 	if (S.worker.joinable()) S.worker.join();
+	S.planners.shutdown();
 	S.hosts.shutdown();
 	for (auto& kv : S.tickets) kv.second->state.reset();
 	(void)hipStreamSynchronize(S.st);

@@ -50,16 +50,21 @@ def test_serve_same_result_alone_and_in_mix():
     tickets = []
     for i, q in enumerate(MIX):          # staggered arrivals: later requests join running ones
         tickets.append(srv.submit(q))
-        time.sleep(0.05)
+        time.sleep(0.005)
+    res = []
     for i, t in enumerate(tickets):
         r = srv.poll(t, wait=True)
         en = srv.energies(t)
-        say("mix", i, r.status, r.sweeps, en.tolist(), "latency", r.t_done - r.t_submit)
+        say("mix", i, r.status, r.sweeps, en.tolist(), "admit", r.t_admit, "done", r.t_done, "latency", r.t_done - r.t_submit)
         assert r.status == 0
         assert len(en) == len(alone[i])
         assert np.max(np.abs(en - alone[i])) <= 1e-10 * max(1.0, np.max(np.abs(alone[i])))
+        res.append((r.t_admit, r.t_done))
     st = srv.stats()
     say("stats", st)
+    # every later request was admitted while an earlier one was still running (the solves overlapped)
+    for i in range(1, len(res)):
+        assert any(res[j][0] <= res[i][0] < res[j][1] for j in range(i)), i
     assert st["max_batch"] >= 2      # work of different requests shared batched launches
     srv.close()
 

@@ -184,12 +184,14 @@ def bench_molecular(ct, n, chi, sweeps, seed, split):
     return out
 
 
-def bench_serve(ct, nreq, rate, L_range, chi_range, seed, budget, max_active, timeout_s=240.0):
+def bench_serve(ct, nreq, rate, L_range, chi_range, seed, budget, max_active, timeout_s=None):
     """Serving loop: nreq ground-state solve requests with Poisson arrivals at `rate` per second,
     each an XXZ chain (Delta ~ U[0,2]) or a Fermi-Hubbard chain (U ~ U[0,8]) with L and chi drawn
     uniformly from the given ranges, Lanczos 10 steps per bond, stop when a sweep changes the energy
     by less than 1e-10 (at most 12 sweeps). Latency = finish - submit."""
     import threading
+    if timeout_s is None:
+        timeout_s = max(240.0, 1.0 * nreq)
     rng = np.random.default_rng(seed)
     reqs, desc = [], []
     for i in range(nreq):
@@ -219,12 +221,16 @@ def bench_serve(ct, nreq, rate, L_range, chi_range, seed, budget, max_active, ti
     th.start()
     th.join()
     res = []
+    t_note = time.perf_counter()
     for i in range(nreq):
         while True:
             r = srv.poll(tickets[i], wait=False)
             if r.status != 1 or time.perf_counter() - t0 > timeout_s:
                 break
             time.sleep(0.01)
+            if time.perf_counter() - t_note > 30:      # progress of long runs
+                t_note = time.perf_counter()
+                log(f"serve: {i}/{nreq} finished in order after {t_note - t0:.0f}s")
         res.append(r)
     makespan = time.perf_counter() - t0
     st = srv.stats()

@@ -68,6 +68,16 @@ void* dev_alloc(size_t nbytes);
 void copy_h2d_async(void* dst, const void* src, size_t nbytes, hipStream_t s);
 void dev_free(void* p);
 void dev_trim();
+// marks a stream capture on the calling thread: the allocator does not trim its cache (a device
+// synchronisation) meanwhile
+struct CaptureScope
+{
+	CaptureScope();
+	~CaptureScope();
+};
+// release the calling thread's pinned staging blocks and its cached device blocks (a long-lived
+// helper thread calls it before it exits, once its stream has completed)
+void release_thread_resources();
 size_t dev_bytes_in_use();
 
 struct DevTensor
@@ -496,6 +506,9 @@ struct BatchPlan
 	LzProb* probs = nullptr;
 	int nblocks = 0;
 	double flops = 0, bytes = 0;
+	// host seconds of the plan's parts: [0] operand buffers and copies, [1] grouped tables,
+	// [2] work items and uploads, [3] Lanczos reduction blocks
+	double host_s[4] = { 0, 0, 0, 0 };
 	BatchLanczosWork lz;
 	~BatchPlan();
 };

@@ -41,17 +41,25 @@ void GemmBatch::add_term(const double* A, const double* B, int K, int sa_i, int 
 
 std::vector<WorkItem> GemmBatch::work_items() const
 {
-	std::vector<WorkItem> items;
+	// tiles in order of decreasing cost, ties in task and tile order: the (cost, tile) keys are
+	// sorted, then the items are written once in that order
+	struct Key { int cost; int32_t task; int m0, n0; };
+	std::vector<Key> keys;
 	for (size_t ti = 0; ti < tasks.size(); ti++) {
 		const GemmTask& t = tasks[ti];
 		int64_t ksum = 0;
 		for (int q = t.t0; q < t.t1; q++) ksum += terms[q].K;
 		const int cost = (int)std::min<int64_t>(ksum + 4, (int64_t)1 << 30);
 		for (int m0 = 0; m0 < t.M; m0 += TILE)
-			for (int n0 = 0; n0 < t.N; n0 += TILE)
-				items.push_back(WorkItem{ t, m0, n0, cost, (int32_t)ti, t.t1 > t.t0 ? terms[t.t0] : GemmTerm{} });
+			for (int n0 = 0; n0 < t.N; n0 += TILE) keys.push_back(Key{ cost, (int32_t)ti, m0, n0 });
 	}
-	std::stable_sort(items.begin(), items.end(), [](const WorkItem& a, const WorkItem& b) { return a.cost > b.cost; });
+	std::stable_sort(keys.begin(), keys.end(), [](const Key& a, const Key& b) { return a.cost > b.cost; });
+	std::vector<WorkItem> items;
+	items.reserve(keys.size());
+	for (const Key& k : keys) {
+		const GemmTask& t = tasks[(size_t)k.task];
+		items.push_back(WorkItem{ t, k.m0, k.n0, k.cost, k.task, t.t1 > t.t0 ? terms[t.t0] : GemmTerm{} });
+	}
 	return items;
 }
 
@@ -73,9 +81,10 @@ void DevGemm::upload(const GemmBatch& b, hipStream_t s)
 	grid = std::max(1, std::min((nitems + 3) / 4, cap));
 	// workgroup-level split of the k-step stream only pays for long streams
 	double steps = 0;
-	for (const WorkItem& w : it) {
-		const GemmTask& t = w.task;
-		for (int q = t.t0; q < t.t1; q++) steps += (b.terms[q].K + 3) / 4;
+	for (const GemmTask& t : b.tasks) {
+		double ts = 0;
+		for (int q = t.t0; q < t.t1; q++) ts += (b.terms[q].K + 3) / 4;
+		steps += ts * (double)(((t.M + TILE - 1) / TILE) * ((t.N + TILE - 1) / TILE));
 	}
 	const double avg = nitems ? steps / nitems : 0;
 	form = (avg > 8.0) ? GEMM_WG_CHUNK : GEMM_WAVE_STREAM;
@@ -146,12 +155,24 @@ static GemmBatch plan_dot(const Meta& s, int axs, const Meta& t, int axt, int nm
 		}
 		b.add_task(off_ptr_mut(r.offset[k]), (int)M, (int)N, (int)N, (int)N, 0, accumulate);
 		for (int i = 0; i < nm; i++) ic[i] = 0;
-		for (ct_long mm = 0; mm < ncontract; mm++) {
-			for (int i = 0; i < nds; i++) is[offset_s + i] = ir[i];
+		// the contracted block indices in row-major order; the last one is fixed by the vanishing
+		// quantum number sum of s, so only the leading nm - 1 are enumerated and the last is found
+		// by binary search (same pairs, same order as scanning all combinations)
+		const ct_long nlast = (nm > 0) ? t.dim_blocks[shift_t + nm - 1] : 1;
+		const ct_long nprefix = (nm > 0) ? ncontract / std::max<ct_long>(nlast, 1) : 1;
+		for (int i = 0; i < nds; i++) is[offset_s + i] = ir[i];
+		for (ct_long pp = 0; pp < nprefix && (nm == 0 || nlast > 0); pp++) {
 			for (int i = 0; i < nm; i++) is[shift_s + i] = ic[i];
-			qnumber qsum = 0;
-			for (int i = 0; i < s.ndim; i++) qsum += s.axis_dir[i] * s.qnums_blocks[i][is[i]];
-			if (qsum == 0) {
+			bool found = true;
+			if (nm > 0) {
+				const int al = shift_s + nm - 1;
+				qnumber rest = 0;
+				for (int i = 0; i < s.ndim; i++) if (i != al) rest += s.axis_dir[i] * s.qnums_blocks[i][is[i]];
+				const ct_long j = s.find_qnum(al, -s.axis_dir[al] * rest);
+				if (j < 0) found = false;
+				else { ic[nm - 1] = j; is[al] = j; }
+			}
+			if (found) {
 				for (int i = 0; i < ndt; i++) it[offset_t + i] = ir[nds + i];
 				for (int i = 0; i < nm; i++) it[shift_t + i] = ic[i];
 				const ct_long sk = s.flat_index(is), tk = t.flat_index(it);
@@ -169,7 +190,8 @@ static GemmBatch plan_dot(const Meta& s, int axs, const Meta& t, int axt, int nm
 					b.add_term(off_ptr(s.offset[sk]), off_ptr(t.offset[tk]), (int)K, sa_i, sa_k, sb_k, sb_j, 0);
 				}
 			}
-			for (int i = nm - 1; i >= 0; i--) { if (++ic[i] < t.dim_blocks[shift_t + i]) break; ic[i] = 0; }
+			if (nm > 0) ic[nm - 1] = 0;
+			for (int i = nm - 2; i >= 0; i--) { if (++ic[i] < t.dim_blocks[shift_t + i]) break; ic[i] = 0; }
 		}
 	}
 	return b;
@@ -370,15 +392,20 @@ DevTensor dev_flatten_axes(const DevTensor& t, int i_ax, int new_dir, hipStream_
 		const std::vector<ct_long> f0 = fanout(m, i_ax, q0), f1 = fanout(m, i_ax + 1, q1);
 		ct_long stride = 1;
 		for (int i = i_ax + 2; i < nd; i++) stride *= bt[i];
+		// one segment per index of the axes before i_ax + 1: for fixed (i_0 .. i_{i_ax}) the entries
+		// of consecutive i_{i_ax+1} of this block are consecutive in the fused sector as well (fused
+		// logical indices with the same row and the same quantum number of axis i_ax + 1 are
+		// exactly those of this block), so the whole run is one contiguous copy
+		const ct_long run = bt[i_ax + 1] * stride;
 		ct_long nsl = 1;
-		for (int i = 0; i < i_ax + 2; i++) { nsl *= bt[i]; is[i] = 0; }
+		for (int i = 0; i < i_ax + 1; i++) { nsl *= bt[i]; is[i] = 0; }
 		for (ct_long j = 0; j < nsl; j++) {
 			for (int i = 0; i < i_ax; i++) isr[i] = is[i];
-			isr[i_ax] = fin[f0[is[i_ax]] * d1 + f1[is[i_ax + 1]]];
+			isr[i_ax] = fin[f0[is[i_ax]] * d1 + f1[0]];
 			ct_long l = 0;
 			for (int i = 0; i <= i_ax; i++) l = l * br[i] + isr[i];
-			segs.push_back(CopySeg{ m.offset[k] + j * stride, rm.offset[kr] + l * stride, stride });
-			for (int i = i_ax + 1; i >= 0; i--) { if (++is[i] < bt[i]) break; is[i] = 0; }
+			segs.push_back(CopySeg{ m.offset[k] + j * run, rm.offset[kr] + l * stride, run });
+			for (int i = i_ax; i >= 0; i--) { if (++is[i] < bt[i]) break; is[i] = 0; }
 		}
 	}
 	r.zero(st);
@@ -417,15 +444,18 @@ DevTensor dev_split_axis(const DevTensor& t, int i_ax, const ct_long* nd2, const
 		const std::vector<ct_long> f0 = fanout(rm, i_ax, q0), f1 = fanout(rm, i_ax + 1, q1);
 		ct_long stride = 1;
 		for (int i = i_ax + 2; i < nd + 1; i++) stride *= br[i];
+		// one segment per index of the axes before i_ax + 1 (the run over axis i_ax + 1 is
+		// contiguous in the fused sector, as in dev_flatten_axes)
+		const ct_long run = br[i_ax + 1] * stride;
 		ct_long nsl = 1;
-		for (int i = 0; i < i_ax + 2; i++) { nsl *= br[i]; is[i] = 0; }
+		for (int i = 0; i < i_ax + 1; i++) { nsl *= br[i]; is[i] = 0; }
 		for (ct_long j = 0; j < nsl; j++) {
 			for (int i = 0; i < i_ax; i++) ist[i] = is[i];
-			ist[i_ax] = fin[f0[is[i_ax]] * nd2[1] + f1[is[i_ax + 1]]];
+			ist[i_ax] = fin[f0[is[i_ax]] * nd2[1] + f1[0]];
 			ct_long l = 0;
 			for (int i = 0; i <= i_ax; i++) l = l * bt[i] + ist[i];
-			segs.push_back(CopySeg{ m.offset[kt] + l * stride, rm.offset[k] + j * stride, stride });
-			for (int i = i_ax + 1; i >= 0; i--) { if (++is[i] < br[i]) break; is[i] = 0; }
+			segs.push_back(CopySeg{ m.offset[kt] + l * stride, rm.offset[k] + j * run, run });
+			for (int i = i_ax; i >= 0; i--) { if (++is[i] < br[i]) break; is[i] = 0; }
 		}
 	}
 	r.zero(st);
@@ -492,15 +522,21 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 	auto t1_key = [&](ct_long A, ct_long P, ct_long N, ct_long C) { return ((A * nP + P) * nN + N) * nC + C; };
 	ct_long t1_elems = 0;
 	{
+		// the (N, C) of the present R blocks of every B, in (N, C) order
+		std::vector<std::vector<std::pair<ct_long, ct_long>>> r_nc((size_t)nB);
+		for (ct_long B = 0; B < nB; B++)
+			for (ct_long N = 0; N < nN; N++) for (ct_long C = 0; C < nC; C++) {
+				const ct_long irr[4] = { B, N, C, 0 };
+				if (R.present(R.flat_index(irr))) r_nc[(size_t)B].emplace_back(N, C);
+			}
 		std::vector<std::tuple<ct_long, ct_long, ct_long, ct_long, ct_long, ct_long>> blocks;  // A P N C Bidx off
 		for (ct_long A = 0; A < nA; A++) for (ct_long P = 0; P < nP; P++) {
 			const ct_long B = th.find_qnum(2, qb(th, 0, A) + qb(th, 1, P));
 			if (B < 0) continue;
 			const ct_long ib[3] = { A, P, B };
 			if (!th.present(th.flat_index(ib))) continue;
-			for (ct_long N = 0; N < nN; N++) for (ct_long C = 0; C < nC; C++) {
-				const ct_long irr[4] = { B, N, C, 0 };
-				if (!R.present(R.flat_index(irr))) continue;
+			for (const auto& nc : r_nc[(size_t)B]) {
+				const ct_long N = nc.first, C = nc.second;
 				t1_off[t1_key(A, P, N, C)] = t1_elems;
 				t1_elems += bc(th, 0, A) * bc(th, 1, P) * bc(R, 1, N) * bc(R, 2, C);
 				blocks.emplace_back(A, P, N, C, B, t1_off[t1_key(A, P, N, C)]);
@@ -523,6 +559,15 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 	// (M,Q) pairs by their quantum number sum
 	std::map<qnumber, std::vector<std::pair<ct_long, ct_long>>> mq_by_sum;
 	for (ct_long M = 0; M < nM; M++) for (ct_long Q = 0; Q < nQ; Q++) mq_by_sum[qb(W, 0, M) + qb(W, 1, Q)].emplace_back(M, Q);
+	// the (P, N) of the present W blocks of every (M, Q), in P order
+	std::vector<std::vector<std::pair<ct_long, ct_long>>> w_pn((size_t)(nM * nQ));
+	for (ct_long M = 0; M < nM; M++) for (ct_long Q = 0; Q < nQ; Q++)
+		for (ct_long P = 0; P < nP; P++) {
+			const ct_long N = W.find_qnum(3, qb(W, 0, M) + qb(W, 1, Q) - qb(W, 2, P));
+			if (N < 0) continue;
+			const ct_long iw[4] = { M, Q, P, N };
+			if (W.present(W.flat_index(iw))) w_pn[(size_t)(M * nQ + Q)].emplace_back(P, N);
+		}
 	std::vector<std::pair<ct_long, ct_long>> tl;  // (P, N)
 	for (ct_long A = 0; A < nA; A++) for (ct_long C = 0; C < nC; C++) {
 		auto grp = mq_by_sum.find(qb(R, 2, C) + qo - qb(th, 0, A));
@@ -530,14 +575,8 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 		for (const auto& mq : grp->second) {
 			const ct_long M = mq.first, Q = mq.second;
 			tl.clear();
-			for (ct_long P = 0; P < nP; P++) {
-				const ct_long N = W.find_qnum(3, qb(W, 0, M) + qb(W, 1, Q) - qb(W, 2, P));
-				if (N < 0) continue;
-				const ct_long iw[4] = { M, Q, P, N };
-				if (!W.present(W.flat_index(iw))) continue;
-				if (t1_off[t1_key(A, P, N, C)] < 0) continue;
-				tl.emplace_back(P, N);
-			}
+			for (const auto& pn : w_pn[(size_t)(M * nQ + Q)])
+				if (t1_off[t1_key(A, pn.first, pn.second, C)] >= 0) tl.push_back(pn);
 			if (tl.empty()) continue;
 			const ct_long dA = bc(th, 0, A), dM = bc(W, 0, M), dQ = bc(W, 1, Q), dC = bc(R, 2, C);
 			const ct_long off = t2_elems;
@@ -675,6 +714,11 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 		}
 	}
 	// stage 3: y(X,Q,C)[x q c] = sum_(A,M) L(O,A,M,X)^T T2(A,M,Q,C)
+	// the present T2 blocks (A, M) of every (Q, C), in (A, M) order; for a given X at most one M
+	// of an A matches (qM = qo - qA + qX)
+	std::vector<std::vector<std::pair<ct_long, ct_long>>> t2_am((size_t)(nQ * nC));
+	for (ct_long A = 0; A < nA; A++) for (ct_long M = 0; M < nM; M++) for (ct_long Q = 0; Q < nQ; Q++) for (ct_long C = 0; C < nC; C++)
+		if (t2_off[t2_key(A, M, Q, C)] >= 0) t2_am[(size_t)(Q * nC + C)].emplace_back(A, M);
 	ct_long iy[3];
 	for (ct_long k = 0; k < th.nblocks; k++) {
 		if (!th.present(k)) continue;
@@ -682,13 +726,12 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 		const ct_long X = iy[0], Q = iy[1], C = iy[2];
 		const ct_long dX = bc(th, 0, X), dQ = bc(th, 1, Q), dC = bc(th, 2, C);
 		s3.add_task(off_ptr_mut(x_off + th.offset[k]), (int)dX, (int)(dQ * dC), (int)(dQ * dC), (int)(dQ * dC), 0, false);
-		for (ct_long A = 0; A < nA; A++) {
-			const ct_long M = L.find_qnum(2, qo - qb(L, 1, A) + qb(L, 3, X));
-			if (M < 0) continue;
+		for (const auto& am : t2_am[(size_t)(Q * nC + C)]) {
+			const ct_long A = am.first, M = am.second;
+			if (qb(L, 2, M) != qo - qb(L, 1, A) + qb(L, 3, X)) continue;
 			const ct_long il[4] = { 0, A, M, X };
 			if (!L.present(L.flat_index(il))) continue;
 			const ct_long o2 = t2_off[t2_key(A, M, Q, C)];
-			if (o2 < 0) continue;
 			const ct_long dA = bc(L, 1, A), dM = bc(L, 2, M);
 			s3.add_term(off_ptr(l_off + L.offset[L.flat_index(il)]), off_ptr(t2_base + o2), (int)(dA * dM), 1, (int)dX, (int)(dQ * dC), 1, 0);
 		}

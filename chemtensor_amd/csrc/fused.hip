@@ -16,6 +16,7 @@
 // LDS offsets in the tables are per column and are multiplied by the unit's k at run time.
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -534,6 +535,9 @@ BatchPlan::~BatchPlan()
 
 BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tune, const ParallelFor* par)
 {
+	using clk = std::chrono::steady_clock;
+	const auto secs = [](clk::time_point a, clk::time_point z) { return std::chrono::duration<double>(z - a).count(); };
+	const clk::time_point c0 = clk::now();
 	std::unique_ptr<BatchPlan> b(new BatchPlan());
 	b->stream = s;
 	b->nprob = (int)in.size();
@@ -560,6 +564,7 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 		if (in[i].l.meta.nelem) launch_copy(in[i].l.d, b->l + probs[i].l_off, in[i].l.meta.nelem, s);
 		if (in[i].r.meta.nelem) launch_copy(in[i].r.d, b->r + probs[i].r_off, in[i].r.meta.nelem, s);
 	}
+	const clk::time_point c1 = clk::now();
 	// grouped three-stage tables over the concatenated operands; without tuning only the tables of
 	// the fixed forms (stage 2 as the column stream) unless the column stream does not apply
 	GroupedTables g;
@@ -582,6 +587,7 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 	};
 	plan_all(!tune);
 	if (!tune && g.wa.kmax > WA_KMAX) plan_all(false);
+	const clk::time_point c2 = clk::now();
 	b->t1 = (double*)dev_alloc((size_t)std::max<int64_t>(g.t1_elems, 1) * sizeof(double));
 	b->t2 = (double*)dev_alloc((size_t)std::max<int64_t>(g.t2_elems, 1) * sizeof(double));
 	if (!tune) b->flops = g.s1.flops + g.s2.flops + g.s3.flops;   // executed count (no replanning of the reference sequence)
@@ -601,6 +607,7 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 		for (int i = 0; i < 3; i++) b->stage[i].form = GEMM_WAVE_STREAM;
 		b->use_wapply = b->wapply.ok;
 	}
+	const clk::time_point c3 = clk::now();
 	// reduction blocks of the batched Lanczos: up to 64 blocks of >= 4096 elements per problem
 	std::vector<LzSeg> segs;
 	std::vector<LzProb> pr;
@@ -617,6 +624,11 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 	b->nblocks = (int)segs.size();
 	b->segs = upload_vec(segs, s);
 	b->probs = upload_vec(pr, s);
+	const clk::time_point c4 = clk::now();
+	b->host_s[0] = secs(c0, c1);
+	b->host_s[1] = secs(c1, c2);
+	b->host_s[2] = secs(c2, c3);
+	b->host_s[3] = secs(c3, c4);
 	// time both forms on the batch itself and keep the faster (without tuning, the caller's
 	// launches follow on the same stream: the staged uploads and copies need no wait)
 	if (tune) {

@@ -80,13 +80,31 @@ Meta make_meta(int ndim, const ct_long* dim, const int* axis_dir, const qnumber*
 		m.nelem = 1;
 		return m;
 	}
-	std::vector<ct_long> idx(ndim, 0);
-	for (ct_long k = 0; k < m.nblocks; k++) {
+	// the blocks in row-major order (bst.c:119-143): for every index of the leading ndim - 1 axes the
+	// quantum number of the last axis is fixed by the vanishing sum, so at most one block of the
+	// last axis exists; it is found by binary search instead of scanning the last axis
+	if (m.nblocks == 0) {
+		m.nelem = 0;
+		return m;
+	}
+	const int last = ndim - 1;
+	const std::vector<qnumber>& qlast = m.qnums_blocks[last];
+	std::vector<ct_long> idx((size_t)ndim, 0);
+	ct_long prefix = 0;
+	const ct_long nprefix = m.nblocks / m.dim_blocks[last];
+	for (ct_long p = 0; p < nprefix; p++, prefix++) {
 		qnumber qsum = 0;
 		ct_long sz = 1;
-		for (int i = 0; i < ndim; i++) { qsum += axis_dir[i] * m.qnums_blocks[i][idx[i]]; sz *= m.block_count[i][idx[i]]; }
-		if (qsum == 0) { m.offset[k] = off; off += sz; }
-		for (int i = ndim - 1; i >= 0; i--) { if (++idx[i] < m.dim_blocks[i]) break; idx[i] = 0; }
+		for (int i = 0; i < last; i++) { qsum += axis_dir[i] * m.qnums_blocks[i][idx[i]]; sz *= m.block_count[i][idx[i]]; }
+		// axis_dir[last] * q + qsum = 0 with axis_dir = +-1
+		const qnumber q = -axis_dir[last] * qsum;
+		const auto it = std::lower_bound(qlast.begin(), qlast.end(), q);
+		if (it != qlast.end() && *it == q) {
+			const ct_long j = (ct_long)(it - qlast.begin());
+			m.offset[prefix * m.dim_blocks[last] + j] = off;
+			off += sz * m.block_count[last][j];
+		}
+		for (int i = last - 1; i >= 0; i--) { if (++idx[i] < m.dim_blocks[i]) break; idx[i] = 0; }
 	}
 	m.nelem = off;
 	return m;

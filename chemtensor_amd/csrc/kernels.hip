@@ -605,21 +605,30 @@ __global__ __launch_bounds__(256) void k_permute_blocks(const double* __restrict
 		const PermBlock pb = blocks[b];
 		for (int64_t e = threadIdx.x; e < pb.n; e += blockDim.x) {
 			int64_t rem = e, so = 0;
-			for (int i = pb.ndim - 1; i >= 0; i--) {
-				const int64_t q = rem / pb.ddim[i];
-				so += (rem - q * pb.ddim[i]) * pb.sstride[i];
-				rem = q;
+			// axes from the last to the first; unrolled with constant indices, so the descriptor
+			// stays in registers (a loop bounded by ndim indexes it dynamically, in scratch memory)
+#pragma unroll
+			for (int t = 0; t < 6; t++) {
+				const int i = 5 - t;
+				if (i < pb.ndim) {
+					const int64_t q = rem / pb.ddim[i];
+					so += (rem - q * pb.ddim[i]) * pb.sstride[i];
+					rem = q;
+				}
 			}
 			dst[pb.dst + e] = src[pb.src + so];
 		}
 	}
 }
 
+// one wavefront per segment (segments are rows of blocks: mostly tens to hundreds of entries)
 __global__ __launch_bounds__(256) void k_copy_segments(const double* __restrict__ src, double* __restrict__ dst, const CopySeg* __restrict__ segs, int nsegs)
 {
-	for (int s = blockIdx.x; s < nsegs; s += gridDim.x) {
+	const int lane = threadIdx.x & 63;
+	const int nw = blockDim.x >> 6;
+	for (int s = blockIdx.x * nw + (threadIdx.x >> 6); s < nsegs; s += gridDim.x * nw) {
 		const CopySeg g = segs[s];
-		for (int64_t i = threadIdx.x; i < g.len; i += blockDim.x) dst[g.dst + i] = src[g.src + i];
+		for (int64_t i = lane; i < g.len; i += 64) dst[g.dst + i] = src[g.src + i];
 	}
 }
 
@@ -703,7 +712,8 @@ void launch_permute_blocks(const double* src, double* dst, const PermBlock* d_bl
 void launch_copy_segments(const double* src, double* dst, const CopySeg* d_segs, int nsegs, hipStream_t s)
 {
 	if (nsegs == 0) return;
-	hipLaunchKernelGGL(k_copy_segments, dim3(nsegs < 8192 ? nsegs : 8192), dim3(256), 0, s, src, dst, d_segs, nsegs);
+	const int nb = (nsegs + 3) / 4;
+	hipLaunchKernelGGL(k_copy_segments, dim3(nb < 8192 ? nb : 8192), dim3(256), 0, s, src, dst, d_segs, nsegs);
 	CTG_HIP(hipGetLastError());
 }
 

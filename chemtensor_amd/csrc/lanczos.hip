@@ -186,6 +186,7 @@ int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatv
 		if (!lw.graph || lw.graph_maxiter != maxiter) {
 			lw.drop_graph();
 			hipGraph_t g = nullptr;
+			CaptureScope cs;
 			CTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
 			try {
 				enqueue_recurrence(lw, n, s, matvec, lw.vs, maxiter);
@@ -408,6 +409,7 @@ int lanczos_ground_state_batch(BatchPlan& b, const double* vstart, int maxiter, 
 		if (!lw.graph || lw.graph_maxiter != maxiter) {
 			lw.drop_graph();
 			hipGraph_t g = nullptr;
+			CaptureScope cs;
 			CTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
 			try {
 				enqueue_batch_recurrence(b, lw.vs, maxiter);

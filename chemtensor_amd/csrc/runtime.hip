@@ -61,12 +61,29 @@ size_t round_size(size_t n)
 	return (n + b - 1) / b * b;
 }
 
+thread_local int tl_capture_depth = 0;
+
+size_t cache_cap()
+{
+	static const size_t cap = [] {
+		size_t fr = 0, tot = 0;
+		if (hipMemGetInfo(&fr, &tot) != hipSuccess || tot == 0) {
+			(void)hipGetLastError();
+			return (size_t)16 << 30;
+		}
+		return tot / 8;
+	}();
+	return cap;
+}
+
 }  // namespace
 
 void* dev_alloc(size_t nbytes)
 {
 	Pool& p = pool();
 	const size_t sz = round_size(nbytes);
+	const size_t cap = cache_cap();
+	bool over = false;
 	{
 		std::lock_guard<std::mutex> g(p.mu);
 		auto& fb = p.free_blocks[std::this_thread::get_id()];
@@ -79,7 +96,11 @@ void* dev_alloc(size_t nbytes)
 			fb.erase(it);
 			return ptr;
 		}
+		over = p.bytes_cached > cap;
 	}
+	// a long-running caller (the serving loop) sees ever new sizes: the cache is released once it
+	// holds more than an eighth of the device memory
+	if (over && tl_capture_depth == 0) dev_trim();
 	void* ptr = nullptr;
 	hipError_t e = hipMalloc(&ptr, sz);
 	if (e != hipSuccess) {
@@ -138,10 +159,23 @@ struct Staging
 	std::deque<StageBlock> busy;
 };
 
+// at most this many staging blocks (pinned buffer + event) per thread: a thread with more copies in
+// flight waits for its oldest one
+constexpr size_t STAGE_MAX_BLOCKS = 32;
+
+thread_local Staging* tl_staging = nullptr;
+
 Staging& staging()
 {
-	thread_local Staging* st = new Staging();
-	return *st;
+	if (!tl_staging) tl_staging = new Staging();
+	return *tl_staging;
+}
+
+size_t stage_cap(size_t nbytes)
+{
+	size_t cap = 64 * 1024;
+	while (cap < nbytes) cap *= 2;
+	return cap;
 }
 
 }  // namespace
@@ -162,19 +196,73 @@ void copy_h2d_async(void* dst, const void* src, size_t nbytes, hipStream_t s)
 		b = S.idle[best];
 		S.idle.erase(S.idle.begin() + best);
 	}
-	else {
-		size_t cap = 64 * 1024;
-		while (cap < nbytes) cap *= 2;
+	else if (S.idle.size() + S.busy.size() < STAGE_MAX_BLOCKS) {
+		const size_t cap = stage_cap(nbytes);
 		void* p = nullptr;
 		CTG_HIP(hipHostMalloc(&p, cap, hipHostMallocDefault));
 		b.p = (char*)p;
 		b.cap = cap;
 		CTG_HIP(hipEventCreateWithFlags(&b.ev, hipEventDisableTiming));
 	}
+	else {
+		// all blocks in use or too small: wait for the oldest copy if none is idle, then grow the
+		// largest idle block's buffer (its copy has completed)
+		if (S.idle.empty()) {
+			CTG_HIP(hipEventSynchronize(S.busy.front().ev));
+			S.idle.push_back(S.busy.front());
+			S.busy.pop_front();
+		}
+		int big = 0;
+		for (int i = 1; i < (int)S.idle.size(); i++) if (S.idle[i].cap > S.idle[big].cap) big = i;
+		b = S.idle[big];
+		S.idle.erase(S.idle.begin() + big);
+		if (b.cap < nbytes) {
+			CTG_HIP(hipHostFree(b.p));
+			const size_t cap = stage_cap(nbytes);
+			void* p = nullptr;
+			CTG_HIP(hipHostMalloc(&p, cap, hipHostMallocDefault));
+			b.p = (char*)p;
+			b.cap = cap;
+		}
+	}
 	memcpy(b.p, src, nbytes);
 	CTG_HIP(hipMemcpyAsync(dst, b.p, nbytes, hipMemcpyHostToDevice, s));
 	CTG_HIP(hipEventRecord(b.ev, s));
 	S.busy.push_back(b);
+}
+
+CaptureScope::CaptureScope() { tl_capture_depth++; }
+CaptureScope::~CaptureScope() { tl_capture_depth--; }
+
+void release_thread_resources()
+{
+	if (tl_staging) {
+		Staging& S = *tl_staging;
+		for (StageBlock& b : S.busy) {
+			(void)hipEventSynchronize(b.ev);
+			S.idle.push_back(b);
+		}
+		S.busy.clear();
+		for (StageBlock& b : S.idle) {
+			(void)hipHostFree(b.p);
+			(void)hipEventDestroy(b.ev);
+		}
+		delete tl_staging;
+		tl_staging = nullptr;
+	}
+	// device blocks cached for this thread (its stream has completed the work that used them)
+	std::multimap<size_t, void*> mine;
+	{
+		Pool& p = pool();
+		std::lock_guard<std::mutex> g(p.mu);
+		auto it = p.free_blocks.find(std::this_thread::get_id());
+		if (it != p.free_blocks.end()) {
+			mine.swap(it->second);
+			p.free_blocks.erase(it);
+			for (auto& b : mine) p.bytes_cached -= b.first;
+		}
+	}
+	for (auto& b : mine) (void)hipFree(b.second);
 }
 
 size_t dev_bytes_in_use()

@@ -80,6 +80,7 @@ struct HostPool
 					f(s);
 					if (s) (void)hipStreamSynchronize(s);
 				}
+				release_thread_resources();
 				if (s) (void)hipStreamDestroy(s);
 			});
 	}
@@ -148,6 +149,7 @@ struct Server
 	// seconds spent in each part of the ticks: start of requests, batch plans, batched Lanczos,
 	// device part of the splits before / after the host SVDs, host SVDs
 	double t_start = 0, t_plan = 0, t_lanczos = 0, t_split_dev = 0, t_split_host = 0, t_splits = 0;
+	double t_plan_part[4] = { 0, 0, 0, 0 };       // the batch plans' parts (BatchPlan::host_s)
 	hipStream_t st = nullptr;
 	std::chrono::steady_clock::time_point t0;
 	std::thread worker;
@@ -220,6 +222,7 @@ struct Server
 		std::unique_ptr<BatchPlan> b(build_batch_plan(in, st, false, &par));
 		double c1 = now();
 		t_plan += c1 - c0;
+		for (int i = 0; i < 4; i++) t_plan_part[i] += b->host_s[i];
 		const int64_t ntot = b->off.back();
 		double* v0 = (double*)dev_alloc((size_t)std::max<int64_t>(ntot, 1) * sizeof(double));
 		double* u = (double*)dev_alloc((size_t)std::max<int64_t>(ntot, 1) * sizeof(double));
@@ -362,7 +365,11 @@ int ct_serve_create(const double hbm_budget_bytes, const int max_active, ct_serv
 		CTG_HIP(hipStreamCreateWithFlags(&S.st, hipStreamNonBlocking));
 		S.hosts.start(S.host_threads);
 		S.planners.start(std::min(4, S.host_threads));
-		S.worker = std::thread([&S] { S.loop(); });
+		S.worker = std::thread([&S] {
+			S.loop();
+			(void)hipStreamSynchronize(S.st);
+			release_thread_resources();
+		});
 		*server = h.release();
 		return 0;
 	}
@@ -462,6 +469,13 @@ int ct_serve_destroy(ct_serve* server)
 	S.cv_work.notify_all();
 	S.cv_done.notify_all();
 	if (S.worker.joinable()) S.worker.join();
+	// CT_SERVE_PROFILE=1: the worker's time by part
+	const char* prof = getenv("CT_SERVE_PROFILE");
+	if (prof && prof[0] == '1')
+		fprintf(stderr, "chemtensor_gpu: serve worker: ticks %lld busy %.3f s: batch plans %.3f s (operands %.3f, tables %.3f, items+uploads %.3f, "
+			"reduction blocks %.3f), batched Lanczos %.3f s, split hand-off %.3f s; pool: starts %.3f s, splits device %.3f s host %.3f s\n",
+			(long long)S.ticks, S.busy_s, S.t_plan, S.t_plan_part[0], S.t_plan_part[1], S.t_plan_part[2], S.t_plan_part[3], S.t_lanczos,
+			S.t_splits, S.t_start, S.t_split_dev, S.t_split_host);
 	S.planners.shutdown();
 	S.hosts.shutdown();
 	for (auto& kv : S.tickets) kv.second->state.reset();

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -442,8 +443,10 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 	int64_t x_off, int64_t w_off, int64_t l_off, int64_t r_off, bool lean = false);
 // host check of the tables (bounds, each intermediate / output entry written once); 0 when valid
 int check_grouped_tables(const Meta& th, const Meta& W, const Meta& L, const Meta& R);
-// append src, planned on its own (intermediates from offset 0), behind the intermediates of dst
-void merge_grouped_tables(GroupedTables& dst, const GroupedTables& src);
+// append src, planned on its own (intermediates from offset 0), behind the intermediates of dst;
+// src's operand offsets (theta / y, W, L, R) move by x_off, w_off, l_off, r_off
+void merge_grouped_tables(GroupedTables& dst, const GroupedTables& src, int64_t x_off = 0, int64_t w_off = 0, int64_t l_off = 0,
+	int64_t r_off = 0);
 
 // contract checks of apply_local_hamiltonian (degrees, unit outer bonds, matching axes); throws HipError
 void validate_matvec_operands(const Meta& theta, const Meta& w, const Meta& l, const Meta& r);
@@ -513,7 +516,8 @@ struct BatchPlan
 	~BatchPlan();
 };
 
-struct BatchInput { Meta theta; DevTensor w, l, r; };
+// pre: the problem's lean grouped tables planned ahead with zero operand offsets (or null)
+struct BatchInput { Meta theta; DevTensor w, l, r; const GroupedTables* pre = nullptr; };
 // tune: time the grouped launch forms and the fused form and keep the fastest; otherwise the
 // grouped form with heuristic launch forms (no timing launches, for plans rebuilt often)
 // par(n, f): run f(0 .. n-1), possibly in parallel host threads (host planning only, no HIP calls)
@@ -561,6 +565,12 @@ struct DmrgState
 	bool trace = false;
 	double phase_s[PH_COUNT] = {};
 	double phase_t0 = 0;
+	// plan_ahead: after init and after every split the lean grouped tables of the next local
+	// eigenproblem are planned on the calling thread (zero operand offsets), for a batch plan that
+	// then only concatenates them (the serving loop)
+	bool plan_ahead = false;
+	std::unique_ptr<GroupedTables> ahead;
+	void plan_ahead_tables();
 	void phase_mark();
 	void phase_end(int ph);
 	void phase_report() const;

@@ -198,8 +198,26 @@ int DmrgState::init(const mpo* H, const mps* psi, int num_sweeps_, int maxiter_,
 	bond = (L > 2) ? 0 : L - 2;
 	done = false;
 	theta = merge_mps(A[bond], A[bond + 1], st);
+	plan_ahead_tables();
 	CTG_HIP(hipStreamSynchronize(st));
 	return 0;
+}
+
+void DmrgState::plan_ahead_tables()
+{
+	ahead.reset();
+	if (!plan_ahead || done) return;
+	const Meta& w = h2[bond].meta;
+	const Meta& l = lb[bond].meta;
+	const Meta& r = rb[bond + 1].meta;
+	validate_matvec_operands(theta.meta, w, l, r);
+	ahead.reset(new GroupedTables());
+	plan_grouped_tables(*ahead, theta.meta, w, l, r, 0, 0, 0, 0, true);
+	if (ahead->wa.kmax > WA_KMAX) {
+		// the column stream does not apply: the full tables (a batch plan then replans anyway)
+		ahead.reset(new GroupedTables());
+		plan_grouped_tables(*ahead, theta.meta, w, l, r, 0, 0, 0, 0, false);
+	}
 }
 
 // split of the optimised two-site tensor (mps_split_tensor_svd, mps.c:1119-1160) in three parts:
@@ -385,6 +403,7 @@ int DmrgState::advance_end(double en)
 	phase_end(PH_ENVIRONMENT);
 	if (!done) theta = merge_mps(A[bond], A[bond + 1], st);
 	else theta = DevTensor();
+	plan_ahead_tables();
 	CTG_HIP(hipStreamSynchronize(st));
 	phase_end(PH_MERGE);
 	return 0;

@@ -746,18 +746,21 @@ void plan_grouped_tables(GroupedTables& g, const Meta& th, const Meta& W, const 
 // Append src (planned with zero intermediate bases) to dst: its T1 / T2 offsets move behind dst's
 static const double* shift_ptr(const double* p, int64_t elems) { return (const double*)((intptr_t)p + (intptr_t)(elems * (int64_t)sizeof(double))); }
 
-void merge_grouped_tables(GroupedTables& dst, const GroupedTables& src)
+void merge_grouped_tables(GroupedTables& dst, const GroupedTables& src, int64_t x_off, int64_t w_off, int64_t l_off, int64_t r_off)
 {
 	const int64_t t1s = dst.t1_elems, t2s = dst.t2_elems;
-	auto append = [](GemmBatch& d, const GemmBatch& b, int64_t c_shift, int64_t b_shift) {
+	auto append = [](GemmBatch& d, const GemmBatch& b, int64_t c_shift, int64_t a_shift, int64_t b_shift) {
 		const int32_t q0 = (int32_t)d.terms.size();
+		d.tasks.reserve(d.tasks.size() + b.tasks.size());
+		d.terms.reserve(d.terms.size() + b.terms.size());
 		for (GemmTask t : b.tasks) { t.C = (double*)shift_ptr(t.C, c_shift); t.t0 += q0; t.t1 += q0; d.tasks.push_back(t); }
-		for (GemmTerm m : b.terms) { m.B = shift_ptr(m.B, b_shift); d.terms.push_back(m); }
+		for (GemmTerm m : b.terms) { m.A = shift_ptr(m.A, a_shift); m.B = shift_ptr(m.B, b_shift); d.terms.push_back(m); }
 		d.flops += b.flops;
 	};
-	append(dst.s1, src.s1, t1s, 0);
-	append(dst.s2, src.s2, t2s, t1s);
-	append(dst.s3, src.s3, 0, t2s);
+	// stage 1: T1 = theta R; stage 2: T2 = W T1; stage 3: y = L^T T2 (y at theta's offset)
+	append(dst.s1, src.s1, t1s, x_off, r_off);
+	append(dst.s2, src.s2, t2s, w_off, t1s);
+	append(dst.s3, src.s3, x_off, l_off, t2s);
 	WApplyHost& a = dst.wa;
 	const WApplyHost& b = src.wa;
 	const int64_t kb0 = (int64_t)a.kbase.size(), rb0 = (int64_t)a.rbase.size(), wr0 = (int64_t)(a.widx.size() / WA_KMAX);
@@ -766,15 +769,19 @@ void merge_grouped_tables(GroupedTables& dst, const GroupedTables& src)
 	for (WaUnit u : b.units) { u.kb += kb0; u.rb += rb0; u.wrow += wr0; u.wcol += wc0; a.units.push_back(u); }
 	for (WaItem it : b.items) { it.unit += u0; it.u.kb += kb0; it.u.rb += rb0; it.u.wrow += wr0; it.u.wcol += wc0; a.items.push_back(it); }
 	const int g0 = (int)a.groups.size();
-	a.groups.insert(a.groups.end(), b.groups.begin(), b.groups.end());
+	for (F12Group gr : b.groups) {
+		if (gr.th_off >= 0) gr.th_off += x_off;
+		if (gr.r_off >= 0) gr.r_off += r_off;
+		a.groups.push_back(gr);
+	}
 	for (F12Item it : b.items12) { it.g0 += g0; it.u.kb += kb0; it.u.rb += rb0; it.u.wrow += wr0; it.u.wcol += wc0; a.items12.push_back(it); }
 	a.f12_ok = a.f12_ok && b.f12_ok;
 	for (int64_t v : b.kbase) a.kbase.push_back(v >= 0 ? v + t1s : v);
 	for (int64_t v : b.rbase) a.rbase.push_back(v >= 0 ? v + t2s : v);
 	a.kstride.insert(a.kstride.end(), b.kstride.begin(), b.kstride.end());
 	a.rstride.insert(a.rstride.end(), b.rstride.begin(), b.rstride.end());
-	a.widx.insert(a.widx.end(), b.widx.begin(), b.widx.end());
-	a.widxT.insert(a.widxT.end(), b.widxT.begin(), b.widxT.end());
+	for (int64_t v : b.widx) a.widx.push_back(v >= 0 ? v + w_off : v);
+	for (int64_t v : b.widxT) a.widxT.push_back(v >= 0 ? v + w_off : v);
 	a.kmax = std::max(a.kmax, b.kmax);
 	dst.t1_elems += src.t1_elems;
 	dst.t2_elems += src.t2_elems;

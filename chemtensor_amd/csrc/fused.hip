@@ -585,8 +585,18 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 					probs[i].r_off, lean);
 		}
 	};
-	plan_all(!tune);
-	if (!tune && g.wa.kmax > WA_KMAX) plan_all(false);
+	// problems whose lean tables were planned ahead (zero operand offsets) are only concatenated
+	bool ahead = !tune;
+	for (const BatchInput& p : in) ahead = ahead && p.pre && p.pre->wa.kmax <= WA_KMAX;
+	if (ahead) {
+		g = GroupedTables();
+		for (size_t i = 0; i < in.size(); i++)
+			merge_grouped_tables(g, *in[i].pre, probs[i].x_off, probs[i].w_off, probs[i].l_off, probs[i].r_off);
+	}
+	else {
+		plan_all(!tune);
+		if (!tune && g.wa.kmax > WA_KMAX) plan_all(false);
+	}
 	const clk::time_point c2 = clk::now();
 	b->t1 = (double*)dev_alloc((size_t)std::max<int64_t>(g.t1_elems, 1) * sizeof(double));
 	b->t2 = (double*)dev_alloc((size_t)std::max<int64_t>(g.t2_elems, 1) * sizeof(double));

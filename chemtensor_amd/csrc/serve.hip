@@ -194,6 +194,7 @@ struct Server
 		construct_random_mps(CT_DOUBLE_REAL, r.nsites, H.d, H.qsite, r.sector, r.max_vdim, &rng, &psi);
 		t.state.reset(new DmrgState());
 		t.state->st = s;
+		t.state->plan_ahead = true;   // each split task plans the next batch tables of its request
 		int ret = CT_ERR_HIP;
 		try {
 			ret = t.state->init(&H, &psi, r.num_sweeps, r.maxiter_lanczos, r.tol_split, r.max_vdim, r.tol_energy);
@@ -216,6 +217,7 @@ struct Server
 			in[i].w = DevTensor::view(S.h2[S.bond], S.h2[S.bond].meta);
 			in[i].l = DevTensor::view(S.lb[S.bond], S.lb[S.bond].meta);
 			in[i].r = DevTensor::view(S.rb[S.bond + 1], S.rb[S.bond + 1].meta);
+			in[i].pre = S.ahead.get();
 		}
 		double c0 = now();
 		const ParallelFor par = [this](int n, const std::function<void(int)>& f) { planners.run(n, [&f](int i, hipStream_t) { f(i); }); };

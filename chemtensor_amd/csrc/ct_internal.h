@@ -67,8 +67,14 @@ std::vector<ct_long> fanout(const Meta& m, int ax, qnumber q);
 void* dev_alloc(size_t nbytes);
 // enqueue a host-to-device copy of host data on s; the host data may be freed on return
 void copy_h2d_async(void* dst, const void* src, size_t nbytes, hipStream_t s);
+// several host arrays into one device buffer with one transfer: part i lands at dst + dst_off
+struct H2DPart { const void* src; size_t nbytes; size_t dst_off; };
+void copy_h2d_parts(void* dst, const H2DPart* parts, int nparts, size_t total, hipStream_t s);
 void dev_free(void* p);
 void dev_trim();
+// after the calling thread has synchronised its stream: its cached blocks become available to
+// every thread
+void dev_cache_to_shared();
 // marks a stream capture on the calling thread: the allocator does not trim its cache (a device
 // synchronisation) meanwhile
 struct CaptureScope
@@ -228,6 +234,7 @@ enum GemmForm
 /// a GemmBatch resident on the device
 struct DevGemm
 {
+	char* buf = nullptr;   // one device allocation holding the three tables
 	GemmTask* tasks = nullptr;
 	GemmTerm* terms = nullptr;
 	WorkItem* items = nullptr;

@@ -71,12 +71,17 @@ void DevGemm::upload(const GemmBatch& b, hipStream_t s)
 	nterms = (int)b.terms.size();
 	nitems = (int)it.size();
 	flops = b.flops;
-	tasks = (GemmTask*)dev_alloc(std::max<size_t>(1, b.tasks.size()) * sizeof(GemmTask));
-	terms = (GemmTerm*)dev_alloc(std::max<size_t>(1, b.terms.size()) * sizeof(GemmTerm));
-	items = (WorkItem*)dev_alloc(std::max<size_t>(1, it.size()) * sizeof(WorkItem));
-	copy_h2d_async(tasks, b.tasks.data(), b.tasks.size() * sizeof(GemmTask), s);
-	copy_h2d_async(terms, b.terms.data(), b.terms.size() * sizeof(GemmTerm), s);
-	copy_h2d_async(items, it.data(), it.size() * sizeof(WorkItem), s);
+	// the three tables in one allocation and one transfer (256-byte aligned parts)
+	auto al = [](size_t n) { return (n + 255) / 256 * 256; };
+	const size_t n_tasks = b.tasks.size() * sizeof(GemmTask), n_terms = b.terms.size() * sizeof(GemmTerm), n_items = it.size() * sizeof(WorkItem);
+	const size_t o_terms = al(std::max<size_t>(n_tasks, 1)), o_items = o_terms + al(std::max<size_t>(n_terms, 1));
+	const size_t total = o_items + std::max<size_t>(n_items, 1);
+	buf = (char*)dev_alloc(total);
+	tasks = (GemmTask*)buf;
+	terms = (GemmTerm*)(buf + o_terms);
+	items = (WorkItem*)(buf + o_items);
+	const H2DPart parts[3] = { { b.tasks.data(), n_tasks, 0 }, { b.terms.data(), n_terms, o_terms }, { it.data(), n_items, o_items } };
+	copy_h2d_parts(buf, parts, 3, o_items + n_items, s);
 	const int cap = num_compute_units() * 8;
 	grid = std::max(1, std::min((nitems + 3) / 4, cap));
 	// workgroup-level split of the k-step stream only pays for long streams
@@ -93,7 +98,8 @@ void DevGemm::upload(const GemmBatch& b, hipStream_t s)
 
 void DevGemm::release()
 {
-	dev_free(tasks); dev_free(terms); dev_free(items);
+	dev_free(buf);
+	buf = nullptr;
 	tasks = nullptr; terms = nullptr; items = nullptr;
 	ntasks = nterms = nitems = 0;
 }

@@ -3,10 +3,13 @@
 // bond, which must not turn into hipMalloc/hipFree calls) and host <-> device transfers of
 // block-sparse tensors in their packed (serialize-order) layout.
 
+#include <algorithm>
+#include <atomic>
 #include <deque>
 #include <map>
 #include <mutex>
 #include <thread>
+#include <unordered_map>
 #include <cstring>
 
 #include "ct_internal.h"
@@ -34,22 +37,60 @@ hipStream_t default_stream()
 
 namespace {
 
-// Freed blocks are cached per host thread and handed out again only to the thread that freed
-// them. A thread's device work is ordered on the stream it uses, and callers that work on several
-// threads synchronise their stream at the end of each task, so a block never returns to use while
-// another stream may still read it.
-struct Pool
+// Freed blocks go to a cache of the freeing thread and are handed out again to that thread: its
+// device work is ordered on the one stream it uses, so a block never returns to use while an
+// earlier launch may still read it. A thread that has synchronised its stream (the serving pool
+// after each task, the worker after each batch) passes its cached blocks to a shared cache, from
+// which any thread may take them. The live map (block -> size) is sharded by address.
+struct LiveShard
 {
 	std::mutex mu;
-	std::map<std::thread::id, std::multimap<size_t, void*>> free_blocks;
-	std::map<void*, size_t> live;
-	size_t bytes_live = 0, bytes_cached = 0;
+	std::unordered_map<void*, size_t> m;
+};
+
+struct ThreadCache
+{
+	std::mutex mu;     // taken by the owner, and by dev_trim
+	std::multimap<size_t, void*> blocks;
+};
+
+// the shared cache: blocks by rounded size, sharded by size so that threads taking blocks of
+// different sizes do not wait for each other
+struct SharedShard
+{
+	std::mutex mu;
+	std::unordered_map<size_t, std::vector<void*>> by_size;
+};
+
+struct Pool
+{
+	static constexpr int NSHARD = 64;
+	LiveShard live[NSHARD];
+	SharedShard shared[NSHARD];
+	std::mutex mu;                         // the list of thread caches
+	std::vector<ThreadCache*> caches;
+	std::atomic<size_t> bytes_live{ 0 }, bytes_cached{ 0 };
+	LiveShard& shard(void* p) { return live[((uintptr_t)p >> 8) % NSHARD]; }
+	SharedShard& shared_shard(size_t sz) { return shared[(sz >> 9) % NSHARD]; }
 };
 
 Pool& pool()
 {
 	static Pool* p = new Pool();
 	return *p;
+}
+
+thread_local ThreadCache* tl_cache = nullptr;
+
+ThreadCache& thread_cache()
+{
+	if (!tl_cache) {
+		tl_cache = new ThreadCache();
+		Pool& p = pool();
+		std::lock_guard<std::mutex> g(p.mu);
+		p.caches.push_back(tl_cache);
+	}
+	return *tl_cache;
 }
 
 size_t round_size(size_t n)
@@ -59,6 +100,43 @@ size_t round_size(size_t n)
 	size_t b = 1;
 	while ((b << 3) < n) b <<= 1;
 	return (n + b - 1) / b * b;
+}
+
+// a shared block of rounded size sz .. 5/4 sz (sizes in the cache are rounded sizes), or null
+void* take_shared(Pool& p, size_t sz, size_t& got)
+{
+	for (size_t c = sz; c <= sz + sz / 4; c = round_size(c + 1)) {
+		SharedShard& sh = p.shared_shard(c);
+		std::lock_guard<std::mutex> g(sh.mu);
+		auto it = sh.by_size.find(c);
+		if (it != sh.by_size.end() && !it->second.empty()) {
+			void* ptr = it->second.back();
+			it->second.pop_back();
+			got = c;
+			return ptr;
+		}
+	}
+	return nullptr;
+}
+
+// a block of at least sz bytes and at most 5/4 sz from a cache, or null
+void* take_fit(std::multimap<size_t, void*>& m, size_t sz, size_t& got)
+{
+	auto it = m.lower_bound(sz);
+	if (it == m.end() || it->first > sz + sz / 4) return nullptr;
+	void* ptr = it->second;
+	got = it->first;
+	m.erase(it);
+	return ptr;
+}
+
+void register_live(void* ptr, size_t sz)
+{
+	Pool& p = pool();
+	LiveShard& sh = p.shard(ptr);
+	std::lock_guard<std::mutex> g(sh.mu);
+	sh.m[ptr] = sz;
+	p.bytes_live += sz;
 }
 
 thread_local int tl_capture_depth = 0;
@@ -82,36 +160,30 @@ void* dev_alloc(size_t nbytes)
 {
 	Pool& p = pool();
 	const size_t sz = round_size(nbytes);
-	const size_t cap = cache_cap();
-	bool over = false;
-	{
-		std::lock_guard<std::mutex> g(p.mu);
-		auto& fb = p.free_blocks[std::this_thread::get_id()];
-		auto it = fb.lower_bound(sz);
-		if (it != fb.end() && it->first <= sz + sz / 4) {
-			void* ptr = it->second;
-			p.live[ptr] = it->first;
-			p.bytes_live += it->first;
-			p.bytes_cached -= it->first;
-			fb.erase(it);
-			return ptr;
-		}
-		over = p.bytes_cached > cap;
-	}
-	// a long-running caller (the serving loop) sees ever new sizes: the cache is released once it
-	// holds more than an eighth of the device memory
-	if (over && tl_capture_depth == 0) dev_trim();
+	size_t got = 0;
 	void* ptr = nullptr;
+	{
+		ThreadCache& tc = thread_cache();
+		std::lock_guard<std::mutex> g(tc.mu);
+		ptr = take_fit(tc.blocks, sz, got);
+	}
+	if (!ptr) ptr = take_shared(p, sz, got);
+	if (ptr) {
+		p.bytes_cached -= got;
+		register_live(ptr, got);
+		return ptr;
+	}
+	// a long-running caller (the serving loop) sees ever new sizes: the caches are released once
+	// they hold more than an eighth of the device memory
+	if (p.bytes_cached > cache_cap() && tl_capture_depth == 0) dev_trim();
 	hipError_t e = hipMalloc(&ptr, sz);
 	if (e != hipSuccess) {
-		// release the cache and retry once
+		// release the caches and retry once
 		dev_trim();
 		(void)hipGetLastError();
 		CTG_HIP(hipMalloc(&ptr, sz));
 	}
-	std::lock_guard<std::mutex> g(p.mu);
-	p.live[ptr] = sz;
-	p.bytes_live += sz;
+	register_live(ptr, sz);
 	return ptr;
 }
 
@@ -119,27 +191,71 @@ void dev_free(void* ptr)
 {
 	if (!ptr) return;
 	Pool& p = pool();
-	std::lock_guard<std::mutex> g(p.mu);
-	auto it = p.live.find(ptr);
-	if (it == p.live.end()) return;
-	p.free_blocks[std::this_thread::get_id()].emplace(it->second, ptr);
-	p.bytes_cached += it->second;
-	p.bytes_live -= it->second;
-	p.live.erase(it);
+	size_t sz = 0;
+	{
+		LiveShard& sh = p.shard(ptr);
+		std::lock_guard<std::mutex> g(sh.mu);
+		auto it = sh.m.find(ptr);
+		if (it == sh.m.end()) return;
+		sz = it->second;
+		sh.m.erase(it);
+	}
+	p.bytes_live -= sz;
+	ThreadCache& tc = thread_cache();
+	std::lock_guard<std::mutex> g(tc.mu);
+	tc.blocks.emplace(sz, ptr);
+	p.bytes_cached += sz;
+}
+
+void dev_cache_to_shared()
+{
+	if (!tl_cache) return;
+	std::multimap<size_t, void*> mine;
+	{
+		std::lock_guard<std::mutex> g(tl_cache->mu);
+		mine.swap(tl_cache->blocks);
+	}
+	Pool& p = pool();
+	// equal sizes are adjacent in the multimap: one lock per run of one size
+	for (auto it = mine.begin(); it != mine.end();) {
+		const size_t sz = it->first;
+		SharedShard& sh = p.shared_shard(sz);
+		std::lock_guard<std::mutex> g(sh.mu);
+		std::vector<void*>& v = sh.by_size[sz];
+		for (; it != mine.end() && it->first == sz; ++it) v.push_back(it->second);
+	}
 }
 
 void dev_trim()
 {
 	Pool& p = pool();
-	std::lock_guard<std::mutex> g(p.mu);
-	bool any = false;
-	for (auto& kv : p.free_blocks) any = any || !kv.second.empty();
-	if (any) (void)hipDeviceSynchronize();
-	for (auto& kv : p.free_blocks) {
-		for (auto& b : kv.second) (void)hipFree(b.second);
-		kv.second.clear();
+	std::vector<void*> blocks;
+	{
+		// take every cached block out first: whatever work used them was enqueued before they
+		// were freed, so the synchronisation below covers it
+		size_t removed = 0;
+		std::lock_guard<std::mutex> g(p.mu);
+		for (SharedShard& sh : p.shared) {
+			std::lock_guard<std::mutex> g2(sh.mu);
+			for (auto& kv : sh.by_size) {
+				for (void* b : kv.second) { blocks.push_back(b); removed += kv.first; }
+				kv.second.clear();
+			}
+		}
+		p.bytes_cached -= removed;
+		removed = 0;
+		for (ThreadCache* tc : p.caches) {
+			std::lock_guard<std::mutex> g2(tc->mu);
+			for (auto& b : tc->blocks) { blocks.push_back(b.second); removed += b.first; }
+			tc->blocks.clear();
+			p.bytes_cached -= removed;
+			removed = 0;
+		}
+		p.bytes_cached -= removed;
 	}
-	p.bytes_cached = 0;
+	if (blocks.empty()) return;
+	(void)hipDeviceSynchronize();
+	for (void* b : blocks) (void)hipFree(b);
 }
 
 //________________________________________________________________________________________________________________________
@@ -180,10 +296,29 @@ size_t stage_cap(size_t nbytes)
 
 }  // namespace
 
+static StageBlock stage_block(Staging& S, size_t nbytes);
+
 void copy_h2d_async(void* dst, const void* src, size_t nbytes, hipStream_t s)
 {
-	if (nbytes == 0) return;
+	const H2DPart part = { src, nbytes, 0 };
+	copy_h2d_parts(dst, &part, 1, nbytes, s);
+}
+
+void copy_h2d_parts(void* dst, const H2DPart* parts, int nparts, size_t total, hipStream_t s)
+{
+	if (total == 0) return;
 	Staging& S = staging();
+	StageBlock b = stage_block(S, total);
+	for (int i = 0; i < nparts; i++)
+		if (parts[i].nbytes) memcpy(b.p + parts[i].dst_off, parts[i].src, parts[i].nbytes);
+	CTG_HIP(hipMemcpyAsync(dst, b.p, total, hipMemcpyHostToDevice, s));
+	CTG_HIP(hipEventRecord(b.ev, s));
+	S.busy.push_back(b);
+}
+
+// a staging block of at least nbytes whose previous copy has completed
+static StageBlock stage_block(Staging& S, size_t nbytes)
+{
 	while (!S.busy.empty() && hipEventQuery(S.busy.front().ev) == hipSuccess) {
 		S.idle.push_back(S.busy.front());
 		S.busy.pop_front();
@@ -225,10 +360,7 @@ void copy_h2d_async(void* dst, const void* src, size_t nbytes, hipStream_t s)
 			b.cap = cap;
 		}
 	}
-	memcpy(b.p, src, nbytes);
-	CTG_HIP(hipMemcpyAsync(dst, b.p, nbytes, hipMemcpyHostToDevice, s));
-	CTG_HIP(hipEventRecord(b.ev, s));
-	S.busy.push_back(b);
+	return b;
 }
 
 CaptureScope::CaptureScope() { tl_capture_depth++; }
@@ -250,26 +382,23 @@ void release_thread_resources()
 		delete tl_staging;
 		tl_staging = nullptr;
 	}
-	// device blocks cached for this thread (its stream has completed the work that used them)
-	std::multimap<size_t, void*> mine;
-	{
+	// device blocks cached for this thread go to the shared cache (its stream has completed the
+	// work that used them); the thread's cache is unregistered
+	if (tl_cache) {
+		dev_cache_to_shared();
 		Pool& p = pool();
-		std::lock_guard<std::mutex> g(p.mu);
-		auto it = p.free_blocks.find(std::this_thread::get_id());
-		if (it != p.free_blocks.end()) {
-			mine.swap(it->second);
-			p.free_blocks.erase(it);
-			for (auto& b : mine) p.bytes_cached -= b.first;
+		{
+			std::lock_guard<std::mutex> g(p.mu);
+			p.caches.erase(std::remove(p.caches.begin(), p.caches.end(), tl_cache), p.caches.end());
 		}
+		delete tl_cache;
+		tl_cache = nullptr;
 	}
-	for (auto& b : mine) (void)hipFree(b.second);
 }
 
 size_t dev_bytes_in_use()
 {
-	Pool& p = pool();
-	std::lock_guard<std::mutex> g(p.mu);
-	return p.bytes_live;
+	return pool().bytes_live;
 }
 
 //________________________________________________________________________________________________________________________

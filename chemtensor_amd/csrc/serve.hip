@@ -79,6 +79,7 @@ struct HostPool
 					}
 					f(s);
 					if (s) (void)hipStreamSynchronize(s);
+					dev_cache_to_shared();
 				}
 				release_thread_resources();
 				if (s) (void)hipStreamDestroy(s);
@@ -251,6 +252,7 @@ struct Server
 		dev_free(v0);
 		dev_free(u);
 		b.reset();
+		dev_cache_to_shared();   // the worker's stream has completed the batch
 		for (size_t i = 0; i < grp.size(); i++) {
 			std::shared_ptr<Ticket> t = grp[i];
 			if (status[i] != 0) { finish(*t, status[i]); continue; }

@@ -90,3 +90,40 @@ def test_serve_budget_limits_admission():
     assert r2.t_admit >= r1.t_done - 1e-6
     assert r1.energy == r2.energy
     srv.close()
+
+
+def test_serve_many_concurrent_requests():
+    # 24 requests in flight at once: their splits run on all pool threads, device blocks freed on
+    # one thread are reused by others (after that thread's stream is synchronised), batches hold
+    # many requests; every request must finish and sampled ones must equal standalone DMRG
+    rng = np.random.default_rng(11)
+    reqs, spec = [], []
+    for i in range(24):
+        if i % 2 == 0:
+            L, chi, delta = int(rng.integers(8, 17)) // 2 * 2, int(rng.integers(12, 41)), float(rng.uniform(0, 2))
+            reqs.append(req_xxz(L, chi, delta, 100 + i, num_sweeps=3, maxiter_lanczos=8, tol_split=1e-12, tol_energy=0.0))
+            spec.append(("xxz", L, chi, delta, 100 + i))
+        else:
+            L, chi, u = int(rng.integers(6, 13)) // 2 * 2, int(rng.integers(12, 41)), float(rng.uniform(0, 8))
+            reqs.append(req_hub(L, chi, u, 100 + i, num_sweeps=3, maxiter_lanczos=8, tol_split=1e-12, tol_energy=0.0))
+            spec.append(("hubbard", L, chi, u, 100 + i))
+    srv = ct.Server(50e9, 32)
+    tickets = [srv.submit(q) for q in reqs]
+    res = [srv.poll(t, wait=True) for t in tickets]
+    ens = [srv.energies(t) for t in tickets]
+    st = srv.stats()
+    srv.close()
+    say("many: statuses", [r.status for r in res], "max_batch", st["max_batch"])
+    assert all(r.status == 0 for r in res)
+    assert all(len(e) == 3 for e in ens)
+    for i in (0, 1, 6, 13):
+        model, L, chi, p, seed = spec[i]
+        if model == "xxz":
+            mpo = ct.MPO.xxz(L, 1.0, p, 0.0)
+            psi = ct.MPS.random(L, 2, mpo.qsite, 0, chi, seed)
+        else:
+            mpo = ct.MPO.hubbard(L, 1.0, p, 0.0)
+            psi = ct.MPS.random(L, 4, mpo.qsite, ct.encode_pair(L, 0), chi, seed)
+        en_ref, _ = ct.dmrg(mpo, psi, num_sweeps=3, maxiter_lanczos=8, tol_split=1e-12, max_vdim=chi)
+        say("many: request", i, spec[i], ens[i].tolist(), en_ref.tolist())
+        assert np.max(np.abs(ens[i] - en_ref)) < 1e-10 * max(1.0, np.max(np.abs(en_ref)))

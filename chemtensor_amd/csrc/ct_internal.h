@@ -154,6 +154,9 @@ DevTensor device_slice_2d(const DevTensor& src, const Meta& out, const std::vect
 // block SVD of a packed block-sparse matrix; iso < 0: one-sided Jacobi (u, vh isometric), iso 0 / 1:
 // host_svd_factor per block (vh = diag(s) Vh, resp. u = U diag(s))
 int host_block_svd(const HostPacked& a, HostPacked& u, std::vector<double>& s, HostPacked& vh, int iso = -1);
+// blocks of the calling thread's splits without helper threads (a caller whose threads already
+// run independent splits on every core)
+void host_svd_serial_on_this_thread(bool on);
 ct_long host_retained_bond_indices(const std::vector<double>& sigma, double tol, bool relative, ct_long max_vdim, std::vector<ct_long>& ind, trunc_info& info);
 HostPacked host_slice(const HostPacked& t, int i_ax, const std::vector<ct_long>& ind);
 void host_scale_axis(HostPacked& t, bool leading, const std::vector<double>& vec);

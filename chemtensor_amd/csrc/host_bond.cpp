@@ -34,6 +34,12 @@ static int host_svd_threads(int want)
 
 static void host_svd_threads_release(int n) { svd_threads_busy -= n; }
 
+// set by threads of a caller that already keeps every core busy with independent splits (the
+// serving pool): their blocks are factorised one after the other, without helper threads
+static thread_local bool tl_svd_serial = false;
+
+void host_svd_serial_on_this_thread(bool on) { tl_svd_serial = on; }
+
 }  // namespace ctg
 
 namespace ctg {
@@ -104,7 +110,7 @@ int host_block_svd(const HostPacked& a, HostPacked& u, std::vector<double>& s, H
 		const int r = (iso < 0) ? host_svd(j.r, j.c, j.a, j.ub, j.sb, j.vb) : host_svd_factor(j.r, j.c, j.a, iso, j.ub, j.sb, j.vb);
 		if (r != 0) ret = -1;
 	};
-	const int nt = (total > 4e6) ? host_svd_threads((int)jobs.size()) : 1;
+	const int nt = (total > 4e6 && !tl_svd_serial) ? host_svd_threads((int)jobs.size()) : 1;
 	if (nt <= 1) {
 		for (size_t i = 0; i < jobs.size(); i++) run(i);
 	}

@@ -68,6 +68,7 @@ struct HostPool
 			th.emplace_back([this] {
 				hipStream_t s = nullptr;
 				if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+				host_svd_serial_on_this_thread(true);   // the pool's threads already split in parallel
 				for (;;) {
 					std::function<void(hipStream_t)> f;
 					{

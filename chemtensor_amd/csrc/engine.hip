@@ -414,7 +414,10 @@ DevTensor dev_flatten_axes(const DevTensor& t, int i_ax, int new_dir, hipStream_
 			for (int i = i_ax; i >= 0; i--) { if (++is[i] < bt[i]) break; is[i] = 0; }
 		}
 	}
-	r.zero(st);
+	// the segments are disjoint; when they cover every entry no zero fill is needed
+	ct_long covered = 0;
+	for (const CopySeg& g : segs) covered += g.len;
+	if (covered != rm.nelem) r.zero(st);
 	run_segments(t.d, r.d, segs, st);
 	return r;
 }
@@ -464,7 +467,11 @@ DevTensor dev_split_axis(const DevTensor& t, int i_ax, const ct_long* nd2, const
 			for (int i = i_ax; i >= 0; i--) { if (++is[i] < br[i]) break; is[i] = 0; }
 		}
 	}
-	r.zero(st);
+	// result blocks without a source block stay zero: fill only when the segments (disjoint) do
+	// not cover every entry
+	ct_long covered = 0;
+	for (const CopySeg& g : segs) covered += g.len;
+	if (covered != rm.nelem) r.zero(st);
 	run_segments(t.d, r.d, segs, st);
 	return r;
 }

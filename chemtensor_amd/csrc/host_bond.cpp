@@ -181,6 +181,15 @@ HostPacked host_slice(const HostPacked& t, int i_ax, const std::vector<ct_long>&
 	HostPacked r;
 	r.meta = make_meta(nd, dl.data(), m.axis_dir.data(), qn.data());
 	r.v.assign((size_t)r.meta.nelem, 0.0);
+	// once per axis: the position of every source index within its block, and the (new) logical
+	// indices of every block of the result axis in order
+	std::vector<ct_long> fin((size_t)m.dim_logical[i_ax]);
+	{
+		std::vector<ct_long> cnt((size_t)m.dim_blocks[i_ax], 0);
+		for (ct_long j = 0; j < m.dim_logical[i_ax]; j++) fin[(size_t)j] = cnt[(size_t)m.find_qnum(i_ax, m.qnums_logical[i_ax][j])]++;
+	}
+	std::vector<std::vector<ct_long>> fo((size_t)r.meta.dim_blocks[i_ax]);
+	for (ct_long j = 0; j < dl[i_ax]; j++) fo[(size_t)r.meta.find_qnum(i_ax, qs[(size_t)j])].push_back(j);
 	ct_long ir[16], it[16], br[16], bt[16];
 	for (ct_long k = 0; k < r.meta.nblocks; k++) {
 		if (!r.meta.present(k)) continue;
@@ -191,18 +200,18 @@ HostPacked host_slice(const HostPacked& t, int i_ax, const std::vector<ct_long>&
 		it[i_ax] = m.find_qnum(i_ax, q);
 		const ct_long kt = m.flat_index(it);
 		m.block_dims(kt, bt);
-		const std::vector<ct_long> fo = fanout(r.meta, i_ax, q);
-		std::vector<ct_long> fin((size_t)m.dim_logical[i_ax], 0);
-		ct_long c = 0;
-		for (ct_long j = 0; j < m.dim_logical[i_ax]; j++) if (m.qnums_logical[i_ax][j] == q) fin[j] = c++;
+		const std::vector<ct_long>& fob = fo[(size_t)ir[i_ax]];
 		ct_long outer = 1, inner = 1;
 		for (int i = 0; i < i_ax; i++) outer *= bt[i];
 		for (int i = i_ax + 1; i < nd; i++) inner *= bt[i];
 		const double* src = t.v.data() + m.offset[kt];
 		double* dst = r.v.data() + r.meta.offset[k];
 		for (ct_long o = 0; o < outer; o++) for (ct_long j = 0; j < br[i_ax]; j++) {
-			const ct_long js = fin[ind[fo[j]]];
-			std::memcpy(dst + (o * br[i_ax] + j) * inner, src + (o * bt[i_ax] + js) * inner, (size_t)inner * sizeof(double));
+			const ct_long js = fin[(size_t)ind[(size_t)fob[(size_t)j]]];
+			const double* sp = src + (o * bt[i_ax] + js) * inner;
+			double* dp = dst + (o * br[i_ax] + j) * inner;
+			if (inner == 1) *dp = *sp;
+			else std::memcpy(dp, sp, (size_t)inner * sizeof(double));
 		}
 	}
 	return r;

@@ -33,6 +33,18 @@ static double nrm2s(ct_long n, const double* x, ct_long inc)
 	return scale * std::sqrt(ssq);
 }
 
+// sqrt(a^2 + b^2) without overflow or underflow: the larger magnitude scaled out (about one ulp;
+// std::hypot rounds correctly but costs several times as much, once per QL rotation)
+static inline double hypot_fast(double a, double b)
+{
+	a = std::fabs(a);
+	b = std::fabs(b);
+	const double m = std::max(a, b), n = std::min(a, b);
+	if (m == 0) return 0;
+	const double r = n / m;
+	return m * std::sqrt(1.0 + r * r);
+}
+
 // Implicit-shift QL iteration on the symmetric tridiagonal (d, offd) (the algorithm of EISPACK
 // tql2), applying the rotations to the columns of z (n x n, column-major) as it goes: with z = I on
 // entry the columns become the eigenvectors of the tridiagonal, with z = Q those of Q T Q^T.
@@ -51,14 +63,14 @@ static int ql_vectors(int n, double* d, const double* offd, double* z, double an
 			if (m == l) break;
 			if (iter >= 60) return -1;
 			double g = (d[l + 1] - d[l]) / (2 * e[l]);
-			double r = std::hypot(g, 1.0);
+			double r = hypot_fast(g, 1.0);
 			g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
 			double s = 1, c = 1, p = 0;
 			bool deflated = false;
 			for (int i = m - 1; i >= l; i--) {
 				double f = s * e[i];
 				const double b = c * e[i];
-				r = std::hypot(f, g);
+				r = hypot_fast(f, g);
 				e[i + 1] = r;
 				if (r == 0) { d[i + 1] -= p; e[m] = 0; deflated = true; break; }
 				s = f / r;

@@ -433,13 +433,12 @@ static void append_fused(FusedHost& a, const FusedHost& b, int64_t x_off, int64_
 // largest dynamic LDS allocation of one workgroup on the current device (160 KB on MI355X)
 static size_t lds_limit()
 {
-	static size_t lim = 0;
-	if (lim == 0) {
+	static const size_t lim = [] {
 		int dev = 0, v = 0;
 		CTG_HIP(hipGetDevice(&dev));
 		CTG_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-		lim = std::min<size_t>(FZ_LDS_MAX, (size_t)v);
-	}
+		return std::min<size_t>(FZ_LDS_MAX, (size_t)v);
+	}();
 	return lim;
 }
 

@@ -648,14 +648,14 @@ __global__ __launch_bounds__(256) void k_fill(double* __restrict__ p, double v, 
 
 int num_compute_units()
 {
-	static int cus = 0;
-	if (cus == 0) {
+	// initialised once (thread-safe): many host threads plan launches concurrently
+	static const int cus = [] {
 		int dev = 0;
 		CTG_HIP(hipGetDevice(&dev));
 		hipDeviceProp_t prop;
 		CTG_HIP(hipGetDeviceProperties(&prop, dev));
-		cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-	}
+		return prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+	}();
 	return cus;
 }
 

@@ -909,7 +909,9 @@ static void build_grouped(MatvecPlan& p, bool lean)
 	p.stage_bases[0] = GemmBases{ nullptr, p.r.d, p.interm[0].d };
 	p.stage_bases[1] = GemmBases{ p.w.d, p.interm[0].d, p.interm[1].d };
 	p.stage_bases[2] = GemmBases{ p.l.d, p.interm[1].d, nullptr };
-	if (p.mode == CT_MATVEC_GROUPED_FP64) p.wapply.upload(g.wa, p.w.d, p.stream);
+	// the column-stream stage 2 (FP64 FMAs, no MFMA) serves the FP64 and the split-FP32 forms: in
+	// both the MFMA GEMMs are stages 1 and 3
+	if (p.mode == CT_MATVEC_GROUPED_FP64 || p.mode == CT_MATVEC_GROUPED_SPLIT) p.wapply.upload(g.wa, p.w.d, p.stream);
 	p.nterms = g.s1.terms.size() + g.s2.terms.size() + g.s3.terms.size();
 	p.ntasks = g.s1.tasks.size() + g.s2.tasks.size() + g.s3.tasks.size();
 	p.nitems = p.stage[0].nitems + p.stage[1].nitems + p.stage[2].nitems;
@@ -1141,6 +1143,12 @@ MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& 
 		if (mode == CT_MATVEC_GROUPED_FP64 && autotune) autotune_stages(*p);
 		// without timing, stage 2 takes the column stream (faster at every size measured)
 		if (mode == CT_MATVEC_GROUPED_FP64 && !autotune) p->use_wapply = p->wapply.ok;
+		// split-FP32: stages 1 and 3 on FP32 MFMA with sliced operands, stage 2 as the FP64 column
+		// stream when it applies (CT_STAGE2=mfma keeps the sliced MFMA stage 2)
+		if (mode == CT_MATVEC_GROUPED_SPLIT) {
+			const char* f2 = getenv("CT_STAGE2");
+			p->use_wapply = p->wapply.ok && !(f2 && strcmp(f2, "mfma") == 0);
+		}
 		for (int i = 0; i < 4; i++) CTG_HIP(hipEventCreate(&p->ev[i]));
 	}
 	catch (...) {
@@ -1179,7 +1187,10 @@ float matvec_stage_time(MatvecPlan& p, int stage, const double* x, double* y, in
 		GemmBases b = p.stage_bases[stage];
 		if (stage == 0) b.A = x;
 		if (stage == 2) b.C = y;
-		if (p.mode == CT_MATVEC_GROUPED_SPLIT) launch_grouped_gemm_split(p.stage[stage], b, s);
+		if (p.mode == CT_MATVEC_GROUPED_SPLIT) {
+			if (stage == 1 && p.use_wapply) launch_wapply(p.wapply, b.B, b.C, s);
+			else launch_grouped_gemm_split(p.stage[stage], b, s);
+		}
 		else if (stage < 2 && p.use_f12 && p.wapply.f12_ok) { if (stage == 0) launch_f12(p.wapply, x, b.B, p.stage_bases[1].C, s); }
 		else if (stage == 1 && p.use_wapply && p.wapply.ok) launch_wapply(p.wapply, b.B, b.C, s);
 		else launch_grouped_gemm_f64(p.stage[stage], b, s);
@@ -1214,7 +1225,8 @@ void matvec_apply(MatvecPlan& p, const double* x, double* y, float* stage_ms)
 			GemmBases b = p.stage_bases[st];
 			if (st == 0) b.A = x;
 			if (st == 2) b.C = y;
-			launch_grouped_gemm_split(p.stage[st], b, s);
+			if (st == 1 && p.use_wapply) launch_wapply(p.wapply, b.B, b.C, s);
+			else launch_grouped_gemm_split(p.stage[st], b, s);
 			if (stage_ms) CTG_HIP(hipEventRecord(p.ev[st + 1], s));
 		}
 	}

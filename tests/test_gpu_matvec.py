@@ -137,6 +137,24 @@ def test_grouped_stage2_forms_match_oracle(stage2, monkeypatch):
         assert err < TOL
 
 
+@pytest.mark.parametrize("stage2", ["mfma", "stream"])
+def test_split_form_stage2_options_match_oracle(stage2, monkeypatch):
+    """The split-FP32 form with stage 2 as sliced FP32 MFMA tiles and as the FP64 column stream
+    (its default), against the oracle."""
+    monkeypatch.setenv("CT_STAGE2", stage2)
+    for model, L, chi in [("hubbard", 32, 512), ("xxz", 32, 256), ("hubbard", 8, 24)]:
+        theta, h2, left, right, _ = orc.matvec_problem(model, L, chi, 4243)
+        want = orc.apply_local_hamiltonian(theta, h2, left, right).serialize()
+        plan = ct.MatvecPlan(theta, h2, left, right, ct.GROUPED_SPLIT)
+        x = ct.DeviceArray.from_numpy(theta.serialize())
+        y = ct.DeviceArray(plan.dim)
+        plan.apply(x, y)
+        plan.synchronize()
+        err = rel(y.numpy(), want)
+        say("  split stage2", stage2, model, L, chi, "rel err", err)
+        assert err < TOL
+
+
 def test_apply_local_hamiltonian_dropin_with_reference_structs():
     """The oracle's structs (reference layout) go straight into the product's drop-in entry point."""
     theta, h2, left, right, _ = orc.matvec_problem("hubbard", 16, 48, 77)

@@ -483,14 +483,17 @@ __device__ __forceinline__ void split_slices(double x, int e, float* sl)
 	}
 }
 
-__device__ __forceinline__ void wave_tile_split(const GemmTask& tk, const GemmTerm* __restrict__ terms, int m0, int n0, int lane, GemmBases bsp)
+// The k-steps of the task's terms with (step index mod nparts) == part, summed into acc (the four
+// 16 x 16 sub-tiles, register r of lane l holding row 4 (l / 16) + r and column l % 16).
+__device__ __forceinline__ void wave_tile_split_part(const GemmTask& tk, const GemmTerm* __restrict__ terms, int m0, int n0, int lane, GemmBases bsp,
+	int part, int nparts, double (&acc)[4][4])
 {
 	const int i16 = lane & 15, kq = lane >> 4;
-	double acc[4][4];
 	#pragma unroll
 	for (int s = 0; s < 4; s++)
 		#pragma unroll
 		for (int r = 0; r < 4; r++) acc[s][r] = 0;
+	int step = 0;
 	const int rr[2] = { m0 + i16, m0 + 16 + i16 };
 	const int cc[2] = { n0 + i16, n0 + 16 + i16 };
 	for (int t = tk.t0; t < tk.t1; t++) {
@@ -501,6 +504,7 @@ __device__ __forceinline__ void wave_tile_split(const GemmTask& tk, const GemmTe
 		#pragma unroll
 		for (int h = 0; h < 2; h++) bo[h] = b_col_offset(cc[h] < tk.N ? cc[h] : 0, tk.nci, tm.sb_batch, tm.sb_j);
 		for (int k0 = 0; k0 < tm.K; k0 += 4) {
+			if ((step++ % nparts) != part) continue;
 			const int k = k0 + kq;
 			const bool vk = k < tm.K;
 			float as[2][SPLIT_TERMS], bs[2][SPLIT_TERMS];
@@ -544,24 +548,28 @@ __device__ __forceinline__ void wave_tile_split(const GemmTask& tk, const GemmTe
 			}
 		}
 	}
+}
+
+// store sub-tile s of a split tile (register r of lane l: row 4 (l / 16) + r, column l % 16)
+__device__ __forceinline__ void store_split_sub(const GemmTask& tk, int m0, int n0, GemmBases bsp, int s, const double (&v)[4], int lane)
+{
+	const int i16 = lane & 15, kq = lane >> 4;
+	const int mi = s >> 1, ni = s & 1;
+	const int col = n0 + 16 * ni + i16;
+	if (col >= tk.N) return;
+	const int jb = col / tk.nci, jc = col - jb * tk.nci;
+	double* __restrict__ C = (double*)rebase(bsp.C, tk.C) + (int64_t)jb * tk.sc_batch + jc;
 	#pragma unroll
-	for (int s = 0; s < 4; s++) {
-		const int mi = s >> 1, ni = s & 1;
-		const int col = n0 + 16 * ni + i16;
-		if (col >= tk.N) continue;
-		const int jb = col / tk.nci, jc = col - jb * tk.nci;
-		double* __restrict__ C = (double*)rebase(bsp.C, tk.C) + (int64_t)jb * tk.sc_batch + jc;
-		#pragma unroll
-		for (int r = 0; r < 4; r++) {
-			const int row = m0 + 16 * mi + 4 * kq + r;
-			if (row < tk.M) {
-				if (tk.accumulate) C[(int64_t)row * tk.sc_i] += acc[s][r];
-				else C[(int64_t)row * tk.sc_i] = acc[s][r];
-			}
+	for (int r = 0; r < 4; r++) {
+		const int row = m0 + 16 * mi + 4 * kq + r;
+		if (row < tk.M) {
+			if (tk.accumulate) C[(int64_t)row * tk.sc_i] += v[r];
+			else C[(int64_t)row * tk.sc_i] = v[r];
 		}
 	}
 }
 
+// one wavefront per work item
 __global__ __launch_bounds__(256) void k_grouped_gemm_split(const GemmTask* __restrict__ tasks, const GemmTerm* __restrict__ terms,
 	const WorkItem* __restrict__ items, int nitems, GemmBases bs)
 {
@@ -570,7 +578,36 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_split(const GemmTask* __re
 	for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < nitems; w += nwaves) {
 		const WorkItem it = items[w];
 		const GemmTask& tk = it.task;
-		wave_tile_split(tk, terms, it.m0, it.n0, lane, bs);
+		double acc[4][4];
+		wave_tile_split_part(tk, terms, it.m0, it.n0, lane, bs, 0, 1, acc);
+		#pragma unroll
+		for (int s = 0; s < 4; s++) store_split_sub(tk, it.m0, it.n0, bs, s, acc[s], lane);
+	}
+}
+
+// one workgroup per work item (long k-step chains): wavefront w takes the k-steps w, w + 4, ...;
+// the four partial tiles are summed through LDS in wavefront order (deterministic), wavefront s
+// storing sub-tile s
+__global__ __launch_bounds__(256) void k_grouped_gemm_split_wg(const GemmTask* __restrict__ tasks, const GemmTerm* __restrict__ terms,
+	const WorkItem* __restrict__ items, int nitems, GemmBases bs)
+{
+	__shared__ double red[4][4][4][64];   // [wavefront][sub-tile][register][lane], 32 KB
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+		const WorkItem it = items[w];
+		const GemmTask& tk = it.task;
+		double acc[4][4];
+		wave_tile_split_part(tk, terms, it.m0, it.n0, lane, bs, wave, 4, acc);
+		#pragma unroll
+		for (int s = 0; s < 4; s++)
+			#pragma unroll
+			for (int r = 0; r < 4; r++) red[wave][s][r][lane] = acc[s][r];
+		__syncthreads();
+		double v[4];
+		#pragma unroll
+		for (int r = 0; r < 4; r++) v[r] = ((red[0][wave][r][lane] + red[1][wave][r][lane]) + red[2][wave][r][lane]) + red[3][wave][r][lane];
+		store_split_sub(tk, it.m0, it.n0, bs, wave, v, lane);
+		__syncthreads();
 	}
 }
 
@@ -675,7 +712,12 @@ void launch_grouped_gemm_f64(const DevGemm& g, GemmBases b, hipStream_t s)
 void launch_grouped_gemm_split(const DevGemm& g, GemmBases b, hipStream_t s)
 {
 	if (g.nitems == 0) return;
-	hipLaunchKernelGGL(k_grouped_gemm_split, dim3(g.grid), dim3(256), 0, s, g.tasks, g.terms, g.items, g.nitems, b);
+	// long k-step chains (the heuristic of the FP64 forms) get a workgroup per tile
+	if (g.avg_steps > 8.0) {
+		const int wg_grid = std::max(1, std::min(g.nitems, num_compute_units() * 8));
+		hipLaunchKernelGGL(k_grouped_gemm_split_wg, dim3(wg_grid), dim3(256), 0, s, g.tasks, g.terms, g.items, g.nitems, b);
+	}
+	else hipLaunchKernelGGL(k_grouped_gemm_split, dim3(g.grid), dim3(256), 0, s, g.tasks, g.terms, g.items, g.nitems, b);
 	CTG_HIP(hipGetLastError());
 }
 

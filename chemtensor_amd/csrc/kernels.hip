@@ -463,6 +463,12 @@ __global__ __launch_bounds__(256) void k_plain_gemm_f64(GemmTask tk, GemmTerm tm
 constexpr int SPLIT_BITS = 10;
 constexpr int SPLIT_TERMS = 5;
 
+// 2^(-SPLIT_BITS (L + 2)) for the levels L = 0 .. 4
+__device__ __forceinline__ constexpr double split_level_scale(int L)
+{
+	return L == 0 ? 0x1p-20 : L == 1 ? 0x1p-30 : L == 2 ? 0x1p-40 : L == 3 ? 0x1p-50 : 0x1p-60;
+}
+
 __device__ __forceinline__ int group_exponent(double x)
 {
 	double m = fabs(x);
@@ -518,6 +524,12 @@ __device__ __forceinline__ void wave_tile_split_part(const GemmTask& tk, const G
 				split_slices(av, ea[h], as[h]);
 				split_slices(bv, eb[h], bs[h]);
 			}
+			// the row scales of this k-step's output rows (register r holds row 4 (l / 16) + r)
+			int erow[2][4];
+			#pragma unroll
+			for (int mi = 0; mi < 2; mi++)
+				#pragma unroll
+				for (int r = 0; r < 4; r++) erow[mi][r] = __shfl(ea[mi], 4 * kq + r);
 			#pragma unroll
 			for (int s = 0; s < 4; s++) {
 				const int mi = s >> 1, ni = s & 1;
@@ -536,14 +548,15 @@ __device__ __forceinline__ void wave_tile_split_part(const GemmTask& tk, const G
 				}
 				#pragma unroll
 				for (int r = 0; r < 4; r++) {
+					// level L's sum scaled by 2^(-SPLIT_BITS (L + 2)): an exact power of two, so one
+					// FMA per level rounds exactly as scaling and adding would
 					double sum = 0;
 					#pragma unroll
 					for (int q = 5; q >= 0; q--) {
 						const int L = (q == 5 ? 4 : q);
-						sum += ldexp((double)lv[q][r], -SPLIT_BITS * (L + 2));
+						sum = fma((double)lv[q][r], split_level_scale(L), sum);
 					}
-					const int erow = __shfl(ea[mi], 4 * kq + r);
-					acc[s][r] += ldexp(sum, erow + eb[ni]);
+					acc[s][r] += ldexp(sum, erow[mi][r] + eb[ni]);
 				}
 			}
 		}

@@ -1,7 +1,7 @@
 // h5_extract: dump the root-level datasets and attributes of an HDF5 fixture file into raw
 // little-endian binary files plus a manifest, so that make_fixtures.py can turn them into
-// committed .npz fixtures. Built and run only in the development container, where the
-// HDF5 C library is available (conda); nothing on the GPU box uses it.
+// committed .npz fixtures. Built and run only where the HDF5 C library is available (conda);
+// the tests read the .npz fixtures and never need it.
 //
 // usage: h5_extract <file.hdf5> <outdir>
 // manifest lines: <kind:D|A> <name> <type:f8|i8|c16> <ndim> <dim0> <dim1> ...

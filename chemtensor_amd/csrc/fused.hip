@@ -612,8 +612,12 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 		b->fused_ok = build_fused_batch(b->fused, probs, s);
 	}
 	else {
-		// one fixed launch form: the summation order of a problem does not depend on the batch
-		for (int i = 0; i < 3; i++) b->stage[i].form = GEMM_WAVE_STREAM;
+		// fixed launch forms per stage (the summation order of a problem does not depend on the
+		// batch): stages 1 and 2 one wavefront per tile, stage 3 (long k-chains, K = dA dM) one
+		// workgroup per tile
+		b->stage[0].form = GEMM_WAVE_STREAM;
+		b->stage[1].form = GEMM_WAVE_STREAM;
+		b->stage[2].form = GEMM_WG_STREAM;
 		b->use_wapply = b->wapply.ok;
 	}
 	const clk::time_point c3 = clk::now();

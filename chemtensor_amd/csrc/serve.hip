@@ -9,9 +9,10 @@
 // request rejoins the ready set when its split is done, so slow splits of large requests do not
 // hold back small ones, and a request that arrives while others run is admitted (in arrival
 // order, while the HBM budget and the slots allow) and starts on the pool at once. Requests that
-// finish free their budget. The grouped launches use one fixed launch form per stage and every output tile is
-// summed by one wavefront in a fixed order, and each request's Lanczos reductions cover only its
-// own segment, so a request's results do not depend on the other requests in the batch.
+// finish free their budget. The grouped launches use one fixed launch form per stage, every output
+// tile is summed in a fixed order (by one wavefront, or by four with a fixed LDS reduction), and
+// each request's Lanczos reductions cover only its own segment, so a request's results do not
+// depend on the other requests in the batch.
 
 #include <algorithm>
 #include <chrono>

@@ -70,8 +70,11 @@ struct Pool
 	std::mutex mu;                         // the list of thread caches
 	std::vector<ThreadCache*> caches;
 	std::atomic<size_t> bytes_live{ 0 }, bytes_cached{ 0 };
-	LiveShard& shard(void* p) { return live[((uintptr_t)p >> 8) % NSHARD]; }
-	SharedShard& shared_shard(size_t sz) { return shared[(sz >> 9) % NSHARD]; }
+	// Fibonacci hashing: block addresses and rounded sizes are multiples of large powers of two,
+	// so their low bits alone would send most of them to one shard
+	static int mix(uint64_t x) { return (int)((x * 0x9E3779B97F4A7C15ull) >> 58); }
+	LiveShard& shard(void* p) { return live[mix((uint64_t)(uintptr_t)p)]; }
+	SharedShard& shared_shard(size_t sz) { return shared[mix((uint64_t)sz)]; }
 };
 
 Pool& pool()

@@ -25,7 +25,7 @@ FP = C.POINTER(C.c_float)
 
 LEADING, TRAILING = 0, 1
 AXIS_IN, AXIS_OUT = -1, 1
-PLAIN_FP64, GROUPED_FP64, GROUPED_SPLIT, FUSED_FP64 = 0, 1, 2, 3
+PLAIN_FP64, GROUPED_FP64, GROUPED_SPLIT, FUSED_FP64, PAIR_FP64 = 0, 1, 2, 3, 4
 
 CT_ERR_HIP, CT_ERR_UNSUPPORTED, CT_ERR_NOMEM, CT_ERR_ARG = -100, -101, -102, -103
 ERRORS = {-1: "Krylov breakdown / factorisation failure", -2: "eigensolver failure", -100: "HIP runtime error",
@@ -78,6 +78,8 @@ def lib():
             "ct_matvec_plan_create": (C.c_int, [P, P, P, P, C.c_int, C.POINTER(P)]),
             "ct_matvec_plan_destroy": (C.c_int, [P]),
             "ct_matvec_plan_check": (C.c_int, [P, P, P, P]),
+            "ct_matvec_pair_check": (C.c_int, [P, P, P, P, C.c_int, DP]),
+            "ct_matvec_pair_stats": (C.c_int, [P, P, P, P, C.c_int, DP]),
             "ct_matvec_plan_dim": (ct_long, [P]),
             "ct_matvec_plan_flops": (C.c_double, [P]),
             "ct_matvec_plan_bytes": (C.c_double, [P]),
@@ -248,6 +250,22 @@ def mpo_merge_tensor_pair(a0, a1):
     lib().mpo_merge_tensor_pair(_ptr(a0), _ptr(a1), r.sptr)
     r.live = True
     return r
+
+
+def matvec_pair_check(a, w, l, r, mfma_cap=0):
+    """Host-only replay of the sector-pair tables (ct_matvec_pair_check): (status, y), status 0 when
+    every index and write-once check holds, y the product computed by the replay."""
+    n = lib().block_sparse_tensor_num_elements_blocks(_ptr(a))
+    y = np.zeros(n, dtype=np.float64)
+    st = lib().ct_matvec_pair_check(_ptr(a), _ptr(w), _ptr(l), _ptr(r), int(mfma_cap), y.ctypes.data_as(DP))
+    return st, y
+
+
+def matvec_pair_stats(a, w, l, r, mfma_cap=0):
+    """Host-only statistics of the sector-pair tables (ct_matvec_pair_stats)."""
+    st = np.zeros(6, dtype=np.float64)
+    check(lib().ct_matvec_pair_stats(_ptr(a), _ptr(w), _ptr(l), _ptr(r), int(mfma_cap), st.ctypes.data_as(DP)), "ct_matvec_pair_stats")
+    return dict(zip(["units", "partial_doubles", "cost_sum", "cost_max", "lds_max", "record_words_max"], st.tolist()))
 
 
 def matvec_plan_check(a, w, l, r):

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libchemtensor_gpu.so")
 
-HIP_SOURCES = ["kernels.hip", "fused.hip", "runtime.hip", "engine.hip", "lanczos.hip", "split.hip", "dmrg.hip", "serve.hip", "capi.hip"]
+HIP_SOURCES = ["kernels.hip", "fused.hip", "pair.hip", "runtime.hip", "engine.hip", "lanczos.hip", "split.hip", "dmrg.hip", "serve.hip", "capi.hip"]
 CPP_SOURCES = ["host_tensor.cpp", "host_linalg.cpp", "host_models.cpp", "host_molecular.cpp", "host_bond.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]

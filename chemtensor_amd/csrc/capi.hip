@@ -92,7 +92,7 @@ void mpo_merge_tensor_pair(const struct block_sparse_tensor* a0, const struct bl
 int ct_matvec_plan_create(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l, const struct block_sparse_tensor* r,
 	const int mode, ct_matvec_plan** plan)
 {
-	if (!a || !w || !l || !r || !plan || mode < 0 || mode > 3) return CT_ERR_ARG;
+	if (!a || !w || !l || !r || !plan || mode < 0 || mode > 4) return CT_ERR_ARG;
 	try {
 		std::unique_ptr<ct_matvec_plan> h(new ct_matvec_plan());
 		const char* env = getenv("CT_AUTOTUNE");
@@ -236,6 +236,55 @@ int ct_matvec_plan_check(const struct block_sparse_tensor* a, const struct block
 		const Meta ma = meta_of(a), mw = meta_of(w), ml = meta_of(l), mr = meta_of(r);
 		validate_matvec_operands(ma, mw, ml, mr);
 		return check_grouped_tables(ma, mw, ml, mr);
+	}
+	catch (const HipError& e) {
+		return map_error(e);
+	}
+}
+
+int ct_matvec_pair_check(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l,
+	const struct block_sparse_tensor* r, const int mfma_cap, double* y)
+{
+	if (!a || !w || !l || !r) return CT_ERR_ARG;
+	try {
+		const Meta ma = meta_of(a), mw = meta_of(w), ml = meta_of(l), mr = meta_of(r);
+		validate_matvec_operands(ma, mw, ml, mr);
+		PairHost h;
+		h.mfma_cap = mfma_cap > 0 ? mfma_cap : pair_default_cap();
+		h.lds = pair_default_lds();
+		if (!plan_pair_tables(h, ma, mw, ml, mr, 0, 0, 0, 0)) return CT_ERR_UNSUPPORTED;
+		std::vector<double> x((size_t)ma.nelem), vw((size_t)mw.nelem), vl((size_t)ml.nelem), vr((size_t)mr.nelem);
+		block_sparse_tensor_serialize_entries(a, x.data());
+		block_sparse_tensor_serialize_entries(w, vw.data());
+		block_sparse_tensor_serialize_entries(l, vl.data());
+		block_sparse_tensor_serialize_entries(r, vr.data());
+		return pair_replay(h, x.data(), ma.nelem, vr.data(), mr.nelem, vl.data(), ml.nelem, vw.data(), mw.nelem, y);
+	}
+	catch (const HipError& e) {
+		return map_error(e);
+	}
+}
+
+int ct_matvec_pair_stats(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l,
+	const struct block_sparse_tensor* r, const int mfma_cap, double* stats6)
+{
+	if (!a || !w || !l || !r || !stats6) return CT_ERR_ARG;
+	try {
+		const Meta ma = meta_of(a), mw = meta_of(w), ml = meta_of(l), mr = meta_of(r);
+		validate_matvec_operands(ma, mw, ml, mr);
+		PairHost h;
+		h.mfma_cap = mfma_cap > 0 ? mfma_cap : pair_default_cap();
+		h.lds = pair_default_lds();
+		if (!plan_pair_tables(h, ma, mw, ml, mr, 0, 0, 0, 0)) return CT_ERR_UNSUPPORTED;
+		double cmax = 0, csum = 0;
+		for (double c : h.cost) { cmax = std::max(cmax, c); csum += c; }
+		stats6[0] = (double)h.recs.size();
+		stats6[1] = (double)h.part_elems;
+		stats6[2] = csum;
+		stats6[3] = cmax;
+		stats6[4] = (double)h.max_lds;
+		stats6[5] = (double)h.max_words;
+		return 0;
 	}
 	catch (const HipError& e) {
 		return map_error(e);

@@ -376,6 +376,55 @@ struct FusedProblem { const Meta* theta; const Meta* w; const Meta* l; const Met
 bool build_fused_batch(FusedPlan& f, const std::vector<FusedProblem>& probs, hipStream_t s);
 void launch_fused_matvec(const FusedPlan& f, const double* x, const double* w, const double* l, const double* r, double* y, hipStream_t s);
 
+// Sector-pair form (pair.hip): one workgroup per chunk (rows a0..a0+ta, columns c0..c0+tc) of a
+// block pair (A, C) of the left and right bond sectors runs all three contractions in LDS and writes
+// its contribution to each output block y(X,Q,C) as a partial block; a reduction launch sums the
+// partial blocks of every y block in (A, row chunk) order.
+constexpr int PR_THREADS = 256;
+constexpr int PR_LDS = 4608;                          // doubles of operand / intermediate LDS per workgroup (36 KB)
+constexpr int PR_REC_WORDS = 512;                     // largest unit record (4-byte words)
+constexpr int PR_RMAX = 32, PR_KMAX = 32;             // largest class W_D (rows x columns), held in registers
+struct PrRed { int64_t y, p; int32_t n, size, nslots, pad; };   // up to 64 entries of one y block
+struct PairHost
+{
+	std::vector<std::vector<uint32_t>> recs;   // one record per unit
+	std::vector<double> cost;                  // estimated cost per unit (MFMA count + staging)
+	std::vector<int64_t> wd_idx;               // class W_D tables (R16 x K4 each) as W element indices, -1: zero
+	std::vector<PrRed> red;
+	int64_t part_elems = 0;
+	double flops = 0;
+	int max_lds = 0, max_words = 0;
+	int mfma_cap = 96;                         // units are cut into chunks of at most this many MFMAs
+	int lds = PR_LDS;                          // LDS budget of a unit (doubles): 2048, 3072 or 4608
+};
+struct PairPlan
+{
+	uint32_t* recs = nullptr;
+	int rec_words = 0, nunits = 0;
+	double* wd = nullptr;
+	PrRed* red = nullptr;
+	int nred = 0;
+	double* part = nullptr;
+	int64_t part_elems = 0;
+	double flops = 0;
+	int max_lds = 0;
+	int lds = PR_LDS;
+	void upload(const PairHost& h, const double* w, hipStream_t s);
+	void release();
+};
+// append one problem (operands at the given element offsets of the concatenated buffers); false
+// when a class W_D exceeds PR_RMAX x PR_KMAX or a single row and column of a pair exceeds the LDS
+bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L, const Meta& R, int64_t x_off, int64_t w_off, int64_t l_off, int64_t r_off);
+bool build_pair_plan(PairPlan& pp, const Meta& th, const Meta& W, const Meta& L, const Meta& R, const double* w, hipStream_t s);
+bool build_pair_batch(PairPlan& pp, const std::vector<FusedProblem>& probs, const double* w, hipStream_t s);
+// host replay of the tables with the kernels' index arithmetic (no device); 0 when every check holds
+int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r, int64_t nr, const double* l, int64_t nl, const double* w, int64_t nw, double* y);
+int pair_default_cap();
+int pair_default_lds();
+void launch_pair_units(const PairPlan& pp, const double* x, const double* r, const double* l, hipStream_t s);
+void launch_pair_reduce(const PairPlan& pp, double* y, hipStream_t s);
+void launch_pair_matvec(const PairPlan& pp, const double* x, const double* r, const double* l, double* y, hipStream_t s);
+
 // Device workspace of the Lanczos iteration: maxiter Krylov vectors, w, the alpha / beta scalars,
 // reduction partials and the Ritz coefficients. Grows on demand; freed with its owner.
 struct LanczosWork
@@ -422,6 +471,8 @@ struct MatvecPlan
 	bool use_f12 = false;                       // stages 1 and 2 fused (no T1 in HBM)
 	// fused form: one launch
 	FusedPlan fused;
+	// sector-pair form: one unit launch and one reduction launch
+	PairPlan pair;
 	// plain form: the reference's operation sequence replayed as individual launches
 	std::vector<std::function<void(const double* x, double* y)>> plain_ops;
 	int plain_stage_end[3] = { 0, 0, 0 };       // index one past the last op of each dot stage

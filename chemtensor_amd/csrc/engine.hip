@@ -485,6 +485,7 @@ MatvecPlan::~MatvecPlan()
 	for (int i = 0; i < 3; i++) stage[i].release();
 	wapply.release();
 	fused.release();
+	pair.release();
 	for (void* p : owned) dev_free(p);
 	for (int i = 0; i < 4; i++) if (ev[i]) (void)hipEventDestroy(ev[i]);
 }
@@ -1139,6 +1140,16 @@ MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& 
 			p->stage_bytes[0] = 8.0 * (2.0 * (double)theta.nelem + (double)p->w.meta.nelem + (double)p->l.meta.nelem + (double)p->r.meta.nelem);
 			p->nunits = p->fused.nunits;
 		}
+		else if (mode == CT_MATVEC_PAIR_FP64) {
+			if (!build_pair_plan(p->pair, theta, p->w.meta, p->l.meta, p->r.meta, p->w.d, s))
+				throw HipError{ hipErrorNotSupported, "sector-pair matvec: a class W_D or a single pair row and column exceeds the unit limits" };
+			p->stage_flops[0] = p->pair.flops;
+			// units read theta, R, L and W_D and write the partial blocks; the reduction reads them and writes y
+			p->stage_bytes[0] = 8.0 * ((double)theta.nelem + (double)p->w.meta.nelem + (double)p->l.meta.nelem + (double)p->r.meta.nelem + (double)p->pair.part_elems);
+			p->stage_bytes[1] = 8.0 * ((double)p->pair.part_elems + (double)theta.nelem);
+			p->nunits = p->pair.nunits;
+			p->interm_elems = (double)p->pair.part_elems;
+		}
 		else build_grouped(*p, mode == CT_MATVEC_GROUPED_FP64 && !autotune);
 		if (mode == CT_MATVEC_GROUPED_FP64 && autotune) autotune_stages(*p);
 		// without timing, stage 2 takes the column stream (faster at every size measured)
@@ -1179,6 +1190,11 @@ float matvec_stage_time(MatvecPlan& p, int stage, const double* x, double* y, in
 			if (stage == 0) launch_fused_matvec(p.fused, x, p.w.d, p.l.d, p.r.d, y, s);
 			return;
 		}
+		if (p.mode == CT_MATVEC_PAIR_FP64) {
+			if (stage == 0) launch_pair_units(p.pair, x, p.r.d, p.l.d, s);
+			else if (stage == 1) launch_pair_reduce(p.pair, y, s);
+			return;
+		}
 		if (p.mode == CT_MATVEC_PLAIN_FP64) {
 			const int i0 = stage == 0 ? 0 : p.plain_stage_end[stage - 1];
 			for (int i = i0; i < p.plain_stage_end[stage]; i++) p.plain_ops[i](x, y);
@@ -1212,6 +1228,12 @@ void matvec_apply(MatvecPlan& p, const double* x, double* y, float* stage_ms)
 	if (p.mode == CT_MATVEC_FUSED_FP64) {
 		launch_fused_matvec(p.fused, x, p.w.d, p.l.d, p.r.d, y, s);
 		if (stage_ms) for (int st = 1; st < 4; st++) CTG_HIP(hipEventRecord(p.ev[st], s));
+	}
+	else if (p.mode == CT_MATVEC_PAIR_FP64) {
+		launch_pair_units(p.pair, x, p.r.d, p.l.d, s);
+		if (stage_ms) CTG_HIP(hipEventRecord(p.ev[1], s));
+		launch_pair_reduce(p.pair, y, s);
+		if (stage_ms) for (int st = 2; st < 4; st++) CTG_HIP(hipEventRecord(p.ev[st], s));
 	}
 	else if (p.mode == CT_MATVEC_PLAIN_FP64) {
 		int i = 0;

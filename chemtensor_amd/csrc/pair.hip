@@ -1,0 +1,793 @@
+// Sector-pair form of the two-site H_eff matvec: the three contractions of apply_local_hamiltonian
+// (src/algorithm/chain_ops.c:353-390) in one launch, plus one launch that sums partial results.
+//
+// With unit outer bonds the reference's sequence is, block by block,
+//   T1(A,P,N,C)[a p n c] = sum_b  theta(A,P,B)[a p b] R(B,N,C,o)[b n c 0]           qB = qA + qP
+//   T2(A,M,Q,C)[a m q c] = sum_pn W(M,Q,P,N)[m q p n] T1(A,P,N,C)[a p n c]           qM + qQ = qP + qN
+//   y(X,Q,C)[x q c]      = sum_am L(o,A,M,X)[0 a m x] T2(A,M,Q,C)[a m q c]           qX = qA + qM - qo
+// and every intermediate block belongs to exactly one block pair (A, C) of the left and right bond
+// sectors, whose class D = qC - qA + qo fixes the (P,N) "routes" and (M,Q) "rows" involved. A
+// workgroup takes one pair, or a chunk of its rows a and columns c: it stages the theta rows, R
+// columns and L rows of the chunk in LDS (every load in flight at once), computes T1, then
+// T2 = W_D T1 with the class's dense W_D held in registers, then L^T T2, all on
+// v_mfma_f64_16x16x4f64 with operands from LDS, and writes its contribution to each y(X,Q,C) as a
+// partial block. A second launch sums the partial blocks of every y block in a fixed order (by A,
+// then row chunk), so the result does not depend on scheduling. Nothing of T1 / T2 reaches HBM.
+//
+// The host resolves the block structure into per-unit tables (staging pieces, MFMA tiles with their
+// LDS bases and valid extents, the stage-2 row map), so the kernel's per-lane work is address
+// arithmetic on uniform descriptors: divisions by small runtime integers use precomputed
+// multipliers, the contracted dimension of every staged panel is zero-padded to a multiple of 4,
+// lanes outside a tile's valid rows or columns read a clamped in-bounds address (their outputs are
+// never stored), and absent T1 rows read an LDS region of zeros.
+
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <map>
+#include <numeric>
+#include "ct_internal.h"
+
+namespace ctg {
+
+typedef double pr_double4 __attribute__((ext_vector_type(4)));
+
+// Unit record (4-byte words, fixed stride per plan):
+//   header (16): n1 n3 npiece R16 | K4 S1 tc mag(tc) | t2 wd off1 off3 | offk offp pad pad
+//   stage-1 tiles at off1 (8 words each): abase sth bbase Nr | nk vr|vc<<16 sbase pad
+//   stage-3 tiles at off3 (16 words each): lbase dX t2b dQ*S1 | dM mag(dM) K3 nk3 |
+//                                          vr|vc<<16 j0 tc|dC<<16 mag(tc) | pbase(2) dQ*dC pad
+//   stage-2 row map at offk (2 words per T1 row k < K4): LDS offset of row k at a = 0, c = 0, and
+//                                          its stride in a (0, 0 for absent rows: the zero region)
+//   pieces at offp (8 words each): src(2) d0 e0|kind<<30 | n|vrows<<16 cols|dstride<<16
+//                                  sstride|vcols<<16 mag(cols)
+// A piece covers entries e0 .. e0+63 of a staged panel of n entries laid out as rows of 'cols'
+// entries: entry (row, col) goes to LDS d0 + row dstride + col and holds
+// src[row sstride + col] (kind 0: theta, 1: R, 2: L) when row < vrows and col < vcols, else 0.
+// mag(d) = ceil(2^31 / d): for 0 <= t < 2^31 / d, t / d = mulhi(2 t, mag(d)).
+constexpr int PR_HDR = 16, PR_T1 = 8, PR_T3 = 16, PR_PIECE = 8;
+constexpr int PR_ZERO = 64;   // LDS doubles [0, 64) hold zeros
+
+__device__ __forceinline__ int pr_q(int t, uint32_t mag) { return (int)__umulhi((uint32_t)t << 1, mag); }
+__device__ __forceinline__ int pr_s(uint32_t v) { return __builtin_amdgcn_readfirstlane((int)v); }
+
+template <int LDS>
+__global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __restrict__ recs, const int rec_words, const double* __restrict__ x,
+	const double* __restrict__ rr, const double* __restrict__ ll, const double* __restrict__ wd, double* __restrict__ part)
+{
+	constexpr int NB = LDS / PR_THREADS;   // staging pieces per wavefront
+	static_assert(NB <= 64, "one piece descriptor per lane");
+	__shared__ double sm[LDS];
+	const int tid = threadIdx.x, lane = tid & 63;
+	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+	const int i16 = lane & 15, kq = lane >> 4;
+	const uint32_t* __restrict__ g = recs + (size_t)blockIdx.x * (size_t)rec_words;
+
+	const int n1 = pr_s(g[0]), n3 = pr_s(g[1]), npiece = pr_s(g[2]), R16 = pr_s(g[3]);
+	const int K4 = pr_s(g[4]), S1 = pr_s(g[5]), tc = pr_s(g[6]);
+	const uint32_t magtc = (uint32_t)pr_s(g[7]);
+	const int t2 = pr_s(g[8]), wdo = pr_s(g[9]), off1 = pr_s(g[10]), off3 = pr_s(g[11]);
+	const int offk = pr_s(g[12]), offp = pr_s(g[13]);
+
+	// lane j of wavefront w holds the descriptor of piece w + 4 j
+	uint4 pa = { 0u, 0u, 0u, 0u }, pb = { 0u, 0u, 0u, 0u };
+	if (lane < NB && wave + 4 * lane < npiece) {
+		const uint4* d = (const uint4*)(g + offp + PR_PIECE * (wave + 4 * lane));
+		pa = d[0];
+		pb = d[1];
+	}
+	// stage-2 row map: T1 row 4 ks + l / 16 of this lane
+	int kofs[PR_KMAX / 4], kstr[PR_KMAX / 4];
+	#pragma unroll
+	for (int ks = 0; ks < PR_KMAX / 4; ks++) {
+		kofs[ks] = 0;
+		kstr[ks] = 0;
+		if (4 * ks < K4) {
+			const uint2 e = *(const uint2*)(g + offk + 2 * (4 * ks + kq));
+			kofs[ks] = (int)e.x;
+			kstr[ks] = (int)e.y;
+		}
+	}
+	// the class's W_D (R16 x K4, zero padded) as MFMA A fragments: row 16 mi + l % 16, column 4 ks + l / 16
+	double wa[PR_RMAX / 16][PR_KMAX / 4];
+	#pragma unroll
+	for (int mi = 0; mi < PR_RMAX / 16; mi++)
+		#pragma unroll
+		for (int ks = 0; ks < PR_KMAX / 4; ks++)
+			wa[mi][ks] = (16 * mi < R16 && 4 * ks < K4) ? wd[wdo + (16 * mi + i16) * K4 + 4 * ks + kq] : 0.0;
+
+	// staging: every load of the wavefront's pieces is issued before the first LDS store
+	if (tid < PR_ZERO) sm[tid] = 0.0;
+	{
+		double v[NB];
+		int dst[NB];
+		#pragma unroll
+		for (int i = 0; i < NB; i++) {
+			dst[i] = -1;
+			v[i] = 0.0;
+			if (wave + 4 * i < npiece) {
+				const uint32_t w0 = (uint32_t)__builtin_amdgcn_readlane((int)pa.x, i), w1 = (uint32_t)__builtin_amdgcn_readlane((int)pa.y, i);
+				const uint32_t w2 = (uint32_t)__builtin_amdgcn_readlane((int)pa.z, i), w3 = (uint32_t)__builtin_amdgcn_readlane((int)pa.w, i);
+				const uint32_t w4 = (uint32_t)__builtin_amdgcn_readlane((int)pb.x, i), w5 = (uint32_t)__builtin_amdgcn_readlane((int)pb.y, i);
+				const uint32_t w6 = (uint32_t)__builtin_amdgcn_readlane((int)pb.z, i), w7 = (uint32_t)__builtin_amdgcn_readlane((int)pb.w, i);
+				const int64_t src = (int64_t)((uint64_t)w0 | ((uint64_t)w1 << 32));
+				const int kind = (int)(w3 >> 30), e = (int)(w3 & 0xffffu) + lane;
+				const int n = (int)(w4 & 0xffffu), vrows = (int)(w4 >> 16);
+				const int cols = (int)(w5 & 0xffffu), dstride = (int)(w5 >> 16);
+				const int sstride = (int)(w6 & 0xffffu), vcols = (int)(w6 >> 16);
+				if (e < n) {
+					const int row = pr_q(e, w7), col = e - row * cols;
+					dst[i] = (int)w2 + row * dstride + col;
+					if (row < vrows && col < vcols) {
+						const double* __restrict__ base = (kind == 0 ? x : (kind == 1 ? rr : ll)) + src;
+						v[i] = base[row * sstride + col];
+					}
+				}
+			}
+		}
+		#pragma unroll
+		for (int i = 0; i < NB; i++)
+			if (dst[i] >= 0) sm[dst[i]] = v[i];
+	}
+	__syncthreads();
+
+	// stage 1: T1 of each route in its own layout [(a p)][(n c)] = theta[(a p), b] R[b, (n c)]
+	for (int t = wave; t < n1; t += 4) {
+		const uint32_t* __restrict__ td = g + off1 + PR_T1 * t;
+		const int abase = pr_s(td[0]), sth = pr_s(td[1]), bbase = pr_s(td[2]), Nr = pr_s(td[3]);
+		const int nk = pr_s(td[4]), vrc = pr_s(td[5]), sbase = pr_s(td[6]);
+		const int vr = vrc & 0xffff, vc = vrc >> 16;
+		const int ar = (i16 < vr ? abase + i16 * sth : abase) + kq;
+		const int bc = (i16 < vc ? bbase + i16 : bbase) + kq * Nr;
+		pr_double4 acc = { 0, 0, 0, 0 };
+		for (int s = 0; s < nk; s++)
+			acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sm[ar + 4 * s], sm[bc + 4 * s * Nr], acc, 0, 0, 0);
+		if (i16 < vc) {
+			#pragma unroll
+			for (int r4 = 0; r4 < 4; r4++)
+				if (kq + 4 * r4 < vr) sm[sbase + (kq + 4 * r4) * Nr + i16] = acc[r4];
+		}
+	}
+	__syncthreads();
+
+	// stage 2: T2[(M Q m q)][(a c)] = W_D T1 (rows of the row map, columns a tc + c)
+	{
+		const int nt2 = (S1 + 15) >> 4;
+		#pragma unroll
+		for (int mi = 0; mi < PR_RMAX / 16; mi++) {
+			if (16 * mi >= R16) break;
+			for (int ni = wave; ni < nt2; ni += 4) {
+				const int j2 = 16 * ni + i16;
+				const bool vj = j2 < S1;
+				const int a = vj ? pr_q(j2, magtc) : 0, c = vj ? j2 - a * tc : 0;
+				pr_double4 acc = { 0, 0, 0, 0 };
+				#pragma unroll
+				for (int ks = 0; ks < PR_KMAX / 4; ks++)
+					if (4 * ks < K4) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[mi][ks], sm[kofs[ks] + a * kstr[ks] + c], acc, 0, 0, 0);
+				if (vj) {
+					#pragma unroll
+					for (int r4 = 0; r4 < 4; r4++) sm[t2 + (16 * mi + kq + 4 * r4) * S1 + j2] = acc[r4];
+				}
+			}
+		}
+	}
+	__syncthreads();
+
+	// stage 3: partial y(X,Q,C)[x, (q c)] = L(o,A,M,X)[(a m), x]^T T2(A,M,Q,C)[(a m), (q c)]
+	for (int t = wave; t < n3; t += 4) {
+		const uint32_t* __restrict__ td = g + off3 + PR_T3 * t;
+		const int lbase = pr_s(td[0]), dX = pr_s(td[1]), t2b = pr_s(td[2]), dQS1 = pr_s(td[3]);
+		const int dM = pr_s(td[4]);
+		const uint32_t magm = (uint32_t)pr_s(td[5]);
+		const int K3 = pr_s(td[6]), nk3 = pr_s(td[7]), vrc = pr_s(td[8]), j0 = pr_s(td[9]);
+		const int tcdc = pr_s(td[10]);
+		const uint32_t mag3 = (uint32_t)pr_s(td[11]);
+		const int64_t pbase = (int64_t)((uint64_t)(uint32_t)pr_s(td[12]) | ((uint64_t)(uint32_t)pr_s(td[13]) << 32));
+		const int dQdC = pr_s(td[14]);
+		const int vr = vrc & 0xffff, vc = vrc >> 16, tc3 = tcdc & 0xffff, dC = tcdc >> 16;
+		const int ax = (i16 < vr ? lbase + i16 : lbase);
+		const bool vj = i16 < vc;
+		const int j3 = j0 + i16;
+		const int q = vj ? pr_q(j3, mag3) : 0, c = vj ? j3 - q * tc3 : 0;
+		const int jofs = t2b + q * S1 + c;
+		pr_double4 acc = { 0, 0, 0, 0 };
+		for (int s = 0; s < nk3; s++) {
+			const int k = 4 * s + kq;
+			const int a = pr_q(k, magm), m = k - a * dM;
+			const bool vk = k < K3;
+			const int la = vk ? ax + k * dX : 0, lb = vk ? jofs + m * dQS1 + a * tc3 : 0;
+			acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sm[la], sm[lb], acc, 0, 0, 0);
+		}
+		if (vj) {
+			double* __restrict__ out = part + pbase + q * dC + c;
+			#pragma unroll
+			for (int r4 = 0; r4 < 4; r4++)
+				if (kq + 4 * r4 < vr) out[(kq + 4 * r4) * dQdC] = acc[r4];
+		}
+	}
+}
+
+// y block chunk of up to 64 entries: y[y + e] = sum_s part[p + s * size + e], s = 0 .. nslots-1
+__global__ __launch_bounds__(256) void k_pair_reduce(const PrRed* __restrict__ items, const int nitems, const double* __restrict__ part, double* __restrict__ y)
+{
+	const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+	const int lane = threadIdx.x & 63;
+	if (w >= nitems) return;
+	const PrRed it = items[w];
+	if (lane >= it.n) return;
+	const double* __restrict__ src = part + it.p + lane;
+	const int64_t sz = it.size;
+	double acc = 0.0;
+	int s = 0;
+	for (; s + 4 <= it.nslots; s += 4) {
+		const double v0 = src[(s + 0) * sz], v1 = src[(s + 1) * sz], v2 = src[(s + 2) * sz], v3 = src[(s + 3) * sz];
+		acc += v0;
+		acc += v1;
+		acc += v2;
+		acc += v3;
+	}
+	for (; s < it.nslots; s++) acc += src[s * sz];
+	y[it.y + lane] = acc;
+}
+
+//________________________________________________________________________________________________________________________
+//
+// host planning
+
+namespace {
+
+struct PrClass
+{
+	std::vector<std::pair<ct_long, ct_long>> rows, cols;   // (M, Q) and (P, N) blocks
+	std::vector<int> rbase, kbase;                          // first W_D row / column of each
+	int Rd = 0, Kd = 0, R16 = 0, K4 = 0;
+	int64_t wd = 0;                                          // offset of W_D in the table
+	bool ok = false;
+};
+
+// a stage-3 tile's partial offset: patched with its slot once all slots are known
+struct PrSlotRef { int64_t yblock; ct_long A; int ia; size_t unit; int word; };
+
+inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+inline int up4(int a) { return (a + 3) / 4 * 4; }
+inline uint32_t magic(int d) { return (uint32_t)((((uint64_t)1 << 31) + (uint64_t)d - 1) / (uint64_t)d); }
+inline int qdiv(int t, uint32_t mag) { return (int)(((uint64_t)((uint32_t)t << 1) * mag) >> 32); }
+
+}  // namespace
+
+bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L, const Meta& R, int64_t x_off, int64_t w_off, int64_t l_off, int64_t r_off)
+{
+	const qnumber qo = L.qnums_blocks[0][0];
+	const ct_long nA = th.dim_blocks[0], nC = R.dim_blocks[2];
+	const ct_long nM = W.dim_blocks[0], nQ = W.dim_blocks[1], nP = W.dim_blocks[2], nN = W.dim_blocks[3];
+	auto bc = [](const Meta& m, int ax, ct_long i) { return (int)m.block_count[ax][i]; };
+	auto qb = [](const Meta& m, int ax, ct_long i) { return m.qnums_blocks[ax][i]; };
+
+	// classes D = qM + qQ = qP + qN: rows (M,Q), columns (P,N), dense W_D (R16 x K4) as W indices
+	std::map<qnumber, PrClass> classes;
+	auto get_class = [&](qnumber D) -> PrClass& {
+		auto it = classes.find(D);
+		if (it != classes.end()) return it->second;
+		PrClass& c = classes[D];
+		for (ct_long M = 0; M < nM; M++) for (ct_long Q = 0; Q < nQ; Q++)
+			if (qb(W, 0, M) + qb(W, 1, Q) == D) { c.rows.emplace_back(M, Q); c.rbase.push_back(c.Rd); c.Rd += bc(W, 0, M) * bc(W, 1, Q); }
+		for (ct_long P = 0; P < nP; P++) for (ct_long N = 0; N < nN; N++)
+			if (qb(W, 2, P) + qb(W, 3, N) == D) { c.cols.emplace_back(P, N); c.kbase.push_back(c.Kd); c.Kd += bc(W, 2, P) * bc(W, 3, N); }
+		c.ok = c.Rd > 0 && c.Kd > 0 && c.Rd <= PR_RMAX && c.Kd <= PR_KMAX;
+		if (!c.ok) return c;
+		c.R16 = (c.Rd + 15) / 16 * 16;
+		c.K4 = up4(c.Kd);
+		c.wd = (int64_t)h.wd_idx.size();
+		h.wd_idx.resize(h.wd_idx.size() + (size_t)c.R16 * c.K4, -1);
+		for (size_t ir = 0; ir < c.rows.size(); ir++) {
+			const ct_long M = c.rows[ir].first, Q = c.rows[ir].second;
+			const int dM = bc(W, 0, M), dQ = bc(W, 1, Q);
+			for (size_t ic = 0; ic < c.cols.size(); ic++) {
+				const ct_long P = c.cols[ic].first, N = c.cols[ic].second;
+				const int dP = bc(W, 2, P), dN = bc(W, 3, N);
+				const ct_long iw[4] = { M, Q, P, N };
+				const ct_long kw = W.flat_index(iw);
+				if (!W.present(kw)) continue;
+				for (int m = 0; m < dM; m++) for (int q = 0; q < dQ; q++)
+					for (int pp = 0; pp < dP; pp++) for (int n = 0; n < dN; n++) {
+						const int row = c.rbase[ir] + m * dQ + q, col = c.kbase[ic] + pp * dN + n;
+						h.wd_idx[(size_t)(c.wd + (int64_t)row * c.K4 + col)] = w_off + W.offset[kw] + ((ct_long)(m * dQ + q) * dP + pp) * dN + n;
+					}
+			}
+		}
+		return c;
+	};
+
+	struct VRoute { int64_t th, r; int dP, dB, dN, kbase; };
+	struct VRow { int64_t l; int64_t yblock; int dM, dQ, dX, rbase; };
+	std::vector<PrSlotRef> refs;
+	const int nbp = 4 * (h.lds / PR_THREADS);   // pieces per unit
+
+	std::vector<VRoute> vr;
+	std::vector<VRow> vq;
+	for (ct_long A = 0; A < nA; A++) {
+		const int dA = bc(th, 0, A);
+		for (ct_long C = 0; C < nC; C++) {
+			const int dC = bc(R, 2, C);
+			const qnumber D = qb(R, 2, C) - qb(th, 0, A) + qo;
+			PrClass& cl = get_class(D);
+			if (cl.rows.empty() || cl.cols.empty()) continue;
+			if (!cl.ok) return false;
+			vr.clear();
+			vq.clear();
+			for (size_t ic = 0; ic < cl.cols.size(); ic++) {
+				const ct_long P = cl.cols[ic].first, N = cl.cols[ic].second;
+				const ct_long B = th.find_qnum(2, qb(th, 0, A) + qb(th, 1, P));
+				if (B < 0) continue;
+				const ct_long ith[3] = { A, P, B }, irr[4] = { B, N, C, 0 };
+				const ct_long kt = th.flat_index(ith), kr = R.flat_index(irr);
+				if (!th.present(kt) || !R.present(kr)) continue;
+				vr.push_back(VRoute{ x_off + th.offset[kt], r_off + R.offset[kr], bc(th, 1, P), bc(th, 2, B), bc(R, 1, N), cl.kbase[ic] });
+			}
+			for (size_t ir = 0; ir < cl.rows.size(); ir++) {
+				const ct_long M = cl.rows[ir].first, Q = cl.rows[ir].second;
+				const ct_long X = L.find_qnum(3, qb(th, 0, A) + qb(W, 0, M) - qo);
+				if (X < 0) continue;
+				const ct_long il[4] = { 0, A, M, X }, iy[3] = { X, Q, C };
+				const ct_long kl = L.flat_index(il), ky = th.flat_index(iy);
+				if (!L.present(kl) || !th.present(ky)) continue;
+				vq.push_back(VRow{ l_off + L.offset[kl], (int64_t)ky, bc(W, 0, M), bc(W, 1, Q), bc(L, 3, X), cl.rbase[ir] });
+			}
+			if (vr.empty() || vq.empty()) continue;
+			// LDS need of a chunk (ta rows a, tc columns c) and its number of staging pieces
+			auto lds_need = [&](int ta, int tc) {
+				int l = PR_ZERO, t1 = 0, st = 0;
+				for (auto& v : vq) l += ta * v.dM * v.dX;
+				for (auto& v : vr) { t1 += ta * v.dP * v.dN * tc; st += ta * v.dP * (up4(v.dB) + 1) + up4(v.dB) * v.dN * tc; }
+				return l + t1 + std::max(cl.R16 * ta * tc, st);
+			};
+			auto npieces = [&](int ta, int tc) {
+				int n = 0;
+				for (auto& v : vr) n += cdiv(ta * v.dP * up4(v.dB), 64) + cdiv(up4(v.dB) * v.dN * tc, 64);
+				for (auto& v : vq) n += cdiv(ta * v.dM * v.dX, 64);
+				return n;
+			};
+			auto mfmas = [&](int ta, int tc) {
+				int n = 0;
+				for (auto& v : vr) n += cdiv(ta * v.dP, 16) * cdiv(v.dN * tc, 16) * (up4(v.dB) / 4);
+				n += (cl.R16 / 16) * cdiv(ta * tc, 16) * (cl.K4 / 4);
+				for (auto& v : vq) n += cdiv(v.dX, 16) * cdiv(v.dQ * tc, 16) * cdiv(ta * v.dM, 4);
+				return n;
+			};
+			int ta = dA, tc = dC;
+			while (lds_need(ta, tc) > h.lds || npieces(ta, tc) > nbp || mfmas(ta, tc) > h.mfma_cap || tc > PR_ZERO) {
+				if (tc > 4 && (tc >= ta || ta <= 4)) tc = (tc + 1) / 2;
+				else if (ta > 1) ta = (ta + 1) / 2;
+				else if (tc > 1) tc = (tc + 1) / 2;
+				else if (lds_need(ta, tc) > h.lds || npieces(ta, tc) > nbp) return false;
+				else break;
+			}
+			for (int a0 = 0, ia = 0; a0 < dA; a0 += ta, ia++) {
+				const int tA = std::min(ta, dA - a0);
+				for (int c0 = 0; c0 < dC; c0 += tc) {
+					const int tC = std::min(tc, dC - c0);
+					const int S1 = tA * tC;
+					// LDS layout: zeros | L rows | T1 of each route | T2 overlaid on the theta and R panels
+					int off = PR_ZERO;
+					std::vector<int> lds_l(vq.size()), t1b(vr.size()), lth(vr.size()), lr(vr.size());
+					for (size_t i = 0; i < vq.size(); i++) { lds_l[i] = off; off += tA * vq[i].dM * vq[i].dX; }
+					for (size_t i = 0; i < vr.size(); i++) { t1b[i] = off; off += tA * vr[i].dP * vr[i].dN * tC; }
+					const int lt2 = off;
+					for (size_t i = 0; i < vr.size(); i++) {
+						lth[i] = off; off += tA * vr[i].dP * (up4(vr[i].dB) + 1);
+						lr[i] = off; off += up4(vr[i].dB) * vr[i].dN * tC;
+					}
+					const int need = std::max(off, lt2 + cl.R16 * S1);
+					if (need > h.lds || need > 0xffff) return false;
+					h.max_lds = std::max(h.max_lds, need);
+					std::vector<uint32_t> t1s, t3s, kt((size_t)cl.K4 * 2, 0u), pcs;
+					// stage-1 tiles
+					for (size_t i = 0; i < vr.size(); i++) {
+						const int Mr = tA * vr[i].dP, Nr = vr[i].dN * tC, sth = up4(vr[i].dB) + 1;
+						for (int mi = 0; mi < cdiv(Mr, 16); mi++)
+							for (int ni = 0; ni < cdiv(Nr, 16); ni++) {
+								const int vrr = std::min(16, Mr - 16 * mi), vcc = std::min(16, Nr - 16 * ni);
+								const uint32_t d[PR_T1] = { (uint32_t)(lth[i] + 16 * mi * sth), (uint32_t)sth, (uint32_t)(lr[i] + 16 * ni), (uint32_t)Nr,
+									(uint32_t)(up4(vr[i].dB) / 4), (uint32_t)vrr | ((uint32_t)vcc << 16), (uint32_t)(t1b[i] + 16 * mi * Nr + 16 * ni), 0u };
+								t1s.insert(t1s.end(), d, d + PR_T1);
+							}
+					}
+					// stage-2 row map: T1 row k = kbase + p dN + n of a route -> the route's T1 at (a dP + p, n tc + c)
+					for (size_t i = 0; i < vr.size(); i++) {
+						const int Nr = vr[i].dN * tC;
+						for (int p = 0; p < vr[i].dP; p++) for (int n = 0; n < vr[i].dN; n++) {
+							const int k = vr[i].kbase + p * vr[i].dN + n;
+							kt[(size_t)(2 * k)] = (uint32_t)(t1b[i] + p * Nr + n * tC);
+							kt[(size_t)(2 * k + 1)] = (uint32_t)(vr[i].dP * Nr);
+						}
+					}
+					// stage-3 tiles; words 12-13 hold the offset within the partial slot until patched
+					for (size_t i = 0; i < vq.size(); i++) {
+						const int dX = vq[i].dX, dM = vq[i].dM, dQ = vq[i].dQ, Nq = dQ * tC, K3 = tA * dM;
+						for (int mi = 0; mi < cdiv(dX, 16); mi++)
+							for (int ni = 0; ni < cdiv(Nq, 16); ni++) {
+								const int vrr = std::min(16, dX - 16 * mi), vcc = std::min(16, Nq - 16 * ni);
+								const int64_t rel = (int64_t)16 * mi * dQ * dC + c0;
+								const uint32_t d[PR_T3] = { (uint32_t)(lds_l[i] + 16 * mi), (uint32_t)dX, (uint32_t)(lt2 + vq[i].rbase * S1), (uint32_t)(dQ * S1),
+									(uint32_t)dM, magic(dM), (uint32_t)K3, (uint32_t)cdiv(K3, 4),
+									(uint32_t)vrr | ((uint32_t)vcc << 16), (uint32_t)(16 * ni), (uint32_t)tC | ((uint32_t)dC << 16), magic(tC),
+									(uint32_t)rel, (uint32_t)((uint64_t)rel >> 32), (uint32_t)(dQ * dC), 0u };
+								refs.push_back(PrSlotRef{ vq[i].yblock, A, ia, h.recs.size(), (int)(t3s.size() + 12) });
+								t3s.insert(t3s.end(), d, d + PR_T3);
+							}
+					}
+					// staging pieces: theta rows zero-padded to a multiple of 4 columns, R rows b
+					// zero-padded to a multiple of 4, L rows contiguous
+					auto add_pieces = [&](int64_t src, int d0, int n, int vrows, int cols, int dstride, int sstride, int vcols, int kind) {
+						for (int e0 = 0; e0 < n; e0 += 64) {
+							const uint32_t d[PR_PIECE] = { (uint32_t)src, (uint32_t)((uint64_t)src >> 32), (uint32_t)d0, (uint32_t)e0 | ((uint32_t)kind << 30),
+								(uint32_t)n | ((uint32_t)vrows << 16), (uint32_t)cols | ((uint32_t)dstride << 16), (uint32_t)sstride | ((uint32_t)vcols << 16), magic(cols) };
+							pcs.insert(pcs.end(), d, d + PR_PIECE);
+						}
+					};
+					for (size_t i = 0; i < vr.size(); i++) {
+						const int dB = vr[i].dB, dB4 = up4(dB), Mr = tA * vr[i].dP;
+						add_pieces(vr[i].th + (int64_t)a0 * vr[i].dP * dB, lth[i], Mr * dB4, Mr, dB4, dB4 + 1, dB, dB, 0);
+						add_pieces(vr[i].r + c0, lr[i], dB4 * vr[i].dN * tC, dB * vr[i].dN, tC, tC, dC, tC, 1);
+					}
+					for (size_t i = 0; i < vq.size(); i++) {
+						const int n = tA * vq[i].dM * vq[i].dX;
+						add_pieces(vq[i].l + (int64_t)a0 * vq[i].dM * vq[i].dX, lds_l[i], n, 1, n, 0, 0, n, 2);
+					}
+					const int np = (int)(pcs.size() / PR_PIECE);
+					if (np > nbp) return false;
+					std::vector<uint32_t> rec(PR_HDR, 0u);
+					rec[0] = (uint32_t)(t1s.size() / PR_T1); rec[1] = (uint32_t)(t3s.size() / PR_T3); rec[2] = (uint32_t)np; rec[3] = (uint32_t)cl.R16;
+					rec[4] = (uint32_t)cl.K4; rec[5] = (uint32_t)S1; rec[6] = (uint32_t)tC; rec[7] = magic(tC);
+					rec[8] = (uint32_t)lt2; rec[9] = (uint32_t)cl.wd;
+					rec[10] = (uint32_t)rec.size();
+					rec.insert(rec.end(), t1s.begin(), t1s.end());
+					rec[11] = (uint32_t)rec.size();
+					for (size_t k = refs.size() - t3s.size() / PR_T3; k < refs.size(); k++) refs[k].word += (int)rec.size();
+					rec.insert(rec.end(), t3s.begin(), t3s.end());
+					rec[12] = (uint32_t)rec.size();
+					rec.insert(rec.end(), kt.begin(), kt.end());
+					while (rec.size() % 4) rec.push_back(0u);
+					rec[13] = (uint32_t)rec.size();
+					rec.insert(rec.end(), pcs.begin(), pcs.end());
+					h.max_words = std::max(h.max_words, (int)rec.size());
+					// flops as executed (algorithmic: no tile padding)
+					double f = 0;
+					for (auto& v : vr) f += 2.0 * tA * v.dP * v.dB * (double)(v.dN * tC);
+					f += 2.0 * (double)cl.Rd * cl.Kd * S1;
+					for (auto& v : vq) f += 2.0 * v.dX * (double)(tA * v.dM) * (v.dQ * tC);
+					h.flops += f;
+					h.cost.push_back((double)mfmas(tA, tC) + (double)np / 4.0 + 4.0);
+					h.recs.push_back(std::move(rec));
+				}
+			}
+		}
+	}
+	// partial slots: one per (y block, A, row chunk), grouped by y block in (A, row chunk) order;
+	// the column chunks of a pair write disjoint columns of the same slot
+	std::vector<size_t> order(refs.size());
+	std::iota(order.begin(), order.end(), (size_t)0);
+	std::sort(order.begin(), order.end(), [&](size_t i, size_t j) {
+		const PrSlotRef& a = refs[i];
+		const PrSlotRef& b = refs[j];
+		if (a.yblock != b.yblock) return a.yblock < b.yblock;
+		if (a.A != b.A) return a.A < b.A;
+		return a.ia < b.ia;
+	});
+	std::vector<int64_t> slot_base((size_t)th.nblocks, -1);
+	std::vector<int> nslots((size_t)th.nblocks, 0);
+	for (size_t k = 0; k < order.size();) {
+		const int64_t yb = refs[order[k]].yblock;
+		const int64_t size = th.block_size(yb);
+		slot_base[(size_t)yb] = h.part_elems;
+		while (k < order.size() && refs[order[k]].yblock == yb) {
+			const PrSlotRef& rs = refs[order[k]];
+			const int64_t slot = h.part_elems;
+			const ct_long A0 = rs.A;
+			const int ia0 = rs.ia;
+			for (; k < order.size() && refs[order[k]].yblock == yb && refs[order[k]].A == A0 && refs[order[k]].ia == ia0; k++) {
+				const PrSlotRef& ru = refs[order[k]];
+				uint32_t* w = &h.recs[ru.unit][(size_t)ru.word];
+				const int64_t pb = slot + (int64_t)((uint64_t)w[0] | ((uint64_t)w[1] << 32));
+				w[0] = (uint32_t)pb;
+				w[1] = (uint32_t)((uint64_t)pb >> 32);
+			}
+			h.part_elems += size;
+			nslots[(size_t)yb]++;
+		}
+	}
+	// reduction items over every y block in serialise order (blocks without slots are zeroed)
+	for (ct_long k = 0; k < th.nblocks; k++) {
+		if (!th.present(k)) continue;
+		const int64_t size = th.block_size(k);
+		for (int64_t e0 = 0; e0 < size; e0 += 64) {
+			PrRed it;
+			it.y = x_off + th.offset[k] + e0;
+			it.p = nslots[(size_t)k] ? slot_base[(size_t)k] + e0 : 0;
+			it.n = (int32_t)std::min<int64_t>(64, size - e0);
+			it.size = (int32_t)size;
+			it.nslots = nslots[(size_t)k];
+			it.pad = 0;
+			h.red.push_back(it);
+		}
+	}
+	return true;
+}
+
+// Host replay of the unit and reduction launches with the kernels' own index arithmetic (the
+// multiplier divisions included), tile order and clamps: every operand, LDS and partial index in
+// bounds, every LDS operand read after it was written in the current phase (the theta / R panels
+// are dead once stage 1 ends), T1 / T2 / partial / y entries written exactly once, staging
+// destinations disjoint. y (optional, length nx) receives the product. Returns 0 or a check number.
+int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r, int64_t nr, const double* l, int64_t nl, const double* w, int64_t nw, double* y)
+{
+	const int LDS = h.lds, NB = h.lds / PR_THREADS;
+	if (NB > 64) return 1;
+	std::vector<double> wd(h.wd_idx.size());
+	for (size_t i = 0; i < wd.size(); i++) {
+		const int64_t q = h.wd_idx[i];
+		if (q >= nw) return 1;
+		wd[i] = q >= 0 ? w[q] : 0.0;
+	}
+	std::vector<double> part((size_t)h.part_elems, 0.0);
+	std::vector<uint8_t> pwr((size_t)h.part_elems, 0);
+	std::vector<double> sm((size_t)LDS);
+	std::vector<uint8_t> sw((size_t)LDS);
+	const int rec_words = std::max(PR_HDR, (h.max_words + 3) / 4 * 4);
+	bool bad = false;
+	auto dv = [&](int t, int d, uint32_t mag) { const int q = qdiv(t, mag); if (t < 0 || d < 1 || q != t / d) bad = true; return q; };
+	for (size_t u = 0; u < h.recs.size(); u++) {
+		const std::vector<uint32_t>& rec = h.recs[u];
+		if (rec.size() < (size_t)PR_HDR || (int)rec.size() > rec_words) return 2;
+		const uint32_t* g = rec.data();
+		const int n1 = (int)g[0], n3 = (int)g[1], npiece = (int)g[2], R16 = (int)g[3];
+		const int K4 = (int)g[4], S1 = (int)g[5], tc = (int)g[6];
+		const uint32_t magtc = g[7];
+		const int t2 = (int)g[8], wdo = (int)g[9], off1 = (int)g[10], off3 = (int)g[11], offk = (int)g[12], offp = (int)g[13];
+		if ((size_t)off1 + (size_t)PR_T1 * n1 > rec.size() || (size_t)off3 + (size_t)PR_T3 * n3 > rec.size() || (size_t)offk + 2 * (size_t)K4 > rec.size()) return 3;
+		if (offp % 4 || (size_t)offp + (size_t)PR_PIECE * npiece != rec.size()) return 3;
+		if (R16 > PR_RMAX || K4 > PR_KMAX || R16 % 16 || K4 % 4 || (size_t)wdo + (size_t)R16 * K4 > wd.size()) return 4;
+		if (npiece > 4 * NB || tc < 1 || tc > PR_ZERO || S1 < tc || S1 % tc) return 5;
+		const uint32_t* kt = g + offk;
+		const uint32_t* pieces = g + offp;
+		std::fill(sw.begin(), sw.end(), 0);
+		auto lw = [&](int a, double v) { if (a < 0 || a >= LDS) { bad = true; return; } sm[(size_t)a] = v; sw[(size_t)a] = 1; };
+		auto lr = [&](int a) -> double { if (a < 0 || a >= LDS || !sw[(size_t)a]) { bad = true; return 0.0; } return sm[(size_t)a]; };
+		for (int i = 0; i < PR_ZERO; i++) lw(i, 0.0);
+		// staging
+		for (int wave = 0; wave < 4; wave++)
+			for (int i = 0; i < NB; i++) {
+				const int pc = wave + 4 * i;
+				if (pc >= npiece) continue;
+				const uint32_t* d = pieces + PR_PIECE * pc;
+				const int64_t src = (int64_t)((uint64_t)d[0] | ((uint64_t)d[1] << 32));
+				const int d0 = (int)d[2], kind = (int)(d[3] >> 30), e0 = (int)(d[3] & 0xffffu);
+				const int n = (int)(d[4] & 0xffffu), vrows = (int)(d[4] >> 16);
+				const int cols = (int)(d[5] & 0xffffu), dstride = (int)(d[5] >> 16);
+				const int sstride = (int)(d[6] & 0xffffu), vcols = (int)(d[6] >> 16);
+				if (kind == 3 || cols < 1) return 6;
+				for (int lane = 0; lane < 64; lane++) {
+					const int e = e0 + lane;
+					if (e >= n) continue;
+					const int row = dv(e, cols, d[7]), col = e - row * cols;
+					const int dd = d0 + row * dstride + col;
+					if (dd < PR_ZERO || dd >= LDS || sw[(size_t)dd]) return 8;
+					double v = 0.0;
+					if (row < vrows && col < vcols) {
+						const int64_t sidx = src + (int64_t)row * sstride + col;
+						const double* base = kind == 0 ? x : (kind == 1 ? r : l);
+						const int64_t nb = kind == 0 ? nx : (kind == 1 ? nr : nl);
+						if (sidx < 0 || sidx >= nb) return 7;
+						v = base[sidx];
+					}
+					lw(dd, v);
+				}
+			}
+		// stage 1
+		std::vector<uint8_t> t1w((size_t)LDS, 0);
+		for (int t = 0; t < n1; t++) {
+			const uint32_t* td = g + off1 + PR_T1 * t;
+			const int abase = (int)td[0], sth = (int)td[1], bbase = (int)td[2], Nr = (int)td[3], nk = (int)td[4];
+			const int vr = (int)(td[5] & 0xffffu), vc = (int)(td[5] >> 16), sbase = (int)td[6];
+			if (vr < 1 || vr > 16 || vc < 1 || vc > 16) return 9;
+			for (int i = 0; i < 16; i++)
+				for (int j = 0; j < 16; j++) {
+					const int ar = i < vr ? abase + i * sth : abase, bcol = j < vc ? bbase + j : bbase;
+					double acc = 0.0;
+					for (int k = 0; k < 4 * nk; k++) acc += lr(ar + k) * lr(bcol + k * Nr);
+					if (i < vr && j < vc) {
+						const int a = sbase + i * Nr + j;
+						if (a < PR_ZERO || a >= t2 || t1w[(size_t)a]++) return 10;
+						lw(a, acc);
+					}
+				}
+		}
+		// the theta / R panels are dead after stage 1; T2 overlays them
+		for (int a = t2; a < LDS; a++) sw[(size_t)a] = 0;
+		// stage 2
+		const int nt2 = (S1 + 15) >> 4;
+		for (int mi = 0; mi < R16 / 16; mi++)
+			for (int ni = 0; ni < nt2; ni++)
+				for (int jl = 0; jl < 16; jl++) {
+					const int j2 = 16 * ni + jl;
+					const bool vj = j2 < S1;
+					const int a = vj ? dv(j2, tc, magtc) : 0, c = vj ? j2 - a * tc : 0;
+					double b[PR_KMAX];
+					for (int k = 0; k < K4; k++) b[k] = lr((int)kt[2 * k] + a * (int)kt[2 * k + 1] + c);
+					if (!vj) continue;
+					for (int il = 0; il < 16; il++) {
+						const int i2 = 16 * mi + il;
+						double acc = 0.0;
+						for (int k = 0; k < K4; k++) acc += wd[(size_t)(wdo + i2 * K4 + k)] * b[k];
+						const int addr = t2 + i2 * S1 + j2;
+						if (addr >= LDS || sw[(size_t)addr]) return 11;
+						lw(addr, acc);
+					}
+				}
+		// stage 3
+		for (int t = 0; t < n3; t++) {
+			const uint32_t* td = g + off3 + PR_T3 * t;
+			const int lbase = (int)td[0], dX = (int)td[1], t2b = (int)td[2], dQS1 = (int)td[3], dM = (int)td[4];
+			const int K3 = (int)td[6], nk3 = (int)td[7], vr = (int)(td[8] & 0xffffu), vc = (int)(td[8] >> 16), j0 = (int)td[9];
+			const int tc3 = (int)(td[10] & 0xffffu), dC = (int)(td[10] >> 16), dQdC = (int)td[14];
+			const int64_t pbase = (int64_t)((uint64_t)td[12] | ((uint64_t)td[13] << 32));
+			if (vr < 1 || vr > 16 || vc < 1 || vc > 16 || nk3 * 4 < K3) return 12;
+			for (int i = 0; i < 16; i++)
+				for (int j = 0; j < 16; j++) {
+					const int ax = i < vr ? lbase + i : lbase;
+					const bool vj = j < vc;
+					const int j3 = j0 + j;
+					const int q = vj ? dv(j3, tc3, td[11]) : 0, c = vj ? j3 - q * tc3 : 0;
+					const int jofs = t2b + q * S1 + c;
+					double acc = 0.0;
+					for (int k = 0; k < 4 * nk3; k++) {
+						const int a = dv(k, dM, td[5]), m = k - a * dM;
+						const bool vk = k < K3;
+						acc += lr(vk ? ax + k * dX : 0) * lr(vk ? jofs + m * dQS1 + a * tc3 : 0);
+					}
+					if (i < vr && vj) {
+						const int64_t o = pbase + (int64_t)q * dC + c + (int64_t)i * dQdC;
+						if (o < 0 || o >= h.part_elems || pwr[(size_t)o]++) return 13;
+						part[(size_t)o] = acc;
+					}
+				}
+		}
+		if (bad) return 17;
+	}
+	for (size_t i = 0; i < pwr.size(); i++) if (pwr[i] != 1) return 18;
+	std::vector<uint8_t> yw((size_t)nx, 0);
+	for (const PrRed& it : h.red) {
+		if (it.n < 1 || it.n > 64 || it.y < 0 || it.y + it.n > nx) return 19;
+		if (it.nslots > 0 && (it.p < 0 || it.p + (int64_t)(it.nslots - 1) * it.size + it.n > h.part_elems)) return 20;
+		for (int e = 0; e < it.n; e++) {
+			double acc = 0.0;
+			for (int s = 0; s < it.nslots; s++) acc += part[(size_t)(it.p + (int64_t)s * it.size + e)];
+			if (yw[(size_t)(it.y + e)]++) return 21;
+			if (y) y[it.y + e] = acc;
+		}
+	}
+	for (size_t i = 0; i < yw.size(); i++) if (yw[i] != 1) return 22;
+	return 0;
+}
+
+template <typename T>
+static T* pr_upload(const std::vector<T>& v, hipStream_t s)
+{
+	T* d = (T*)dev_alloc(std::max<size_t>(1, v.size()) * sizeof(T));
+	copy_h2d_async(d, v.data(), v.size() * sizeof(T), s);
+	return d;
+}
+
+__global__ __launch_bounds__(256) void k_pair_gather_w(const double* __restrict__ w, const int64_t* __restrict__ idx, int64_t n, double* __restrict__ wd)
+{
+	for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+		const int64_t q = idx[i];
+		wd[i] = q >= 0 ? w[q] : 0.0;
+	}
+}
+
+void PairPlan::upload(const PairHost& h, const double* w, hipStream_t s)
+{
+	release();
+	nunits = (int)h.recs.size();
+	rec_words = std::max(PR_HDR, (h.max_words + 3) / 4 * 4);
+	// units in descending cost order: the largest start first
+	std::vector<size_t> ord(h.recs.size());
+	std::iota(ord.begin(), ord.end(), (size_t)0);
+	std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return h.cost[a] > h.cost[b]; });
+	std::vector<uint32_t> flat((size_t)nunits * (size_t)rec_words, 0u);
+	for (size_t u = 0; u < ord.size(); u++) {
+		const auto& r = h.recs[ord[u]];
+		std::copy(r.begin(), r.end(), flat.begin() + (ptrdiff_t)(u * (size_t)rec_words));
+	}
+	recs = pr_upload(flat, s);
+	red = pr_upload(h.red, s);
+	nred = (int)h.red.size();
+	part_elems = h.part_elems;
+	part = (double*)dev_alloc((size_t)std::max<int64_t>(1, part_elems) * sizeof(double));
+	const int64_t n = (int64_t)h.wd_idx.size();
+	wd = (double*)dev_alloc((size_t)std::max<int64_t>(1, n) * sizeof(double));
+	if (n > 0) {
+		int64_t* idx = pr_upload(h.wd_idx, s);
+		hipLaunchKernelGGL(k_pair_gather_w, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1024)), dim3(256), 0, s, w, (const int64_t*)idx, n, wd);
+		CTG_HIP(hipGetLastError());
+		dev_free(idx);   // reused only behind the gather kernel on s
+	}
+	flops = h.flops;
+	max_lds = h.max_lds;
+	lds = h.lds;
+}
+
+void PairPlan::release()
+{
+	dev_free(recs); dev_free(red); dev_free(part); dev_free(wd);
+	recs = nullptr; red = nullptr; part = nullptr; wd = nullptr;
+	nunits = 0; nred = 0;
+}
+
+// LDS budget (doubles) of a unit: 2048, 3072 or 4608 (CT_PAIR_LDS overrides)
+int pair_default_lds()
+{
+	const char* e = getenv("CT_PAIR_LDS");
+	const int v = e ? atoi(e) : 0;
+	return (v == 2048 || v == 3072 || v == 4608) ? v : PR_LDS;
+}
+
+// MFMAs per unit above which a pair is cut into chunks (CT_PAIR_CAP overrides)
+int pair_default_cap()
+{
+	const char* e = getenv("CT_PAIR_CAP");
+	const int v = e ? atoi(e) : 0;
+	return v > 0 ? v : 96;
+}
+
+bool build_pair_plan(PairPlan& pp, const Meta& th, const Meta& W, const Meta& L, const Meta& R, const double* w, hipStream_t s)
+{
+	PairHost h;
+	h.mfma_cap = pair_default_cap();
+	h.lds = pair_default_lds();
+	if (!plan_pair_tables(h, th, W, L, R, 0, 0, 0, 0)) return false;
+	pp.upload(h, w, s);
+	return true;
+}
+
+bool build_pair_batch(PairPlan& pp, const std::vector<FusedProblem>& probs, const double* w, hipStream_t s)
+{
+	PairHost h;
+	h.mfma_cap = pair_default_cap();
+	h.lds = pair_default_lds();
+	for (const auto& pb : probs)
+		if (!plan_pair_tables(h, *pb.theta, *pb.w, *pb.l, *pb.r, pb.x_off, pb.w_off, pb.l_off, pb.r_off)) return false;
+	pp.upload(h, w, s);
+	return true;
+}
+
+void launch_pair_units(const PairPlan& pp, const double* x, const double* r, const double* l, hipStream_t s)
+{
+	if (pp.nunits <= 0) return;
+	const dim3 grid((unsigned)pp.nunits), block(PR_THREADS);
+	const uint32_t* recs = pp.recs;
+	const double* wd = pp.wd;
+	switch (pp.lds) {
+	case 2048: hipLaunchKernelGGL(k_pair_units<2048>, grid, block, 0, s, recs, pp.rec_words, x, r, l, wd, pp.part); break;
+	case 3072: hipLaunchKernelGGL(k_pair_units<3072>, grid, block, 0, s, recs, pp.rec_words, x, r, l, wd, pp.part); break;
+	case 4608: hipLaunchKernelGGL(k_pair_units<4608>, grid, block, 0, s, recs, pp.rec_words, x, r, l, wd, pp.part); break;
+	default: throw HipError{ hipErrorInvalidValue, "sector-pair units: unsupported LDS budget" };
+	}
+	CTG_HIP(hipGetLastError());
+}
+
+void launch_pair_reduce(const PairPlan& pp, double* y, hipStream_t s)
+{
+	if (pp.nred > 0) {
+		hipLaunchKernelGGL(k_pair_reduce, dim3((unsigned)((pp.nred + 3) / 4)), dim3(256), 0, s, (const PrRed*)pp.red, pp.nred, (const double*)pp.part, y);
+		CTG_HIP(hipGetLastError());
+	}
+}
+
+void launch_pair_matvec(const PairPlan& pp, const double* x, const double* r, const double* l, double* y, hipStream_t s)
+{
+	launch_pair_units(pp, x, r, l, s);
+	launch_pair_reduce(pp, y, s);
+}
+
+}  // namespace ctg

@@ -158,6 +158,7 @@ enum ct_matvec_mode
 	CT_MATVEC_GROUPED_FP64 = 1,  ///< three persistent grouped FP64 MFMA launches, transposes folded into addressing
 	CT_MATVEC_GROUPED_SPLIT = 2, ///< as GROUPED_FP64, operands split into FP32 terms on FP32 MFMA with FP64 accumulation
 	CT_MATVEC_FUSED_FP64   = 3,  ///< one launch per matvec: intermediates of a right-bond column chunk stay in LDS (FP64)
+	CT_MATVEC_PAIR_FP64    = 4,  ///< one launch for the three contractions of every (left sector, right sector) block pair in LDS (FP64 MFMA) + one partial-sum launch
 };
 
 /// Build a plan for H_eff = apply_local_hamiltonian(., w, l, r) on vectors with the block
@@ -171,6 +172,18 @@ int ct_matvec_plan_destroy(ct_matvec_plan* plan);
 /// Returns 0 when valid, a positive check number otherwise, or a CT_ERR_* code.
 int ct_matvec_plan_check(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l,
 	const struct block_sparse_tensor* r);
+/// Host-only replay of the sector-pair tables (CT_MATVEC_PAIR_FP64) with the kernels' index
+/// arithmetic and tile order (no device needed): operand, LDS and partial indices in bounds, LDS
+/// operands read only after they were written, every intermediate, partial and result entry written
+/// exactly once. mfma_cap (<= 0: the default) bounds the MFMAs of one unit. y (optional, length
+/// block_sparse_tensor_num_elements_blocks(a)) receives H_eff a as computed by the replay.
+/// Returns 0 when valid, a positive check number otherwise, or a CT_ERR_* code.
+int ct_matvec_pair_check(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l,
+	const struct block_sparse_tensor* r, const int mfma_cap, double* y);
+/// Host-only statistics of the sector-pair tables: [0] units, [1] partial-block doubles, [2] sum and
+/// [3] largest unit cost (MFMAs + staging), [4] largest unit LDS (doubles), [5] largest record (words)
+int ct_matvec_pair_stats(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l,
+	const struct block_sparse_tensor* r, const int mfma_cap, double* stats6);
 /// SVD of one dense matrix a (m x n, row-major, host memory) on the device, with the kernel that
 /// factorises the blocks of the DMRG split (one-sided Jacobi in the LDS of one workgroup, the
 /// device counterpart of the block loop of block_sparse_tensor_svd, block_sparse_tensor.c:2686-2848):

@@ -18,8 +18,9 @@ HIP_SOURCES = ["kernels.hip", "fused.hip", "pair.hip", "runtime.hip", "engine.hi
 CPP_SOURCES = ["host_tensor.cpp", "host_linalg.cpp", "host_models.cpp", "host_molecular.cpp", "host_bond.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
-# gfx950 device code; packed FP32 instructions are disabled for all HIP code
-HIP_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# gfx950 device code; packed FP32 instructions are disabled for all HIP code; MFMA accumulators in
+# VGPRs (the AGPR form copies the accumulator VGPR <-> AGPR around every MFMA of a k-loop)
+HIP_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form"]
 # host-only sources (block bookkeeping, host linear algebra): AVX2 + FMA servers
 HOST_FLAGS = ["-march=x86-64-v3"]
 

@@ -217,7 +217,7 @@ int eigensystem_krylov_symmetric(const ct_long n, lanczos_linear_func_d afunc, c
 			afunc(n, adata, hx.data(), hy.data());
 			CTG_HIP(hipMemcpyAsync(y, hy.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
 		};
-		const int ret = lanczos_eigensystem(lw, n, s, mv, dv.get(), maxiter, numeig, lambda, du.get());
+		const int ret = lanczos_eigensystem(lw, n, s, mv, dv.get(), maxiter, numeig, lambda, du.get(), false, true);
 		if (ret != 0) return ret;
 		CTG_HIP(hipMemcpyAsync(u_ritz, du.get(), (size_t)n * numeig * sizeof(double), hipMemcpyDeviceToHost, s));
 		CTG_HIP(hipStreamSynchronize(s));

@@ -452,9 +452,10 @@ using DevMatvec = std::function<void(const double* x, double* y)>;
 // numeig) or -2 (tridiagonal eigensolver failure).
 // With use_graph, the recurrence (maxiter matvecs and the vector updates) is captured once into a
 // HIP graph held by lw and replayed on later calls with the same maxiter; matvec must then only
-// enqueue kernels on s (no host synchronisation or copies).
+// enqueue kernels on s (no host synchronisation or copies). With sync_stop (eager only) the host
+// reads the breakdown flag before each matvec and stops there, so matvec runs numiter times.
 int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart,
-	int maxiter, int numeig, double* lambda, double* u, bool use_graph = false);
+	int maxiter, int numeig, double* lambda, double* u, bool use_graph = false, bool sync_stop = false);
 
 struct MatvecPlan
 {

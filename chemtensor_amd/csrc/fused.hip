@@ -23,6 +23,7 @@
 #include <numeric>
 #include <tuple>
 
+#include <mutex>
 #include "ct_internal.h"
 
 namespace ctg {
@@ -504,11 +505,13 @@ void launch_fused_matvec(const FusedPlan& f, const double* x, const double* w, c
 	if (f.grid == 0) return;
 	FzArgs a = { f.sectors, f.units, f.order, f.map, f.copies, f.t1, f.t2, f.u2, f.t3, f.u3, x, w, l, r, y };
 	if (f.lds_bytes > 64 * 1024) {
-		static size_t granted = 64 * 1024;
-		if (f.lds_bytes > granted) {
-			CTG_HIP(hipFuncSetAttribute((const void*)k_fused_matvec, hipFuncAttributeMaxDynamicSharedMemorySize, (int)f.lds_bytes));
-			granted = f.lds_bytes;
-		}
+		// the kernel is allowed the device's whole LDS once, before any launch above 64 KB (plans of
+		// several host threads launch concurrently; no thread can be left with a smaller grant)
+		static std::once_flag granted;
+		std::call_once(granted, [] {
+			CTG_HIP(hipFuncSetAttribute((const void*)k_fused_matvec, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit()));
+		});
+		if (f.lds_bytes > lds_limit()) throw HipError{ hipErrorInvalidValue, "fused matvec: LDS request above the device limit" };
 	}
 	hipLaunchKernelGGL(k_fused_matvec, dim3(f.grid), dim3(FZ_THREADS), f.lds_bytes, s, a);
 	CTG_HIP(hipGetLastError());

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cfloat>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -39,6 +40,9 @@ static inline double hypot_fast(double a, double b)
 {
 	a = std::fabs(a);
 	b = std::fabs(b);
+	// NaN and infinite input propagate (NaN stays NaN, an infinity gives infinity) instead of being
+	// dropped by max / min or turned into inf / inf
+	if (!(a <= DBL_MAX) || !(b <= DBL_MAX)) return a + b;
 	const double m = std::max(a, b), n = std::min(a, b);
 	if (m == 0) return 0;
 	const double r = n / m;

@@ -150,8 +150,12 @@ LanczosWork::~LanczosWork()
 	dev_free(vs);
 }
 
-// v0 = vstart / |vstart|, then maxiter steps of the recurrence; only kernel launches on s
-static void enqueue_recurrence(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart, int maxiter)
+// v0 = vstart / |vstart|, then maxiter steps of the recurrence; only kernel launches on s. With
+// sync_stop (a matvec that synchronises anyway, such as a host callback) the stop flag is read back
+// before every matvec after the first, and the loop ends at a breakdown: the matvec then runs exactly
+// numiter times, on normalised Krylov vectors only (krylov.c:34-68 calls afunc numiter times).
+static void enqueue_recurrence(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart, int maxiter,
+	bool sync_stop = false)
 {
 	const int G = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 511) / 512));
 	double* V = lw.v;
@@ -166,6 +170,12 @@ static void enqueue_recurrence(LanczosWork& lw, int64_t n, hipStream_t s, const 
 	CTG_HIP(hipGetLastError());
 	for (int j = 0; j < maxiter; j++) {
 		const double* vj = V + (int64_t)j * n;
+		if (sync_stop && j > 0) {
+			double stop = 0;
+			CTG_HIP(hipMemcpyAsync(&stop, scal + 2 * maxiter + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+			CTG_HIP(hipStreamSynchronize(s));
+			if (stop != 0) break;
+		}
 		matvec(vj, w);
 		hipLaunchKernelGGL(k_lz_dot_partial, dim3(G), dim3(LZ_THREADS), 0, s, w, vj, n, part, scal + 2 * maxiter + 1);
 		hipLaunchKernelGGL(k_lz_alpha_update, dim3(G), dim3(LZ_THREADS), 0, s, w, vj, j > 0 ? vj - n : vj, n, j, maxiter, part, G, part2, scal);
@@ -176,7 +186,7 @@ static void enqueue_recurrence(LanczosWork& lw, int64_t n, hipStream_t s, const 
 }
 
 int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart,
-	int maxiter, int numeig, double* lambda, double* u, bool use_graph)
+	int maxiter, int numeig, double* lambda, double* u, bool use_graph, bool sync_stop)
 {
 	if (maxiter < 1 || numeig < 1 || numeig > maxiter || n < 1) return CT_ERR_ARG;
 	lw.reserve(n, maxiter, numeig);
@@ -204,7 +214,7 @@ int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatv
 		}
 		CTG_HIP(hipGraphLaunch(lw.graph, s));
 	}
-	else enqueue_recurrence(lw, n, s, matvec, vstart, maxiter);
+	else enqueue_recurrence(lw, n, s, matvec, vstart, maxiter, sync_stop);
 	double* V = lw.v;
 	std::vector<double> hs((size_t)(2 * maxiter + 2));
 	CTG_HIP(hipMemcpyAsync(hs.data(), lw.scal, hs.size() * sizeof(double), hipMemcpyDeviceToHost, s));

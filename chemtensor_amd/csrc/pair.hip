@@ -23,6 +23,7 @@
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <numeric>
 #include "ct_internal.h"
@@ -32,27 +33,45 @@ namespace ctg {
 typedef double pr_double4 __attribute__((ext_vector_type(4)));
 
 // Unit record (4-byte words, fixed stride per plan):
-//   header (16): n1 n3 npiece R16 | K4 S1 tc mag(tc) | t2 wd off1 off3 | offk offp pad pad
-//   stage-1 tiles at off1 (8 words each): abase sth bbase Nr | nk vr|vc<<16 sbase pad
-//   stage-3 tiles at off3 (16 words each): lbase dX t2b dQ*S1 | dM mag(dM) K3 nk3 |
-//                                          vr|vc<<16 j0 tc|dC<<16 mag(tc) | pbase(2) dQ*dC pad
-//   stage-2 row map at offk (2 words per T1 row k < K4): LDS offset of row k at a = 0, c = 0, and
-//                                          its stride in a (0, 0 for absent rows: the zero region)
-//   pieces at offp (8 words each): src(2) d0 e0|kind<<30 | n|vrows<<16 cols|dstride<<16
-//                                  sstride|vcols<<16 mag(cols)
-// A piece covers entries e0 .. e0+63 of a staged panel of n entries laid out as rows of 'cols'
-// entries: entry (row, col) goes to LDS d0 + row dstride + col and holds
-// src[row sstride + col] (kind 0: theta, 1: R, 2: L) when row < vrows and col < vcols, else 0.
-// mag(d) = ceil(2^31 / d): for 0 <= t < 2^31 / d, t / d = mulhi(2 t, mag(d)).
-constexpr int PR_HDR = 16, PR_T1 = 8, PR_T3 = 16, PR_PIECE = 8;
+//   header (16): n1 n3 npiece R16 | K4 S1 tc mag(tc) | t2 pad off1 off3 | offk offp wb kb
+//   (t2, wb, kb: LDS offsets of T2, the class's W_D copy, the stage-2 row map)
+//   stage-1 tiles at off1 (8 words each): abase sth bbase Nr nk vr vc sbase
+//   stage-3 tiles at off3 (16 words each): lbase dX t2b dQ*S1 dM ta nka vr vc j0 tc dC mag(tc)
+//                                          pbase(2) dQ*dC
+//   stage-2 row map at offk (even; 2 words per T1 row k < K4): LDS offset of row k at a = 0, c = 0,
+//                                          and its stride in a (0, 0 for absent rows: the zero region)
+//   pieces at offp (4 words each), 64 consecutive entries e = e0 .. e0+63 of a staged panel whose
+//   entry (row, col) = divmod(e, cols) goes to LDS d0 + row dstride + col from source
+//   s0 + row sstride + col (or is zero): with (row0, col0) = divmod(e0, cols),
+//     w0 = s0 + row0 sstride    w1 = (d0 + row0 dstride) | sstride << 16
+//     w2 = dstride | cols << 8 | col0 << 16 | lanes << 24
+//     w3 = mag16(cols) | vcols << 17 | kind << 25
+//   lane l < lanes: t = col0 + l, row - row0 = t / cols (= t mag16 >> 16), col = t - (row - row0) cols;
+//   col < vcols loads, else writes 0. kind: 0 theta (x), 1 R, 2 L, 3 zeros, 4 the W_D table, 5 the
+//   unit's own record read as doubles (the row map).
+// mag(d) = ceil(2^31 / d): t / d = mulhi(2 t, mag(d)) for 0 <= t < 2^31 / d; mag16(d) = ceil(2^16 / d):
+// t / d = (t mag16(d)) >> 16 for t (t + 1) < 2^16 ... exact whenever t cols < 2^16.
+constexpr int PR_HDR = 16, PR_T1 = 8, PR_T3 = 16, PR_PIECE = 4;
 constexpr int PR_ZERO = 64;   // LDS doubles [0, 64) hold zeros
+constexpr int PR_DUMP = 64;   // LDS double taking the stores of lanes outside a tile (never read)
+constexpr int PR_WB = 80;     // LDS offset of the W_D copy
 
 __device__ __forceinline__ int pr_q(int t, uint32_t mag) { return (int)__umulhi((uint32_t)t << 1, mag); }
 __device__ __forceinline__ int pr_s(uint32_t v) { return __builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ int pr_lane(uint32_t v, int l) { return __builtin_amdgcn_readlane((int)v, l); }
+// identity DPP moves give a loaded descriptor registers of their own: the readlanes of it then wait for
+// that load only, not for loads issued later in branches the wait counter cannot count
+__device__ __forceinline__ void pr_fresh(uint4& d)
+{
+	d.x = (uint32_t)__builtin_amdgcn_mov_dpp((int)d.x, 0xE4, 0xF, 0xF, false);
+	d.y = (uint32_t)__builtin_amdgcn_mov_dpp((int)d.y, 0xE4, 0xF, 0xF, false);
+	d.z = (uint32_t)__builtin_amdgcn_mov_dpp((int)d.z, 0xE4, 0xF, 0xF, false);
+	d.w = (uint32_t)__builtin_amdgcn_mov_dpp((int)d.w, 0xE4, 0xF, 0xF, false);
+}
 
 template <int LDS>
 __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __restrict__ recs, const int rec_words, const double* __restrict__ x,
-	const double* __restrict__ rr, const double* __restrict__ ll, const double* __restrict__ wd, double* __restrict__ part)
+	const double* __restrict__ rr, const double* __restrict__ ll, const double* __restrict__ wd, double* __restrict__ part, double* __restrict__ dump)
 {
 	constexpr int NB = LDS / PR_THREADS;   // staging pieces per wavefront
 	static_assert(NB <= 64, "one piece descriptor per lane");
@@ -65,35 +84,33 @@ __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __res
 	const int n1 = pr_s(g[0]), n3 = pr_s(g[1]), npiece = pr_s(g[2]), R16 = pr_s(g[3]);
 	const int K4 = pr_s(g[4]), S1 = pr_s(g[5]), tc = pr_s(g[6]);
 	const uint32_t magtc = (uint32_t)pr_s(g[7]);
-	const int t2 = pr_s(g[8]), wdo = pr_s(g[9]), off1 = pr_s(g[10]), off3 = pr_s(g[11]);
-	const int offk = pr_s(g[12]), offp = pr_s(g[13]);
+	const int t2 = pr_s(g[8]), off1 = pr_s(g[10]), off3 = pr_s(g[11]);
+	const int offp = pr_s(g[13]), wb = pr_s(g[14]), kb = pr_s(g[15]);
 
 	// lane j of wavefront w holds the descriptor of piece w + 4 j
-	uint4 pa = { 0u, 0u, 0u, 0u }, pb = { 0u, 0u, 0u, 0u };
-	if (lane < NB && wave + 4 * lane < npiece) {
-		const uint4* d = (const uint4*)(g + offp + PR_PIECE * (wave + 4 * lane));
-		pa = d[0];
-		pb = d[1];
+	uint4 pd = { 0u, 0u, 0u, 0u };
+	if (lane < NB && wave + 4 * lane < npiece) pd = reinterpret_cast<const uint4*>(g + offp)[wave + 4 * lane];
+	// the wavefront's stage-1 tiles w + 4 j and stage-3 tiles w + 4 j, one per lane, in the same round trip
+	uint4 d1a = { 0u, 0u, 0u, 0u }, d1b = d1a, d3a = d1a, d3b = d1a, d3c = d1a, d3d = d1a;
+	if (wave + 4 * lane < n1) {
+		const uint4* d = reinterpret_cast<const uint4*>(g + off1 + PR_T1 * (wave + 4 * lane));
+		d1a = d[0];
+		d1b = d[1];
 	}
-	// stage-2 row map: T1 row 4 ks + l / 16 of this lane
-	int kofs[PR_KMAX / 4], kstr[PR_KMAX / 4];
-	#pragma unroll
-	for (int ks = 0; ks < PR_KMAX / 4; ks++) {
-		kofs[ks] = 0;
-		kstr[ks] = 0;
-		if (4 * ks < K4) {
-			const uint2 e = *(const uint2*)(g + offk + 2 * (4 * ks + kq));
-			kofs[ks] = (int)e.x;
-			kstr[ks] = (int)e.y;
-		}
+	if (wave + 4 * lane < n3) {
+		const uint4* d = reinterpret_cast<const uint4*>(g + off3 + PR_T3 * (wave + 4 * lane));
+		d3a = d[0];
+		d3b = d[1];
+		d3c = d[2];
+		d3d = d[3];
 	}
-	// the class's W_D (R16 x K4, zero padded) as MFMA A fragments: row 16 mi + l % 16, column 4 ks + l / 16
-	double wa[PR_RMAX / 16][PR_KMAX / 4];
-	#pragma unroll
-	for (int mi = 0; mi < PR_RMAX / 16; mi++)
-		#pragma unroll
-		for (int ks = 0; ks < PR_KMAX / 4; ks++)
-			wa[mi][ks] = (16 * mi < R16 && 4 * ks < K4) ? wd[wdo + (16 * mi + i16) * K4 + 4 * ks + kq] : 0.0;
+	pr_fresh(pd);
+	pr_fresh(d1a);
+	pr_fresh(d1b);
+	pr_fresh(d3a);
+	pr_fresh(d3b);
+	pr_fresh(d3c);
+	pr_fresh(d3d);
 
 	// staging: every load of the wavefront's pieces is issued before the first LDS store
 	if (tid < PR_ZERO) sm[tid] = 0.0;
@@ -102,55 +119,65 @@ __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __res
 		int dst[NB];
 		#pragma unroll
 		for (int i = 0; i < NB; i++) {
-			dst[i] = -1;
+			dst[i] = PR_DUMP;
 			v[i] = 0.0;
 			if (wave + 4 * i < npiece) {
-				const uint32_t w0 = (uint32_t)__builtin_amdgcn_readlane((int)pa.x, i), w1 = (uint32_t)__builtin_amdgcn_readlane((int)pa.y, i);
-				const uint32_t w2 = (uint32_t)__builtin_amdgcn_readlane((int)pa.z, i), w3 = (uint32_t)__builtin_amdgcn_readlane((int)pa.w, i);
-				const uint32_t w4 = (uint32_t)__builtin_amdgcn_readlane((int)pb.x, i), w5 = (uint32_t)__builtin_amdgcn_readlane((int)pb.y, i);
-				const uint32_t w6 = (uint32_t)__builtin_amdgcn_readlane((int)pb.z, i), w7 = (uint32_t)__builtin_amdgcn_readlane((int)pb.w, i);
-				const int64_t src = (int64_t)((uint64_t)w0 | ((uint64_t)w1 << 32));
-				const int kind = (int)(w3 >> 30), e = (int)(w3 & 0xffffu) + lane;
-				const int n = (int)(w4 & 0xffffu), vrows = (int)(w4 >> 16);
-				const int cols = (int)(w5 & 0xffffu), dstride = (int)(w5 >> 16);
-				const int sstride = (int)(w6 & 0xffffu), vcols = (int)(w6 >> 16);
-				if (e < n) {
-					const int row = pr_q(e, w7), col = e - row * cols;
-					dst[i] = (int)w2 + row * dstride + col;
-					if (row < vrows && col < vcols) {
-						const double* __restrict__ base = (kind == 0 ? x : (kind == 1 ? rr : ll)) + src;
-						v[i] = base[row * sstride + col];
+				const uint32_t w0 = (uint32_t)__builtin_amdgcn_readlane((int)pd.x, i), w1 = (uint32_t)__builtin_amdgcn_readlane((int)pd.y, i);
+				const uint32_t w2 = (uint32_t)__builtin_amdgcn_readlane((int)pd.z, i), w3 = (uint32_t)__builtin_amdgcn_readlane((int)pd.w, i);
+				if (lane < (int)(w2 >> 24)) {
+					const int cols = (int)((w2 >> 8) & 0xffu);
+					const int t = (int)((w2 >> 16) & 0xffu) + lane;
+					const int q = __mul24(t, (int)(w3 & 0x1ffffu)) >> 16;
+					const int c = t - __mul24(q, cols);
+					dst[i] = (int)(w1 & 0xffffu) + __mul24(q, (int)(w2 & 0xffu)) + c;
+					const int kind = (int)(w3 >> 25);
+					if (kind != 3 && c < (int)((w3 >> 17) & 0xffu)) {
+						const double* __restrict__ base =
+							kind == 0 ? x : (kind == 1 ? rr : (kind == 2 ? ll : (kind == 4 ? wd : reinterpret_cast<const double*>(g))));
+						v[i] = base[w0 + (uint32_t)(__mul24(q, (int)(w1 >> 16)) + c)];
 					}
 				}
 			}
 		}
 		#pragma unroll
 		for (int i = 0; i < NB; i++)
-			if (dst[i] >= 0) sm[dst[i]] = v[i];
+			if (wave + 4 * i < npiece) sm[dst[i]] = v[i];
 	}
 	__syncthreads();
 
 	// stage 1: T1 of each route in its own layout [(a p)][(n c)] = theta[(a p), b] R[b, (n c)]
-	for (int t = wave; t < n1; t += 4) {
-		const uint32_t* __restrict__ td = g + off1 + PR_T1 * t;
-		const int abase = pr_s(td[0]), sth = pr_s(td[1]), bbase = pr_s(td[2]), Nr = pr_s(td[3]);
-		const int nk = pr_s(td[4]), vrc = pr_s(td[5]), sbase = pr_s(td[6]);
-		const int vr = vrc & 0xffff, vc = vrc >> 16;
-		const int ar = (i16 < vr ? abase + i16 * sth : abase) + kq;
-		const int bc = (i16 < vc ? bbase + i16 : bbase) + kq * Nr;
+	for (int t = wave, tl = 0; t < n1; t += 4, tl++) {
+		const int abase = pr_lane(d1a.x, tl), sth = pr_lane(d1a.y, tl), bbase = pr_lane(d1a.z, tl), Nr = pr_lane(d1a.w, tl);
+		const int nk = pr_lane(d1b.x, tl), vr = pr_lane(d1b.y, tl), vc = pr_lane(d1b.z, tl), sbase = pr_lane(d1b.w, tl);
+		const int ar = (i16 < vr ? abase + __mul24(i16, sth) : abase) + kq;
+		const int bc = (i16 < vc ? bbase + i16 : bbase) + __mul24(kq, Nr);
 		pr_double4 acc = { 0, 0, 0, 0 };
 		for (int s = 0; s < nk; s++)
-			acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sm[ar + 4 * s], sm[bc + 4 * s * Nr], acc, 0, 0, 0);
-		if (i16 < vc) {
-			#pragma unroll
-			for (int r4 = 0; r4 < 4; r4++)
-				if (kq + 4 * r4 < vr) sm[sbase + (kq + 4 * r4) * Nr + i16] = acc[r4];
+			acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sm[ar + 4 * s], sm[bc + __mul24(4 * s, Nr)], acc, 0, 0, 0);
+		#pragma unroll
+		for (int r4 = 0; r4 < 4; r4++) {
+			const int i = kq + 4 * r4;
+			sm[(i16 < vc && i < vr) ? sbase + __mul24(i, Nr) + i16 : PR_DUMP] = acc[r4];
 		}
 	}
 	__syncthreads();
 
 	// stage 2: T2[(M Q m q)][(a c)] = W_D T1 (rows of the row map, columns a tc + c)
 	{
+		// the class's W_D (R16 x K4 at row stride K4 + 1, zero padded) as MFMA A fragments (row
+		// 16 mi + l % 16, column 4 ks + l / 16) and the row map of this lane's T1 rows 4 ks + l / 16
+		double wa[PR_RMAX / 16][PR_KMAX / 4];
+		int kofs[PR_KMAX / 4], kstr[PR_KMAX / 4];
+		const int2* __restrict__ kmap = reinterpret_cast<const int2*>(sm + kb);
+		#pragma unroll
+		for (int ks = 0; ks < PR_KMAX / 4; ks++) {
+			const bool vk = 4 * ks < K4;
+			const int2 e = vk ? kmap[4 * ks + kq] : int2{ 0, 0 };
+			kofs[ks] = e.x;
+			kstr[ks] = e.y;
+			#pragma unroll
+			for (int mi = 0; mi < PR_RMAX / 16; mi++) wa[mi][ks] = (vk && 16 * mi < R16) ? sm[wb + (16 * mi + i16) * (K4 + 1) + 4 * ks + kq] : 0.0;
+		}
 		const int nt2 = (S1 + 15) >> 4;
 		#pragma unroll
 		for (int mi = 0; mi < PR_RMAX / 16; mi++) {
@@ -159,49 +186,55 @@ __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __res
 				const int j2 = 16 * ni + i16;
 				const bool vj = j2 < S1;
 				const int a = vj ? pr_q(j2, magtc) : 0, c = vj ? j2 - a * tc : 0;
+				// rows k >= K4 read the zero region (row map 0, 0) against zero W_D columns
 				pr_double4 acc = { 0, 0, 0, 0 };
 				#pragma unroll
-				for (int ks = 0; ks < PR_KMAX / 4; ks++)
-					if (4 * ks < K4) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[mi][ks], sm[kofs[ks] + a * kstr[ks] + c], acc, 0, 0, 0);
-				if (vj) {
+				for (int h = 0; h < PR_KMAX / 16; h++) {
+					if (16 * h >= K4) break;
+					double b[4];
 					#pragma unroll
-					for (int r4 = 0; r4 < 4; r4++) sm[t2 + (16 * mi + kq + 4 * r4) * S1 + j2] = acc[r4];
+					for (int u = 0; u < 4; u++) b[u] = sm[kofs[4 * h + u] + __mul24(a, kstr[4 * h + u]) + c];
+					#pragma unroll
+					for (int u = 0; u < 4; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[mi][4 * h + u], b[u], acc, 0, 0, 0);
 				}
+				#pragma unroll
+				for (int r4 = 0; r4 < 4; r4++) sm[vj ? t2 + __mul24(16 * mi + kq + 4 * r4, S1) + j2 : PR_DUMP] = acc[r4];
 			}
 		}
 	}
 	__syncthreads();
 
 	// stage 3: partial y(X,Q,C)[x, (q c)] = L(o,A,M,X)[(a m), x]^T T2(A,M,Q,C)[(a m), (q c)]
-	for (int t = wave; t < n3; t += 4) {
-		const uint32_t* __restrict__ td = g + off3 + PR_T3 * t;
-		const int lbase = pr_s(td[0]), dX = pr_s(td[1]), t2b = pr_s(td[2]), dQS1 = pr_s(td[3]);
-		const int dM = pr_s(td[4]);
-		const uint32_t magm = (uint32_t)pr_s(td[5]);
-		const int K3 = pr_s(td[6]), nk3 = pr_s(td[7]), vrc = pr_s(td[8]), j0 = pr_s(td[9]);
-		const int tcdc = pr_s(td[10]);
-		const uint32_t mag3 = (uint32_t)pr_s(td[11]);
-		const int64_t pbase = (int64_t)((uint64_t)(uint32_t)pr_s(td[12]) | ((uint64_t)(uint32_t)pr_s(td[13]) << 32));
-		const int dQdC = pr_s(td[14]);
-		const int vr = vrc & 0xffff, vc = vrc >> 16, tc3 = tcdc & 0xffff, dC = tcdc >> 16;
+	for (int t = wave, tl = 0; t < n3; t += 4, tl++) {
+		const int lbase = pr_lane(d3a.x, tl), dX = pr_lane(d3a.y, tl), t2b = pr_lane(d3a.z, tl), dQS1 = pr_lane(d3a.w, tl);
+		const int dM = pr_lane(d3b.x, tl), ta = pr_lane(d3b.y, tl), nka = pr_lane(d3b.z, tl), vr = pr_lane(d3b.w, tl);
+		const int vc = pr_lane(d3c.x, tl), j0 = pr_lane(d3c.y, tl), tc3 = pr_lane(d3c.z, tl), dC = pr_lane(d3c.w, tl);
+		const uint32_t mag3 = (uint32_t)pr_lane(d3d.x, tl);
+		const int64_t pbase = (int64_t)((uint64_t)(uint32_t)pr_lane(d3d.y, tl) | ((uint64_t)(uint32_t)pr_lane(d3d.z, tl) << 32));
+		const int dQdC = pr_lane(d3d.w, tl);
 		const int ax = (i16 < vr ? lbase + i16 : lbase);
 		const bool vj = i16 < vc;
 		const int j3 = j0 + i16;
 		const int q = vj ? pr_q(j3, mag3) : 0, c = vj ? j3 - q * tc3 : 0;
-		const int jofs = t2b + q * S1 + c;
+		const int jofs = t2b + __mul24(q, S1) + c;
+		// k runs over (m, a): L row (a dM + m) of the staged [a m x] panel, T2 row (m dQ + q) column
+		// a tc + c; lanes past the chunk's rows a read the zero region
 		pr_double4 acc = { 0, 0, 0, 0 };
-		for (int s = 0; s < nk3; s++) {
-			const int k = 4 * s + kq;
-			const int a = pr_q(k, magm), m = k - a * dM;
-			const bool vk = k < K3;
-			const int la = vk ? ax + k * dX : 0, lb = vk ? jofs + m * dQS1 + a * tc3 : 0;
-			acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sm[la], sm[lb], acc, 0, 0, 0);
+		const int lstep = __mul24(4 * dM, dX), bstep = 4 * tc3;
+		for (int m = 0; m < dM; m++) {
+			int la = ax + __mul24(kq * dM + m, dX), lb = jofs + __mul24(m, dQS1) + __mul24(kq, tc3);
+			for (int s = 0; s < nka; s++) {
+				const bool vk = 4 * s + kq < ta;
+				acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sm[vk ? la : 0], sm[vk ? lb : 0], acc, 0, 0, 0);
+				la += lstep;
+				lb += bstep;
+			}
 		}
 		if (vj) {
-			double* __restrict__ out = part + pbase + q * dC + c;
+			double* __restrict__ out = part + pbase + (uint32_t)(__mul24(q, dC) + c);
 			#pragma unroll
 			for (int r4 = 0; r4 < 4; r4++)
-				if (kq + 4 * r4 < vr) out[(kq + 4 * r4) * dQdC] = acc[r4];
+				if (kq + 4 * r4 < vr) out[(uint32_t)__mul24(kq + 4 * r4, dQdC)] = acc[r4];
 		}
 	}
 }
@@ -277,7 +310,7 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 		c.R16 = (c.Rd + 15) / 16 * 16;
 		c.K4 = up4(c.Kd);
 		c.wd = (int64_t)h.wd_idx.size();
-		h.wd_idx.resize(h.wd_idx.size() + (size_t)c.R16 * c.K4, -1);
+		h.wd_idx.resize(h.wd_idx.size() + (size_t)c.R16 * (c.K4 + 1), -1);
 		for (size_t ir = 0; ir < c.rows.size(); ir++) {
 			const ct_long M = c.rows[ir].first, Q = c.rows[ir].second;
 			const int dM = bc(W, 0, M), dQ = bc(W, 1, Q);
@@ -290,7 +323,7 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 				for (int m = 0; m < dM; m++) for (int q = 0; q < dQ; q++)
 					for (int pp = 0; pp < dP; pp++) for (int n = 0; n < dN; n++) {
 						const int row = c.rbase[ir] + m * dQ + q, col = c.kbase[ic] + pp * dN + n;
-						h.wd_idx[(size_t)(c.wd + (int64_t)row * c.K4 + col)] = w_off + W.offset[kw] + ((ct_long)(m * dQ + q) * dP + pp) * dN + n;
+						h.wd_idx[(size_t)(c.wd + (int64_t)row * (c.K4 + 1) + col)] = w_off + W.offset[kw] + ((ct_long)(m * dQ + q) * dP + pp) * dN + n;
 					}
 			}
 		}
@@ -335,30 +368,38 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 			if (vr.empty() || vq.empty()) continue;
 			// LDS need of a chunk (ta rows a, tc columns c) and its number of staging pieces
 			auto lds_need = [&](int ta, int tc) {
-				int l = PR_ZERO, t1 = 0, st = 0;
+				int l = PR_WB + cl.R16 * (cl.K4 + 1) + cl.K4, t1 = 0, st = 0;
 				for (auto& v : vq) l += ta * v.dM * v.dX;
 				for (auto& v : vr) { t1 += ta * v.dP * v.dN * tc; st += ta * v.dP * (up4(v.dB) + 1) + up4(v.dB) * v.dN * tc; }
 				return l + t1 + std::max(cl.R16 * ta * tc, st);
 			};
 			auto npieces = [&](int ta, int tc) {
-				int n = 0;
-				for (auto& v : vr) n += cdiv(ta * v.dP * up4(v.dB), 64) + cdiv(up4(v.dB) * v.dN * tc, 64);
+				int n = cdiv(cl.R16 * (cl.K4 + 1), 64) + cdiv(cl.K4, 64);
+				for (auto& v : vr) n += cdiv(ta * v.dP * up4(v.dB), 64) + cdiv(v.dB * v.dN * tc, 64) + cdiv((up4(v.dB) - v.dB) * v.dN * tc, 64);
 				for (auto& v : vq) n += cdiv(ta * v.dM * v.dX, 64);
 				return n;
+			};
+			// tiles of stages 1 and 3 (one descriptor per lane of a wavefront: at most 4 x 64 each)
+			auto tiles = [&](int ta, int tc) {
+				int n1 = 0, n3 = 0;
+				for (auto& v : vr) n1 += cdiv(ta * v.dP, 16) * cdiv(v.dN * tc, 16);
+				for (auto& v : vq) n3 += cdiv(v.dX, 16) * cdiv(v.dQ * tc, 16);
+				return std::max(n1, n3);
 			};
 			auto mfmas = [&](int ta, int tc) {
 				int n = 0;
 				for (auto& v : vr) n += cdiv(ta * v.dP, 16) * cdiv(v.dN * tc, 16) * (up4(v.dB) / 4);
 				n += (cl.R16 / 16) * cdiv(ta * tc, 16) * (cl.K4 / 4);
-				for (auto& v : vq) n += cdiv(v.dX, 16) * cdiv(v.dQ * tc, 16) * cdiv(ta * v.dM, 4);
+				for (auto& v : vq) n += cdiv(v.dX, 16) * cdiv(v.dQ * tc, 16) * v.dM * cdiv(ta, 4);
 				return n;
 			};
 			int ta = dA, tc = dC;
-			while (lds_need(ta, tc) > h.lds || npieces(ta, tc) > nbp || mfmas(ta, tc) > h.mfma_cap || tc > PR_ZERO) {
+			for (auto& v : vr) if (up4(v.dB) > 223 || v.dB > 0xffff || dC > 0xffff) return false;
+			while (lds_need(ta, tc) > h.lds || npieces(ta, tc) > nbp || mfmas(ta, tc) > h.mfma_cap || tc > PR_ZERO || tiles(ta, tc) > 4 * 64) {
 				if (tc > 4 && (tc >= ta || ta <= 4)) tc = (tc + 1) / 2;
 				else if (ta > 1) ta = (ta + 1) / 2;
 				else if (tc > 1) tc = (tc + 1) / 2;
-				else if (lds_need(ta, tc) > h.lds || npieces(ta, tc) > nbp) return false;
+				else if (lds_need(ta, tc) > h.lds || npieces(ta, tc) > nbp || tiles(ta, tc) > 4 * 64) return false;
 				else break;
 			}
 			for (int a0 = 0, ia = 0; a0 < dA; a0 += ta, ia++) {
@@ -366,8 +407,10 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 				for (int c0 = 0; c0 < dC; c0 += tc) {
 					const int tC = std::min(tc, dC - c0);
 					const int S1 = tA * tC;
-					// LDS layout: zeros | L rows | T1 of each route | T2 overlaid on the theta and R panels
-					int off = PR_ZERO;
+					// LDS layout: zeros, dump | W_D | row map | L rows | T1 of each route | T2 overlaid on
+					// the theta and R panels
+					const int lwb = PR_WB, lkb = lwb + cl.R16 * (cl.K4 + 1);
+					int off = lkb + cl.K4;
 					std::vector<int> lds_l(vq.size()), t1b(vr.size()), lth(vr.size()), lr(vr.size());
 					for (size_t i = 0; i < vq.size(); i++) { lds_l[i] = off; off += tA * vq[i].dM * vq[i].dX; }
 					for (size_t i = 0; i < vr.size(); i++) { t1b[i] = off; off += tA * vr[i].dP * vr[i].dN * tC; }
@@ -385,9 +428,9 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 						const int Mr = tA * vr[i].dP, Nr = vr[i].dN * tC, sth = up4(vr[i].dB) + 1;
 						for (int mi = 0; mi < cdiv(Mr, 16); mi++)
 							for (int ni = 0; ni < cdiv(Nr, 16); ni++) {
-								const int vrr = std::min(16, Mr - 16 * mi), vcc = std::min(16, Nr - 16 * ni);
 								const uint32_t d[PR_T1] = { (uint32_t)(lth[i] + 16 * mi * sth), (uint32_t)sth, (uint32_t)(lr[i] + 16 * ni), (uint32_t)Nr,
-									(uint32_t)(up4(vr[i].dB) / 4), (uint32_t)vrr | ((uint32_t)vcc << 16), (uint32_t)(t1b[i] + 16 * mi * Nr + 16 * ni), 0u };
+									(uint32_t)(up4(vr[i].dB) / 4), (uint32_t)std::min(16, Mr - 16 * mi), (uint32_t)std::min(16, Nr - 16 * ni),
+									(uint32_t)(t1b[i] + 16 * mi * Nr + 16 * ni) };
 								t1s.insert(t1s.end(), d, d + PR_T1);
 							}
 					}
@@ -400,51 +443,64 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 							kt[(size_t)(2 * k + 1)] = (uint32_t)(vr[i].dP * Nr);
 						}
 					}
-					// stage-3 tiles; words 12-13 hold the offset within the partial slot until patched
+					// stage-3 tiles; words 13-14 hold the offset within the partial slot until patched
 					for (size_t i = 0; i < vq.size(); i++) {
-						const int dX = vq[i].dX, dM = vq[i].dM, dQ = vq[i].dQ, Nq = dQ * tC, K3 = tA * dM;
+						const int dX = vq[i].dX, dM = vq[i].dM, dQ = vq[i].dQ, Nq = dQ * tC;
 						for (int mi = 0; mi < cdiv(dX, 16); mi++)
 							for (int ni = 0; ni < cdiv(Nq, 16); ni++) {
-								const int vrr = std::min(16, dX - 16 * mi), vcc = std::min(16, Nq - 16 * ni);
 								const int64_t rel = (int64_t)16 * mi * dQ * dC + c0;
 								const uint32_t d[PR_T3] = { (uint32_t)(lds_l[i] + 16 * mi), (uint32_t)dX, (uint32_t)(lt2 + vq[i].rbase * S1), (uint32_t)(dQ * S1),
-									(uint32_t)dM, magic(dM), (uint32_t)K3, (uint32_t)cdiv(K3, 4),
-									(uint32_t)vrr | ((uint32_t)vcc << 16), (uint32_t)(16 * ni), (uint32_t)tC | ((uint32_t)dC << 16), magic(tC),
-									(uint32_t)rel, (uint32_t)((uint64_t)rel >> 32), (uint32_t)(dQ * dC), 0u };
-								refs.push_back(PrSlotRef{ vq[i].yblock, A, ia, h.recs.size(), (int)(t3s.size() + 12) });
+									(uint32_t)dM, (uint32_t)tA, (uint32_t)cdiv(tA, 4), (uint32_t)std::min(16, dX - 16 * mi), (uint32_t)std::min(16, Nq - 16 * ni),
+									(uint32_t)(16 * ni), (uint32_t)tC, (uint32_t)dC, magic(tC),
+									(uint32_t)rel, (uint32_t)((uint64_t)rel >> 32), (uint32_t)(dQ * dC) };
+								refs.push_back(PrSlotRef{ vq[i].yblock, A, ia, h.recs.size(), (int)(t3s.size() + 13) });
 								t3s.insert(t3s.end(), d, d + PR_T3);
 							}
 					}
 					// staging pieces: theta rows zero-padded to a multiple of 4 columns, R rows b
-					// zero-padded to a multiple of 4, L rows contiguous
-					auto add_pieces = [&](int64_t src, int d0, int n, int vrows, int cols, int dstride, int sstride, int vcols, int kind) {
+					// zero-padded to a multiple of 4 (zero pieces), L rows, W_D and the row map contiguous
+					auto add_pieces = [&](int64_t src, int d0, int n, int cols, int dstride, int sstride, int vcols, int kind) {
 						for (int e0 = 0; e0 < n; e0 += 64) {
-							const uint32_t d[PR_PIECE] = { (uint32_t)src, (uint32_t)((uint64_t)src >> 32), (uint32_t)d0, (uint32_t)e0 | ((uint32_t)kind << 30),
-								(uint32_t)n | ((uint32_t)vrows << 16), (uint32_t)cols | ((uint32_t)dstride << 16), (uint32_t)sstride | ((uint32_t)vcols << 16), magic(cols) };
+							const int row0 = e0 / cols, col0 = e0 % cols;
+							const int64_t s0 = src + (int64_t)row0 * sstride;
+							if (s0 < 0 || s0 > 0xffffffffLL) return false;
+							const uint32_t d[PR_PIECE] = { (uint32_t)s0, (uint32_t)(d0 + row0 * dstride) | ((uint32_t)sstride << 16),
+								(uint32_t)dstride | ((uint32_t)cols << 8) | ((uint32_t)col0 << 16) | ((uint32_t)std::min(64, n - e0) << 24),
+								((uint32_t)((65536 + cols - 1) / cols)) | ((uint32_t)vcols << 17) | ((uint32_t)kind << 25) };
 							pcs.insert(pcs.end(), d, d + PR_PIECE);
 						}
+						return true;
 					};
+					bool ok = true;
 					for (size_t i = 0; i < vr.size(); i++) {
-						const int dB = vr[i].dB, dB4 = up4(dB), Mr = tA * vr[i].dP;
-						add_pieces(vr[i].th + (int64_t)a0 * vr[i].dP * dB, lth[i], Mr * dB4, Mr, dB4, dB4 + 1, dB, dB, 0);
-						add_pieces(vr[i].r + c0, lr[i], dB4 * vr[i].dN * tC, dB * vr[i].dN, tC, tC, dC, tC, 1);
+						const int dB = vr[i].dB, dB4 = up4(dB), Mr = tA * vr[i].dP, Nr = vr[i].dN * tC;
+						ok = ok && add_pieces(vr[i].th + (int64_t)a0 * vr[i].dP * dB, lth[i], Mr * dB4, dB4, dB4 + 1, dB, dB, 0);
+						ok = ok && add_pieces(vr[i].r + c0, lr[i], dB * vr[i].dN * tC, tC, tC, dC, tC, 1);
+						if (dB4 > dB) ok = ok && add_pieces(0, lr[i] + dB * Nr, (dB4 - dB) * Nr, 64, 64, 64, 64, 3);
 					}
 					for (size_t i = 0; i < vq.size(); i++) {
 						const int n = tA * vq[i].dM * vq[i].dX;
-						add_pieces(vq[i].l + (int64_t)a0 * vq[i].dM * vq[i].dX, lds_l[i], n, 1, n, 0, 0, n, 2);
+						ok = ok && add_pieces(vq[i].l + (int64_t)a0 * vq[i].dM * vq[i].dX, lds_l[i], n, 64, 64, 64, 64, 2);
 					}
+					ok = ok && add_pieces(cl.wd, lwb, cl.R16 * (cl.K4 + 1), 64, 64, 64, 64, 4);
+					// the row map: the record's words from offk, copied as doubles (offk even); its
+					// source offset is set below once offk is known
+					const size_t kpiece = pcs.size();
+					ok = ok && add_pieces(0, lkb, cl.K4, 64, 64, 64, 64, 5);
 					const int np = (int)(pcs.size() / PR_PIECE);
-					if (np > nbp) return false;
+					if (!ok || np > nbp) return false;
 					std::vector<uint32_t> rec(PR_HDR, 0u);
 					rec[0] = (uint32_t)(t1s.size() / PR_T1); rec[1] = (uint32_t)(t3s.size() / PR_T3); rec[2] = (uint32_t)np; rec[3] = (uint32_t)cl.R16;
 					rec[4] = (uint32_t)cl.K4; rec[5] = (uint32_t)S1; rec[6] = (uint32_t)tC; rec[7] = magic(tC);
-					rec[8] = (uint32_t)lt2; rec[9] = (uint32_t)cl.wd;
+					rec[8] = (uint32_t)lt2; rec[14] = (uint32_t)lwb; rec[15] = (uint32_t)lkb;
 					rec[10] = (uint32_t)rec.size();
 					rec.insert(rec.end(), t1s.begin(), t1s.end());
 					rec[11] = (uint32_t)rec.size();
 					for (size_t k = refs.size() - t3s.size() / PR_T3; k < refs.size(); k++) refs[k].word += (int)rec.size();
 					rec.insert(rec.end(), t3s.begin(), t3s.end());
+					if (rec.size() % 2) rec.push_back(0u);
 					rec[12] = (uint32_t)rec.size();
+					pcs[kpiece] = rec[12] / 2;
 					rec.insert(rec.end(), kt.begin(), kt.end());
 					while (rec.size() % 4) rec.push_back(0u);
 					rec[13] = (uint32_t)rec.size();
@@ -514,10 +570,11 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 }
 
 // Host replay of the unit and reduction launches with the kernels' own index arithmetic (the
-// multiplier divisions included), tile order and clamps: every operand, LDS and partial index in
-// bounds, every LDS operand read after it was written in the current phase (the theta / R panels
-// are dead once stage 1 ends), T1 / T2 / partial / y entries written exactly once, staging
-// destinations disjoint. y (optional, length nx) receives the product. Returns 0 or a check number.
+// multiplier divisions included), tile order, clamps and dump stores: every operand, LDS and
+// partial index in bounds, every LDS operand read after it was written in the current phase (the
+// theta / R panels are dead once stage 1 ends), T1 / T2 / partial / y entries written exactly once,
+// staging destinations disjoint. y (optional, length nx) receives the product. Returns 0 or a check
+// number.
 int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r, int64_t nr, const double* l, int64_t nl, const double* w, int64_t nw, double* y)
 {
 	const int LDS = h.lds, NB = h.lds / PR_THREADS;
@@ -542,16 +599,17 @@ int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r,
 		const int n1 = (int)g[0], n3 = (int)g[1], npiece = (int)g[2], R16 = (int)g[3];
 		const int K4 = (int)g[4], S1 = (int)g[5], tc = (int)g[6];
 		const uint32_t magtc = g[7];
-		const int t2 = (int)g[8], wdo = (int)g[9], off1 = (int)g[10], off3 = (int)g[11], offk = (int)g[12], offp = (int)g[13];
+		const int t2 = (int)g[8], off1 = (int)g[10], off3 = (int)g[11], offk = (int)g[12], offp = (int)g[13], wb = (int)g[14], kb = (int)g[15];
 		if ((size_t)off1 + (size_t)PR_T1 * n1 > rec.size() || (size_t)off3 + (size_t)PR_T3 * n3 > rec.size() || (size_t)offk + 2 * (size_t)K4 > rec.size()) return 3;
-		if (offp % 4 || (size_t)offp + (size_t)PR_PIECE * npiece != rec.size()) return 3;
-		if (R16 > PR_RMAX || K4 > PR_KMAX || R16 % 16 || K4 % 4 || (size_t)wdo + (size_t)R16 * K4 > wd.size()) return 4;
-		if (npiece > 4 * NB || tc < 1 || tc > PR_ZERO || S1 < tc || S1 % tc) return 5;
-		const uint32_t* kt = g + offk;
+		if (offp % 4 || offk % 2 || (size_t)offp + (size_t)PR_PIECE * npiece != rec.size()) return 3;
+		if (R16 > PR_RMAX || K4 > PR_KMAX || R16 % 16 || K4 % 4 || wb != PR_WB || kb != PR_WB + R16 * (K4 + 1)) return 4;
+		if (npiece > 4 * NB || tc < 1 || tc > PR_ZERO || S1 < tc || S1 % tc || n1 > 4 * 64 || n3 > 4 * 64) return 5;
+		std::vector<double> recd(rec.size() / 2);
+		memcpy(recd.data(), rec.data(), recd.size() * sizeof(double));
 		const uint32_t* pieces = g + offp;
 		std::fill(sw.begin(), sw.end(), 0);
 		auto lw = [&](int a, double v) { if (a < 0 || a >= LDS) { bad = true; return; } sm[(size_t)a] = v; sw[(size_t)a] = 1; };
-		auto lr = [&](int a) -> double { if (a < 0 || a >= LDS || !sw[(size_t)a]) { bad = true; return 0.0; } return sm[(size_t)a]; };
+		auto lr = [&](int a) -> double { if (a < 0 || a >= LDS || a == PR_DUMP || !sw[(size_t)a]) { bad = true; return 0.0; } return sm[(size_t)a]; };
 		for (int i = 0; i < PR_ZERO; i++) lw(i, 0.0);
 		// staging
 		for (int wave = 0; wave < 4; wave++)
@@ -559,23 +617,21 @@ int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r,
 				const int pc = wave + 4 * i;
 				if (pc >= npiece) continue;
 				const uint32_t* d = pieces + PR_PIECE * pc;
-				const int64_t src = (int64_t)((uint64_t)d[0] | ((uint64_t)d[1] << 32));
-				const int d0 = (int)d[2], kind = (int)(d[3] >> 30), e0 = (int)(d[3] & 0xffffu);
-				const int n = (int)(d[4] & 0xffffu), vrows = (int)(d[4] >> 16);
-				const int cols = (int)(d[5] & 0xffffu), dstride = (int)(d[5] >> 16);
-				const int sstride = (int)(d[6] & 0xffffu), vcols = (int)(d[6] >> 16);
-				if (kind == 3 || cols < 1) return 6;
-				for (int lane = 0; lane < 64; lane++) {
-					const int e = e0 + lane;
-					if (e >= n) continue;
-					const int row = dv(e, cols, d[7]), col = e - row * cols;
-					const int dd = d0 + row * dstride + col;
-					if (dd < PR_ZERO || dd >= LDS || sw[(size_t)dd]) return 8;
+				const int lanes = (int)(d[2] >> 24), cols = (int)((d[2] >> 8) & 0xffu), col0 = (int)((d[2] >> 16) & 0xffu), dstride = (int)(d[2] & 0xffu);
+				const int d0 = (int)(d[1] & 0xffffu), sstride = (int)(d[1] >> 16);
+				const int mag = (int)(d[3] & 0x1ffffu), vcols = (int)((d[3] >> 17) & 0xffu), kind = (int)(d[3] >> 25);
+				if (kind > 5 || cols < 1 || lanes < 1 || lanes > 64 || col0 >= cols) return 6;
+				for (int lane = 0; lane < lanes; lane++) {
+					const int t = col0 + lane;
+					const int q = (t * mag) >> 16, c = t - q * cols;
+					if (q != t / cols) return 6;
+					const int dd = d0 + q * dstride + c;
+					if (dd < PR_WB || dd >= LDS || sw[(size_t)dd]) return 8;
 					double v = 0.0;
-					if (row < vrows && col < vcols) {
-						const int64_t sidx = src + (int64_t)row * sstride + col;
-						const double* base = kind == 0 ? x : (kind == 1 ? r : l);
-						const int64_t nb = kind == 0 ? nx : (kind == 1 ? nr : nl);
+					if (kind != 3 && c < vcols) {
+						const int64_t sidx = (int64_t)d[0] + (int64_t)q * sstride + c;
+						const double* base = kind == 0 ? x : (kind == 1 ? r : (kind == 2 ? l : (kind == 4 ? wd.data() : recd.data())));
+						const int64_t nb = kind == 0 ? nx : (kind == 1 ? nr : (kind == 2 ? nl : (kind == 4 ? (int64_t)wd.size() : (int64_t)recd.size())));
 						if (sidx < 0 || sidx >= nb) return 7;
 						v = base[sidx];
 					}
@@ -587,7 +643,7 @@ int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r,
 		for (int t = 0; t < n1; t++) {
 			const uint32_t* td = g + off1 + PR_T1 * t;
 			const int abase = (int)td[0], sth = (int)td[1], bbase = (int)td[2], Nr = (int)td[3], nk = (int)td[4];
-			const int vr = (int)(td[5] & 0xffffu), vc = (int)(td[5] >> 16), sbase = (int)td[6];
+			const int vr = (int)td[5], vc = (int)td[6], sbase = (int)td[7];
 			if (vr < 1 || vr > 16 || vc < 1 || vc > 16) return 9;
 			for (int i = 0; i < 16; i++)
 				for (int j = 0; j < 16; j++) {
@@ -596,14 +652,24 @@ int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r,
 					for (int k = 0; k < 4 * nk; k++) acc += lr(ar + k) * lr(bcol + k * Nr);
 					if (i < vr && j < vc) {
 						const int a = sbase + i * Nr + j;
-						if (a < PR_ZERO || a >= t2 || t1w[(size_t)a]++) return 10;
+						if (a < PR_WB || a >= t2 || t1w[(size_t)a]++) return 10;
 						lw(a, acc);
 					}
 				}
 		}
 		// the theta / R panels are dead after stage 1; T2 overlays them
 		for (int a = t2; a < LDS; a++) sw[(size_t)a] = 0;
-		// stage 2
+		// stage 2: W_D and the row map from their LDS copies
+		std::vector<int> kofs((size_t)K4), kstr((size_t)K4);
+		for (int k = 0; k < K4; k++) {
+			const double v = lr(kb + k);
+			int32_t e[2];
+			memcpy(e, &v, sizeof(e));
+			kofs[(size_t)k] = e[0];
+			kstr[(size_t)k] = e[1];
+		}
+		std::vector<double> wl((size_t)R16 * K4);
+		for (int i = 0; i < R16; i++) for (int k = 0; k < K4; k++) wl[(size_t)(i * K4 + k)] = lr(wb + i * (K4 + 1) + k);
 		const int nt2 = (S1 + 15) >> 4;
 		for (int mi = 0; mi < R16 / 16; mi++)
 			for (int ni = 0; ni < nt2; ni++)
@@ -612,12 +678,12 @@ int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r,
 					const bool vj = j2 < S1;
 					const int a = vj ? dv(j2, tc, magtc) : 0, c = vj ? j2 - a * tc : 0;
 					double b[PR_KMAX];
-					for (int k = 0; k < K4; k++) b[k] = lr((int)kt[2 * k] + a * (int)kt[2 * k + 1] + c);
+					for (int k = 0; k < K4; k++) b[k] = lr(kofs[(size_t)k] + a * kstr[(size_t)k] + c);
 					if (!vj) continue;
 					for (int il = 0; il < 16; il++) {
 						const int i2 = 16 * mi + il;
 						double acc = 0.0;
-						for (int k = 0; k < K4; k++) acc += wd[(size_t)(wdo + i2 * K4 + k)] * b[k];
+						for (int k = 0; k < K4; k++) acc += wl[(size_t)(i2 * K4 + k)] * b[k];
 						const int addr = t2 + i2 * S1 + j2;
 						if (addr >= LDS || sw[(size_t)addr]) return 11;
 						lw(addr, acc);
@@ -627,23 +693,23 @@ int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r,
 		for (int t = 0; t < n3; t++) {
 			const uint32_t* td = g + off3 + PR_T3 * t;
 			const int lbase = (int)td[0], dX = (int)td[1], t2b = (int)td[2], dQS1 = (int)td[3], dM = (int)td[4];
-			const int K3 = (int)td[6], nk3 = (int)td[7], vr = (int)(td[8] & 0xffffu), vc = (int)(td[8] >> 16), j0 = (int)td[9];
-			const int tc3 = (int)(td[10] & 0xffffu), dC = (int)(td[10] >> 16), dQdC = (int)td[14];
-			const int64_t pbase = (int64_t)((uint64_t)td[12] | ((uint64_t)td[13] << 32));
-			if (vr < 1 || vr > 16 || vc < 1 || vc > 16 || nk3 * 4 < K3) return 12;
+			const int ta3 = (int)td[5], nka = (int)td[6], vr = (int)td[7], vc = (int)td[8], j0 = (int)td[9];
+			const int tc3 = (int)td[10], dC = (int)td[11], dQdC = (int)td[15];
+			const int64_t pbase = (int64_t)((uint64_t)td[13] | ((uint64_t)td[14] << 32));
+			if (vr < 1 || vr > 16 || vc < 1 || vc > 16 || nka * 4 < ta3 || ta3 * tc != S1 || tc3 != tc) return 12;
 			for (int i = 0; i < 16; i++)
 				for (int j = 0; j < 16; j++) {
 					const int ax = i < vr ? lbase + i : lbase;
 					const bool vj = j < vc;
 					const int j3 = j0 + j;
-					const int q = vj ? dv(j3, tc3, td[11]) : 0, c = vj ? j3 - q * tc3 : 0;
+					const int q = vj ? dv(j3, tc3, td[12]) : 0, c = vj ? j3 - q * tc3 : 0;
 					const int jofs = t2b + q * S1 + c;
 					double acc = 0.0;
-					for (int k = 0; k < 4 * nk3; k++) {
-						const int a = dv(k, dM, td[5]), m = k - a * dM;
-						const bool vk = k < K3;
-						acc += lr(vk ? ax + k * dX : 0) * lr(vk ? jofs + m * dQS1 + a * tc3 : 0);
-					}
+					for (int m = 0; m < dM; m++)
+						for (int a = 0; a < 4 * nka; a++) {
+							const bool vk = a < ta3;
+							acc += lr(vk ? ax + (a * dM + m) * dX : 0) * lr(vk ? jofs + m * dQS1 + a * tc3 : 0);
+						}
 					if (i < vr && vj) {
 						const int64_t o = pbase + (int64_t)q * dC + c + (int64_t)i * dQdC;
 						if (o < 0 || o >= h.part_elems || pwr[(size_t)o]++) return 13;
@@ -703,7 +769,8 @@ void PairPlan::upload(const PairHost& h, const double* w, hipStream_t s)
 	red = pr_upload(h.red, s);
 	nred = (int)h.red.size();
 	part_elems = h.part_elems;
-	part = (double*)dev_alloc((size_t)std::max<int64_t>(1, part_elems) * sizeof(double));
+	// 64 extra doubles take the stores of lanes outside a tile
+	part = (double*)dev_alloc((size_t)(part_elems + 64) * sizeof(double));
 	const int64_t n = (int64_t)h.wd_idx.size();
 	wd = (double*)dev_alloc((size_t)std::max<int64_t>(1, n) * sizeof(double));
 	if (n > 0) {
@@ -729,7 +796,7 @@ int pair_default_lds()
 {
 	const char* e = getenv("CT_PAIR_LDS");
 	const int v = e ? atoi(e) : 0;
-	return (v == 2048 || v == 3072 || v == 4608) ? v : PR_LDS;
+	return (v == 2048 || v == 3072 || v == 4608 || (getenv("CT_PAIR_LDS_ANY") && v > 0 && v <= 65536)) ? v : PR_LDS;
 }
 
 // MFMAs per unit above which a pair is cut into chunks (CT_PAIR_CAP overrides)
@@ -768,9 +835,9 @@ void launch_pair_units(const PairPlan& pp, const double* x, const double* r, con
 	const uint32_t* recs = pp.recs;
 	const double* wd = pp.wd;
 	switch (pp.lds) {
-	case 2048: hipLaunchKernelGGL(k_pair_units<2048>, grid, block, 0, s, recs, pp.rec_words, x, r, l, wd, pp.part); break;
-	case 3072: hipLaunchKernelGGL(k_pair_units<3072>, grid, block, 0, s, recs, pp.rec_words, x, r, l, wd, pp.part); break;
-	case 4608: hipLaunchKernelGGL(k_pair_units<4608>, grid, block, 0, s, recs, pp.rec_words, x, r, l, wd, pp.part); break;
+	case 2048: hipLaunchKernelGGL(k_pair_units<2048>, grid, block, 0, s, recs, pp.rec_words, x, r, l, wd, pp.part, pp.part + pp.part_elems); break;
+	case 3072: hipLaunchKernelGGL(k_pair_units<3072>, grid, block, 0, s, recs, pp.rec_words, x, r, l, wd, pp.part, pp.part + pp.part_elems); break;
+	case 4608: hipLaunchKernelGGL(k_pair_units<4608>, grid, block, 0, s, recs, pp.rec_words, x, r, l, wd, pp.part, pp.part + pp.part_elems); break;
 	default: throw HipError{ hipErrorInvalidValue, "sector-pair units: unsupported LDS budget" };
 	}
 	CTG_HIP(hipGetLastError());

@@ -105,7 +105,9 @@ void block_sparse_tensor_transpose(const int* perm, const struct block_sparse_te
 // chain operations and DMRG (replaces src/algorithm/chain_ops.c, src/algorithm/dmrg.c)
 
 /// chain_ops.c:353-390 (include/algorithm/chain_ops.h:31): b = H_eff a on the GPU. Requires
-/// outer (dummy) bond dimension 1 on l and r; b is allocated. Host buffers in, host buffers out
+/// outer (dummy) bond dimension 1 on l and r, and equal bra and ket axes (l axis 3 = a axis 0,
+/// w axis 1 = a axis 1, r axis 2 = a axis 2: b has the block structure of a, as in every DMRG
+/// call); other operands abort like the reference's assertions. b is allocated. Host buffers in, host buffers out
 /// (the input tensors are copied to the device per call: use ct_matvec_plan_* to keep them resident).
 void apply_local_hamiltonian(const struct block_sparse_tensor* a, const struct block_sparse_tensor* w, const struct block_sparse_tensor* l, const struct block_sparse_tensor* r, struct block_sparse_tensor* b);
 

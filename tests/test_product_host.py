@@ -175,3 +175,53 @@ def test_host_split_factor_rank_deficient_and_fixtures(golden, tmp_path):
             assert np.all(np.diff(s) <= 0) and np.all(s >= 0)
             s_np = np.linalg.svd(a, compute_uv=False)
             assert np.max(np.abs(s ** 2 - s_np ** 2)) < 1e-13 * scale ** 2
+
+
+def test_host_tridiag_eigh_nonfinite_and_numpy(tmp_path):
+    """The product's tridiagonal eigensolver (host_tridiag_eigh, the dsteqr replacement of
+    krylov.c:230): eigenvalues equal numpy's on finite input; a NaN or an infinity in the input is not
+    rotated away as if finite (the result is an error status or carries a non-finite value)."""
+    import os
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        import pytest
+        pytest.skip("g++ not available")
+    here = os.path.dirname(os.path.abspath(__file__))
+    csrc = os.path.join(os.path.dirname(here), "chemtensor_amd", "csrc")
+    exe = tmp_path / "tridiag_check"
+    subprocess.run([gxx, "-O2", "-std=c++17", "-march=x86-64-v3", "-I", csrc, "-I", "/opt/rocm/include",
+                    "-D__HIP_PLATFORM_AMD__", "-x", "c++", os.path.join(here, "host_svd_check.cpp"),
+                    "-x", "c++", os.path.join(csrc, "host_linalg.cpp"), "-o", str(exe)], check=True)
+    rng = np.random.default_rng(11)
+    n = 7
+    cases = []
+    d, e = rng.standard_normal(n), rng.standard_normal(n)
+    cases.append(("finite", d, e))
+    for what, val in (("nan_off", np.nan), ("inf_off", np.inf), ("nan_diag", np.nan), ("inf_diag", -np.inf)):
+        dd, ee = d.copy(), e.copy()
+        if what.endswith("off"):
+            ee[2] = val
+        else:
+            dd[3] = val
+        cases.append((what, dd, ee))
+    with open(tmp_path / "in.bin", "wb") as f:
+        np.array([len(cases)], dtype=np.int64).tofile(f)
+        for _, dd, ee in cases:
+            np.array([2, n], dtype=np.int64).tofile(f)
+            np.concatenate([dd, ee]).astype(np.float64).tofile(f)
+    subprocess.run([str(exe), str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), "tridiag"], check=True, timeout=60)
+    raw = open(tmp_path / "out.bin", "rb").read()
+    pos = 0
+    for what, dd, ee in cases:
+        rc = int(np.frombuffer(raw, np.int32, 1, pos)[0]); pos += 4
+        lam = np.frombuffer(raw, np.float64, n, pos); pos += 8 * n
+        z = np.frombuffer(raw, np.float64, n * n, pos); pos += 8 * n * n
+        if what == "finite":
+            t = np.diag(dd) + np.diag(ee[:-1], 1) + np.diag(ee[:-1], -1)
+            assert rc == 0
+            assert np.max(np.abs(np.sort(lam) - np.linalg.eigvalsh(t))) < 1e-12
+        else:
+            assert rc != 0 or not (np.all(np.isfinite(lam)) and np.all(np.isfinite(z))), what
+    assert pos == len(raw)

@@ -5,8 +5,9 @@ chain, L = 32, t = 1, U = 4, half filling, U(1) x U(1) quantum numbers (N, 2Sz),
 chi = 512 on both bonds; bond quantum numbers drawn as construct_random_mps draws them, theta and
 the environment blocks N(0,1) from seeds (synthetic data). A step is one matvec on device-resident
 inputs. The value is algorithmic TFLOP/s (2 m n k summed over the reference's GEMM calls) of the
-fastest FP64 batched form (fused single launch, or three grouped launches); the plain per-block FP64
-form and the split-FP32 form are timed beside it.
+fastest FP64 batched form (fused single launch, three grouped launches, or the sector-pair unit
+launch plus its partial-sum launch); the plain per-block FP64 form and the split-FP32 form are timed
+beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--chi 512] [--no-cpu]
 """
@@ -286,7 +287,7 @@ def main():
                     help="FETCH_SIZE,WRITE_SIZE (KiB per launch) of the dominant kernel from rocprofv3 --pmc runs")
     ap.add_argument("--fetch-scale", type=float, default=2.0,
                     help="FETCH_SIZE scale from the plain-copy calibration (bench.py --calibrate-copy)")
-    ap.add_argument("--forms", default="fused_fp64,grouped_fp64,split_fp32,plain_fp64",
+    ap.add_argument("--forms", default="fused_fp64,grouped_fp64,pair_fp64,split_fp32,plain_fp64",
                     help="comma list of matvec forms to time (grouped_fp64 is always included)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB ten times (scale for HBM counters) and exit")
@@ -315,7 +316,8 @@ def main():
 
     forms = {}
     wanted = set(args.forms.split(",")) | {"grouped_fp64"}
-    for name, mode in (("fused_fp64", ct.FUSED_FP64), ("grouped_fp64", ct.GROUPED_FP64), ("split_fp32", ct.GROUPED_SPLIT), ("plain_fp64", ct.PLAIN_FP64)):
+    for name, mode in (("fused_fp64", ct.FUSED_FP64), ("grouped_fp64", ct.GROUPED_FP64), ("pair_fp64", ct.PAIR_FP64), ("split_fp32", ct.GROUPED_SPLIT),
+                       ("plain_fp64", ct.PLAIN_FP64)):
         if name not in wanted:
             continue
         t0 = time.perf_counter()
@@ -327,7 +329,7 @@ def main():
         forms[name] = dict(plan=plan, y=y, ms=ms, stage_ms=stage_ms, steps=steps, plan_s=time.perf_counter() - t0)
         log(f"{name}: {ms:.4f} ms/matvec, stages {np.round(stage_ms, 4).tolist()} ms, plan {forms[name]['plan_s']:.2f}s")
 
-    head = min((k for k in ("fused_fp64", "grouped_fp64") if k in forms), key=lambda k: forms[k]["ms"])
+    head = min((k for k in ("fused_fp64", "grouped_fp64", "pair_fp64") if k in forms), key=lambda k: forms[k]["ms"])
     g = forms[head]
     plan = g["plan"]
     flops = plan.flops
@@ -336,7 +338,7 @@ def main():
     value = flops / (ms * 1e-3) / 1e12
     sbytes = plan.stage_bytes()
     # per-launch kernel durations: each stage launched back to back on the plan's stream
-    nstage = 1 if head == "fused_fp64" else 3
+    nstage = {"fused_fp64": 1, "pair_fp64": 2}.get(head, 3)
     kernel_ms = [plan.stage_time(i, x, g["y"], 100) for i in range(nstage)]
     dom = int(np.argmax(kernel_ms))
     # every stage is below the FP64 ridge point (about 10 flop/byte), so HBM is its roofline
@@ -349,7 +351,12 @@ def main():
         traffic_source = "--hbm-traffic-kib"
     y_g = forms["grouped_fp64"]["y"].numpy()
     names = ["k_grouped_gemm_f64", "k_stream_gemm_f64_wave", "k_grouped_gemm_f64_wg", "k_stream_gemm_f64_wg", "k_wapply", "k_f12"]
-    kernel_name = "k_fused_matvec" if head == "fused_fp64" else names[plan.forms()[0][dom]]
+    if head == "fused_fp64":
+        kernel_name = "k_fused_matvec"
+    elif head == "pair_fp64":
+        kernel_name = ["k_pair_units", "k_pair_reduce"][dom]
+    else:
+        kernel_name = names[plan.forms()[0][dom]]
     workload = (f"single two-site H_eff matvec, Fermi-Hubbard L={args.nsites} t=1 U=4 half filling, centre bond, chi={args.chi}"
                 if args.model == "hubbard" else f"single two-site H_eff matvec, XXZ L={args.nsites}, centre bond, chi={args.chi}")
     if traffic is None:
@@ -366,6 +373,10 @@ def main():
     if head == "fused_fp64":
         kernel_desc = "k_fused_matvec (one launch per matvec: theta.R, W.T1, L.T2 with T1/T2 in LDS)"
         form_desc = "fused FP64 (one launch per matvec, intermediates in LDS)"
+    elif head == "pair_fp64":
+        kernel_desc = [f"k_pair_units (theta.R, W.T1, L^T.T2 of every sector-pair chunk in LDS; {plan.stats()['tasks']} units, partial blocks out)",
+                       "k_pair_reduce (sums the partial blocks of every y block in a fixed order)"][dom]
+        form_desc = "sector-pair FP64 MFMA (one unit launch with T1/T2 in LDS + one partial-sum launch per matvec)"
     else:
         kernel_desc = f"{kernel_name}: grouped stage {dom + 1} ({['theta.R' if kernel_name != 'k_f12' else 'theta.R and W.T1 fused', 'W.T1', 'L.T2'][dom]})"
         form_desc = "batched grouped FP64 MFMA (3 launches per matvec)"

@@ -1148,6 +1148,8 @@ MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& 
 			p->stage_bytes[0] = 8.0 * ((double)theta.nelem + (double)p->w.meta.nelem + (double)p->l.meta.nelem + (double)p->r.meta.nelem + (double)p->pair.part_elems);
 			p->stage_bytes[1] = 8.0 * ((double)p->pair.part_elems + (double)theta.nelem);
 			p->nunits = p->pair.nunits;
+			p->ntasks = p->pair.nunits;      // plan statistics: units and reduction items
+			p->nitems = p->pair.nred;
 			p->interm_elems = (double)p->pair.part_elems;
 		}
 		else build_grouped(*p, mode == CT_MATVEC_GROUPED_FP64 && !autotune);

@@ -88,6 +88,7 @@ def lib():
             "ct_matvec_plan_stage_bytes": (C.c_int, [P, DP]),
             "ct_matvec_plan_forms": (C.c_int, [P, IP, FP]),
             "ct_matvec_apply": (C.c_int, [P, P, P]),
+            "ct_matvec_apply_noalloc": (C.c_int, [P, P, P]),
             "ct_matvec_plan_stage_time": (C.c_int, [P, C.c_int, P, P, C.c_int, FP]),
             "ct_matvec_apply_timed": (C.c_int, [P, P, P, FP]),
             "ct_matvec_plan_stream": (P, [P]),

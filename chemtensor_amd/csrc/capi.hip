@@ -173,6 +173,25 @@ int ct_matvec_apply(ct_matvec_plan* plan, const double* d_x, double* d_y)
 	}
 }
 
+int ct_matvec_apply_noalloc(ct_matvec_plan* plan, const double* d_x, double* d_y)
+{
+	if (!plan || !d_x || !d_y) return CT_ERR_ARG;
+	const MatvecPlan& p = *plan->p;
+	if (p.mode != CT_MATVEC_PAIR_FP64) return CT_ERR_UNSUPPORTED;
+	if (p.pair.part_elems + 64 > PR_SCRATCH_DOUBLES) {
+		fprintf(stderr, "ct_matvec_apply_noalloc: the plan needs %lld doubles of partial blocks, the scratch array holds %lld\n",
+			(long long)(p.pair.part_elems + 64), (long long)PR_SCRATCH_DOUBLES);
+		return CT_ERR_NOMEM;
+	}
+	try {
+		launch_pair_matvec_noalloc(p.pair, d_x, p.r.d, p.l.d, d_y, p.stream);
+		return 0;
+	}
+	catch (const HipError&) {
+		return CT_ERR_HIP;
+	}
+}
+
 int ct_matvec_apply_timed(ct_matvec_plan* plan, const double* d_x, double* d_y, float* stage_ms3)
 {
 	try {

@@ -424,6 +424,10 @@ int pair_default_lds();
 void launch_pair_units(const PairPlan& pp, const double* x, const double* r, const double* l, hipStream_t s);
 void launch_pair_reduce(const PairPlan& pp, double* y, hipStream_t s);
 void launch_pair_matvec(const PairPlan& pp, const double* x, const double* r, const double* l, double* y, hipStream_t s);
+// the same launches with the partial blocks in a file-scope device array of PR_SCRATCH_DOUBLES
+// (no allocation; concurrent calls on one GPU share the array and race)
+constexpr int64_t PR_SCRATCH_DOUBLES = (int64_t)1 << 22;
+void launch_pair_matvec_noalloc(const PairPlan& pp, const double* x, const double* r, const double* l, double* y, hipStream_t s);
 
 // Device workspace of the Lanczos iteration: maxiter Krylov vectors, w, the alpha / beta scalars,
 // reduction partials and the Ritz coefficients. Grows on demand; freed with its owner.

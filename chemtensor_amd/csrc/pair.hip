@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include "ct_internal.h"
 
@@ -855,6 +856,59 @@ void launch_pair_matvec(const PairPlan& pp, const double* x, const double* r, co
 {
 	launch_pair_units(pp, x, r, l, s);
 	launch_pair_reduce(pp, y, s);
+}
+
+//________________________________________________________________________________________________________________________
+//
+// matvec without allocation: the partial blocks in a file-scope device array instead of the plan's buffer
+
+// Each GPU holds its own copy of this array until the process ends. In host code its name is a
+// host-side stand-in, not a device address: host code uses only the per-device address from
+// hipGetSymbolAddress below.
+static __device__ __align__(256) double scratch[PR_SCRATCH_DOUBLES];
+
+namespace {
+std::mutex scratch_mu;
+std::vector<double*> scratch_ptr;   // per device: the address of 'scratch' (null until looked up)
+}
+
+// the device address of this device's scratch array, looked up once per device with that device current
+static double* scratch_address()
+{
+	int dev = 0;
+	CTG_HIP(hipGetDevice(&dev));
+	std::lock_guard<std::mutex> lk(scratch_mu);
+	if ((int)scratch_ptr.size() <= dev) scratch_ptr.resize((size_t)dev + 1, nullptr);
+	if (!scratch_ptr[(size_t)dev]) {
+		void* p = nullptr;
+		CTG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+		if (!p) throw HipError{ hipErrorInvalidSymbol, "scratch array has no device address" };
+		scratch_ptr[(size_t)dev] = (double*)p;
+	}
+	return scratch_ptr[(size_t)dev];
+}
+
+void launch_pair_matvec_noalloc(const PairPlan& pp, const double* x, const double* r, const double* l, double* y, hipStream_t s)
+{
+	if (pp.part_elems + 64 > PR_SCRATCH_DOUBLES)
+		throw HipError{ hipErrorInvalidValue, "matvec without allocation: the plan's partial blocks exceed the scratch array" };
+	double* part = scratch_address();
+	// every partial entry is written by the unit launch before the reduction reads it: nothing to clear
+	if (pp.nunits > 0) {
+		const dim3 grid((unsigned)pp.nunits), block(PR_THREADS);
+		double* dump = part + pp.part_elems;
+		switch (pp.lds) {
+		case 2048: hipLaunchKernelGGL(k_pair_units<2048>, grid, block, 0, s, (const uint32_t*)pp.recs, pp.rec_words, x, r, l, (const double*)pp.wd, part, dump); break;
+		case 3072: hipLaunchKernelGGL(k_pair_units<3072>, grid, block, 0, s, (const uint32_t*)pp.recs, pp.rec_words, x, r, l, (const double*)pp.wd, part, dump); break;
+		case 4608: hipLaunchKernelGGL(k_pair_units<4608>, grid, block, 0, s, (const uint32_t*)pp.recs, pp.rec_words, x, r, l, (const double*)pp.wd, part, dump); break;
+		default: throw HipError{ hipErrorInvalidValue, "sector-pair units: unsupported LDS budget" };
+		}
+		CTG_HIP(hipGetLastError());
+	}
+	if (pp.nred > 0) {
+		hipLaunchKernelGGL(k_pair_reduce, dim3((unsigned)((pp.nred + 3) / 4)), dim3(256), 0, s, (const PrRed*)pp.red, pp.nred, (const double*)part, y);
+		CTG_HIP(hipGetLastError());
+	}
 }
 
 }  // namespace ctg

@@ -214,6 +214,14 @@ int ct_matvec_plan_forms(const ct_matvec_plan* plan, int* forms3, float* tuned_m
 int ct_matvec_plan_stage_time(ct_matvec_plan* plan, const int stage, const double* d_x, double* d_y, const int reps, float* ms);
 /// y = H_eff x on device vectors (packed in serialize order) on the plan's stream
 int ct_matvec_apply(ct_matvec_plan* plan, const double* d_x, double* d_y);
+/// As ct_matvec_apply, without any allocation (no counterpart in the reference: for callers that
+/// cannot allocate device memory, such as code under HIP graph capture or tight serving loops). Only
+/// CT_MATVEC_PAIR_FP64 plans (CT_ERR_UNSUPPORTED otherwise): the partial blocks go to a file-scope
+/// device array of 2^22 doubles (32 MiB) that each GPU holds until the process ends, instead of the
+/// plan's own buffer; a plan needing more returns CT_ERR_NOMEM. Calls that overlap on one GPU share
+/// the array and race: serialise them (for example on one stream). Nothing in the array needs
+/// clearing: each call writes every partial entry before reading it.
+int ct_matvec_apply_noalloc(ct_matvec_plan* plan, const double* d_x, double* d_y);
 /// as ct_matvec_apply, recording the device time (ms) of each of the three stages
 int ct_matvec_apply_timed(ct_matvec_plan* plan, const double* d_x, double* d_y, float* stage_ms3);
 /// the plan's HIP stream (hipStream_t) as an opaque pointer

@@ -424,6 +424,10 @@ int pair_default_lds();
 void launch_pair_units(const PairPlan& pp, const double* x, const double* r, const double* l, hipStream_t s);
 void launch_pair_reduce(const PairPlan& pp, double* y, hipStream_t s);
 void launch_pair_matvec(const PairPlan& pp, const double* x, const double* r, const double* l, double* y, hipStream_t s);
+// the matvec with the Lanczos alpha partials of <y, x> from its partial-sum launch: dpart[0 .. G),
+// G = pair_dot_partials(pp)
+int pair_dot_partials(const PairPlan& pp);
+void launch_pair_matvec_dot(const PairPlan& pp, const double* x, const double* r, const double* l, double* y, double* dpart, hipStream_t s);
 // the same launches with the partial blocks in a file-scope device array of PR_SCRATCH_DOUBLES
 // (no allocation; concurrent calls on one GPU share the array and race)
 constexpr int64_t PR_SCRATCH_DOUBLES = (int64_t)1 << 22;
@@ -436,20 +440,26 @@ struct LanczosWork
 	double* v = nullptr;
 	double* w = nullptr;
 	double* scal = nullptr;
-	double* part = nullptr;
+	double* part = nullptr;            // reduction partials: [0, pcap) alpha / norms, [pcap, 2 pcap) norms of w
 	double* z = nullptr;
 	double* vs = nullptr;              // staged start vector (input of the captured recurrence)
 	int64_t n = 0;
-	int cap = 0, zcap = 0;
+	int cap = 0, zcap = 0, pcap = 0;
 	hipGraphExec_t graph = nullptr;    // captured recurrence for 'graph_maxiter' steps
 	int graph_maxiter = 0;
-	void reserve(int64_t n, int maxiter, int numeig);
+	void reserve(int64_t n, int maxiter, int numeig, int partials = 1024);
 	void drop_graph();
 	~LanczosWork();
 };
 
 // y = A x on device vectors of the Lanczos length, enqueued on the Lanczos stream
 using DevMatvec = std::function<void(const double* x, double* y)>;
+// y = A x that also writes partial sums of <y, x> to part (in a fixed order); returns their number
+struct DevMatvecDot
+{
+	std::function<void(const double* x, double* y, double* part)> apply;
+	int partials = 0;
+};
 
 // Lanczos eigensystem (src/util/krylov.c:24-89 and :172-251): lambda[0:numeig] ascending,
 // u (device, n x numeig row-major) the Ritz vectors. Returns 0, -1 (fewer Lanczos iterations than
@@ -458,8 +468,10 @@ using DevMatvec = std::function<void(const double* x, double* y)>;
 // HIP graph held by lw and replayed on later calls with the same maxiter; matvec must then only
 // enqueue kernels on s (no host synchronisation or copies). With sync_stop (eager only) the host
 // reads the breakdown flag before each matvec and stops there, so matvec runs numiter times.
+// With matvec_dot (partials > 0) each step's matvec also yields the alpha partials, and the step
+// is the matvec plus two vector launches.
 int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart,
-	int maxiter, int numeig, double* lambda, double* u, bool use_graph = false, bool sync_stop = false);
+	int maxiter, int numeig, double* lambda, double* u, bool use_graph = false, bool sync_stop = false, const DevMatvecDot* matvec_dot = nullptr);
 
 struct MatvecPlan
 {

@@ -58,18 +58,43 @@ __global__ __launch_bounds__(LZ_THREADS) void k_lz_normalize(const double* __res
 	for (int64_t i = (int64_t)blockIdx.x * LZ_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * LZ_THREADS) v0[i] = inv * vs[i];
 }
 
+// The vector-update kernels run on LZ_UPD_GRID(n) blocks; the first LZ_PRE entries of a thread are
+// loaded before the partials are summed (one memory round trip instead of two), the rest after.
+constexpr int LZ_PRE = 4;
+static inline int lz_upd_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + LZ_THREADS - 1) / LZ_THREADS)); }
+
 // alpha_j = sum(part); w -= alpha_j v_j + beta_{j-1} v_{j-1}; part2 = partial sums of w^2
 __global__ __launch_bounds__(LZ_THREADS) void k_lz_alpha_update(double* __restrict__ w, const double* __restrict__ vj, const double* __restrict__ vjm1,
 	int64_t n, int j, int maxiter, const double* __restrict__ part, int G, double* __restrict__ part2, double* __restrict__ scal)
 {
 	__shared__ double sh[LZ_THREADS];
 	if (scal[2 * maxiter + 1] != 0) return;
+	const int64_t i0 = (int64_t)blockIdx.x * LZ_THREADS + threadIdx.x, step = (int64_t)gridDim.x * LZ_THREADS;
+	const bool upd = j < maxiter - 1;   // final iteration: alpha only
+	double wv[LZ_PRE], a[LZ_PRE], b[LZ_PRE];
+	#pragma unroll
+	for (int k = 0; k < LZ_PRE; k++) {
+		const int64_t i = i0 + k * step;
+		const bool in = upd && i < n;
+		wv[k] = in ? w[i] : 0.0;
+		a[k] = in ? vj[i] : 0.0;
+		b[k] = (in && j > 0) ? vjm1[i] : 0.0;
+	}
 	const double alpha = sum_partials(part, G, sh);
 	if (blockIdx.x == 0 && threadIdx.x == 0) scal[j] = alpha;
-	if (j == maxiter - 1) return;   // final iteration: alpha only
+	if (!upd) return;
 	const double bprev = (j > 0) ? scal[maxiter + j - 1] : 0.0;
 	double v = 0;
-	for (int64_t i = (int64_t)blockIdx.x * LZ_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * LZ_THREADS) {
+	#pragma unroll
+	for (int k = 0; k < LZ_PRE; k++) {
+		const int64_t i = i0 + k * step;
+		if (i < n) {
+			const double wi = (j > 0) ? wv[k] - (alpha * a[k] + bprev * b[k]) : wv[k] - alpha * a[k];
+			w[i] = wi;
+			v += wi * wi;
+		}
+	}
+	for (int64_t i = i0 + LZ_PRE * step; i < n; i += step) {
 		double wi = w[i];
 		if (j > 0) wi -= alpha * vj[i] + bprev * vjm1[i];
 		else wi -= alpha * vj[i];
@@ -86,6 +111,13 @@ __global__ __launch_bounds__(LZ_THREADS) void k_lz_beta_scale(const double* __re
 {
 	__shared__ double sh[LZ_THREADS];
 	if (scal[2 * maxiter + 1] != 0) return;
+	const int64_t i0 = (int64_t)blockIdx.x * LZ_THREADS + threadIdx.x, step = (int64_t)gridDim.x * LZ_THREADS;
+	double wv[LZ_PRE];
+	#pragma unroll
+	for (int k = 0; k < LZ_PRE; k++) {
+		const int64_t i = i0 + k * step;
+		wv[k] = i < n ? w[i] : 0.0;
+	}
 	const double beta = sqrt(sum_partials(part2, G, sh));
 	// every block has read the stop flag before block 0 may set it (it is only read at kernel entry)
 	if (beta < thresh) {
@@ -94,7 +126,12 @@ __global__ __launch_bounds__(LZ_THREADS) void k_lz_beta_scale(const double* __re
 	}
 	if (blockIdx.x == 0 && threadIdx.x == 0) scal[maxiter + j] = beta;
 	const double inv = 1 / beta;
-	for (int64_t i = (int64_t)blockIdx.x * LZ_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * LZ_THREADS) vnext[i] = inv * w[i];
+	#pragma unroll
+	for (int k = 0; k < LZ_PRE; k++) {
+		const int64_t i = i0 + k * step;
+		if (i < n) vnext[i] = inv * wv[k];
+	}
+	for (int64_t i = i0 + LZ_PRE * step; i < n; i += step) vnext[i] = inv * w[i];
 }
 
 // u[i, k] = sum_j V[j, i] z[j, k]  (u: n x numeig row-major, z: numiter x numeig row-major)
@@ -109,7 +146,7 @@ __global__ __launch_bounds__(LZ_THREADS) void k_lz_ritz(const double* __restrict
 	}
 }
 
-void LanczosWork::reserve(int64_t nn, int maxiter, int numeig)
+void LanczosWork::reserve(int64_t nn, int maxiter, int numeig, int partials)
 {
 	if (nn != n || cap < maxiter) {
 		drop_graph();
@@ -124,7 +161,13 @@ void LanczosWork::reserve(int64_t nn, int maxiter, int numeig)
 		n = nn;
 		cap = maxiter;
 	}
-	if (!part) part = (double*)dev_alloc(2 * 1024 * sizeof(double));
+	partials = std::max(partials, 1024);
+	if (pcap < partials) {
+		drop_graph();
+		dev_free(part);
+		part = (double*)dev_alloc(2 * (size_t)partials * sizeof(double));
+		pcap = partials;
+	}
 	if (zcap < maxiter * numeig) {
 		dev_free(z);
 		z = (double*)dev_alloc((size_t)maxiter * numeig * sizeof(double));
@@ -155,14 +198,14 @@ LanczosWork::~LanczosWork()
 // before every matvec after the first, and the loop ends at a breakdown: the matvec then runs exactly
 // numiter times, on normalised Krylov vectors only (krylov.c:34-68 calls afunc numiter times).
 static void enqueue_recurrence(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart, int maxiter,
-	bool sync_stop = false)
+	bool sync_stop = false, const DevMatvecDot* mvd = nullptr)
 {
 	const int G = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 511) / 512));
 	double* V = lw.v;
 	double* w = lw.w;
 	double* scal = lw.scal;
 	double* part = lw.part;
-	double* part2 = lw.part + 1024;
+	double* part2 = lw.part + lw.pcap;
 	launch_fill(scal, 0.0, 2 * maxiter + 2, s);
 	const double thresh = 100.0 * (double)n * 2.2204460492503131e-16;   // krylov.c:58
 	hipLaunchKernelGGL(k_lz_dot_partial, dim3(G), dim3(LZ_THREADS), 0, s, vstart, vstart, n, part, (const double*)nullptr);
@@ -176,20 +219,30 @@ static void enqueue_recurrence(LanczosWork& lw, int64_t n, hipStream_t s, const 
 			CTG_HIP(hipStreamSynchronize(s));
 			if (stop != 0) break;
 		}
-		matvec(vj, w);
-		hipLaunchKernelGGL(k_lz_dot_partial, dim3(G), dim3(LZ_THREADS), 0, s, w, vj, n, part, scal + 2 * maxiter + 1);
-		hipLaunchKernelGGL(k_lz_alpha_update, dim3(G), dim3(LZ_THREADS), 0, s, w, vj, j > 0 ? vj - n : vj, n, j, maxiter, part, G, part2, scal);
+		int Ga = G;
+		if (mvd) {
+			// the matvec's last launch also writes the alpha partials
+			mvd->apply(vj, w, part);
+			Ga = mvd->partials;
+		}
+		else {
+			matvec(vj, w);
+			hipLaunchKernelGGL(k_lz_dot_partial, dim3(G), dim3(LZ_THREADS), 0, s, w, vj, n, part, scal + 2 * maxiter + 1);
+		}
+		const int Gu = lz_upd_grid(n);
+		hipLaunchKernelGGL(k_lz_alpha_update, dim3(Gu), dim3(LZ_THREADS), 0, s, w, vj, j > 0 ? vj - n : vj, n, j, maxiter, part, Ga, part2, scal);
 		if (j < maxiter - 1)
-			hipLaunchKernelGGL(k_lz_beta_scale, dim3(G), dim3(LZ_THREADS), 0, s, w, V + (int64_t)(j + 1) * n, n, j, maxiter, part2, G, thresh, scal);
+			hipLaunchKernelGGL(k_lz_beta_scale, dim3(Gu), dim3(LZ_THREADS), 0, s, w, V + (int64_t)(j + 1) * n, n, j, maxiter, part2, Gu, thresh, scal);
 		CTG_HIP(hipGetLastError());
 	}
 }
 
 int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart,
-	int maxiter, int numeig, double* lambda, double* u, bool use_graph, bool sync_stop)
+	int maxiter, int numeig, double* lambda, double* u, bool use_graph, bool sync_stop, const DevMatvecDot* matvec_dot)
 {
 	if (maxiter < 1 || numeig < 1 || numeig > maxiter || n < 1) return CT_ERR_ARG;
-	lw.reserve(n, maxiter, numeig);
+	if (matvec_dot && matvec_dot->partials < 1) matvec_dot = nullptr;
+	lw.reserve(n, maxiter, numeig, matvec_dot ? matvec_dot->partials : 1024);
 	if (use_graph) {
 		if (!lw.vs) lw.vs = (double*)dev_alloc((size_t)n * sizeof(double));
 		launch_copy(vstart, lw.vs, n, s);
@@ -199,7 +252,7 @@ int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatv
 			CaptureScope cs;
 			CTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
 			try {
-				enqueue_recurrence(lw, n, s, matvec, lw.vs, maxiter);
+				enqueue_recurrence(lw, n, s, matvec, lw.vs, maxiter, false, matvec_dot);
 			}
 			catch (...) {
 				(void)hipStreamEndCapture(s, &g);
@@ -214,7 +267,7 @@ int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatv
 		}
 		CTG_HIP(hipGraphLaunch(lw.graph, s));
 	}
-	else enqueue_recurrence(lw, n, s, matvec, vstart, maxiter, sync_stop);
+	else enqueue_recurrence(lw, n, s, matvec, vstart, maxiter, sync_stop, matvec_dot);
 	double* V = lw.v;
 	std::vector<double> hs((size_t)(2 * maxiter + 2));
 	CTG_HIP(hipMemcpyAsync(hs.data(), lw.scal, hs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -254,8 +307,15 @@ int lanczos_ground_state(MatvecPlan& p, const double* vstart, int maxiter, doubl
 {
 	// the plain form replays thousands of per-block launches; it stays eager
 	const bool graph = p.mode != CT_MATVEC_PLAIN_FP64 && graph_enabled();
+	// the sector-pair form computes the alpha partials in its partial-sum launch
+	DevMatvecDot mvd;
+	if (p.mode == CT_MATVEC_PAIR_FP64) {
+		mvd.partials = pair_dot_partials(p.pair);
+		mvd.apply = [&p](const double* x, double* y, double* part) { launch_pair_matvec_dot(p.pair, x, p.r.d, p.l.d, y, part, p.stream); };
+	}
 	return lanczos_eigensystem(p.lz, p.theta_meta.nelem, p.stream,
-		[&p](const double* x, double* y) { matvec_apply(p, x, y, nullptr); }, vstart, maxiter, 1, lambda, u, graph);
+		[&p](const double* x, double* y) { matvec_apply(p, x, y, nullptr); }, vstart, maxiter, 1, lambda, u, graph, false,
+		p.mode == CT_MATVEC_PAIR_FP64 ? &mvd : nullptr);
 }
 
 //________________________________________________________________________________________________________________________

@@ -263,6 +263,42 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const PrRed* __restrict__ i
 	y[it.y + lane] = acc;
 }
 
+// The same sums, and dpart[block] = sum over the block's entries of y[e] v[e] (the Lanczos alpha
+// partials, v the vector the matvec was applied to): wavefront butterflies, then the four
+// wavefront sums in order, so every launch sums in the same order
+__global__ __launch_bounds__(256) void k_pair_reduce_dot(const PrRed* __restrict__ items, const int nitems, const double* __restrict__ part,
+	double* __restrict__ y, const double* __restrict__ v, double* __restrict__ dpart)
+{
+	__shared__ double sh[4];
+	const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+	const int lane = threadIdx.x & 63;
+	double d = 0.0;
+	if (w < nitems) {
+		const PrRed it = items[w];
+		if (lane < it.n) {
+			const double* __restrict__ src = part + it.p + lane;
+			const int64_t sz = it.size;
+			double acc = 0.0;
+			int s = 0;
+			for (; s + 4 <= it.nslots; s += 4) {
+				const double v0 = src[(s + 0) * sz], v1 = src[(s + 1) * sz], v2 = src[(s + 2) * sz], v3 = src[(s + 3) * sz];
+				acc += v0;
+				acc += v1;
+				acc += v2;
+				acc += v3;
+			}
+			for (; s < it.nslots; s++) acc += src[s * sz];
+			y[it.y + lane] = acc;
+			d = acc * v[it.y + lane];
+		}
+	}
+	#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
+	if (lane == 0) sh[threadIdx.x >> 6] = d;
+	__syncthreads();
+	if (threadIdx.x == 0) dpart[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
 //________________________________________________________________________________________________________________________
 //
 // host planning
@@ -856,6 +892,16 @@ void launch_pair_matvec(const PairPlan& pp, const double* x, const double* r, co
 {
 	launch_pair_units(pp, x, r, l, s);
 	launch_pair_reduce(pp, y, s);
+}
+
+int pair_dot_partials(const PairPlan& pp) { return std::max(1, (pp.nred + 3) / 4); }
+
+void launch_pair_matvec_dot(const PairPlan& pp, const double* x, const double* r, const double* l, double* y, double* dpart, hipStream_t s)
+{
+	launch_pair_units(pp, x, r, l, s);
+	hipLaunchKernelGGL(k_pair_reduce_dot, dim3((unsigned)pair_dot_partials(pp)), dim3(256), 0, s, (const PrRed*)pp.red, pp.nred, (const double*)pp.part,
+		y, x, dpart);
+	CTG_HIP(hipGetLastError());
 }
 
 //________________________________________________________________________________________________________________________

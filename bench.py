@@ -185,6 +185,53 @@ def bench_molecular(ct, n, chi, sweeps, seed, split):
     return out
 
 
+def split_mfma_use(plan, stage_ms):
+    """The split-FP32 form's matrix-core rate: every FP64 product of the sliced stages (1 and 3) is 15
+    exact FP32 slice products on v_mfma_f32_16x16x4f32; against the FP32 MFMA peak."""
+    sf = plan.stage_flops()
+    t = (stage_ms[0] + stage_ms[2]) * 1e-3
+    tf = 15.0 * (sf[0] + sf[2]) / t / 1e12 if t > 0 else 0.0
+    return {"slice_products_per_fp64_product": 15, "stages": [1, 3], "fp32_mfma_tflops_executed": round(tf, 3),
+            "fp32_mfma_peak_tflops": FP32_MFMA_PEAK, "fp32_mfma_peak_frac": round(tf / FP32_MFMA_PEAK, 4)}
+
+
+def bench_molecular_matvec(ct, n, chi, seed, steps, warmup):
+    """configuration 3's compute-heavy kernel: one H_eff matvec at the centre bond pair of the spin
+    molecular Hamiltonian (n orbitals, h2 = [712, 16, 16, 712] at n = 20, the MPO bond between them 862),
+    chi on both outer bonds, bond quantum numbers as construct_random_mps draws them in the sector
+    N = n, 2Sz = 0, theta / L / R N(0,1); every FP64 form that plans it and the split-FP32 form"""
+    t, v = ct.molecular_coefficients(n, seed)
+    mpo = ct.MPO.spin_molecular(t, v)
+    theta, h2, left, right = ct.generate_matvec_problem_mpo(mpo, ct.encode_pair(n, 0), chi, seed)
+    x = ct.DeviceArray.from_numpy(theta.serialize())
+    out = {"workload": f"single two-site H_eff matvec, spin molecular Hamiltonian n={n} (seeded integrals), centre bond pair, "
+                       f"chi={chi}, h2 {h2.meta()[0]}",
+           "theta_elements": int(theta.nelem()), "forms": {}}
+    ys = {}
+    for name, mode in (("grouped_fp64", ct.GROUPED_FP64), ("pair_fp64", ct.PAIR_FP64), ("fused_fp64", ct.FUSED_FP64),
+                       ("split_fp32", ct.GROUPED_SPLIT)):
+        try:
+            plan = ct.MatvecPlan(theta, h2, left, right, mode)
+        except RuntimeError as e:
+            out["forms"][name] = {"refused": str(e)}
+            continue
+        y = ct.DeviceArray(plan.dim)
+        ms, stage_ms = time_form(ct, plan, x, y, steps, warmup)
+        f = {"ms_per_matvec": round(ms, 5), "tflops": round(plan.flops / (ms * 1e-3) / 1e12, 4),
+             "fp64_mfma_peak_frac": round(plan.flops / (ms * 1e-3) / 1e12 / FP64_MFMA_PEAK, 4),
+             "stage_ms": [round(float(v), 5) for v in stage_ms]}
+        if name == "split_fp32":
+            f["matrix_core_use"] = split_mfma_use(plan, stage_ms)
+        out["forms"][name] = f
+        out["flops_per_matvec"] = plan.flops
+        ys[name] = y.numpy()
+        log(f"molecular matvec n={n} chi={chi} {name}: {ms:.4f} ms ({f['tflops']} TFLOP/s)")
+    for k in ys:
+        if k != "grouped_fp64":
+            out["forms"][k]["rel_err_vs_grouped"] = float(np.linalg.norm(ys[k] - ys["grouped_fp64"]) / np.linalg.norm(ys["grouped_fp64"]))
+    return out
+
+
 def bench_serve(ct, nreq, rate, L_range, chi_range, seed, budget, max_active, timeout_s=None):
     """Serving loop: nreq ground-state solve requests with Poisson arrivals at `rate` per second,
     each an XXZ chain (Delta ~ U[0,2]) or a Fermi-Hubbard chain (U ~ U[0,8]) with L and chi drawn
@@ -283,6 +330,7 @@ def main():
     ap.add_argument("--molecular-chi", type=int, default=512)
     ap.add_argument("--molecular-sweeps", type=int, default=6)
     ap.add_argument("--molecular-split", type=int, default=1, help="also run the sweeps with split-FP32 matvecs")
+    ap.add_argument("--molecular-matvec", type=int, default=1, help="time the molecular centre-bond matvec (n, chi of --molecular)")
     ap.add_argument("--hbm-traffic-kib", default="",
                     help="FETCH_SIZE,WRITE_SIZE (KiB per launch) of the dominant kernel from rocprofv3 --pmc runs")
     ap.add_argument("--fetch-scale", type=float, default=2.0,
@@ -431,6 +479,8 @@ def main():
                              "ms_per_form": [[round(x, 5) for x in r] for r in forms["grouped_fp64"]["plan"].forms()[1]],
                      "form_names": ["wave_simple", "wave_stream", "wg_chunk", "wg_stream", "column_stream", "fused_stages_1_2"]},
     }
+    if "split_fp32" in forms:
+        result["forms"]["split_fp32"]["matrix_core_use"] = split_mfma_use(forms["split_fp32"]["plan"], forms["split_fp32"]["stage_ms"])
     ys = {k: v["y"].numpy() for k, v in forms.items()}
     for k in ys:
         if k != "grouped_fp64":
@@ -459,6 +509,9 @@ def main():
     if args.molecular > 0:
         result["molecular"] = bench_molecular(ct, args.molecular, args.molecular_chi, args.molecular_sweeps, args.seed,
                                               args.molecular_split)
+        if args.molecular_matvec:
+            result["molecular"]["centre_matvec"] = bench_molecular_matvec(ct, args.molecular, args.molecular_chi, args.seed,
+                                                                          max(10, args.steps // 5), 3)
 
     if args.batch > 0:
         result["batched"] = bench_batch(ct, args.batch, args.seed, max(10, args.steps // 5), 3, args.lanczos_iter)

@@ -123,12 +123,15 @@ def lib():
             "ct_stream_synchronize": (C.c_int, [P]),
             "ct_device_copy": (C.c_int, [P, P, ct_long]),
             "ct_generate_matvec_problem": (C.c_int, [C.c_int, C.c_int, ct_long, C.c_uint64, bst, bst, bst, bst]),
+            "ct_generate_matvec_problem_mpo": (C.c_int, [C.POINTER(MPOStruct), C.c_int32, ct_long, C.c_uint64, bst, bst, bst, bst]),
             "ct_construct_heisenberg_xxz_1d_mpo": (None, [C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(MPOStruct)]),
             "ct_construct_fermi_hubbard_1d_mpo": (None, [C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(MPOStruct)]),
             "ct_construct_spin_molecular_hamiltonian_mpo": (None, [C.c_int, DP, DP, C.POINTER(MPOStruct)]),
             "delete_mpo": (None, [C.POINTER(MPOStruct)]),
             "ct_dense_svd_device": (C.c_int, [ct_long, ct_long, DP, DP, DP, DP]),
             "delete_mps": (None, [C.POINTER(MPSStruct)]),
+            "save_mps": (C.c_int, [C.c_char_p, C.POINTER(MPSStruct)]),
+            "load_mps": (C.c_int, [C.c_char_p, C.POINTER(MPSStruct)]),
             "seed_rng_state": (None, [C.c_uint64, P]),
             "construct_random_mps": (None, [C.c_int, C.c_int, ct_long, IP, C.c_int32, ct_long, P, C.POINTER(MPSStruct)]),
             "dmrg_twosite": (C.c_int, [C.POINTER(MPOStruct), C.c_int, C.c_int, C.c_double, ct_long, C.POINTER(MPSStruct), DP, DP]),
@@ -278,6 +281,17 @@ def generate_matvec_problem(model, nsites, chi, seed):
     """theta, h2, left, right of the synthetic H_eff matvec at the centre bond pair (model 'xxz' or 'hubbard')."""
     ts = [HostTensor() for _ in range(4)]
     check(lib().ct_generate_matvec_problem(0 if model == "xxz" else 1, nsites, chi, seed, *[t.sptr for t in ts]), "ct_generate_matvec_problem")
+    for t in ts:
+        t.live = True
+    return tuple(ts)
+
+
+def generate_matvec_problem_mpo(mpo, sector, chi, seed):
+    """theta, h2, left, right of the synthetic H_eff matvec at the centre bond pair of any MPO built by
+    the library (the spin molecular Hamiltonian, for instance), quantum-number sector `sector`."""
+    ts = [HostTensor() for _ in range(4)]
+    check(lib().ct_generate_matvec_problem_mpo(C.pointer(mpo.s), sector, chi, seed, *[t.sptr for t in ts]),
+          "ct_generate_matvec_problem_mpo")
     for t in ts:
         t.live = True
     return tuple(ts)
@@ -648,6 +662,21 @@ class MPS:
         lib().construct_random_mps(1, nsites, d, q.ctypes.data_as(IP), sector, max_vdim, rng, C.pointer(m.s))
         m.live = True
         return m
+
+
+def save_mps(filename, mps):
+    """save_mps (src/state/mps.c:1219): the MPS in an HDF5 file with the reference's layout."""
+    if lib().save_mps(os.fsencode(filename), C.pointer(mps.s)) != 0:
+        raise RuntimeError(f"save_mps failed for '{filename}'")
+
+
+def load_mps(filename):
+    """load_mps (src/state/mps.c:1309): an MPS from an HDF5 file written by either library."""
+    m = MPS()
+    if lib().load_mps(os.fsencode(filename), C.pointer(m.s)) != 0:
+        raise RuntimeError(f"load_mps failed for '{filename}'")
+    m.live = True
+    return m
 
 
 def dmrg(mpo, psi, num_sweeps=5, maxiter_lanczos=20, tol_split=1e-10, max_vdim=256):

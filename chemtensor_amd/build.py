@@ -15,7 +15,11 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libchemtensor_gpu.so")
 
 HIP_SOURCES = ["kernels.hip", "fused.hip", "pair.hip", "runtime.hip", "engine.hip", "lanczos.hip", "split.hip", "dmrg.hip", "serve.hip", "capi.hip"]
-CPP_SOURCES = ["host_tensor.cpp", "host_linalg.cpp", "host_models.cpp", "host_molecular.cpp", "host_bond.cpp"]
+CPP_SOURCES = ["host_tensor.cpp", "host_linalg.cpp", "host_models.cpp", "host_molecular.cpp", "host_bond.cpp", "host_io.cpp"]
+# save_mps / load_mps use the HDF5 C library through libchemtensor_h5.so, built only where HDF5's
+# header and library are found (HDF5_ROOT, else the prefixes below)
+H5_LIB = os.path.join(HERE, "libchemtensor_h5.so")
+H5_PREFIXES = ["/opt/conda", "/usr", "/usr/local"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
 # gfx950 device code; packed FP32 instructions are disabled for all HIP code; MFMA accumulators in
@@ -54,7 +58,7 @@ def build(verbose=False):
         sys.stderr.write(e.stderr or "")
         raise
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-shared", "-o", tmp] + objs + ["--offload-arch=gfx950", "-Wl,-Bsymbolic", "-lpthread"]
+    cmd = [hipcc(), "-shared", "-o", tmp] + objs + ["--offload-arch=gfx950", "-Wl,-Bsymbolic", "-lpthread", "-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stderr)
@@ -62,7 +66,41 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    h5 = build_h5()
+    if verbose:
+        print("built", h5 if h5 else "no HDF5 support (HDF5 headers or library not found)")
     return LIB
+
+
+def find_hdf5():
+    """(include dir, library dir) of an HDF5 C installation, or None"""
+    roots = ([os.environ["HDF5_ROOT"]] if os.environ.get("HDF5_ROOT") else []) + H5_PREFIXES
+    for root in roots:
+        for inc in (os.path.join(root, "include"), os.path.join(root, "include", "hdf5", "serial")):
+            if not os.path.exists(os.path.join(inc, "hdf5.h")):
+                continue
+            for lib in (os.path.join(root, "lib"), os.path.join(root, "lib", "x86_64-linux-gnu"),
+                        os.path.join(root, "lib", "x86_64-linux-gnu", "hdf5", "serial")):
+                if os.path.exists(os.path.join(lib, "libhdf5.so")):
+                    return inc, lib
+    return None
+
+
+def build_h5():
+    """libchemtensor_h5.so (the HDF5 calls of save_mps / load_mps) where HDF5 exists; None otherwise"""
+    found = find_hdf5()
+    if found is None:
+        return None
+    inc, libdir = found
+    tmp = H5_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-I", inc, os.path.join(CSRC, "h5_shim.cpp"), "-o", tmp,
+           "-L", libdir, "-lhdf5", "-Wl,-rpath," + libdir]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stderr)
+        raise RuntimeError("HDF5 shim build failed")
+    os.replace(tmp, H5_LIB)
+    return H5_LIB
 
 
 if __name__ == "__main__":

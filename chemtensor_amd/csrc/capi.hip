@@ -433,14 +433,12 @@ extern "C" {
 void ct_fill_random_normal(struct block_sparse_tensor* t, double alpha, struct rng_state* rng);
 void ct_random_bond_qnums(const int nsites, const ct_long d, const qnumber* qsite, const qnumber sector, const ct_long max_vdim, const uint64_t seed, ct_long* dims_out, qnumber* qbonds_out);
 
-int ct_generate_matvec_problem(const int model, const int nsites, const ct_long chi, const uint64_t seed,
+int ct_generate_matvec_problem_mpo(const struct mpo* Hp, const qnumber sector, const ct_long chi, const uint64_t seed,
 	struct block_sparse_tensor* theta, struct block_sparse_tensor* h2, struct block_sparse_tensor* left, struct block_sparse_tensor* right)
 {
-	if (nsites < 4 || chi < 1 || (model != 0 && model != 1)) return CT_ERR_ARG;
-	struct mpo H;
-	qnumber sector;
-	if (model == 0) { ct_construct_heisenberg_xxz_1d_mpo(nsites, 1.0, 0.5, 0.0, &H); sector = 0; }
-	else { ct_construct_fermi_hubbard_1d_mpo(nsites, 1.0, 4.0, 0.0, &H); sector = nsites * 65536; }
+	if (Hp == nullptr || Hp->nsites < 4 || chi < 1) return CT_ERR_ARG;
+	const struct mpo& H = *Hp;
+	const int nsites = H.nsites;
 	const ct_long d = H.d;
 	std::vector<ct_long> dims((size_t)nsites + 1);
 	std::vector<qnumber> qall((size_t)(nsites + 1) * (size_t)chi + 8);
@@ -480,8 +478,20 @@ int ct_generate_matvec_problem(const int model, const int nsites, const ct_long 
 		seed_rng_state(seed + 3, &rng);
 		ct_fill_random_normal(right, 1.0, &rng);
 	}
-	delete_mpo(&H);
 	return 0;
+}
+
+int ct_generate_matvec_problem(const int model, const int nsites, const ct_long chi, const uint64_t seed,
+	struct block_sparse_tensor* theta, struct block_sparse_tensor* h2, struct block_sparse_tensor* left, struct block_sparse_tensor* right)
+{
+	if (nsites < 4 || chi < 1 || (model != 0 && model != 1)) return CT_ERR_ARG;
+	struct mpo H;
+	qnumber sector;
+	if (model == 0) { ct_construct_heisenberg_xxz_1d_mpo(nsites, 1.0, 0.5, 0.0, &H); sector = 0; }
+	else { ct_construct_fermi_hubbard_1d_mpo(nsites, 1.0, 4.0, 0.0, &H); sector = nsites * 65536; }
+	const int ret = ct_generate_matvec_problem_mpo(&H, sector, chi, seed, theta, h2, left, right);
+	delete_mpo(&H);
+	return ret;
 }
 
 }  // extern "C"

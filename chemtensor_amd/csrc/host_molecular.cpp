@@ -134,6 +134,16 @@ extern "C" void ct_construct_spin_molecular_hamiltonian_mpo(const int nsites, co
 				std::swap(ops[y], ops[y - 1]);
 				coeff = -coeff;
 			}
+		// two creators (or two annihilators) at one site: spin up first, so that a+_up a+_dn and
+		// a+_dn a+_up (distinct modes, anticommuting) share one state, as the reference's graph
+		// keys its pair chains by ordered (i, s) < (j, u)
+		for (int x = 0; x + 1 < m; x++) {
+			const Op p = op_of(ops[x]), q = op_of(ops[x + 1]);
+			if (p.site == q.site && p.dag == q.dag && p.spin > q.spin) {
+				std::swap(ops[x], ops[x + 1]);
+				coeff = -coeff;
+			}
+		}
 		int cn = 0, cs = 0;   // charge of the left part
 		// the state of the term after site b with 'cnt' of its operators left of the bond
 		auto key_at = [&](int bond, int cnt) -> Key {

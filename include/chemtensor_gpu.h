@@ -128,6 +128,14 @@ double randn(struct rng_state* state);
 void construct_random_mps(const enum numeric_type dtype, const int nsites, const ct_long d, const qnumber* qsite, const qnumber qnum_sector, const ct_long max_vdim, struct rng_state* rng_state, struct mps* mps);
 /// mps.c:62-74 (mps.h:32)
 void delete_mps(struct mps* mps);
+/// mps.c:1219-1299 (mps.h:129): the MPS in an HDF5 file with the reference's layout (root attributes
+/// nsites, qsite, qbond_0 .. qbond_nsites; datasets tensor_i = dense site tensors, float64). Real
+/// double MPS only. Returns 0, or -1 (message on stderr) on any failure, including a build without
+/// HDF5 support (libchemtensor_h5.so absent).
+int save_mps(const char* filename, const struct mps* mps);
+/// mps.c:1309-1461 (mps.h:131): read an MPS saved by save_mps (either library's); the caller frees it
+/// with delete_mps. Returns 0, or -1 (message on stderr).
+int load_mps(const char* filename, struct mps* mps);
 /// mpo.c:182-196 (include/operator/mpo.h:52)
 void delete_mpo(struct mpo* mpo);
 /// The operator of construct_heisenberg_xxz_1d_mpo_assembly + mpo_from_assembly
@@ -360,6 +368,12 @@ void ct_random_bond_qnums(const int nsites, const ct_long d, const qnumber* qsit
 /// construct_random_mps(seed), theta / left / right N(0,1) with seeds seed+1 / +2 / +3, h2 the merged
 /// MPO pair. The caller frees the four tensors with delete_block_sparse_tensor.
 int ct_generate_matvec_problem(const int model, const int nsites, const ct_long chi, const uint64_t seed,
+	struct block_sparse_tensor* theta, struct block_sparse_tensor* h2, struct block_sparse_tensor* left, struct block_sparse_tensor* right);
+/// The same synthetic H_eff problem for any MPO H (the spin molecular Hamiltonian of
+/// ct_construct_spin_molecular_hamiltonian_mpo, for instance): centre bond i = nsites/2 - 1, bond
+/// quantum numbers as construct_random_mps(H's site quantum numbers, sector, chi, seed) draws them,
+/// theta / left / right N(0,1) with seeds seed+1 / +2 / +3, h2 = H.a[i] merged with H.a[i+1].
+int ct_generate_matvec_problem_mpo(const struct mpo* H, const qnumber sector, const ct_long chi, const uint64_t seed,
 	struct block_sparse_tensor* theta, struct block_sparse_tensor* h2, struct block_sparse_tensor* left, struct block_sparse_tensor* right);
 
 #ifdef __cplusplus

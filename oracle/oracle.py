@@ -321,6 +321,22 @@ def matvec_problem(model, nsites, chi, seed):
     else:
         mpo = MPO.hubbard(nsites, 1.0, 4.0, 0.0)
         sector = encode_pair(nsites, 0)
+    theta, h2, left, right = matvec_problem_mpo(mpo, sector, chi, seed)
+    return theta, h2, left, right, mpo
+
+
+class MPOView(MPO):
+    """An MPO struct owned elsewhere (the product library's, which has the reference's layout)."""
+
+    def __del__(self):
+        self.ptr = None
+
+
+def matvec_problem_mpo(mpo, sector, chi, seed):
+    """matvec_problem's inputs for any MPO: centre bond pair i = nsites/2 - 1, bond quantum numbers
+    as construct_random_mps(mpo's site quantum numbers, sector, chi, seed) draws them, theta / left /
+    right N(0,1) with seeds seed+1..seed+3, h2 the merged MPO pair."""
+    nsites = mpo.nsites
     d = mpo.d
     qb = random_bond_qnums(nsites, d, mpo.qsite, sector, chi, seed)
     i = nsites // 2 - 1
@@ -332,7 +348,7 @@ def matvec_problem(model, nsites, chi, seed):
     zero = np.zeros(1, dtype=np.int32)
     left = BST.zeros([1, len(qb[i]), len(ml), len(qb[i])], [AXIS_OUT, AXIS_IN, AXIS_IN, AXIS_OUT], [zero, qb[i], ml, qb[i]]).fill_random(seed + 2)
     right = BST.zeros([len(qb[i + 2]), len(mr), len(qb[i + 2]), 1], [AXIS_OUT, AXIS_OUT, AXIS_IN, AXIS_IN], [qb[i + 2], mr, qb[i + 2], zero]).fill_random(seed + 3)
-    return theta, h2, left, right, mpo
+    return theta, h2, left, right
 
 
 def ref_rng_lib():

@@ -6,6 +6,7 @@ coefficients; bond quantum numbers consistent with the blocks."""
 import numpy as np
 import pytest
 
+import chemtensor_amd as ct
 import chemtensor_amd.compat as chemtensor
 
 
@@ -71,5 +72,17 @@ def test_spin_molecular_mpo_bond_dimensions():
     # symmetric about the centre and grow quadratically towards it
     assert dims == dims[::-1]
     for b in range(n - 1):
+        # the reference's vertex count per bond (hamiltonian.c:1796-1809): 2 identity chains,
+        # 4 (nl + nr) single-operator chains, 2 s (2 s - 1) creator / annihilator pairs with
+        # (i, s) < (j, u) and 4 s^2 creator-annihilator pairs on the shorter side (s sites)
         short = min(b + 1, n - 1 - b)
-        assert dims[b + 1] > 4 * short
+        assert dims[b + 1] == 2 + 4 * n + 2 * short * (2 * short - 1) + 4 * short * short
+
+
+def test_spin_molecular_mpo_centre_bond_n20():
+    # configuration 3's operator: 862 at the centre bond, the reference's count 2 + 80 + 780
+    t, v = ct.molecular_coefficients(20, 5)
+    mpo = ct.MPO.spin_molecular(t, v)
+    dims = [mpo.s.a[0].dim_logical[0]] + [mpo.s.a[i].dim_logical[3] for i in range(20)]
+    assert dims[10] == 862
+    assert max(dims) == 862

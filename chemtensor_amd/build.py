@@ -3,6 +3,7 @@
     python -m chemtensor_amd.build        (or chemtensor_amd.build.build())
 """
 
+import json
 import os
 import shutil
 import subprocess
@@ -36,16 +37,37 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+RESOURCES = os.path.join(BUILD, "kernel_resources.json")
+
+
 def _compile(src):
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, src.replace(".", "_") + ".o")
     cmd = [hipcc(), "-c", path, "-o", obj] + COMMON
     if src.endswith(".hip"):
-        cmd += HIP_FLAGS
+        # the compiler's per-kernel resource report (registers, scratch, LDS, occupancy)
+        cmd += HIP_FLAGS + ["-Rpass-analysis=kernel-resource-usage"]
     else:
         cmd += HOST_FLAGS
-    subprocess.run(cmd, check=True, capture_output=True, text=True)
-    return obj
+    r = subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return obj, (parse_resources(r.stderr) if src.endswith(".hip") else {})
+
+
+def parse_resources(text):
+    """{kernel symbol: {field: value}} from the kernel-resource-usage remarks of one compile"""
+    out, cur = {}, None
+    for line in text.splitlines():
+        if "remark:" not in line or "kernel-resource-usage" not in line:
+            continue
+        body = line.split("remark:", 1)[1].rsplit("[-Rpass-analysis", 1)[0].strip()
+        if body.startswith("Function Name:"):
+            cur = body.split(":", 1)[1].strip()
+            out[cur] = {}
+        elif cur is not None and ":" in body:
+            k, v = body.rsplit(":", 1)
+            v = v.strip()
+            out[cur][k.strip()] = int(v) if v.lstrip("-").isdigit() else v
+    return out
 
 
 def build(verbose=False):
@@ -53,10 +75,16 @@ def build(verbose=False):
     srcs = [s for s in HIP_SOURCES + CPP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     try:
         with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-            objs = list(ex.map(_compile, srcs))
+            done = list(ex.map(_compile, srcs))
     except subprocess.CalledProcessError as e:
         sys.stderr.write(e.stderr or "")
         raise
+    objs = [o for o, _ in done]
+    resources = {}
+    for _, res in done:
+        resources.update(res)
+    with open(RESOURCES, "w") as f:
+        json.dump(resources, f, indent=1, sort_keys=True)
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "-shared", "-o", tmp] + objs + ["--offload-arch=gfx950", "-Wl,-Bsymbolic", "-lpthread", "-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -13,6 +13,7 @@
 #include <cstring>
 
 #include "ct_internal.h"
+#include "staging.h"
 
 namespace ctg {
 
@@ -265,23 +266,25 @@ void dev_trim()
 //
 // host-to-device copies of host-side tables without host synchronisation: the data is copied into
 // a pinned staging block owned by the calling thread and the DMA is enqueued on the stream; a block
-// is reused only after the event recorded behind its copy has completed. Staging blocks live until
-// the process ends.
+// is reused only after the event recorded behind its copy has completed, and a thread holds at most
+// STAGE_MAX_BLOCKS of them (staging.h). A thread's blocks are freed by release_thread_resources.
 
 namespace {
 
-struct StageBlock { char* p; size_t cap; hipEvent_t ev; };
-
-struct Staging
+// HIP operations of the staging pool; with quiet set (thread teardown) errors are ignored
+struct HipStageOps
 {
-	std::vector<StageBlock> idle;
-	std::deque<StageBlock> busy;
+	typedef hipEvent_t Event;
+	bool quiet = false;
+	void* host_alloc(size_t n) { void* p = nullptr; CTG_HIP(hipHostMalloc(&p, n, hipHostMallocDefault)); return p; }
+	void host_free(void* p) { const hipError_t e = hipHostFree(p); if (!quiet) CTG_HIP(e); }
+	Event event_create() { hipEvent_t e; CTG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming)); return e; }
+	void event_destroy(Event e) { (void)hipEventDestroy(e); }
+	bool event_done(Event e) { return hipEventQuery(e) == hipSuccess; }
+	void event_wait(Event e) { const hipError_t r = hipEventSynchronize(e); if (!quiet) CTG_HIP(r); }
 };
 
-// at most this many staging blocks (pinned buffer + event) per thread: a thread with more copies in
-// flight waits for its oldest one
-constexpr size_t STAGE_MAX_BLOCKS = 32;
-
+typedef StagePool<HipStageOps> Staging;
 thread_local Staging* tl_staging = nullptr;
 
 Staging& staging()
@@ -290,16 +293,7 @@ Staging& staging()
 	return *tl_staging;
 }
 
-size_t stage_cap(size_t nbytes)
-{
-	size_t cap = 64 * 1024;
-	while (cap < nbytes) cap *= 2;
-	return cap;
-}
-
 }  // namespace
-
-static StageBlock stage_block(Staging& S, size_t nbytes);
 
 void copy_h2d_async(void* dst, const void* src, size_t nbytes, hipStream_t s)
 {
@@ -311,59 +305,13 @@ void copy_h2d_parts(void* dst, const H2DPart* parts, int nparts, size_t total, h
 {
 	if (total == 0) return;
 	Staging& S = staging();
-	StageBlock b = stage_block(S, total);
+	HipStageOps ops;
+	Staging::Block b = S.acquire(ops, total);
 	for (int i = 0; i < nparts; i++)
 		if (parts[i].nbytes) memcpy(b.p + parts[i].dst_off, parts[i].src, parts[i].nbytes);
 	CTG_HIP(hipMemcpyAsync(dst, b.p, total, hipMemcpyHostToDevice, s));
 	CTG_HIP(hipEventRecord(b.ev, s));
-	S.busy.push_back(b);
-}
-
-// a staging block of at least nbytes whose previous copy has completed
-static StageBlock stage_block(Staging& S, size_t nbytes)
-{
-	while (!S.busy.empty() && hipEventQuery(S.busy.front().ev) == hipSuccess) {
-		S.idle.push_back(S.busy.front());
-		S.busy.pop_front();
-	}
-	int best = -1;
-	for (int i = 0; i < (int)S.idle.size(); i++)
-		if (S.idle[i].cap >= nbytes && (best < 0 || S.idle[i].cap < S.idle[best].cap)) best = i;
-	StageBlock b;
-	if (best >= 0) {
-		b = S.idle[best];
-		S.idle.erase(S.idle.begin() + best);
-	}
-	else if (S.idle.size() + S.busy.size() < STAGE_MAX_BLOCKS) {
-		const size_t cap = stage_cap(nbytes);
-		void* p = nullptr;
-		CTG_HIP(hipHostMalloc(&p, cap, hipHostMallocDefault));
-		b.p = (char*)p;
-		b.cap = cap;
-		CTG_HIP(hipEventCreateWithFlags(&b.ev, hipEventDisableTiming));
-	}
-	else {
-		// all blocks in use or too small: wait for the oldest copy if none is idle, then grow the
-		// largest idle block's buffer (its copy has completed)
-		if (S.idle.empty()) {
-			CTG_HIP(hipEventSynchronize(S.busy.front().ev));
-			S.idle.push_back(S.busy.front());
-			S.busy.pop_front();
-		}
-		int big = 0;
-		for (int i = 1; i < (int)S.idle.size(); i++) if (S.idle[i].cap > S.idle[big].cap) big = i;
-		b = S.idle[big];
-		S.idle.erase(S.idle.begin() + big);
-		if (b.cap < nbytes) {
-			CTG_HIP(hipHostFree(b.p));
-			const size_t cap = stage_cap(nbytes);
-			void* p = nullptr;
-			CTG_HIP(hipHostMalloc(&p, cap, hipHostMallocDefault));
-			b.p = (char*)p;
-			b.cap = cap;
-		}
-	}
-	return b;
+	S.submit(b);
 }
 
 CaptureScope::CaptureScope() { tl_capture_depth++; }
@@ -372,16 +320,9 @@ CaptureScope::~CaptureScope() { tl_capture_depth--; }
 void release_thread_resources()
 {
 	if (tl_staging) {
-		Staging& S = *tl_staging;
-		for (StageBlock& b : S.busy) {
-			(void)hipEventSynchronize(b.ev);
-			S.idle.push_back(b);
-		}
-		S.busy.clear();
-		for (StageBlock& b : S.idle) {
-			(void)hipHostFree(b.p);
-			(void)hipEventDestroy(b.ev);
-		}
+		HipStageOps ops;
+		ops.quiet = true;
+		tl_staging->release(ops);
 		delete tl_staging;
 		tl_staging = nullptr;
 	}

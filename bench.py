@@ -92,6 +92,24 @@ def bench_batch(ct, nprob, seed, steps, warmup, lanczos_iter):
         batch.apply(x, y)
     e1.record(batch.stream)
     ms = e0.elapsed_ms(e1) / steps
+    # the precision option on the same batch: split-FP32 stages 1 and 3
+    t0 = time.perf_counter()
+    sbatch = ct.MatvecBatch(probs, mode=ct.GROUPED_SPLIT)
+    t_splan = time.perf_counter() - t0
+    ys = ct.DeviceArray(batch.dim)
+    for _ in range(warmup):
+        sbatch.apply(x, ys)
+    sbatch.synchronize()
+    e0.record(sbatch.stream)
+    for _ in range(steps):
+        sbatch.apply(x, ys)
+    e1.record(sbatch.stream)
+    ms_split = e0.elapsed_ms(e1) / steps
+    batch.apply(x, y)
+    batch.synchronize()
+    y64, y32 = y.numpy(), ys.numpy()
+    split_err = float(np.linalg.norm(y32 - y64) / np.linalg.norm(y64))
+    log(f"batch of {nprob} split-FP32: {ms_split:.4f} ms per batched matvec (rel err vs FP64 batch {split_err:.1e}), plan {t_splan:.2f}s")
     batch.lanczos(x, lanczos_iter)
     t0 = time.perf_counter()
     reps = 5
@@ -115,6 +133,8 @@ def bench_batch(ct, nprob, seed, steps, warmup, lanczos_iter):
         "plan_seconds": round(t_plan, 3),
         "form": "fused (one launch)" if batch.fused else "grouped (three launches)",
         "form_ms_grouped_fused": [round(v, 5) for v in batch.form_ms],
+        "split_fp32": {"ms_per_batched_matvec": round(ms_split, 5), "tflops": round(batch.flops / (ms_split * 1e-3) / 1e12, 4),
+                       "rel_err_vs_fp64_batch": split_err, "speedup_vs_fp64": round(ms / ms_split, 3)},
     }
 
 
@@ -287,6 +307,8 @@ def bench_serve(ct, nreq, rate, L_range, chi_range, seed, budget, max_active, ti
     lat = np.array([r.t_done - r.t_submit for r in done]) if done else np.zeros(1)
     log(f"serve: {len(done)}/{nreq} solved in {makespan:.1f}s, p50 {np.percentile(lat, 50):.2f}s p99 {np.percentile(lat, 99):.2f}s")
     return {
+        "sample": (f"bounded sample: {nreq} of the 2000 requests configuration 5 states (run --serve 2000 for the full configuration)"
+                   if nreq < 2000 else "configuration 5 as stated (2000 requests)"),
         "workload": f"{nreq} DMRG ground-state solves, Poisson arrivals at {rate}/s, XXZ (Delta~U[0,2]) or Hubbard (U~U[0,8]), "
                     f"L~U{{{L_range[0]}..{L_range[1]}}}, chi~U{{{chi_range[0]}..{chi_range[1]}}}, stop at |dE_sweep| < 1e-10",
         "requests": nreq, "solved": len(done), "failed": int(sum(1 for r in res if r.status < 0)),

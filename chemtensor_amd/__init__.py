@@ -94,6 +94,7 @@ def lib():
             "ct_matvec_plan_stream": (P, [P]),
             "ct_lanczos_ground_state": (C.c_int, [P, P, C.c_int, DP, P]),
             "ct_matvec_batch_create": (C.c_int, [C.c_int, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
+            "ct_matvec_batch_create_mode": (C.c_int, [C.c_int, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), C.c_int, C.POINTER(P)]),
             "ct_matvec_batch_destroy": (C.c_int, [P]),
             "ct_matvec_batch_dim": (ct_long, [P]),
             "ct_matvec_batch_offsets": (C.c_int, [P, LP]),
@@ -423,11 +424,15 @@ class MatvecBatch:
     """Independent H_eff problems applied in one fused launch (ct_matvec_batch_*); problems are
     (theta, w, l, r) tuples of block-sparse tensors."""
 
-    def __init__(self, problems):
+    def __init__(self, problems, mode=None):
         n = len(problems)
         arrs = [(P * n)(*[_ptr(p[i]) for p in problems]) for i in range(4)]
         self.h = P()
-        check(lib().ct_matvec_batch_create(n, *arrs, C.byref(self.h)), "ct_matvec_batch_create")
+        if mode is None:
+            check(lib().ct_matvec_batch_create(n, *arrs, C.byref(self.h)), "ct_matvec_batch_create")
+        else:
+            check(lib().ct_matvec_batch_create_mode(n, *arrs, mode, C.byref(self.h)), "ct_matvec_batch_create_mode")
+        self.mode = GROUPED_FP64 if mode is None else mode
         self.nprob = n
         self.dim = lib().ct_matvec_batch_dim(self.h)
         off = np.zeros(n + 1, dtype=np.int64)

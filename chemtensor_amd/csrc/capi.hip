@@ -313,7 +313,14 @@ int ct_matvec_pair_stats(const struct block_sparse_tensor* a, const struct block
 int ct_matvec_batch_create(const int nprob, const struct block_sparse_tensor* const* a, const struct block_sparse_tensor* const* w,
 	const struct block_sparse_tensor* const* l, const struct block_sparse_tensor* const* r, ct_matvec_batch** batch)
 {
+	return ct_matvec_batch_create_mode(nprob, a, w, l, r, CT_MATVEC_GROUPED_FP64, batch);
+}
+
+int ct_matvec_batch_create_mode(const int nprob, const struct block_sparse_tensor* const* a, const struct block_sparse_tensor* const* w,
+	const struct block_sparse_tensor* const* l, const struct block_sparse_tensor* const* r, const int mode, ct_matvec_batch** batch)
+{
 	if (nprob < 1 || !a || !w || !l || !r || !batch) return CT_ERR_ARG;
+	if (mode != CT_MATVEC_GROUPED_FP64 && mode != CT_MATVEC_GROUPED_SPLIT) return CT_ERR_UNSUPPORTED;
 	try {
 		hipStream_t s = default_stream();
 		std::vector<BatchInput> in((size_t)nprob);
@@ -325,7 +332,7 @@ int ct_matvec_batch_create(const int nprob, const struct block_sparse_tensor* co
 			in[p].r = upload(r[p], s);
 		}
 		std::unique_ptr<ct_matvec_batch> h(new ct_matvec_batch());
-		h->b.reset(build_batch_plan(in, s));
+		h->b.reset(build_batch_plan(in, s, true, nullptr, mode == CT_MATVEC_GROUPED_SPLIT));
 		*batch = h.release();
 		return 0;
 	}

@@ -582,6 +582,7 @@ struct BatchPlan
 	double* t1 = nullptr;
 	double* t2 = nullptr;
 	bool use_fused = false;                     // the faster of the two forms, timed at plan creation
+	bool split = false;                         // split-FP32 stages 1 and 3 (stage 2 the FP64 column stream)
 	float form_ms[2] = { 0, 0 };                // ms per batched matvec: [0] grouped, [1] fused
 	LzSeg* segs = nullptr;
 	LzProb* probs = nullptr;
@@ -600,7 +601,8 @@ struct BatchInput { Meta theta; DevTensor w, l, r; const GroupedTables* pre = nu
 // grouped form with heuristic launch forms (no timing launches, for plans rebuilt often)
 // par(n, f): run f(0 .. n-1), possibly in parallel host threads (host planning only, no HIP calls)
 using ParallelFor = std::function<void(int, const std::function<void(int)>&)>;
-BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tune = true, const ParallelFor* par = nullptr);
+// split: the split-FP32 form (the grouped tables, stages 1 and 3 on sliced FP32 MFMA, no timing)
+BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tune = true, const ParallelFor* par = nullptr, bool split = false);
 void batch_apply(BatchPlan& b, const double* x, double* y);
 // per-problem ground states: lambda[nprob], status[nprob] (0, -1 breakdown, -2 eigensolver), u concatenated
 int lanczos_ground_state_batch(BatchPlan& b, const double* vstart, int maxiter, double* lambda, int* status, double* u, bool use_graph);

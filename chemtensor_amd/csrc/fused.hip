@@ -535,8 +535,10 @@ BatchPlan::~BatchPlan()
 	dev_free(probs);
 }
 
-BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tune, const ParallelFor* par)
+BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tune, const ParallelFor* par, bool split)
 {
+	// the split form has one launch form: nothing to time
+	if (split) tune = false;
 	using clk = std::chrono::steady_clock;
 	const auto secs = [](clk::time_point a, clk::time_point z) { return std::chrono::duration<double>(z - a).count(); };
 	const clk::time_point c0 = clk::now();
@@ -555,7 +557,7 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 		nl += p.l.meta.nelem;
 		nr += p.r.meta.nelem;
 		b->off.push_back(nx);
-		if (tune) b->flops += reference_flops(th, p.w.meta, p.l.meta, p.r.meta);
+		if (tune || split) b->flops += reference_flops(th, p.w.meta, p.l.meta, p.r.meta);
 		b->bytes += 8.0 * (2.0 * (double)th.nelem + (double)(p.w.meta.nelem + p.l.meta.nelem + p.r.meta.nelem));
 	}
 	b->w = (double*)dev_alloc((size_t)std::max<int64_t>(nw, 1) * sizeof(double));
@@ -602,7 +604,7 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 	const clk::time_point c2 = clk::now();
 	b->t1 = (double*)dev_alloc((size_t)std::max<int64_t>(g.t1_elems, 1) * sizeof(double));
 	b->t2 = (double*)dev_alloc((size_t)std::max<int64_t>(g.t2_elems, 1) * sizeof(double));
-	if (!tune) b->flops = g.s1.flops + g.s2.flops + g.s3.flops;   // executed count (no replanning of the reference sequence)
+	if (!tune && !split) b->flops = g.s1.flops + g.s2.flops + g.s3.flops;   // executed count (no replanning of the reference sequence)
 	b->stage[0].upload(g.s1, s);
 	b->stage[1].upload(g.s2, s);
 	b->stage[2].upload(g.s3, s);
@@ -622,6 +624,7 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 		b->stage[1].form = GEMM_WAVE_STREAM;
 		b->stage[2].form = GEMM_WG_STREAM;
 		b->use_wapply = b->wapply.ok;
+		b->split = split;
 	}
 	const clk::time_point c3 = clk::now();
 	// reduction blocks of the batched Lanczos: up to 64 blocks of >= 4096 elements per problem
@@ -680,6 +683,16 @@ BatchPlan* build_batch_plan(std::vector<BatchInput>& in, hipStream_t s, bool tun
 
 void batch_apply(BatchPlan& b, const double* x, double* y)
 {
+	if (b.split) {
+		for (int st = 0; st < 3; st++) {
+			GemmBases g = b.stage_bases[st];
+			if (st == 0) g.A = x;
+			if (st == 2) g.C = y;
+			if (st == 1 && b.use_wapply) launch_wapply(b.wapply, g.B, g.C, b.stream);
+			else launch_grouped_gemm_split(b.stage[st], g, b.stream);
+		}
+		return;
+	}
 	if (b.use_fused) {
 		launch_fused_matvec(b.fused, x, b.w, b.l, b.r, y, b.stream);
 		return;

@@ -246,6 +246,11 @@ typedef struct ct_matvec_batch ct_matvec_batch;
 /// problems p = 0 .. nprob-1 as in ct_matvec_plan_create (fused FP64 form); operands are uploaded
 int ct_matvec_batch_create(const int nprob, const struct block_sparse_tensor* const* a, const struct block_sparse_tensor* const* w,
 	const struct block_sparse_tensor* const* l, const struct block_sparse_tensor* const* r, ct_matvec_batch** batch);
+/// the same with a precision form: CT_MATVEC_GROUPED_FP64 (as ct_matvec_batch_create: grouped or
+/// fused FP64, the faster timed at creation) or CT_MATVEC_GROUPED_SPLIT (stages 1 and 3 on FP32 MFMA
+/// with exactly sliced FP64 operands, stage 2 the FP64 column stream); other modes: CT_ERR_UNSUPPORTED
+int ct_matvec_batch_create_mode(const int nprob, const struct block_sparse_tensor* const* a, const struct block_sparse_tensor* const* w,
+	const struct block_sparse_tensor* const* l, const struct block_sparse_tensor* const* r, const int mode, ct_matvec_batch** batch);
 int ct_matvec_batch_destroy(ct_matvec_batch* batch);
 /// total length of the concatenated vectors
 ct_long ct_matvec_batch_dim(const ct_matvec_batch* batch);

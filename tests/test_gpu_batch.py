@@ -77,3 +77,32 @@ def test_batched_lanczos_matches_single_plans(problems, form, monkeypatch):
         assert abs(lam_g[i] - lam1) <= 1e-12 * abs(lam1)
         a, b = ug[off[i]:off[i + 1]], u1.numpy()
         assert rel(np.sign(np.dot(a, b)) * a, b) < 1e-8
+
+
+def test_batched_split_fp32_matches_oracle(problems):
+    """the precision option in batches (ct_matvec_batch_create_mode, CT_MATVEC_GROUPED_SPLIT): every
+    problem of the batch against the oracle at 1e-12, bit-reproducible, and the batched Lanczos on the
+    split batch within 1e-10 of the FP64 batch's ground energies"""
+    say("split batch of", len(problems))
+    batch = ct.MatvecBatch([p[:4] for p in problems], mode=ct.GROUPED_SPLIT)
+    assert not batch.fused
+    off = batch.offsets
+    x = np.concatenate([p[0].serialize() for p in problems])
+    dx = ct.DeviceArray.from_numpy(x)
+    dy = ct.DeviceArray(batch.dim)
+    batch.apply(dx, dy)
+    batch.synchronize()
+    y = dy.numpy()
+    for i, p in enumerate(problems):
+        want = orc.apply_local_hamiltonian(*p[:4]).serialize()
+        err = rel(y[off[i]:off[i + 1]], want)
+        say("  problem", CASES[i], "rel err", err)
+        assert err < TOL
+    batch.apply(dx, dy)
+    batch.synchronize()
+    assert np.array_equal(dy.numpy(), y)
+    lam32, st32, _ = batch.lanczos(dx, 20)
+    lam64, st64, _ = ct.MatvecBatch([p[:4] for p in problems]).lanczos(dx, 20)
+    assert np.all(st32 == 0) and np.all(st64 == 0)
+    say("  lanczos split", lam32, "fp64", lam64)
+    assert np.max(np.abs(np.array(lam32) - np.array(lam64)) / np.maximum(1.0, np.abs(lam64))) < 1e-10

@@ -130,6 +130,7 @@ def lib():
             "ct_construct_spin_molecular_hamiltonian_mpo": (None, [C.c_int, DP, DP, C.POINTER(MPOStruct)]),
             "delete_mpo": (None, [C.POINTER(MPOStruct)]),
             "ct_dense_svd_device": (C.c_int, [ct_long, ct_long, DP, DP, DP, DP]),
+            "ct_dense_gram_factor_device": (C.c_int, [ct_long, ct_long, DP, C.c_int, DP, DP, DP]),
             "delete_mps": (None, [C.POINTER(MPSStruct)]),
             "save_mps": (C.c_int, [C.c_char_p, C.POINTER(MPSStruct)]),
             "load_mps": (C.c_int, [C.c_char_p, C.POINTER(MPSStruct)]),
@@ -247,6 +248,19 @@ def dense_svd_device(a):
     u = np.zeros((m, k)); s = np.zeros(k); vh = np.zeros((k, n))
     ret = lib().ct_dense_svd_device(m, n, a.ctypes.data_as(DP), u.ctypes.data_as(DP), s.ctypes.data_as(DP), vh.ctypes.data_as(DP))
     check(ret, "ct_dense_svd_device")
+    return u, s, vh
+
+
+def dense_gram_factor_device(a, left):
+    """(u, s, vh) of a dense matrix through the device Gram factorisation of the DMRG split: u
+    orthonormal and vh = s Vh when left, else vh orthonormal and u = U s"""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    m, n = a.shape
+    k = min(m, n)
+    u = np.zeros((m, k)); s = np.zeros(k); vh = np.zeros((k, n))
+    ret = lib().ct_dense_gram_factor_device(m, n, a.ctypes.data_as(DP), 1 if left else 0, u.ctypes.data_as(DP), s.ctypes.data_as(DP),
+                                            vh.ctypes.data_as(DP))
+    check(ret, "ct_dense_gram_factor_device")
     return u, s, vh
 
 

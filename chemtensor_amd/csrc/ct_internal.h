@@ -142,6 +142,13 @@ size_t block_svd_lds_limit();
 // each block (not sorted); status: one int per block (0 converged, -1 not). False (nothing launched)
 // when a block does not fit the LDS of one workgroup.
 bool launch_block_svd(const SvdLayout& lay, const double* a, double* u, double* vh, double* s, int* status, hipStream_t st);
+// device block factorisation through the Gram matrix of the isometric side (rows when 'left'): u, vh
+// packed as lay.um / lay.vhm with the singular values already in the non-isometric factor, s the norms
+// of its rows / columns (device column order, not sorted); status as launch_block_svd. False when a
+// Gram matrix does not fit.
+size_t block_eig_lds_bytes(const SvdLayout& lay, bool left);
+size_t block_eig_lds_limit();
+bool launch_block_eig(const SvdLayout& lay, const double* a, double* u, double* vh, double* s, int* status, bool left, hipStream_t st);
 // the slice of a packed matrix keeping logical indices 'ind' of axis i_ax (as host_slice), reading
 // logical index x from within-block position srcpos[x]; scale (indexed like ind) multiplies the kept
 // rows / columns when given
@@ -659,7 +666,10 @@ struct DmrgState
 	void advance_host();
 	int advance_end(double en);
 	int advance(double en, DevTensor& a_opt);
-	static bool device_split_enabled();
+	enum { SPLIT_HOST = 0, SPLIT_DEVICE_EIG = 1, SPLIT_DEVICE_SVD = 2 };
+	int split_choice = -1;                     // a split mode fixed for this state (-1: CT_SPLIT decides)
+	bool split_prescaled = false;              // the device factors carry the singular values already
+	int split_mode() const;
 	void download_mps(mps* psi) const;
 	size_t device_bytes() const;
 };

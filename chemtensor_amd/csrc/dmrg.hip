@@ -263,16 +263,24 @@ void DmrgState::advance_begin(const DevTensor& a_opt)
 	DevTensor am = dev_flatten_axes(tmp, 1, TENSOR_AXIS_IN, st);
 	split_meta = a2.meta;
 	split_on_device = false;
-	if (device_split_enabled()) {
+	split_prescaled = false;
+	const int mode = split_mode();
+	// the isometric factor: U when sweeping left-to-right (the singular values go right), else Vh
+	const bool left_iso = (dir == 0);
+	if (mode != SPLIT_HOST) {
 		SvdLayout lay = plan_block_svd(am.meta);
-		if (!lay.dummy && block_svd_lds_bytes(lay) <= block_svd_lds_limit()) {
+		const bool fits = !lay.dummy && (mode == SPLIT_DEVICE_EIG ? block_eig_lds_bytes(lay, left_iso) <= block_eig_lds_limit()
+			: block_svd_lds_bytes(lay) <= block_svd_lds_limit());
+
+		if (fits) {
 			const size_t ni = lay.qi.size(), nj = lay.jobs.size();
 			DevTensor u(lay.um), vh(lay.vhm);
 			double* sd = (double*)dev_alloc(ni * sizeof(double));
 			int* stat = (int*)dev_alloc(nj * sizeof(int));
 			std::vector<double> sv(ni);
 			std::vector<int> status(nj);
-			bool ok = launch_block_svd(lay, am.d, u.d, vh.d, sd, stat, st);
+			bool ok = (mode == SPLIT_DEVICE_EIG) ? launch_block_eig(lay, am.d, u.d, vh.d, sd, stat, left_iso, st)
+				: launch_block_svd(lay, am.d, u.d, vh.d, sd, stat, st);
 			if (ok) {
 				CTG_HIP(hipMemcpyAsync(sv.data(), sd, ni * sizeof(double), hipMemcpyDeviceToHost, st));
 				CTG_HIP(hipMemcpyAsync(status.data(), stat, nj * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -300,6 +308,7 @@ void DmrgState::advance_begin(const DevTensor& a_opt)
 				split_vh = std::move(vh);
 				split_am = std::move(am);
 				split_on_device = true;
+				split_prescaled = (mode == SPLIT_DEVICE_EIG);
 			}
 		}
 	}
@@ -321,13 +330,17 @@ void DmrgState::advance_host()
 	if (trace) phase_s[PH_SPLIT_HOST] += wall_now() - t0;
 }
 
-// the device split with CT_SPLIT=device (read at every split). It is not the default: at the block
-// sizes of these sweeps the LDS Jacobi (latency-bound steps, ~10 sweeps) takes longer than the host
-// factorisation, and the blocks of large bonds exceed the LDS of a workgroup.
-bool DmrgState::device_split_enabled()
+// how the split is factorised (CT_SPLIT, read at every split; the state's own choice when set):
+// "device" (the default): Gram eigenvectors on the device (k_block_eig), the host factorisation for
+// splits whose Gram matrices exceed the LDS of a workgroup or that do not converge; "svd": the
+// one-sided Jacobi SVD kernel (k_block_svd), same fallback; "host": the host factorisation
+int DmrgState::split_mode() const
 {
+	if (split_choice >= 0) return split_choice;
 	const char* e = getenv("CT_SPLIT");
-	return e && std::string(e) == "device";
+	if (e && std::string(e) == "host") return SPLIT_HOST;
+	if (e && std::string(e) == "svd") return SPLIT_DEVICE_SVD;
+	return SPLIT_DEVICE_EIG;
 }
 
 int DmrgState::advance_end(double en)
@@ -354,10 +367,12 @@ int DmrgState::advance_end(double en)
 			std::vector<SliceBlock> b;
 			std::vector<int> ix;
 			std::vector<double> sc;
-			plan_slice_2d(split_lay.um, 1, split_ind, split_pos, left ? &sr : nullptr, m0, b, ix, sc);
-			d0 = device_slice_2d(split_u, m0, b, ix, sc, left, st);
-			plan_slice_2d(split_lay.vhm, 0, split_ind, split_pos, left ? nullptr : &sr, m1, b, ix, sc);
-			d1 = device_slice_2d(split_vh, m1, b, ix, sc, !left, st);
+			// (the Gram factorisation's non-isometric factor carries them already)
+			const bool sc0 = left && !split_prescaled, sc1 = !left && !split_prescaled;
+			plan_slice_2d(split_lay.um, 1, split_ind, split_pos, sc0 ? &sr : nullptr, m0, b, ix, sc);
+			d0 = device_slice_2d(split_u, m0, b, ix, sc, sc0, st);
+			plan_slice_2d(split_lay.vhm, 0, split_ind, split_pos, sc1 ? &sr : nullptr, m1, b, ix, sc);
+			d1 = device_slice_2d(split_vh, m1, b, ix, sc, sc1, st);
 			split_u = DevTensor();
 			split_vh = DevTensor();
 			split_am = DevTensor();

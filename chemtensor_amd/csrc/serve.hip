@@ -15,6 +15,7 @@
 // depend on the other requests in the batch.
 
 #include <algorithm>
+#include <string>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -198,6 +199,13 @@ struct Server
 		t.state.reset(new DmrgState());
 		t.state->st = s;
 		t.state->plan_ahead = true;   // each split task plans the next batch tables of its request
+		// the split's factorisation in the server: on the host unless CT_SERVE_SPLIT=device. The device
+		// factorisation's long one-workgroup kernels slow the batched Lanczos of the other requests
+		// more than they relieve the host (300 requests at 20/s: 3.7 against 8.2 solves/s)
+		{
+			const char* e = getenv("CT_SERVE_SPLIT");
+			t.state->split_choice = (e && std::string(e) == "device") ? DmrgState::SPLIT_DEVICE_EIG : DmrgState::SPLIT_HOST;
+		}
 		int ret = CT_ERR_HIP;
 		try {
 			ret = t.state->init(&H, &psi, r.num_sweeps, r.maxiter_lanczos, r.tol_split, r.max_vdim, r.tol_energy);

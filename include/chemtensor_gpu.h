@@ -201,6 +201,13 @@ int ct_matvec_pair_stats(const struct block_sparse_tensor* a, const struct block
 /// Returns 0, -1 when the sweeps did not converge, CT_ERR_UNSUPPORTED when the matrix does not fit
 /// the LDS of one workgroup ((max(m,n) + min(m,n)) min(m,n) doubles).
 int ct_dense_svd_device(const ct_long m, const ct_long n, const double* a, double* u, double* s, double* vh);
+/// One dense matrix through the default device factorisation of the DMRG split (the Gram matrix of
+/// the isometric side diagonalised by Jacobi rotations in LDS, the other factor by projection): with
+/// left != 0, a = u vh with u (m x k) orthonormal and vh (k x n) = s Vh; with left == 0, vh (k x n)
+/// orthonormal and u = U s. s (k) descending, k = min(m, n); row-major host arrays. Returns 0, -1 when
+/// the sweeps did not converge, CT_ERR_UNSUPPORTED when the Gram matrix (m x m or n x n) does not fit
+/// the LDS of one workgroup.
+int ct_dense_gram_factor_device(const ct_long m, const ct_long n, const double* a, const int left, double* u, double* s, double* vh);
 /// length of the packed vectors (= block_sparse_tensor_num_elements_blocks(a))
 ct_long ct_matvec_plan_dim(const ct_matvec_plan* plan);
 /// algorithmic FP64 flops of one matvec: sum over the reference's GEMM calls of 2 m n k

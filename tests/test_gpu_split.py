@@ -1,7 +1,8 @@
-"""GPU parity of the device split (block_sparse_tensor_svd's per-block factorisation as one-sided
-Jacobi in LDS, block_sparse_tensor.c:2686-2848, with the truncation of bond_ops.c:15-138): the
-kernel against numpy's LAPACK SVD on hard inputs and the reference's SVD fixture, and whole DMRG
-sweeps with the device split against the host split and the oracle."""
+"""GPU parity of the device split (block_sparse_tensor_svd's per-block factorisation,
+block_sparse_tensor.c:2686-2848, with the truncation of bond_ops.c:15-138) in its two device forms:
+the Gram-eigenvector factorisation (the default) and the one-sided Jacobi SVD. The kernels against
+numpy's LAPACK SVD on hard inputs and the reference's SVD fixture, and whole DMRG sweeps with each
+device split against the host split and the oracle."""
 
 import os
 import sys
@@ -40,11 +41,41 @@ def test_dense_svd_device_matches_lapack(golden):
         assert np.max(np.abs(vh[nz] @ vh[nz].T - np.eye(k)), initial=0) < 1e-12
 
 
+def test_dense_gram_factor_device_matches_lapack(golden):
+    """The default device factorisation on the same inputs, both isometric sides: the isometric factor
+    orthonormal (1e-12), the other factor's row / column norms equal to s (1e-13 ||a||), s^2 within
+    1e-13 ||a||^2 of numpy's (the eigenvalue accuracy of the Gram matrix), descending. a = u vh to
+    1e-13 ||a|| when the isometric side is the shorter one; when it is the longer one (its Gram matrix
+    is singular) directions with s^2 below the Gram matrix's accuracy eps ||a||^2 are not resolved,
+    so the reconstruction holds to sqrt(n eps) ||a||."""
+    mats = _svd_stress_matrices(np.random.default_rng(7))
+    z = golden("test_dense_tensor_svd")
+    mats += [z[k] for k in sorted(z.files) if z[k].dtype == np.float64 and z[k].ndim == 2]
+    for a in mats:
+        for left in (True, False):
+            m, n = a.shape
+            u, s, vh = ct.dense_gram_factor_device(a, left)
+            scale = max(1.0, np.linalg.norm(a))
+            k = min(m, n)
+            rec = np.max(np.abs(u @ vh - a))
+            iso = u.T @ u if left else vh @ vh.T
+            other = np.linalg.norm(vh, axis=1) if left else np.linalg.norm(u, axis=0)
+            s_np = np.linalg.svd(a, compute_uv=False)
+            say("matrix", a.shape, "left" if left else "right", "rec %.1e" % (rec / scale))
+            assert np.max(np.abs(iso - np.eye(k))) < 1e-12
+            assert np.max(np.abs(other - s)) < 1e-13 * scale
+            assert np.all(np.diff(s) <= 0) and np.all(s >= 0)
+            assert np.max(np.abs(s ** 2 - s_np ** 2)) < 1e-13 * scale ** 2
+            longer = (m if left else n) > k
+            assert rec < (np.sqrt(max(m, n) * 2.2e-16) if longer else 1e-13) * scale
+
+
+@pytest.mark.parametrize("split", ["device", "svd"])
 @pytest.mark.parametrize("model,L,chi,sweeps,maxiter,tol,seed", [
     ("xxz", 16, 32, 3, 10, 1e-12, 1),
     ("hubbard", 10, 48, 2, 8, 1e-10, 7),
 ])
-def test_dmrg_device_split_matches_host_split_and_oracle(model, L, chi, sweeps, maxiter, tol, seed):
+def test_dmrg_device_split_matches_host_split_and_oracle(model, L, chi, sweeps, maxiter, tol, seed, split):
     def run(split):
         os.environ["CT_SPLIT"] = split
         try:
@@ -58,8 +89,8 @@ def test_dmrg_device_split_matches_host_split_and_oracle(model, L, chi, sweeps, 
             return en, ent, psi.bond_dims()
         finally:
             os.environ.pop("CT_SPLIT", None)
-    en_d, ent_d, bd_d = run("device")
-    en_h, ent_h, bd_h = run("default")
+    en_d, ent_d, bd_d = run(split)
+    en_h, ent_h, bd_h = run("host")
     say("device", en_d.tolist(), "host", en_h.tolist())
     assert np.max(np.abs(en_d - en_h)) < 1e-10
     assert np.max(np.abs(ent_d - ent_h)) < 1e-8

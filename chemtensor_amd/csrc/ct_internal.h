@@ -447,7 +447,8 @@ struct LanczosWork
 	double* v = nullptr;
 	double* w = nullptr;
 	double* scal = nullptr;
-	double* part = nullptr;            // reduction partials: [0, pcap) alpha / norms, [pcap, 2 pcap) norms of w
+	double* part = nullptr;            // reduction partials: [0, pcap) alpha / norms, [pcap, 3 pcap) norms of w (two, alternating)
+	double* u = nullptr;               // the unnormalised next vector of the recurrence with deferred normalisation
 	double* z = nullptr;
 	double* vs = nullptr;              // staged start vector (input of the captured recurrence)
 	int64_t n = 0;
@@ -476,9 +477,11 @@ struct DevMatvecDot
 // enqueue kernels on s (no host synchronisation or copies). With sync_stop (eager only) the host
 // reads the breakdown flag before each matvec and stops there, so matvec runs numiter times.
 // With matvec_dot (partials > 0) each step's matvec also yields the alpha partials, and the step
-// is the matvec plus two vector launches.
+// is the matvec plus two vector launches. With deferred (not with sync_stop) the matvec runs on the
+// unnormalised next vector and one launch per step finalises beta, alpha and the next vector.
 int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart,
-	int maxiter, int numeig, double* lambda, double* u, bool use_graph = false, bool sync_stop = false, const DevMatvecDot* matvec_dot = nullptr);
+	int maxiter, int numeig, double* lambda, double* u, bool use_graph = false, bool sync_stop = false, const DevMatvecDot* matvec_dot = nullptr,
+	bool deferred = false);
 
 struct MatvecPlan
 {

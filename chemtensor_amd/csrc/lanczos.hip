@@ -134,6 +134,75 @@ __global__ __launch_bounds__(LZ_THREADS) void k_lz_beta_scale(const double* __re
 	for (int64_t i = i0 + LZ_PRE * step; i < n; i += step) vnext[i] = inv * w[i];
 }
 
+// One step of the recurrence with deferred normalisation (device plans): the matvec ran on the
+// unnormalised u_j = beta_{j-1} v_j (u_0 = v_0), so y = beta_{j-1} H v_j and dpart holds the partial
+// sums of <y, u_j> = beta_{j-1}^2 alpha_j. This kernel finalises beta_{j-1} = ||u_j|| from the previous
+// step's partials (breakdown test of krylov.c:58), alpha_j, writes v_j = u_j / beta_{j-1} into the
+// Krylov basis and the next unnormalised vector u_{j+1} = H v_j - alpha_j v_j - beta_{j-1} v_{j-1}
+// with its partial squared norms. It takes the place of the beta-scale launch: one vector kernel per
+// step beyond the matvec. uj and unext may be the same buffer (every entry is read, then written, by
+// one thread).
+__global__ __launch_bounds__(LZ_THREADS) void k_lz_step(const double* uj, double* unext, double* __restrict__ vj_out, const double* __restrict__ vjm1,
+	const double* __restrict__ y, int64_t n, int j, int maxiter, const double* __restrict__ dpart, int Gd, const double* __restrict__ npart, int Gn,
+	double thresh, double* __restrict__ npart_out, double* __restrict__ scal)
+{
+	__shared__ double sh[LZ_THREADS];
+	if (scal[2 * maxiter + 1] != 0) return;
+	const int64_t i0 = (int64_t)blockIdx.x * LZ_THREADS + threadIdx.x, step = (int64_t)gridDim.x * LZ_THREADS;
+	const bool upd = j < maxiter - 1;   // final iteration: alpha and v_j only
+	double uv[LZ_PRE], yv[LZ_PRE], bv[LZ_PRE];
+	#pragma unroll
+	for (int k = 0; k < LZ_PRE; k++) {
+		const int64_t i = i0 + k * step;
+		const bool in = i < n;
+		uv[k] = in ? uj[i] : 0.0;
+		yv[k] = (in && upd) ? y[i] : 0.0;
+		bv[k] = (in && upd && j > 0) ? vjm1[i] : 0.0;
+	}
+	double beta = 1.0;
+	if (j > 0) {
+		beta = sqrt(sum_partials(npart, Gn, sh));
+		// every block has read the stop flag before block 0 may set it (it is only read at kernel entry)
+		if (beta < thresh) {
+			if (blockIdx.x == 0 && threadIdx.x == 0) { scal[maxiter + j - 1] = beta; scal[2 * maxiter] = j; scal[2 * maxiter + 1] = 1; }
+			return;
+		}
+	}
+	const double inv = 1 / beta;
+	const double alpha = sum_partials(dpart, Gd, sh) * inv * inv;
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		scal[j] = alpha;
+		if (j > 0) scal[maxiter + j - 1] = beta;
+	}
+	double acc = 0;
+	#pragma unroll
+	for (int k = 0; k < LZ_PRE; k++) {
+		const int64_t i = i0 + k * step;
+		if (i < n) {
+			const double v = inv * uv[k];
+			if (j > 0) vj_out[i] = v;
+			if (upd) {
+				const double wi = (j > 0) ? inv * yv[k] - (alpha * v + beta * bv[k]) : yv[k] - alpha * v;
+				unext[i] = wi;
+				acc += wi * wi;
+			}
+		}
+	}
+	for (int64_t i = i0 + LZ_PRE * step; i < n; i += step) {
+		const double v = inv * uj[i];
+		if (j > 0) vj_out[i] = v;
+		if (upd) {
+			const double wi = (j > 0) ? inv * y[i] - (alpha * v + beta * vjm1[i]) : y[i] - alpha * v;
+			unext[i] = wi;
+			acc += wi * wi;
+		}
+	}
+	if (upd) {
+		const double s2 = block_sum(acc, sh);
+		if (threadIdx.x == 0) npart_out[blockIdx.x] = s2;
+	}
+}
+
 // u[i, k] = sum_j V[j, i] z[j, k]  (u: n x numeig row-major, z: numiter x numeig row-major)
 __global__ __launch_bounds__(LZ_THREADS) void k_lz_ritz(const double* __restrict__ V, const double* __restrict__ z, int numiter, int numeig, int64_t n, double* __restrict__ u)
 {
@@ -154,8 +223,10 @@ void LanczosWork::reserve(int64_t nn, int maxiter, int numeig, int partials)
 		dev_free(w);
 		dev_free(scal);
 		dev_free(vs);
+		dev_free(u);
 		v = (double*)dev_alloc((size_t)maxiter * (size_t)nn * sizeof(double));
 		w = (double*)dev_alloc((size_t)nn * sizeof(double));
+		u = (double*)dev_alloc((size_t)nn * sizeof(double));
 		scal = (double*)dev_alloc((size_t)(2 * maxiter + 2) * sizeof(double));
 		vs = nullptr;
 		n = nn;
@@ -165,7 +236,7 @@ void LanczosWork::reserve(int64_t nn, int maxiter, int numeig, int partials)
 	if (pcap < partials) {
 		drop_graph();
 		dev_free(part);
-		part = (double*)dev_alloc(2 * (size_t)partials * sizeof(double));
+		part = (double*)dev_alloc(3 * (size_t)partials * sizeof(double));
 		pcap = partials;
 	}
 	if (zcap < maxiter * numeig) {
@@ -187,6 +258,7 @@ LanczosWork::~LanczosWork()
 	drop_graph();
 	dev_free(v);
 	dev_free(w);
+	dev_free(u);
 	dev_free(scal);
 	dev_free(part);
 	dev_free(z);
@@ -198,7 +270,7 @@ LanczosWork::~LanczosWork()
 // before every matvec after the first, and the loop ends at a breakdown: the matvec then runs exactly
 // numiter times, on normalised Krylov vectors only (krylov.c:34-68 calls afunc numiter times).
 static void enqueue_recurrence(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart, int maxiter,
-	bool sync_stop = false, const DevMatvecDot* mvd = nullptr)
+	bool sync_stop = false, const DevMatvecDot* mvd = nullptr, bool deferred = false)
 {
 	const int G = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 511) / 512));
 	double* V = lw.v;
@@ -211,6 +283,28 @@ static void enqueue_recurrence(LanczosWork& lw, int64_t n, hipStream_t s, const 
 	hipLaunchKernelGGL(k_lz_dot_partial, dim3(G), dim3(LZ_THREADS), 0, s, vstart, vstart, n, part, (const double*)nullptr);
 	hipLaunchKernelGGL(k_lz_normalize, dim3(G), dim3(LZ_THREADS), 0, s, vstart, V, n, part, G);
 	CTG_HIP(hipGetLastError());
+	if (deferred) {
+		// matvec on the unnormalised u_j (u_0 = v_0); k_lz_step finalises beta_{j-1} and alpha_j, writes
+		// v_j and u_{j+1}; the squared-norm partials alternate between two areas
+		const int Gu = lz_upd_grid(n);
+		double* nparts[2] = { lw.part + lw.pcap, lw.part + 2 * (size_t)lw.pcap };
+		for (int j = 0; j < maxiter; j++) {
+			const double* x = (j == 0) ? V : lw.u;
+			int Ga = G;
+			if (mvd) {
+				mvd->apply(x, w, part);
+				Ga = mvd->partials;
+			}
+			else {
+				matvec(x, w);
+				hipLaunchKernelGGL(k_lz_dot_partial, dim3(G), dim3(LZ_THREADS), 0, s, w, x, n, part, scal + 2 * maxiter + 1);
+			}
+			hipLaunchKernelGGL(k_lz_step, dim3(Gu), dim3(LZ_THREADS), 0, s, x, lw.u, V + (int64_t)j * n, j > 0 ? V + (int64_t)(j - 1) * n : V,
+				(const double*)w, n, j, maxiter, (const double*)part, Ga, (const double*)nparts[(j + 1) & 1], Gu, thresh, nparts[j & 1], scal);
+			CTG_HIP(hipGetLastError());
+		}
+		return;
+	}
 	for (int j = 0; j < maxiter; j++) {
 		const double* vj = V + (int64_t)j * n;
 		if (sync_stop && j > 0) {
@@ -238,8 +332,9 @@ static void enqueue_recurrence(LanczosWork& lw, int64_t n, hipStream_t s, const 
 }
 
 int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatvec& matvec, const double* vstart,
-	int maxiter, int numeig, double* lambda, double* u, bool use_graph, bool sync_stop, const DevMatvecDot* matvec_dot)
+	int maxiter, int numeig, double* lambda, double* u, bool use_graph, bool sync_stop, const DevMatvecDot* matvec_dot, bool deferred)
 {
+	if (sync_stop) deferred = false;   // the host callback sees normalised Krylov vectors
 	if (maxiter < 1 || numeig < 1 || numeig > maxiter || n < 1) return CT_ERR_ARG;
 	if (matvec_dot && matvec_dot->partials < 1) matvec_dot = nullptr;
 	lw.reserve(n, maxiter, numeig, matvec_dot ? matvec_dot->partials : 1024);
@@ -252,7 +347,7 @@ int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatv
 			CaptureScope cs;
 			CTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
 			try {
-				enqueue_recurrence(lw, n, s, matvec, lw.vs, maxiter, false, matvec_dot);
+				enqueue_recurrence(lw, n, s, matvec, lw.vs, maxiter, false, matvec_dot, deferred);
 			}
 			catch (...) {
 				(void)hipStreamEndCapture(s, &g);
@@ -267,7 +362,7 @@ int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatv
 		}
 		CTG_HIP(hipGraphLaunch(lw.graph, s));
 	}
-	else enqueue_recurrence(lw, n, s, matvec, vstart, maxiter, sync_stop, matvec_dot);
+	else enqueue_recurrence(lw, n, s, matvec, vstart, maxiter, sync_stop, matvec_dot, deferred);
 	double* V = lw.v;
 	std::vector<double> hs((size_t)(2 * maxiter + 2));
 	CTG_HIP(hipMemcpyAsync(hs.data(), lw.scal, hs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -297,6 +392,14 @@ int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatv
 	return 0;
 }
 
+// device plans normalise each Lanczos vector inside the next step's update (CT_LANCZOS_DEFERRED=0:
+// the separate beta-scale launch)
+static bool lanczos_deferred()
+{
+	const char* e = getenv("CT_LANCZOS_DEFERRED");
+	return !(e && e[0] == '0');
+}
+
 static bool graph_enabled()
 {
 	const char* e = getenv("CT_LANCZOS_GRAPH");
@@ -315,7 +418,7 @@ int lanczos_ground_state(MatvecPlan& p, const double* vstart, int maxiter, doubl
 	}
 	return lanczos_eigensystem(p.lz, p.theta_meta.nelem, p.stream,
 		[&p](const double* x, double* y) { matvec_apply(p, x, y, nullptr); }, vstart, maxiter, 1, lambda, u, graph, false,
-		p.mode == CT_MATVEC_PAIR_FP64 ? &mvd : nullptr);
+		p.mode == CT_MATVEC_PAIR_FP64 ? &mvd : nullptr, lanczos_deferred());
 }
 
 //________________________________________________________________________________________________________________________

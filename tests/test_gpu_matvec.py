@@ -207,6 +207,46 @@ def test_lanczos_ground_state_matches_oracle():
         assert rel(s * ug, u_o) < 1e-9
 
 
+@pytest.mark.parametrize("deferred", ["1", "0"])
+@pytest.mark.parametrize("mode", [ct.GROUPED_FP64, ct.PAIR_FP64])
+def test_lanczos_breakdown_device_plan(mode, deferred, monkeypatch):
+    """Device plans at a Krylov breakdown (maxiter above the dimension of the local problem): the
+    recurrence stops where the oracle's does (krylov.c:58) and the ground energy and Ritz vector agree
+    with it, with each vector's normalisation deferred into the next step's update (the default) and
+    in a launch of its own (CT_LANCZOS_DEFERRED=0), eagerly and replayed from a graph."""
+    monkeypatch.setenv("CT_LANCZOS_DEFERRED", deferred)
+    mpo = orc.MPO.xxz(6, 1.0, 0.5, 0.0)
+    psi = orc.MPS.random(6, 2, mpo.qsite, 0, 4, 3)
+    theta, h2, left, right = orc.local_problem(mpo, psi, 1)
+    n = theta.nelem()
+    vstart = theta.serialize()
+    ham = theta.copy()
+
+    def afunc(v):
+        ham.deserialize(v)
+        return orc.apply_local_hamiltonian(ham, h2, left, right).serialize()
+    H = np.stack([afunc(e) for e in np.eye(n)], axis=1)
+    for maxiter in (n + 3, n + 12):
+        lam_o = np.zeros(1)
+        u_o = np.zeros(n)
+        assert orc.lib().orc_h_eigensystem_dense(n, orc.dptr(np.ascontiguousarray(H)), orc.dptr(vstart), maxiter, 1, orc.dptr(lam_o),
+                                                 orc.dptr(u_o)) == 0
+        for graph in ("1", "0"):
+            monkeypatch.setenv("CT_LANCZOS_GRAPH", graph)
+            plan = ct.MatvecPlan(theta, h2, left, right, mode)
+            lam, u = plan.lanczos(ct.DeviceArray.from_numpy(vstart), maxiter)
+            say("n", n, "maxiter", maxiter, "graph", graph, "lambda", lam, "oracle", lam_o[0])
+            assert abs(lam - lam_o[0]) < 1e-10 * max(1, abs(lam_o[0]))
+            ug = u.numpy()
+            # without re-orthogonalisation the Krylov vectors drift from orthogonality near a
+            # breakdown, and the last of them carry rounding noise amplified by 1 / beta (beta near
+            # the breakdown threshold): the Ritz vector (norm included) agrees with the oracle's to
+            # 1e-8, not to rounding
+            say("  |u| - |u_oracle|", np.linalg.norm(ug) - np.linalg.norm(u_o), "rel", rel(np.sign(np.dot(ug, u_o)) * ug, u_o))
+            assert abs(np.linalg.norm(ug) - np.linalg.norm(u_o)) < 1e-8
+            assert rel(np.sign(np.dot(ug, u_o)) * ug, u_o) < 1e-8
+
+
 def test_eigensystem_krylov_symmetric_fixture(golden):
     """The exported eigensystem_krylov_symmetric (host callback, several eigenpairs) reproduces the
     reference fixture test/util/data/test_eigensystem_krylov_symmetric.hdf5 (maxiter 35)."""

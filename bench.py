@@ -420,7 +420,8 @@ def main():
         traffic = round((fetch_kib * args.fetch_scale + write_kib) * 1024.0)
         traffic_source = "--hbm-traffic-kib"
     y_g = forms["grouped_fp64"]["y"].numpy()
-    names = ["k_grouped_gemm_f64", "k_stream_gemm_f64_wave", "k_grouped_gemm_f64_wg", "k_stream_gemm_f64_wg", "k_wapply", "k_f12"]
+    names = ["k_grouped_gemm_f64", "k_stream_gemm_f64_wave", "k_grouped_gemm_f64_wg", "k_stream_gemm_f64_wg", "k_wapply", "k_f12",
+             "k_grouped_gemm_f64_lds"]
     if head == "fused_fp64":
         kernel_name = "k_fused_matvec"
     elif head == "pair_fp64":
@@ -499,7 +500,7 @@ def main():
         "plan_stats": plan.stats(),
         "autotune_grouped": {"forms": forms["grouped_fp64"]["plan"].forms()[0],
                              "ms_per_form": [[round(x, 5) for x in r] for r in forms["grouped_fp64"]["plan"].forms()[1]],
-                     "form_names": ["wave_simple", "wave_stream", "wg_chunk", "wg_stream", "column_stream", "fused_stages_1_2"]},
+                     "form_names": ["wave_simple", "wave_stream", "wg_chunk", "wg_stream", "column_stream", "fused_stages_1_2", "lds64_large_tasks"]},
     }
     if "split_fp32" in forms:
         result["forms"]["split_fp32"]["matrix_core_use"] = split_mfma_use(forms["split_fp32"]["plan"], forms["split_fp32"]["stage_ms"])

@@ -134,7 +134,8 @@ int ct_matvec_plan_forms(const ct_matvec_plan* plan, int* forms3, float* tuned_m
 {
 	const MatvecPlan& p = *plan->p;
 	for (int i = 0; i < 3; i++) {
-		forms3[i] = (i < 2 && p.use_f12 && p.wapply.f12_ok) ? GEMM_NUM_FORMS + 1 : (i == 1 && p.use_wapply) ? GEMM_NUM_FORMS : p.stage[i].form;
+		forms3[i] = (i < 2 && p.use_f12 && p.wapply.f12_ok) ? GEMM_NUM_FORMS + 1 : (i == 1 && p.use_wapply) ? GEMM_NUM_FORMS
+			: (p.stage[i].use_lds && p.stage[i].nlitems > 0) ? GEMM_NUM_FORMS + 2 : p.stage[i].form;
 		for (int f = 0; f < GEMM_NUM_FORMS; f++) tuned_ms12[i * GEMM_NUM_FORMS + f] = p.tuned_ms[i][f];
 	}
 	return 0;

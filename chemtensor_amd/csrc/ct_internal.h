@@ -253,9 +253,18 @@ struct DevGemm
 	int form = 1;          // one of GemmForm
 	double avg_steps = 0;  // mean k-steps per work item
 	double flops = 0;
+	// LDS-tiled form for the large tasks (both edges >= LDS_TASK_MIN): 64 x 64 tiles, one workgroup
+	// each (litems), the other tasks' 32 x 32 tiles (ritems) in the chosen form; used when use_lds
+	WorkItem* litems = nullptr;
+	WorkItem* ritems = nullptr;
+	int nlitems = 0, nritems = 0;
+	bool use_lds = false;
 	void upload(const GemmBatch& b, hipStream_t s);
 	void release();
 };
+constexpr int LDS_TILE = 64;       // output tile edge of the LDS-tiled form
+void split_lds_items(const GemmBatch& b, const std::vector<WorkItem>& it, std::vector<WorkItem>& lit, std::vector<WorkItem>& rit);
+constexpr int LDS_TASK_MIN = 48;   // tasks with M and N at least this take the LDS-tiled form
 
 // Operand pointers in tasks/terms are byte offsets added to per-launch base pointers
 // (absolute addresses when the base is null), so one descriptor list serves any input vector.

@@ -63,6 +63,34 @@ std::vector<WorkItem> GemmBatch::work_items() const
 	return items;
 }
 
+// the LDS-tiled form's 64 x 64 tiles of the large tasks (decreasing cost, ties in task and tile
+// order) and the other tasks' items; both empty-handed when no task is large
+void split_lds_items(const GemmBatch& b, const std::vector<WorkItem>& it, std::vector<WorkItem>& lit, std::vector<WorkItem>& rit)
+{
+	lit.clear();
+	rit.clear();
+	std::vector<uint8_t> big(b.tasks.size(), 0);
+	struct Key { int cost; int32_t task; int m0, n0; };
+	std::vector<Key> keys;
+	for (size_t ti = 0; ti < b.tasks.size(); ti++) {
+		const GemmTask& t = b.tasks[ti];
+		if (t.M < LDS_TASK_MIN || t.N < LDS_TASK_MIN || t.t1 <= t.t0) continue;
+		big[ti] = 1;
+		int64_t panels = 0;
+		for (int q = t.t0; q < t.t1; q++) panels += (b.terms[q].K + 15) / 16;
+		const int cost = (int)std::min<int64_t>(panels + 1, (int64_t)1 << 30);
+		for (int m0 = 0; m0 < t.M; m0 += LDS_TILE)
+			for (int n0 = 0; n0 < t.N; n0 += LDS_TILE) keys.push_back(Key{ cost, (int32_t)ti, m0, n0 });
+	}
+	std::stable_sort(keys.begin(), keys.end(), [](const Key& x, const Key& y) { return x.cost > y.cost; });
+	for (const Key& k : keys) {
+		const GemmTask& t = b.tasks[(size_t)k.task];
+		lit.push_back(WorkItem{ t, k.m0, k.n0, k.cost, k.task, b.terms[t.t0] });
+	}
+	if (!lit.empty())
+		for (const WorkItem& w : it) if (!big[(size_t)w.task_index]) rit.push_back(w);
+}
+
 void DevGemm::upload(const GemmBatch& b, hipStream_t s)
 {
 	release();
@@ -72,16 +100,26 @@ void DevGemm::upload(const GemmBatch& b, hipStream_t s)
 	nitems = (int)it.size();
 	flops = b.flops;
 	// the three tables in one allocation and one transfer (256-byte aligned parts)
+	std::vector<WorkItem> lit, rit;
+	split_lds_items(b, it, lit, rit);
 	auto al = [](size_t n) { return (n + 255) / 256 * 256; };
 	const size_t n_tasks = b.tasks.size() * sizeof(GemmTask), n_terms = b.terms.size() * sizeof(GemmTerm), n_items = it.size() * sizeof(WorkItem);
+	const size_t n_lit = lit.size() * sizeof(WorkItem), n_rit = rit.size() * sizeof(WorkItem);
 	const size_t o_terms = al(std::max<size_t>(n_tasks, 1)), o_items = o_terms + al(std::max<size_t>(n_terms, 1));
-	const size_t total = o_items + std::max<size_t>(n_items, 1);
+	const size_t o_lit = o_items + al(std::max<size_t>(n_items, 1)), o_rit = o_lit + al(n_lit);
+	const size_t total = o_rit + std::max<size_t>(n_rit, 1);
 	buf = (char*)dev_alloc(total);
 	tasks = (GemmTask*)buf;
 	terms = (GemmTerm*)(buf + o_terms);
 	items = (WorkItem*)(buf + o_items);
-	const H2DPart parts[3] = { { b.tasks.data(), n_tasks, 0 }, { b.terms.data(), n_terms, o_terms }, { it.data(), n_items, o_items } };
-	copy_h2d_parts(buf, parts, 3, o_items + n_items, s);
+	litems = (WorkItem*)(buf + o_lit);
+	ritems = (WorkItem*)(buf + o_rit);
+	nlitems = (int)lit.size();
+	nritems = (int)rit.size();
+	use_lds = false;
+	const H2DPart parts[5] = { { b.tasks.data(), n_tasks, 0 }, { b.terms.data(), n_terms, o_terms }, { it.data(), n_items, o_items },
+		{ lit.data(), n_lit, o_lit }, { rit.data(), n_rit, o_rit } };
+	copy_h2d_parts(buf, parts, 5, o_rit + n_rit, s);
 	const int cap = num_compute_units() * 8;
 	grid = std::max(1, std::min((nitems + 3) / 4, cap));
 	// workgroup-level split of the k-step stream only pays for long streams
@@ -100,8 +138,9 @@ void DevGemm::release()
 {
 	dev_free(buf);
 	buf = nullptr;
-	tasks = nullptr; terms = nullptr; items = nullptr;
-	ntasks = nterms = nitems = 0;
+	tasks = nullptr; terms = nullptr; items = nullptr; litems = nullptr; ritems = nullptr;
+	ntasks = nterms = nitems = nlitems = nritems = 0;
+	use_lds = false;
 }
 
 static void run_batch_plain(const GemmBatch& b, GemmBases bases, hipStream_t s)
@@ -835,6 +874,29 @@ int check_grouped_tables(const Meta& th, const Meta& W, const Meta& L, const Met
 	if (!check_batch(g.s1, th.nelem, R.nelem, g.t1_elems, &w1) || !once(w1)) return 1;
 	if (!check_batch(g.s2, W.nelem, g.t1_elems, g.t2_elems, &w2) || !once(w2)) return 2;
 	if (!check_batch(g.s3, L.nelem, g.t2_elems, th.nelem, &wy) || !once(wy)) return 3;
+	// the LDS-tiled split: every output entry of every task in exactly one 64 x 64 tile of a large
+	// task or one 32 x 32 item of another task
+	for (const GemmBatch* b : { &g.s1, &g.s2, &g.s3 }) {
+		const std::vector<WorkItem> it = b->work_items();
+		std::vector<WorkItem> lit, rit;
+		split_lds_items(*b, it, lit, rit);
+		if (lit.empty()) continue;
+		std::vector<int64_t> base(b->tasks.size() + 1, 0);
+		for (size_t ti = 0; ti < b->tasks.size(); ti++) base[ti + 1] = base[ti] + (int64_t)b->tasks[ti].M * b->tasks[ti].N;
+		std::vector<int> cover((size_t)base.back(), 0);
+		auto mark = [&](const WorkItem& w, int edge) {
+			const GemmTask& t = b->tasks[(size_t)w.task_index];
+			for (int i = w.m0; i < std::min(t.M, w.m0 + edge); i++)
+				for (int j = w.n0; j < std::min(t.N, w.n0 + edge); j++) cover[(size_t)(base[(size_t)w.task_index] + (int64_t)i * t.N + j)]++;
+		};
+		for (const WorkItem& w : lit) {
+			const GemmTask& t = b->tasks[(size_t)w.task_index];
+			if (t.M < LDS_TASK_MIN || t.N < LDS_TASK_MIN || w.first.K != b->terms[t.t0].K) return 8;
+			mark(w, LDS_TILE);
+		}
+		for (const WorkItem& w : rit) mark(w, TILE);
+		if (!once(cover)) return 8;
+	}
 	const WApplyHost& wa = g.wa;
 	if (wa.kmax <= WA_KMAX) {
 		std::vector<int> ws((size_t)g.t2_elems, 0);
@@ -1047,6 +1109,16 @@ void autotune_grouped(DevGemm* stage, const GemmBases* bases, const WApply* wa, 
 			if (ms < best) { best = ms; best_form = f; }
 		}
 		g.form = best_form;
+		// the large tasks on the LDS-tiled form, the rest on the best form (CT_GEMM_LDS=0|1 overrides)
+		if (g.nlitems > 0) {
+			g.use_lds = true;
+			const float ms = time3([&] { launch_grouped_gemm_f64(g, b, s); });
+			g.use_lds = ms < best;
+			if (g.use_lds) best = ms;
+			const char* force = getenv("CT_GEMM_LDS");
+			if (force && force[0] == '0') g.use_lds = false;
+			if (force && force[0] == '1') g.use_lds = true;
+		}
 		if (st == 0) best1 = best;
 		if (st == 1) best2 = best;
 	}

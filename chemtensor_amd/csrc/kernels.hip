@@ -694,6 +694,133 @@ __global__ __launch_bounds__(256) void k_fill(double* __restrict__ p, double v, 
 
 //________________________________________________________________________________________________________________________
 //
+// LDS-tiled form for large tasks: one workgroup per 64 x 64 output tile, each wavefront a 32 x 32
+// quadrant (2 x 2 MFMA 16x16x4 f64 sub-tiles). The k range of every term is cut into panels of 16;
+// the A panel (64 rows x 16) and the B panel (16 x 64 columns) of the next panel are loaded into
+// registers while the current panel's MFMAs run from LDS (double buffer, one barrier per panel).
+// Entries past a term's K or past the task's rows / columns are zeros. The k-steps run in the
+// order of the wavefront forms (terms in order, k ascending in steps of four).
+constexpr int LK = 16;                 // k-panel
+constexpr int LA_LD = LK + 1;          // A panel row stride in LDS (doubles)
+constexpr int LB_LD = LDS_TILE + 1;    // B panel row stride in LDS
+
+struct LdsPanelLoad
+{
+	double a[4], b[4];
+};
+
+__global__ __launch_bounds__(256) void k_grouped_gemm_f64_lds(const GemmTerm* __restrict__ terms, const WorkItem* __restrict__ items, GemmBases bs)
+{
+	__shared__ double sa[2][LDS_TILE * LA_LD];
+	__shared__ double sb[2][LK * LB_LD];
+	const int tid = threadIdx.x, lane = tid & 63;
+	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+	const WorkItem it = items[blockIdx.x];
+	const GemmTask& tk = it.task;
+	const int m0 = it.m0, n0 = it.n0;
+	// loads of this thread: A rows ra (4 consecutive k from ka), B column cb (k rows kb + 4 i)
+	const int ra = tid >> 2, ka = (tid & 3) * 4;
+	const int cb = tid & 63, kb = tid >> 6;
+	const int arow = m0 + ra, bcol = n0 + cb;
+	const bool va = arow < tk.M, vb = bcol < tk.N;
+	TermCache tc;
+	term_cache_start(tc, terms, tk, lane);
+	// the panel (t, k0) into registers
+	auto load_panel = [&](int t, int k0, LdsPanelLoad& r) {
+		if (t - tc.base >= 64) term_cache_load(tc, terms, t, tk.t1, lane);
+		const TermF f = term_fields(tc, t, it.first, tk.t0);
+		const double* __restrict__ A = rebase(bs.A, f.A);
+		const double* __restrict__ B = rebase(bs.B, f.B);
+		const int64_t ao = (int64_t)(va ? arow : 0) * f.sa_i;
+		const int64_t bo = b_col_offset(vb ? bcol : 0, tk.nci, f.sb_batch, f.sb_j);
+		#pragma unroll
+		for (int i = 0; i < 4; i++) {
+			const int k = k0 + ka + i;
+			r.a[i] = (va && k < f.K) ? A[ao + (int64_t)k * f.sa_k] : 0.0;
+		}
+		#pragma unroll
+		for (int i = 0; i < 4; i++) {
+			const int k = k0 + kb + 4 * i;
+			r.b[i] = (vb && k < f.K) ? B[bo + (int64_t)k * f.sb_k] : 0.0;
+		}
+	};
+	auto store_panel = [&](int buf, const LdsPanelLoad& r) {
+		#pragma unroll
+		for (int i = 0; i < 4; i++) sa[buf][ra * LA_LD + ka + i] = r.a[i];
+		#pragma unroll
+		for (int i = 0; i < 4; i++) sb[buf][(kb + 4 * i) * LB_LD + cb] = r.b[i];
+	};
+	// panels in order: (t, k0) with k0 = 0, 16, ... < K of term t
+	int t = tk.t0, k0 = 0;
+	// K of term t (the cache moved to cover t first)
+	auto term_k = [&](int tt) {
+		if (tt - tc.base >= 64) term_cache_load(tc, terms, tt, tk.t1, lane);
+		return term_K_at(tc, tt, it.first, tk.t0);
+	};
+	auto advance = [&]() {
+		k0 += LK;
+		while (t < tk.t1 && k0 >= term_k(t)) { t++; k0 = 0; }
+	};
+	// a term with K = 0 contributes nothing: skip to the first panel with entries
+	while (t < tk.t1 && term_k(t) <= 0) t++;
+	double4_t acc00 = { 0, 0, 0, 0 }, acc01 = acc00, acc10 = acc00, acc11 = acc00;
+	const int i16 = lane & 15, kq = lane >> 4;
+	const int wm = wave >> 1, wn = wave & 1;
+	const int ar0 = (32 * wm + i16) * LA_LD + kq, ar1 = ar0 + 16 * LA_LD;
+	const int bc0 = kq * LB_LD + 32 * wn + i16, bc1 = bc0 + 16;
+	if (t < tk.t1) {
+		LdsPanelLoad r;
+		load_panel(t, k0, r);
+		advance();
+		store_panel(0, r);
+		__syncthreads();
+		int buf = 0;
+		while (true) {
+			const bool more = t < tk.t1;
+			if (more) {
+				load_panel(t, k0, r);
+				advance();
+			}
+			const double* __restrict__ pa = sa[buf];
+			const double* __restrict__ pb = sb[buf];
+			#pragma unroll
+			for (int st = 0; st < LK / 4; st++) {
+				const double a0 = pa[ar0 + 4 * st], a1 = pa[ar1 + 4 * st];
+				const double b0 = pb[bc0 + 4 * st * LB_LD], b1 = pb[bc1 + 4 * st * LB_LD];
+				acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
+				acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
+				acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
+				acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+			}
+			if (!more) break;
+			store_panel(buf ^ 1, r);
+			__syncthreads();
+			buf ^= 1;
+		}
+	}
+	// store: register r of lane l holds D(l / 16 + 4 r, l % 16) of each 16 x 16 sub-tile
+	const int nci = tk.nci;
+	#pragma unroll
+	for (int sub = 0; sub < 4; sub++) {
+		const int mi = sub >> 1, ni = sub & 1;
+		const double4_t acc = (sub == 0 ? acc00 : sub == 1 ? acc01 : sub == 2 ? acc10 : acc11);
+		const int col = n0 + 32 * wn + 16 * ni + i16;
+		if (col >= tk.N) continue;
+		const int jb = col / nci, jc = col - jb * nci;
+		double* __restrict__ C = (double*)rebase(bs.C, tk.C) + (int64_t)jb * tk.sc_batch + jc;
+		#pragma unroll
+		for (int rr = 0; rr < 4; rr++) {
+			const int row = m0 + 32 * wm + 16 * mi + kq + 4 * rr;
+			if (row < tk.M) {
+				if (tk.accumulate) C[(int64_t)row * tk.sc_i] += acc[rr];
+				else C[(int64_t)row * tk.sc_i] = acc[rr];
+			}
+		}
+	}
+}
+
+//________________________________________________________________________________________________________________________
+//
 // launchers
 
 int num_compute_units()
@@ -709,9 +836,32 @@ int num_compute_units()
 	return cus;
 }
 
+// the wavefront / workgroup forms over an item list
+static void launch_gemm_form(const DevGemm& g, const WorkItem* items, int nitems, GemmBases b, hipStream_t s)
+{
+	if (nitems == 0) return;
+	const int cap = num_compute_units() * 8;
+	const int wave_grid = std::max(1, std::min((nitems + 3) / 4, cap));
+	const int wg_grid = std::max(1, std::min(nitems, cap));
+	switch (g.form) {
+		case GEMM_WAVE_STREAM: hipLaunchKernelGGL(k_stream_gemm_f64_wave, dim3(wave_grid), dim3(256), 0, s, g.tasks, g.terms, items, nitems, b); break;
+		case GEMM_WG_CHUNK: hipLaunchKernelGGL(k_grouped_gemm_f64_wg, dim3(wg_grid), dim3(256), 0, s, g.tasks, g.terms, items, nitems, b); break;
+		case GEMM_WG_STREAM: hipLaunchKernelGGL(k_stream_gemm_f64_wg, dim3(wg_grid), dim3(256), 0, s, g.tasks, g.terms, items, nitems, b); break;
+		default: hipLaunchKernelGGL(k_grouped_gemm_f64, dim3(wave_grid), dim3(256), 0, s, g.tasks, g.terms, items, nitems, b); break;
+	}
+	CTG_HIP(hipGetLastError());
+}
+
 void launch_grouped_gemm_f64(const DevGemm& g, GemmBases b, hipStream_t s)
 {
 	if (g.nitems == 0) return;
+	if (g.use_lds && g.nlitems > 0) {
+		// the large tasks on the LDS-tiled form, the others on the chosen form (disjoint outputs)
+		hipLaunchKernelGGL(k_grouped_gemm_f64_lds, dim3((unsigned)g.nlitems), dim3(256), 0, s, g.terms, g.litems, b);
+		CTG_HIP(hipGetLastError());
+		launch_gemm_form(g, g.ritems, g.nritems, b, s);
+		return;
+	}
 	const int wg_grid = std::max(1, std::min(g.nitems, num_compute_units() * 8));
 	switch (g.form) {
 		case GEMM_WAVE_STREAM: hipLaunchKernelGGL(k_stream_gemm_f64_wave, dim3(g.grid), dim3(256), 0, s, g.tasks, g.terms, g.items, g.nitems, b); break;

@@ -221,7 +221,8 @@ int ct_matvec_plan_stage_flops(const ct_matvec_plan* plan, double* flops3);
 /// per-stage algorithmic bytes: 8 (|inputs| + |output|) of each stage's launch
 int ct_matvec_plan_stage_bytes(const ct_matvec_plan* plan, double* bytes3);
 /// launch form chosen per stage (0 wave/simple, 1 wave/stream, 2 workgroup/chunk, 3 workgroup/stream,
-/// 4 column stream of stage 2, 5 stages 1 and 2 fused in one launch)
+/// 4 column stream of stage 2, 5 stages 1 and 2 fused in one launch, 6 LDS-tiled 64 x 64 workgroup
+/// tiles for the large tasks and the best of 0-3 for the others)
 /// and the autotuner's time (ms) of every form, [stage * 4 + form]; 0 where not timed
 int ct_matvec_plan_forms(const ct_matvec_plan* plan, int* forms3, float* tuned_ms12);
 /// average device time (ms) of one stage (0, 1, 2; the fused form has only stage 0) over 'reps'

@@ -119,3 +119,32 @@ def test_molecular_matvec_forms_match_oracle(n, chi):
         err = np.max(np.abs(got - want)) / np.max(np.abs(want))
         say("  mode", mode, "rel err", err)
         assert err < 1e-12
+
+
+@pytest.mark.parametrize("n,chi", [(12, 128), (20, 512)])
+def test_molecular_matvec_lds_form(n, chi, monkeypatch):
+    """The LDS-tiled GEMM form (64 x 64 workgroup tiles for the large tasks of the molecular
+    operator) forced on and off, against the oracle at 1e-12 and against each other (the forms
+    chosen without it split the k-steps of a tile differently, so the sums agree to rounding)."""
+    from oracle import oracle as orc
+    t, v = ct.molecular_coefficients(n, 5)
+    pm = ct.MPO.spin_molecular(t, v)
+    om = orc.MPOView(pm.ptr, pm.nsites, pm.d, pm.qsite)
+    theta, h2, left, right = orc.matvec_problem_mpo(om, orc.encode_pair(n, 0), chi, 2024)
+    want = orc.apply_local_hamiltonian(theta, h2, left, right).serialize()
+    x = ct.DeviceArray.from_numpy(theta.serialize())
+    ys = {}
+    for lds in ("1", "0"):
+        monkeypatch.setenv("CT_GEMM_LDS", lds)
+        plan = ct.MatvecPlan(theta, h2, left, right, ct.GROUPED_FP64)
+        forms = plan.forms()[0]
+        y = ct.DeviceArray(plan.dim)
+        plan.apply(x, y)
+        plan.synchronize()
+        ys[lds] = y.numpy()
+        err = np.max(np.abs(ys[lds] - want)) / np.max(np.abs(want))
+        say("molecular n", n, "CT_GEMM_LDS", lds, "forms", list(forms), "rel err", err)
+        assert err < 1e-12
+        if lds == "1":
+            assert 6 in list(forms)
+    assert np.max(np.abs(ys["1"] - ys["0"])) < 1e-13 * np.max(np.abs(ys["0"]))

@@ -29,3 +29,15 @@ def test_plan_check_rejects_incompatible_operands():
     theta, h2, left, right, _ = orc.matvec_problem("xxz", 12, 16, 1)
     other = orc.matvec_problem("xxz", 12, 24, 9)
     assert ct.matvec_plan_check(theta, h2, left, other[3]) == ct.CT_ERR_ARG
+
+
+@pytest.mark.parametrize("n,chi", [(12, 128), (20, 512)])
+def test_plan_tables_molecular(n, chi):
+    """The spin molecular operator's centre-bond problem (large W blocks): the grouped tables and the
+    LDS-tiled split of the large tasks (64 x 64 tiles, the other tasks' 32 x 32 items) cover every
+    output entry once."""
+    t, v = ct.molecular_coefficients(n, 5)
+    m = ct.MPO.spin_molecular(t, v)
+    om = orc.MPOView(m.ptr, m.nsites, m.d, m.qsite)
+    theta, h2, left, right = orc.matvec_problem_mpo(om, orc.encode_pair(n, 0), chi, 2024)
+    assert ct.matvec_plan_check(theta, h2, left, right) == 0

@@ -63,32 +63,61 @@ std::vector<WorkItem> GemmBatch::work_items() const
 	return items;
 }
 
-// the LDS-tiled form's 64 x 64 tiles of the large tasks (decreasing cost, ties in task and tile
-// order) and the other tasks' items; both empty-handed when no task is large
+// the LDS-tiled form's 64 x 64 tiles of the large tasks, and the other tasks' items (both empty when
+// no task is large). Workgroup i runs on XCD i mod 8, and each XCD has its own L2: the tiles of a
+// task are cut into groups of up to 4 x 8 tiles (sharing A rows and B columns), the groups are
+// dealt to the eight XCDs by load (largest first), and the item order interleaves the XCDs' queues,
+// so a group's tiles run together on one XCD and fetch their operand panels into one L2.
 void split_lds_items(const GemmBatch& b, const std::vector<WorkItem>& it, std::vector<WorkItem>& lit, std::vector<WorkItem>& rit)
 {
+	constexpr int NXCD = 8, GM = 4, GN = 8;
 	lit.clear();
 	rit.clear();
 	std::vector<uint8_t> big(b.tasks.size(), 0);
-	struct Key { int cost; int32_t task; int m0, n0; };
-	std::vector<Key> keys;
+	struct Group { int64_t cost; int32_t task; int mi0, mi1, ni0, ni1; };
+	std::vector<Group> groups;
+	// cost of a tile: its k-panels times the fraction of its 16 x 16 sub-tiles inside the task (the
+	// kernel runs the MFMAs of those only), plus one for the store
+	auto tile_cost = [](const GemmTask& t, int64_t panels, int mi, int ni) {
+		const int sr = std::min(4, (t.M - mi * LDS_TILE + 15) / 16), sc = std::min(4, (t.N - ni * LDS_TILE + 15) / 16);
+		return std::min<int64_t>((panels * sr * sc + 15) / 16 + 1, (int64_t)1 << 30);
+	};
+	std::vector<int64_t> task_panels(b.tasks.size(), 0);
 	for (size_t ti = 0; ti < b.tasks.size(); ti++) {
 		const GemmTask& t = b.tasks[ti];
 		if (t.M < LDS_TASK_MIN || t.N < LDS_TASK_MIN || t.t1 <= t.t0) continue;
 		big[ti] = 1;
 		int64_t panels = 0;
 		for (int q = t.t0; q < t.t1; q++) panels += (b.terms[q].K + 15) / 16;
-		const int cost = (int)std::min<int64_t>(panels + 1, (int64_t)1 << 30);
-		for (int m0 = 0; m0 < t.M; m0 += LDS_TILE)
-			for (int n0 = 0; n0 < t.N; n0 += LDS_TILE) keys.push_back(Key{ cost, (int32_t)ti, m0, n0 });
+		task_panels[ti] = panels;
+		const int mt = (t.M + LDS_TILE - 1) / LDS_TILE, nt = (t.N + LDS_TILE - 1) / LDS_TILE;
+		for (int mi = 0; mi < mt; mi += GM)
+			for (int ni = 0; ni < nt; ni += GN) {
+				const int mi1 = std::min(mt, mi + GM), ni1 = std::min(nt, ni + GN);
+				int64_t c = 0;
+				for (int a = mi; a < mi1; a++)
+					for (int bb = ni; bb < ni1; bb++) c += tile_cost(t, panels, a, bb);
+				groups.push_back(Group{ c, (int32_t)ti, mi, mi1, ni, ni1 });
+			}
 	}
-	std::stable_sort(keys.begin(), keys.end(), [](const Key& x, const Key& y) { return x.cost > y.cost; });
-	for (const Key& k : keys) {
-		const GemmTask& t = b.tasks[(size_t)k.task];
-		lit.push_back(WorkItem{ t, k.m0, k.n0, k.cost, k.task, b.terms[t.t0] });
+	if (groups.empty()) return;
+	std::stable_sort(groups.begin(), groups.end(), [](const Group& x, const Group& y) { return x.cost > y.cost; });
+	std::vector<std::vector<WorkItem>> q(NXCD);
+	std::vector<int64_t> load(NXCD, 0);
+	for (const Group& g : groups) {
+		const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+		load[(size_t)x] += g.cost;
+		const GemmTask& t = b.tasks[(size_t)g.task];
+		for (int mi = g.mi0; mi < g.mi1; mi++)
+			for (int ni = g.ni0; ni < g.ni1; ni++)
+				q[(size_t)x].push_back(WorkItem{ t, mi * LDS_TILE, ni * LDS_TILE, (int)tile_cost(t, task_panels[(size_t)g.task], mi, ni), g.task, b.terms[t.t0] });
 	}
-	if (!lit.empty())
-		for (const WorkItem& w : it) if (!big[(size_t)w.task_index]) rit.push_back(w);
+	size_t longest = 0;
+	for (const auto& v : q) longest = std::max(longest, v.size());
+	for (size_t jj = 0; jj < longest; jj++)
+		for (int x = 0; x < NXCD; x++)
+			if (jj < q[(size_t)x].size()) lit.push_back(q[(size_t)x][jj]);
+	for (const WorkItem& w : it) if (!big[(size_t)w.task_index]) rit.push_back(w);
 }
 
 void DevGemm::upload(const GemmBatch& b, hipStream_t s)

@@ -12,6 +12,7 @@
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <type_traits>
 #include "ct_internal.h"
 
 namespace ctg {
@@ -701,8 +702,8 @@ __global__ __launch_bounds__(256) void k_fill(double* __restrict__ p, double v, 
 // Entries past a term's K or past the task's rows / columns are zeros. The k-steps run in the
 // order of the wavefront forms (terms in order, k ascending in steps of four).
 constexpr int LK = 16;                 // k-panel
-constexpr int LA_LD = LK + 1;          // A panel row stride in LDS (doubles)
-constexpr int LB_LD = LDS_TILE + 1;    // B panel row stride in LDS
+constexpr int LA_LD = LDS_TILE;        // A panel stored k-major: entry (row, k) at k LA_LD + row
+constexpr int LB_LD = LDS_TILE;        // B panel: entry (k, col) at k LB_LD + col
 
 struct LdsPanelLoad
 {
@@ -711,7 +712,9 @@ struct LdsPanelLoad
 
 __global__ __launch_bounds__(256) void k_grouped_gemm_f64_lds(const GemmTerm* __restrict__ terms, const WorkItem* __restrict__ items, GemmBases bs)
 {
-	__shared__ double sa[2][LDS_TILE * LA_LD];
+	// both panels k-major, unpadded (the MFMA fragments read 16 consecutive rows / columns per k):
+	// 32 KB, five workgroups per CU
+	__shared__ double sa[2][LK * LA_LD];
 	__shared__ double sb[2][LK * LB_LD];
 	const int tid = threadIdx.x, lane = tid & 63;
 	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -746,7 +749,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64_lds(const GemmTerm* __
 	};
 	auto store_panel = [&](int buf, const LdsPanelLoad& r) {
 		#pragma unroll
-		for (int i = 0; i < 4; i++) sa[buf][ra * LA_LD + ka + i] = r.a[i];
+		for (int i = 0; i < 4; i++) sa[buf][(ka + i) * LA_LD + ra] = r.a[i];
 		#pragma unroll
 		for (int i = 0; i < 4; i++) sb[buf][(kb + 4 * i) * LB_LD + cb] = r.b[i];
 	};
@@ -766,9 +769,15 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64_lds(const GemmTerm* __
 	double4_t acc00 = { 0, 0, 0, 0 }, acc01 = acc00, acc10 = acc00, acc11 = acc00;
 	const int i16 = lane & 15, kq = lane >> 4;
 	const int wm = wave >> 1, wn = wave & 1;
-	const int ar0 = (32 * wm + i16) * LA_LD + kq, ar1 = ar0 + 16 * LA_LD;
+	const int ar0 = kq * LA_LD + 32 * wm + i16, ar1 = ar0 + 16;
 	const int bc0 = kq * LB_LD + 32 * wn + i16, bc1 = bc0 + 16;
-	if (t < tk.t1) {
+	// 16 x 16 sub-tiles of this wave holding output rows / columns (0, 1 or 2 each way): a wave runs
+	// only their MFMAs (the edge tiles of a task whose M or N is not a multiple of 64), and a wave with
+	// none still loads and stores its share of the panels
+	const int rows = tk.M - m0 - 32 * wm, cols = tk.N - n0 - 32 * wn;
+	const int nm = rows <= 0 ? 0 : rows <= 16 ? 1 : 2, nn = cols <= 0 ? 0 : cols <= 16 ? 1 : 2;
+	auto mainloop = [&](auto cm, auto cn) {
+		constexpr int NM = decltype(cm)::value, NN = decltype(cn)::value;
 		LdsPanelLoad r;
 		load_panel(t, k0, r);
 		advance();
@@ -783,20 +792,41 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64_lds(const GemmTerm* __
 			}
 			const double* __restrict__ pa = sa[buf];
 			const double* __restrict__ pb = sb[buf];
-			#pragma unroll
-			for (int st = 0; st < LK / 4; st++) {
-				const double a0 = pa[ar0 + 4 * st], a1 = pa[ar1 + 4 * st];
-				const double b0 = pb[bc0 + 4 * st * LB_LD], b1 = pb[bc1 + 4 * st * LB_LD];
-				acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
-				acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
-				acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
-				acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+			if constexpr (NM > 0 && NN > 0) {
+				#pragma unroll
+				for (int st = 0; st < LK / 4; st++) {
+					const double a0 = pa[ar0 + 4 * st * LA_LD], b0 = pb[bc0 + 4 * st * LB_LD];
+					acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
+					if constexpr (NN > 1) {
+						const double b1 = pb[bc1 + 4 * st * LB_LD];
+						acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
+						if constexpr (NM > 1) {
+							const double a1 = pa[ar1 + 4 * st * LA_LD];
+							acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
+							acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+						}
+					}
+					else if constexpr (NM > 1) {
+						const double a1 = pa[ar1 + 4 * st * LA_LD];
+						acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
+					}
+				}
 			}
 			if (!more) break;
 			store_panel(buf ^ 1, r);
 			__syncthreads();
 			buf ^= 1;
 		}
+	};
+	typedef std::integral_constant<int, 0> c0;
+	typedef std::integral_constant<int, 1> c1;
+	typedef std::integral_constant<int, 2> c2;
+	if (t < tk.t1) {
+		if (nm == 2 && nn == 2) mainloop(c2(), c2());
+		else if (nm == 2 && nn == 1) mainloop(c2(), c1());
+		else if (nm == 1 && nn == 2) mainloop(c1(), c2());
+		else if (nm == 1 && nn == 1) mainloop(c1(), c1());
+		else mainloop(c0(), c0());
 	}
 	// store: register r of lane l holds D(l / 16 + 4 r, l % 16) of each 16 x 16 sub-tile
 	const int nci = tk.nci;

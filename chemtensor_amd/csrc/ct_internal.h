@@ -399,7 +399,7 @@ void launch_fused_matvec(const FusedPlan& f, const double* x, const double* w, c
 constexpr int PR_THREADS = 256;
 constexpr int PR_LDS = 4608;                          // doubles of operand / intermediate LDS per workgroup (36 KB)
 constexpr int PR_REC_WORDS = 512;                     // largest unit record (4-byte words)
-constexpr int PR_RMAX = 32, PR_KMAX = 32;             // largest class W_D (rows x columns), held in registers
+constexpr int PR_RMAX = 32, PR_KMAX = 32;             // largest class W_D (rows x columns)
 struct PrRed { int64_t y, p; int32_t n, size, nslots, pad; };   // up to 64 entries of one y block
 struct PairHost
 {

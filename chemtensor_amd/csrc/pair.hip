@@ -6,12 +6,12 @@
 //   T2(A,M,Q,C)[a m q c] = sum_pn W(M,Q,P,N)[m q p n] T1(A,P,N,C)[a p n c]           qM + qQ = qP + qN
 //   y(X,Q,C)[x q c]      = sum_am L(o,A,M,X)[0 a m x] T2(A,M,Q,C)[a m q c]           qX = qA + qM - qo
 // and every intermediate block belongs to exactly one block pair (A, C) of the left and right bond
-// sectors, whose class D = qC - qA + qo fixes the (P,N) "routes" and (M,Q) "rows" involved. A
-// workgroup takes one pair, or a chunk of its rows a and columns c: it stages the theta rows, R
-// columns and L rows of the chunk in LDS (every load in flight at once), computes T1, then
-// T2 = W_D T1 with the class's dense W_D held in registers, then L^T T2, all on
-// v_mfma_f64_16x16x4f64 with operands from LDS, and writes its contribution to each y(X,Q,C) as a
-// partial block. A second launch sums the partial blocks of every y block in a fixed order (by A,
+// sectors, whose class D = qC - qA + qo fixes the (P,N) "routes" and (M,Q) "rows" involved. A pair,
+// or a chunk of its rows a and columns c, is staged in LDS (its theta rows, R columns, L rows and the
+// class's dense W_D); a workgroup takes one or several such chunks (small ones packed together up to
+// the LDS budget, so that every unit is resident at once), loads all of their data at once, computes
+// T1, then T2 = W_D T1, then L^T T2, all on v_mfma_f64_16x16x4f64 with operands from LDS, and writes
+// each chunk's contribution to each y(X,Q,C) as a partial block. A second launch sums the partial blocks of every y block in a fixed order (by A,
 // then row chunk), so the result does not depend on scheduling. Nothing of T1 / T2 reaches HBM.
 //
 // The host resolves the block structure into per-unit tables (staging pieces, MFMA tiles with their
@@ -33,14 +33,20 @@ namespace ctg {
 
 typedef double pr_double4 __attribute__((ext_vector_type(4)));
 
-// Unit record (4-byte words, fixed stride per plan):
-//   header (16): n1 n3 npiece R16 | K4 S1 tc mag(tc) | t2 pad off1 off3 | offk offp wb kb
-//   (t2, wb, kb: LDS offsets of T2, the class's W_D copy, the stage-2 row map)
+// Unit record (4-byte words, fixed stride per plan). A unit holds one or more chunks of block pairs
+// (small chunks are packed together up to the LDS budget), each in its own LDS region above the
+// shared zero / dump area; every LDS offset below is absolute.
+//   header (16): n1 n3 npiece n2 | 0 0 0 0 | 0 0 off1 off3 | off2 offp 0 0
+//   (offp = 16: the pieces follow the header, so their descriptors load in the header's round trip)
 //   stage-1 tiles at off1 (8 words each): abase sth bbase Nr nk vr vc sbase
-//   stage-3 tiles at off3 (16 words each): lbase dX t2b dQ*S1 dM ta nka vr vc j0 tc dC mag(tc)
-//                                          pbase(2) dQ*dC
-//   stage-2 row map at offk (even; 2 words per T1 row k < K4): LDS offset of row k at a = 0, c = 0,
-//                                          and its stride in a (0, 0 for absent rows: the zero region)
+//   stage-2 tiles at off2 (8 words each): wbase wld|nk4<<16 kb tc | mag(tc) S1 j0 t2out
+//     (W_D rows of the tile from LDS wbase at stride wld, nk4 k-steps of four, the chunk's row map at
+//     kb, columns j0 .. j0+15 of the chunk's S1 = ta tc columns (a c), output rows at t2out, stride S1)
+//   stage-3 tiles at off3 (16 words each): lbase dX t2b dQ*S1 | dM ta nka vr|vc<<16 | S1 j0 tc dC |
+//                                          mag(tc) pbase(2) dQ*dC
+//   row maps after the tiles (even offsets; 2 words per T1 row k < K4 of a chunk): LDS offset of row k
+//                                          at a = 0, c = 0, and its stride in a (0, 0 for absent rows:
+//                                          the zero region), staged by the chunk's kind-5 piece
 //   pieces at offp (4 words each), 64 consecutive entries e = e0 .. e0+63 of a staged panel whose
 //   entry (row, col) = divmod(e, cols) goes to LDS d0 + row dstride + col from source
 //   s0 + row sstride + col (or is zero): with (row0, col0) = divmod(e0, cols),
@@ -52,7 +58,9 @@ typedef double pr_double4 __attribute__((ext_vector_type(4)));
 //   unit's own record read as doubles (the row map).
 // mag(d) = ceil(2^31 / d): t / d = mulhi(2 t, mag(d)) for 0 <= t < 2^31 / d; mag16(d) = ceil(2^16 / d):
 // t / d = (t mag16(d)) >> 16 for t (t + 1) < 2^16 ... exact whenever t cols < 2^16.
-constexpr int PR_HDR = 16, PR_T1 = 8, PR_T3 = 16, PR_PIECE = 4;
+constexpr int PR_HDR = 16, PR_T1 = 8, PR_T2 = 8, PR_T3 = 16, PR_PIECE = 4;
+// words after the last record: the kernel's piece-slot loads (NB <= 64 slots of 4 words from PR_HDR)
+constexpr size_t PR_REC_PAD = PR_HDR + PR_PIECE * 4 * 64;
 constexpr int PR_ZERO = 64;   // LDS doubles [0, 64) hold zeros
 constexpr int PR_DUMP = 64;   // LDS double taking the stores of lanes outside a tile (never read)
 constexpr int PR_WB = 80;     // LDS offset of the W_D copy
@@ -82,21 +90,25 @@ __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __res
 	const int i16 = lane & 15, kq = lane >> 4;
 	const uint32_t* __restrict__ g = recs + (size_t)blockIdx.x * (size_t)rec_words;
 
-	const int n1 = pr_s(g[0]), n3 = pr_s(g[1]), npiece = pr_s(g[2]), R16 = pr_s(g[3]);
-	const int K4 = pr_s(g[4]), S1 = pr_s(g[5]), tc = pr_s(g[6]);
-	const uint32_t magtc = (uint32_t)pr_s(g[7]);
-	const int t2 = pr_s(g[8]), off1 = pr_s(g[10]), off3 = pr_s(g[11]);
-	const int offp = pr_s(g[13]), wb = pr_s(g[14]), kb = pr_s(g[15]);
+	const int n1 = pr_s(g[0]), n3 = pr_s(g[1]), npiece = pr_s(g[2]), n2 = pr_s(g[3]);
+	const int off1 = pr_s(g[10]), off3 = pr_s(g[11]), off2 = pr_s(g[12]);
 
-	// lane j of wavefront w holds the descriptor of piece w + 4 j
+	// lane j of wavefront w holds the descriptor of piece w + 4 j: loaded at the fixed offset PR_HDR
+	// without waiting for the header (the record table is padded so that the NB slots of the last
+	// record stay in bounds; slots past npiece are never used)
 	uint4 pd = { 0u, 0u, 0u, 0u };
-	if (lane < NB && wave + 4 * lane < npiece) pd = reinterpret_cast<const uint4*>(g + offp)[wave + 4 * lane];
+	if (lane < NB) pd = reinterpret_cast<const uint4*>(g + PR_HDR)[wave + 4 * lane];
 	// the wavefront's stage-1 tiles w + 4 j and stage-3 tiles w + 4 j, one per lane, in the same round trip
-	uint4 d1a = { 0u, 0u, 0u, 0u }, d1b = d1a, d3a = d1a, d3b = d1a, d3c = d1a, d3d = d1a;
+	uint4 d1a = { 0u, 0u, 0u, 0u }, d1b = d1a, d2a = d1a, d2b = d1a, d3a = d1a, d3b = d1a, d3c = d1a, d3d = d1a;
 	if (wave + 4 * lane < n1) {
 		const uint4* d = reinterpret_cast<const uint4*>(g + off1 + PR_T1 * (wave + 4 * lane));
 		d1a = d[0];
 		d1b = d[1];
+	}
+	if (wave + 4 * lane < n2) {
+		const uint4* d = reinterpret_cast<const uint4*>(g + off2 + PR_T2 * (wave + 4 * lane));
+		d2a = d[0];
+		d2b = d[1];
 	}
 	if (wave + 4 * lane < n3) {
 		const uint4* d = reinterpret_cast<const uint4*>(g + off3 + PR_T3 * (wave + 4 * lane));
@@ -108,6 +120,8 @@ __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __res
 	pr_fresh(pd);
 	pr_fresh(d1a);
 	pr_fresh(d1b);
+	pr_fresh(d2a);
+	pr_fresh(d2b);
 	pr_fresh(d3a);
 	pr_fresh(d3b);
 	pr_fresh(d3c);
@@ -163,53 +177,35 @@ __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __res
 	}
 	__syncthreads();
 
-	// stage 2: T2[(M Q m q)][(a c)] = W_D T1 (rows of the row map, columns a tc + c)
-	{
-		// the class's W_D (R16 x K4 at row stride K4 + 1, zero padded) as MFMA A fragments (row
-		// 16 mi + l % 16, column 4 ks + l / 16) and the row map of this lane's T1 rows 4 ks + l / 16
-		double wa[PR_RMAX / 16][PR_KMAX / 4];
-		int kofs[PR_KMAX / 4], kstr[PR_KMAX / 4];
-		const int2* __restrict__ kmap = reinterpret_cast<const int2*>(sm + kb);
-		#pragma unroll
-		for (int ks = 0; ks < PR_KMAX / 4; ks++) {
-			const bool vk = 4 * ks < K4;
-			const int2 e = vk ? kmap[4 * ks + kq] : int2{ 0, 0 };
-			kofs[ks] = e.x;
-			kstr[ks] = e.y;
-			#pragma unroll
-			for (int mi = 0; mi < PR_RMAX / 16; mi++) wa[mi][ks] = (vk && 16 * mi < R16) ? sm[wb + (16 * mi + i16) * (K4 + 1) + 4 * ks + kq] : 0.0;
+	// stage 2: T2[(M Q m q)][(a c)] = W_D T1 per tile of 16 rows (M Q m q) and 16 columns (a c) of a
+	// pair: W_D row i, column k at LDS wbase + i wld + k; T1 row k of column (a, c) at LDS
+	// kofs(k) + a kstr(k) + c from the pair's row map (absent rows: 0, 0, the zero region)
+	for (int t = wave, tl = 0; t < n2; t += 4, tl++) {
+		const int wbase = pr_lane(d2a.x, tl), wq = pr_lane(d2a.y, tl), kbb = pr_lane(d2a.z, tl), tc2 = pr_lane(d2a.w, tl);
+		const uint32_t mag2 = (uint32_t)pr_lane(d2b.x, tl);
+		const int S1 = pr_lane(d2b.y, tl), j0 = pr_lane(d2b.z, tl), t2o = pr_lane(d2b.w, tl);
+		const int wld = wq & 0xffff, nk4 = wq >> 16;
+		const int j2 = j0 + i16;
+		const bool vj = j2 < S1;
+		const int a = vj ? pr_q(j2, mag2) : 0, c = vj ? j2 - a * tc2 : 0;
+		const int2* __restrict__ kmap = reinterpret_cast<const int2*>(sm + kbb) + kq;
+		const int wrow = wbase + __mul24(i16, wld) + kq;
+		pr_double4 acc = { 0, 0, 0, 0 };
+		for (int ks = 0; ks < nk4; ks++) {
+			const int2 e = kmap[4 * ks];
+			acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sm[wrow + 4 * ks], sm[e.x + __mul24(a, e.y) + c], acc, 0, 0, 0);
 		}
-		const int nt2 = (S1 + 15) >> 4;
 		#pragma unroll
-		for (int mi = 0; mi < PR_RMAX / 16; mi++) {
-			if (16 * mi >= R16) break;
-			for (int ni = wave; ni < nt2; ni += 4) {
-				const int j2 = 16 * ni + i16;
-				const bool vj = j2 < S1;
-				const int a = vj ? pr_q(j2, magtc) : 0, c = vj ? j2 - a * tc : 0;
-				// rows k >= K4 read the zero region (row map 0, 0) against zero W_D columns
-				pr_double4 acc = { 0, 0, 0, 0 };
-				#pragma unroll
-				for (int h = 0; h < PR_KMAX / 16; h++) {
-					if (16 * h >= K4) break;
-					double b[4];
-					#pragma unroll
-					for (int u = 0; u < 4; u++) b[u] = sm[kofs[4 * h + u] + __mul24(a, kstr[4 * h + u]) + c];
-					#pragma unroll
-					for (int u = 0; u < 4; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[mi][4 * h + u], b[u], acc, 0, 0, 0);
-				}
-				#pragma unroll
-				for (int r4 = 0; r4 < 4; r4++) sm[vj ? t2 + __mul24(16 * mi + kq + 4 * r4, S1) + j2 : PR_DUMP] = acc[r4];
-			}
-		}
+		for (int r4 = 0; r4 < 4; r4++) sm[vj ? t2o + __mul24(kq + 4 * r4, S1) + j2 : PR_DUMP] = acc[r4];
 	}
 	__syncthreads();
 
 	// stage 3: partial y(X,Q,C)[x, (q c)] = L(o,A,M,X)[(a m), x]^T T2(A,M,Q,C)[(a m), (q c)]
 	for (int t = wave, tl = 0; t < n3; t += 4, tl++) {
 		const int lbase = pr_lane(d3a.x, tl), dX = pr_lane(d3a.y, tl), t2b = pr_lane(d3a.z, tl), dQS1 = pr_lane(d3a.w, tl);
-		const int dM = pr_lane(d3b.x, tl), ta = pr_lane(d3b.y, tl), nka = pr_lane(d3b.z, tl), vr = pr_lane(d3b.w, tl);
-		const int vc = pr_lane(d3c.x, tl), j0 = pr_lane(d3c.y, tl), tc3 = pr_lane(d3c.z, tl), dC = pr_lane(d3c.w, tl);
+		const int dM = pr_lane(d3b.x, tl), ta = pr_lane(d3b.y, tl), nka = pr_lane(d3b.z, tl), vrc = pr_lane(d3b.w, tl);
+		const int S1 = pr_lane(d3c.x, tl), j0 = pr_lane(d3c.y, tl), tc3 = pr_lane(d3c.z, tl), dC = pr_lane(d3c.w, tl);
+		const int vr = vrc & 0xffff, vc = vrc >> 16;
 		const uint32_t mag3 = (uint32_t)pr_lane(d3d.x, tl);
 		const int64_t pbase = (int64_t)((uint64_t)(uint32_t)pr_lane(d3d.y, tl) | ((uint64_t)(uint32_t)pr_lane(d3d.z, tl) << 32));
 		const int dQdC = pr_lane(d3d.w, tl);
@@ -317,6 +313,16 @@ struct PrClass
 // a stage-3 tile's partial offset: patched with its slot once all slots are known
 struct PrSlotRef { int64_t yblock; ct_long A; int ia; size_t unit; int word; };
 
+// one chunk of a block pair, its LDS offsets laid out from PR_WB (moved up when packed into a unit)
+struct PrPart
+{
+	int need = 0;                                  // LDS end offset (doubles)
+	int np = 0, n1 = 0, n2 = 0, n3 = 0, mfma = 0;
+	size_t kpiece = 0;                             // word of pcs holding the row-map piece's source
+	std::vector<uint32_t> pcs, t1s, t2s, t3s, kt;
+	std::vector<PrSlotRef> yref;                   // per stage-3 tile: its y block, A and row chunk
+};
+
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 inline int up4(int a) { return (a + 3) / 4 * 4; }
 inline uint32_t magic(int d) { return (uint32_t)((((uint64_t)1 << 31) + (uint64_t)d - 1) / (uint64_t)d); }
@@ -370,6 +376,7 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 	struct VRoute { int64_t th, r; int dP, dB, dN, kbase; };
 	struct VRow { int64_t l; int64_t yblock; int dM, dQ, dX, rbase; };
 	std::vector<PrSlotRef> refs;
+	std::vector<PrPart> parts;
 	const int nbp = 4 * (h.lds / PR_THREADS);   // pieces per unit
 
 	std::vector<VRoute> vr;
@@ -416,12 +423,12 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 				for (auto& v : vq) n += cdiv(ta * v.dM * v.dX, 64);
 				return n;
 			};
-			// tiles of stages 1 and 3 (one descriptor per lane of a wavefront: at most 4 x 64 each)
+			// tiles of stages 1, 2 and 3 (one descriptor per lane of a wavefront: at most 4 x 64 each)
 			auto tiles = [&](int ta, int tc) {
 				int n1 = 0, n3 = 0;
 				for (auto& v : vr) n1 += cdiv(ta * v.dP, 16) * cdiv(v.dN * tc, 16);
 				for (auto& v : vq) n3 += cdiv(v.dX, 16) * cdiv(v.dQ * tc, 16);
-				return std::max(n1, n3);
+				return std::max(std::max(n1, n3), (cl.R16 / 16) * cdiv(ta * tc, 16));
 			};
 			auto mfmas = [&](int ta, int tc) {
 				int n = 0;
@@ -458,8 +465,14 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 					}
 					const int need = std::max(off, lt2 + cl.R16 * S1);
 					if (need > h.lds || need > 0xffff) return false;
-					h.max_lds = std::max(h.max_lds, need);
-					std::vector<uint32_t> t1s, t3s, kt((size_t)cl.K4 * 2, 0u), pcs;
+					PrPart pt;
+					pt.need = need;
+					std::vector<uint32_t>& t1s = pt.t1s;
+					std::vector<uint32_t>& t2s = pt.t2s;
+					std::vector<uint32_t>& t3s = pt.t3s;
+					std::vector<uint32_t>& kt = pt.kt;
+					std::vector<uint32_t>& pcs = pt.pcs;
+					kt.assign((size_t)cl.K4 * 2, 0u);
 					// stage-1 tiles
 					for (size_t i = 0; i < vr.size(); i++) {
 						const int Mr = tA * vr[i].dP, Nr = vr[i].dN * tC, sth = up4(vr[i].dB) + 1;
@@ -471,6 +484,13 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 								t1s.insert(t1s.end(), d, d + PR_T1);
 							}
 					}
+					// stage-2 tiles: 16 rows of W_D by 16 columns (a c) of the chunk
+					for (int mi = 0; mi < cl.R16 / 16; mi++)
+						for (int ni = 0; ni < cdiv(S1, 16); ni++) {
+							const uint32_t d[PR_T2] = { (uint32_t)(lwb + 16 * mi * (cl.K4 + 1)), (uint32_t)(cl.K4 + 1) | ((uint32_t)(cl.K4 / 4) << 16),
+								(uint32_t)lkb, (uint32_t)tC, magic(tC), (uint32_t)S1, (uint32_t)(16 * ni), (uint32_t)(lt2 + 16 * mi * S1) };
+							t2s.insert(t2s.end(), d, d + PR_T2);
+						}
 					// stage-2 row map: T1 row k = kbase + p dN + n of a route -> the route's T1 at (a dP + p, n tc + c)
 					for (size_t i = 0; i < vr.size(); i++) {
 						const int Nr = vr[i].dN * tC;
@@ -487,10 +507,10 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 							for (int ni = 0; ni < cdiv(Nq, 16); ni++) {
 								const int64_t rel = (int64_t)16 * mi * dQ * dC + c0;
 								const uint32_t d[PR_T3] = { (uint32_t)(lds_l[i] + 16 * mi), (uint32_t)dX, (uint32_t)(lt2 + vq[i].rbase * S1), (uint32_t)(dQ * S1),
-									(uint32_t)dM, (uint32_t)tA, (uint32_t)cdiv(tA, 4), (uint32_t)std::min(16, dX - 16 * mi), (uint32_t)std::min(16, Nq - 16 * ni),
-									(uint32_t)(16 * ni), (uint32_t)tC, (uint32_t)dC, magic(tC),
+									(uint32_t)dM, (uint32_t)tA, (uint32_t)cdiv(tA, 4), (uint32_t)std::min(16, dX - 16 * mi) | ((uint32_t)std::min(16, Nq - 16 * ni) << 16),
+									(uint32_t)S1, (uint32_t)(16 * ni), (uint32_t)tC, (uint32_t)dC, magic(tC),
 									(uint32_t)rel, (uint32_t)((uint64_t)rel >> 32), (uint32_t)(dQ * dC) };
-								refs.push_back(PrSlotRef{ vq[i].yblock, A, ia, h.recs.size(), (int)(t3s.size() + 13) });
+								pt.yref.push_back(PrSlotRef{ vq[i].yblock, A, ia, 0, 0 });
 								t3s.insert(t3s.end(), d, d + PR_T3);
 							}
 					}
@@ -520,40 +540,121 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 						ok = ok && add_pieces(vq[i].l + (int64_t)a0 * vq[i].dM * vq[i].dX, lds_l[i], n, 64, 64, 64, 64, 2);
 					}
 					ok = ok && add_pieces(cl.wd, lwb, cl.R16 * (cl.K4 + 1), 64, 64, 64, 64, 4);
-					// the row map: the record's words from offk, copied as doubles (offk even); its
-					// source offset is set below once offk is known
-					const size_t kpiece = pcs.size();
+					// the row map: the record's own words, copied as doubles (even offset); its source
+					// offset is set when the unit's record is assembled
+					pt.kpiece = pcs.size();
 					ok = ok && add_pieces(0, lkb, cl.K4, 64, 64, 64, 64, 5);
 					const int np = (int)(pcs.size() / PR_PIECE);
-					if (!ok || np > nbp) return false;
-					std::vector<uint32_t> rec(PR_HDR, 0u);
-					rec[0] = (uint32_t)(t1s.size() / PR_T1); rec[1] = (uint32_t)(t3s.size() / PR_T3); rec[2] = (uint32_t)np; rec[3] = (uint32_t)cl.R16;
-					rec[4] = (uint32_t)cl.K4; rec[5] = (uint32_t)S1; rec[6] = (uint32_t)tC; rec[7] = magic(tC);
-					rec[8] = (uint32_t)lt2; rec[14] = (uint32_t)lwb; rec[15] = (uint32_t)lkb;
-					rec[10] = (uint32_t)rec.size();
-					rec.insert(rec.end(), t1s.begin(), t1s.end());
-					rec[11] = (uint32_t)rec.size();
-					for (size_t k = refs.size() - t3s.size() / PR_T3; k < refs.size(); k++) refs[k].word += (int)rec.size();
-					rec.insert(rec.end(), t3s.begin(), t3s.end());
-					if (rec.size() % 2) rec.push_back(0u);
-					rec[12] = (uint32_t)rec.size();
-					pcs[kpiece] = rec[12] / 2;
-					rec.insert(rec.end(), kt.begin(), kt.end());
-					while (rec.size() % 4) rec.push_back(0u);
-					rec[13] = (uint32_t)rec.size();
-					rec.insert(rec.end(), pcs.begin(), pcs.end());
-					h.max_words = std::max(h.max_words, (int)rec.size());
+					if (!ok || np > nbp || cl.K4 > 64) return false;
+					pt.np = np;
+					pt.n1 = (int)(t1s.size() / PR_T1);
+					pt.n2 = (int)(t2s.size() / PR_T2);
+					pt.n3 = (int)(t3s.size() / PR_T3);
+					pt.mfma = mfmas(tA, tC);
 					// flops as executed (algorithmic: no tile padding)
 					double f = 0;
 					for (auto& v : vr) f += 2.0 * tA * v.dP * v.dB * (double)(v.dN * tC);
 					f += 2.0 * (double)cl.Rd * cl.Kd * S1;
 					for (auto& v : vq) f += 2.0 * v.dX * (double)(tA * v.dM) * (v.dQ * tC);
 					h.flops += f;
-					h.cost.push_back((double)mfmas(tA, tC) + (double)np / 4.0 + 4.0);
-					h.recs.push_back(std::move(rec));
+					parts.push_back(std::move(pt));
 				}
 			}
 		}
+	}
+	// units: the chunks, largest LDS need first, each into the first unit that still holds it within
+	// the LDS, piece, tile and MFMA limits (first fit decreasing), else into a new unit
+	std::vector<size_t> pord(parts.size());
+	std::iota(pord.begin(), pord.end(), (size_t)0);
+	std::stable_sort(pord.begin(), pord.end(), [&](size_t i, size_t j) { return parts[i].need > parts[j].need; });
+	struct PrBin { int used, np, n1, n2, n3, mfma; std::vector<size_t> parts; };
+	std::vector<PrBin> bins;
+	std::vector<size_t> open;   // units that may take another chunk (each chunk has at least 5 pieces)
+	for (size_t pi : pord) {
+		const PrPart& pt = parts[pi];
+		const int sz = pt.need - PR_WB;
+		size_t bi = bins.size();
+		for (size_t oi = 0; oi < open.size(); oi++) {
+			const PrBin& b = bins[open[oi]];
+			if (b.used + sz <= h.lds && b.np + pt.np <= nbp && b.n1 + pt.n1 <= 4 * 64 && b.n2 + pt.n2 <= 4 * 64 && b.n3 + pt.n3 <= 4 * 64 &&
+				b.mfma + pt.mfma <= h.mfma_cap) {
+				bi = open[oi];
+				break;
+			}
+		}
+		if (bi == bins.size()) {
+			bins.push_back(PrBin{ PR_WB, 0, 0, 0, 0, 0, {} });
+			open.push_back(bi);
+		}
+		PrBin& b = bins[bi];
+		b.used += sz;
+		b.np += pt.np; b.n1 += pt.n1; b.n2 += pt.n2; b.n3 += pt.n3; b.mfma += pt.mfma;
+		b.parts.push_back(pi);
+		if (b.np + 5 > nbp || h.lds - b.used < 64) open.erase(std::find(open.begin(), open.end(), bi));
+	}
+	// records: header | pieces | stage-1 tiles | stage-2 tiles | stage-3 tiles | row maps, every chunk
+	// moved up by the LDS of the chunks before it in its unit
+	for (PrBin& b : bins) {
+		std::vector<int> delta(b.parts.size());
+		for (size_t q = 0, at = 0; q < b.parts.size(); q++) { delta[q] = (int)at; at += (size_t)(parts[b.parts[q]].need - PR_WB); }
+		auto moved = [&](const std::vector<uint32_t>& v, int stride, std::initializer_list<int> words, int dl) {
+			std::vector<uint32_t> o = v;
+			for (size_t e = 0; e < o.size(); e += (size_t)stride)
+				for (int wdx : words) o[e + (size_t)wdx] += (uint32_t)dl;
+			return o;
+		};
+		std::vector<uint32_t> rec(PR_HDR, 0u);
+		std::vector<size_t> kword(b.parts.size());
+		rec[13] = (uint32_t)rec.size();
+		for (size_t q = 0; q < b.parts.size(); q++) {
+			const PrPart& pt = parts[b.parts[q]];
+			kword[q] = rec.size() + pt.kpiece;
+			for (size_t e = 0; e < pt.pcs.size(); e += PR_PIECE) {
+				uint32_t w1 = pt.pcs[e + 1];
+				const uint32_t d0 = (w1 & 0xffffu) + (uint32_t)delta[q];
+				if (d0 > 0xffffu) return false;
+				w1 = (w1 & 0xffff0000u) | d0;
+				rec.insert(rec.end(), { pt.pcs[e], w1, pt.pcs[e + 2], pt.pcs[e + 3] });
+			}
+		}
+		rec[10] = (uint32_t)rec.size();
+		for (size_t q = 0; q < b.parts.size(); q++) {
+			const std::vector<uint32_t> v = moved(parts[b.parts[q]].t1s, PR_T1, { 0, 2, 7 }, delta[q]);
+			rec.insert(rec.end(), v.begin(), v.end());
+		}
+		rec[12] = (uint32_t)rec.size();
+		for (size_t q = 0; q < b.parts.size(); q++) {
+			const std::vector<uint32_t> v = moved(parts[b.parts[q]].t2s, PR_T2, { 0, 2, 7 }, delta[q]);
+			rec.insert(rec.end(), v.begin(), v.end());
+		}
+		rec[11] = (uint32_t)rec.size();
+		for (size_t q = 0; q < b.parts.size(); q++) {
+			const PrPart& pt = parts[b.parts[q]];
+			for (size_t t = 0; t < pt.yref.size(); t++) {
+				PrSlotRef rf = pt.yref[t];
+				rf.unit = h.recs.size();
+				rf.word = (int)(rec.size() + 13);
+				refs.push_back(rf);
+				std::vector<uint32_t> d(pt.t3s.begin() + (ptrdiff_t)(t * PR_T3), pt.t3s.begin() + (ptrdiff_t)((t + 1) * PR_T3));
+				d[0] += (uint32_t)delta[q];
+				d[2] += (uint32_t)delta[q];
+				rec.insert(rec.end(), d.begin(), d.end());
+			}
+		}
+		for (size_t q = 0; q < b.parts.size(); q++) {
+			std::vector<uint32_t> kt = parts[b.parts[q]].kt;
+			for (size_t k = 0; k + 1 < kt.size(); k += 2)
+				if (kt[k] != 0u || kt[k + 1] != 0u) kt[k] += (uint32_t)delta[q];
+			if (rec.size() % 2) rec.push_back(0u);
+			rec[kword[q]] = (uint32_t)(rec.size() / 2);
+			rec.insert(rec.end(), kt.begin(), kt.end());
+		}
+		while (rec.size() % 4) rec.push_back(0u);
+		rec[0] = (uint32_t)b.n1; rec[1] = (uint32_t)b.n3; rec[2] = (uint32_t)b.np; rec[3] = (uint32_t)b.n2;
+		h.max_words = std::max(h.max_words, (int)rec.size());
+		h.max_lds = std::max(h.max_lds, b.used);
+		h.cost.push_back((double)b.mfma + (double)b.np / 4.0 + 4.0);
+		h.recs.push_back(std::move(rec));
 	}
 	// partial slots: one per (y block, A, row chunk), grouped by y block in (A, row chunk) order;
 	// the column chunks of a pair write disjoint columns of the same slot
@@ -633,18 +734,19 @@ int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r,
 		const std::vector<uint32_t>& rec = h.recs[u];
 		if (rec.size() < (size_t)PR_HDR || (int)rec.size() > rec_words) return 2;
 		const uint32_t* g = rec.data();
-		const int n1 = (int)g[0], n3 = (int)g[1], npiece = (int)g[2], R16 = (int)g[3];
-		const int K4 = (int)g[4], S1 = (int)g[5], tc = (int)g[6];
-		const uint32_t magtc = g[7];
-		const int t2 = (int)g[8], off1 = (int)g[10], off3 = (int)g[11], offk = (int)g[12], offp = (int)g[13], wb = (int)g[14], kb = (int)g[15];
-		if ((size_t)off1 + (size_t)PR_T1 * n1 > rec.size() || (size_t)off3 + (size_t)PR_T3 * n3 > rec.size() || (size_t)offk + 2 * (size_t)K4 > rec.size()) return 3;
-		if (offp % 4 || offk % 2 || (size_t)offp + (size_t)PR_PIECE * npiece != rec.size()) return 3;
-		if (R16 > PR_RMAX || K4 > PR_KMAX || R16 % 16 || K4 % 4 || wb != PR_WB || kb != PR_WB + R16 * (K4 + 1)) return 4;
-		if (npiece > 4 * NB || tc < 1 || tc > PR_ZERO || S1 < tc || S1 % tc || n1 > 4 * 64 || n3 > 4 * 64) return 5;
+		const int n1 = (int)g[0], n3 = (int)g[1], npiece = (int)g[2], n2 = (int)g[3];
+		const int off1 = (int)g[10], off3 = (int)g[11], off2 = (int)g[12], offp = (int)g[13];
+		if ((size_t)off1 + (size_t)PR_T1 * n1 > rec.size() || (size_t)off2 + (size_t)PR_T2 * n2 > rec.size() || (size_t)off3 + (size_t)PR_T3 * n3 > rec.size()) return 3;
+		if (offp != PR_HDR || (size_t)PR_HDR + (size_t)PR_PIECE * npiece > (size_t)off1 || off1 % 4 || off2 % 4 || off3 % 4) return 3;
+		// the kernel loads NB piece slots from PR_HDR whatever npiece is: within the padded table
+		if ((size_t)u * rec_words + PR_HDR + (size_t)PR_PIECE * 4 * NB > h.recs.size() * (size_t)rec_words + PR_REC_PAD) return 3;
+		if (npiece > 4 * NB || n1 > 4 * 64 || n2 > 4 * 64 || n3 > 4 * 64) return 5;
 		std::vector<double> recd(rec.size() / 2);
 		memcpy(recd.data(), rec.data(), recd.size() * sizeof(double));
 		const uint32_t* pieces = g + offp;
 		std::fill(sw.begin(), sw.end(), 0);
+		// LDS entries staged from theta, R or zero pieces: the panels of stage 1, dead afterwards
+		std::vector<uint8_t> panel((size_t)LDS, 0);
 		auto lw = [&](int a, double v) { if (a < 0 || a >= LDS) { bad = true; return; } sm[(size_t)a] = v; sw[(size_t)a] = 1; };
 		auto lr = [&](int a) -> double { if (a < 0 || a >= LDS || a == PR_DUMP || !sw[(size_t)a]) { bad = true; return 0.0; } return sm[(size_t)a]; };
 		for (int i = 0; i < PR_ZERO; i++) lw(i, 0.0);
@@ -664,6 +766,7 @@ int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r,
 					if (q != t / cols) return 6;
 					const int dd = d0 + q * dstride + c;
 					if (dd < PR_WB || dd >= LDS || sw[(size_t)dd]) return 8;
+					if (kind == 0 || kind == 1 || kind == 3) panel[(size_t)dd] = 1;
 					double v = 0.0;
 					if (kind != 3 && c < vcols) {
 						const int64_t sidx = (int64_t)d[0] + (int64_t)q * sstride + c;
@@ -689,51 +792,52 @@ int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r,
 					for (int k = 0; k < 4 * nk; k++) acc += lr(ar + k) * lr(bcol + k * Nr);
 					if (i < vr && j < vc) {
 						const int a = sbase + i * Nr + j;
-						if (a < PR_WB || a >= t2 || t1w[(size_t)a]++) return 10;
+						if (a < PR_WB || a >= LDS || sw[(size_t)a] || t1w[(size_t)a]++) return 10;
 						lw(a, acc);
 					}
 				}
 		}
 		// the theta / R panels are dead after stage 1; T2 overlays them
-		for (int a = t2; a < LDS; a++) sw[(size_t)a] = 0;
-		// stage 2: W_D and the row map from their LDS copies
-		std::vector<int> kofs((size_t)K4), kstr((size_t)K4);
-		for (int k = 0; k < K4; k++) {
-			const double v = lr(kb + k);
-			int32_t e[2];
-			memcpy(e, &v, sizeof(e));
-			kofs[(size_t)k] = e[0];
-			kstr[(size_t)k] = e[1];
-		}
-		std::vector<double> wl((size_t)R16 * K4);
-		for (int i = 0; i < R16; i++) for (int k = 0; k < K4; k++) wl[(size_t)(i * K4 + k)] = lr(wb + i * (K4 + 1) + k);
-		const int nt2 = (S1 + 15) >> 4;
-		for (int mi = 0; mi < R16 / 16; mi++)
-			for (int ni = 0; ni < nt2; ni++)
-				for (int jl = 0; jl < 16; jl++) {
-					const int j2 = 16 * ni + jl;
-					const bool vj = j2 < S1;
-					const int a = vj ? dv(j2, tc, magtc) : 0, c = vj ? j2 - a * tc : 0;
-					double b[PR_KMAX];
-					for (int k = 0; k < K4; k++) b[k] = lr(kofs[(size_t)k] + a * kstr[(size_t)k] + c);
+		for (int a = 0; a < LDS; a++) if (panel[(size_t)a]) sw[(size_t)a] = 0;
+		// stage 2: per tile, W_D rows and the chunk's row map from their LDS copies
+		for (int t = 0; t < n2; t++) {
+			const uint32_t* td = g + off2 + PR_T2 * t;
+			const int wbase = (int)td[0], wld = (int)(td[1] & 0xffffu), nk4 = (int)(td[1] >> 16), kbb = (int)td[2], tc2 = (int)td[3];
+			const uint32_t mag2 = td[4];
+			const int S1 = (int)td[5], j0 = (int)td[6], t2o = (int)td[7];
+			if (tc2 < 1 || tc2 > PR_ZERO || S1 < tc2 || S1 % tc2 || nk4 < 1 || 4 * nk4 >= wld || j0 % 16 || j0 >= S1) return 4;
+			std::vector<int> kofs((size_t)(4 * nk4)), kstr((size_t)(4 * nk4));
+			for (int k = 0; k < 4 * nk4; k++) {
+				const double v = lr(kbb + k);
+				int32_t e[2];
+				memcpy(e, &v, sizeof(e));
+				kofs[(size_t)k] = e[0];
+				kstr[(size_t)k] = e[1];
+			}
+			for (int jl = 0; jl < 16; jl++) {
+				const int j2 = j0 + jl;
+				const bool vj = j2 < S1;
+				const int a = vj ? dv(j2, tc2, mag2) : 0, c = vj ? j2 - a * tc2 : 0;
+				std::vector<double> bk((size_t)(4 * nk4));
+				for (int k = 0; k < 4 * nk4; k++) bk[(size_t)k] = lr(kofs[(size_t)k] + a * kstr[(size_t)k] + c);
+				for (int il = 0; il < 16; il++) {
+					double acc = 0.0;
+					for (int k = 0; k < 4 * nk4; k++) acc += lr(wbase + il * wld + k) * bk[(size_t)k];
 					if (!vj) continue;
-					for (int il = 0; il < 16; il++) {
-						const int i2 = 16 * mi + il;
-						double acc = 0.0;
-						for (int k = 0; k < K4; k++) acc += wl[(size_t)(i2 * K4 + k)] * b[k];
-						const int addr = t2 + i2 * S1 + j2;
-						if (addr >= LDS || sw[(size_t)addr]) return 11;
-						lw(addr, acc);
-					}
+					const int addr = t2o + il * S1 + j2;
+					if (addr < PR_WB || addr >= LDS || sw[(size_t)addr]) return 11;
+					lw(addr, acc);
 				}
+			}
+		}
 		// stage 3
 		for (int t = 0; t < n3; t++) {
 			const uint32_t* td = g + off3 + PR_T3 * t;
 			const int lbase = (int)td[0], dX = (int)td[1], t2b = (int)td[2], dQS1 = (int)td[3], dM = (int)td[4];
-			const int ta3 = (int)td[5], nka = (int)td[6], vr = (int)td[7], vc = (int)td[8], j0 = (int)td[9];
+			const int ta3 = (int)td[5], nka = (int)td[6], vr = (int)(td[7] & 0xffffu), vc = (int)(td[7] >> 16), S1 = (int)td[8], j0 = (int)td[9];
 			const int tc3 = (int)td[10], dC = (int)td[11], dQdC = (int)td[15];
 			const int64_t pbase = (int64_t)((uint64_t)td[13] | ((uint64_t)td[14] << 32));
-			if (vr < 1 || vr > 16 || vc < 1 || vc > 16 || nka * 4 < ta3 || ta3 * tc != S1 || tc3 != tc) return 12;
+			if (vr < 1 || vr > 16 || vc < 1 || vc > 16 || nka * 4 < ta3 || tc3 < 1 || ta3 * tc3 != S1 || dQS1 % S1) return 12;
 			for (int i = 0; i < 16; i++)
 				for (int j = 0; j < 16; j++) {
 					const int ax = i < vr ? lbase + i : lbase;
@@ -797,7 +901,7 @@ void PairPlan::upload(const PairHost& h, const double* w, hipStream_t s)
 	std::vector<size_t> ord(h.recs.size());
 	std::iota(ord.begin(), ord.end(), (size_t)0);
 	std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return h.cost[a] > h.cost[b]; });
-	std::vector<uint32_t> flat((size_t)nunits * (size_t)rec_words, 0u);
+	std::vector<uint32_t> flat((size_t)nunits * (size_t)rec_words + PR_REC_PAD, 0u);
 	for (size_t u = 0; u < ord.size(); u++) {
 		const auto& r = h.recs[ord[u]];
 		std::copy(r.begin(), r.end(), flat.begin() + (ptrdiff_t)(u * (size_t)rec_words));

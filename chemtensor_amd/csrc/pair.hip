@@ -236,7 +236,24 @@ __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __res
 	}
 }
 
-// y block chunk of up to 64 entries: y[y + e] = sum_s part[p + s * size + e], s = 0 .. nslots-1
+// y block chunk of up to 64 entries: y[y + e] = sum_s part[p + s * size + e], s = 0 .. nslots-1,
+// summed in slot order; the loads of up to PR_RED_BATCH slots are in flight at once (slot indices
+// clamped to the last slot, whose repeated values are not added)
+constexpr int PR_RED_BATCH = 16;
+__device__ __forceinline__ double pr_slot_sum(const double* __restrict__ src, int64_t sz, int nslots)
+{
+	double acc = 0.0;
+	for (int s0 = 0; s0 < nslots; s0 += PR_RED_BATCH) {
+		double v[PR_RED_BATCH];
+		#pragma unroll
+		for (int u = 0; u < PR_RED_BATCH; u++) v[u] = src[(int64_t)min(s0 + u, nslots - 1) * sz];
+		#pragma unroll
+		for (int u = 0; u < PR_RED_BATCH; u++)
+			if (s0 + u < nslots) acc += v[u];
+	}
+	return acc;
+}
+
 __global__ __launch_bounds__(256) void k_pair_reduce(const PrRed* __restrict__ items, const int nitems, const double* __restrict__ part, double* __restrict__ y)
 {
 	const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
@@ -244,19 +261,7 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const PrRed* __restrict__ i
 	if (w >= nitems) return;
 	const PrRed it = items[w];
 	if (lane >= it.n) return;
-	const double* __restrict__ src = part + it.p + lane;
-	const int64_t sz = it.size;
-	double acc = 0.0;
-	int s = 0;
-	for (; s + 4 <= it.nslots; s += 4) {
-		const double v0 = src[(s + 0) * sz], v1 = src[(s + 1) * sz], v2 = src[(s + 2) * sz], v3 = src[(s + 3) * sz];
-		acc += v0;
-		acc += v1;
-		acc += v2;
-		acc += v3;
-	}
-	for (; s < it.nslots; s++) acc += src[s * sz];
-	y[it.y + lane] = acc;
+	y[it.y + lane] = pr_slot_sum(part + it.p + lane, it.size, it.nslots);
 }
 
 // The same sums, and dpart[block] = sum over the block's entries of y[e] v[e] (the Lanczos alpha
@@ -272,18 +277,7 @@ __global__ __launch_bounds__(256) void k_pair_reduce_dot(const PrRed* __restrict
 	if (w < nitems) {
 		const PrRed it = items[w];
 		if (lane < it.n) {
-			const double* __restrict__ src = part + it.p + lane;
-			const int64_t sz = it.size;
-			double acc = 0.0;
-			int s = 0;
-			for (; s + 4 <= it.nslots; s += 4) {
-				const double v0 = src[(s + 0) * sz], v1 = src[(s + 1) * sz], v2 = src[(s + 2) * sz], v3 = src[(s + 3) * sz];
-				acc += v0;
-				acc += v1;
-				acc += v2;
-				acc += v3;
-			}
-			for (; s < it.nslots; s++) acc += src[s * sz];
+			const double acc = pr_slot_sum(part + it.p + lane, it.size, it.nslots);
 			y[it.y + lane] = acc;
 			d = acc * v[it.y + lane];
 		}

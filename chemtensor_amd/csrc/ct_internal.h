@@ -411,6 +411,7 @@ struct PairHost
 	double flops = 0;
 	int max_lds = 0, max_words = 0;
 	int mfma_cap = 96;                         // units are cut into chunks of at most this many MFMAs
+	int slots = 1024;                          // workgroups of the unit launch resident at once (chunks are packed into units only to fit these)
 	int lds = PR_LDS;                          // LDS budget of a unit (doubles): 2048, 3072 or 4608
 };
 struct PairPlan

@@ -557,35 +557,65 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 		}
 	}
 	// units: the chunks, largest LDS need first, each into the first unit that still holds it within
-	// the LDS, piece, tile and MFMA limits (first fit decreasing), else into a new unit
+	// an LDS budget and the piece, tile and MFMA limits (first fit decreasing), else into a new unit.
+	// Packing pays only when it saves a round of workgroups: when the chunks exceed the device's
+	// resident units (h.slots) but fit them once packed, the budget is the smallest that does (small
+	// units finish sooner); otherwise every chunk is a unit of its own.
 	std::vector<size_t> pord(parts.size());
 	std::iota(pord.begin(), pord.end(), (size_t)0);
 	std::stable_sort(pord.begin(), pord.end(), [&](size_t i, size_t j) { return parts[i].need > parts[j].need; });
 	struct PrBin { int used, np, n1, n2, n3, mfma; std::vector<size_t> parts; };
+	auto pack = [&](int budget, size_t limit) {
+		std::vector<PrBin> bins;
+		std::vector<size_t> open;   // units that may take another chunk (each chunk has at least 5 pieces)
+		for (size_t pi : pord) {
+			const PrPart& pt = parts[pi];
+			const int sz = pt.need - PR_WB;
+			size_t bi = bins.size();
+			for (size_t oi = 0; oi < open.size(); oi++) {
+				const PrBin& b = bins[open[oi]];
+				if (b.used + sz <= budget && b.np + pt.np <= nbp && b.n1 + pt.n1 <= 4 * 64 && b.n2 + pt.n2 <= 4 * 64 && b.n3 + pt.n3 <= 4 * 64 &&
+					b.mfma + pt.mfma <= h.mfma_cap) {
+					bi = open[oi];
+					break;
+				}
+			}
+			if (bi == bins.size()) {
+				if (bins.size() == limit) return std::vector<PrBin>();   // more units than wanted: give up
+				bins.push_back(PrBin{ PR_WB, 0, 0, 0, 0, 0, {} });
+				open.push_back(bi);
+			}
+			PrBin& b = bins[bi];
+			b.used += sz;
+			b.np += pt.np; b.n1 += pt.n1; b.n2 += pt.n2; b.n3 += pt.n3; b.mfma += pt.mfma;
+			b.parts.push_back(pi);
+			if (b.np + 5 > nbp || budget - b.used < 64) open.erase(std::find(open.begin(), open.end(), bi));
+		}
+		return bins;
+	};
 	std::vector<PrBin> bins;
-	std::vector<size_t> open;   // units that may take another chunk (each chunk has at least 5 pieces)
-	for (size_t pi : pord) {
-		const PrPart& pt = parts[pi];
-		const int sz = pt.need - PR_WB;
-		size_t bi = bins.size();
-		for (size_t oi = 0; oi < open.size(); oi++) {
-			const PrBin& b = bins[open[oi]];
-			if (b.used + sz <= h.lds && b.np + pt.np <= nbp && b.n1 + pt.n1 <= 4 * 64 && b.n2 + pt.n2 <= 4 * 64 && b.n3 + pt.n3 <= 4 * 64 &&
-				b.mfma + pt.mfma <= h.mfma_cap) {
-				bi = open[oi];
-				break;
+	const size_t slots = (size_t)std::max(1, h.slots);
+	int64_t total = 0;
+	int lo = PR_WB;   // a budget below a chunk's need leaves that chunk alone in its unit
+	for (const PrPart& pt : parts) total += pt.need - PR_WB;
+	if (parts.size() > slots && total <= (int64_t)slots * (h.lds - PR_WB)) {
+		bins = pack(h.lds, slots);
+		if (!bins.empty()) {
+			// the smallest budget that still packs into the resident units
+			int hi = h.lds;
+			while (hi - lo > 32) {
+				const int mid = (lo + hi) / 2;
+				std::vector<PrBin> t = pack(mid, slots);
+				if (t.empty()) lo = mid;
+				else { hi = mid; bins = std::move(t); }
 			}
 		}
-		if (bi == bins.size()) {
-			bins.push_back(PrBin{ PR_WB, 0, 0, 0, 0, 0, {} });
-			open.push_back(bi);
-		}
-		PrBin& b = bins[bi];
-		b.used += sz;
-		b.np += pt.np; b.n1 += pt.n1; b.n2 += pt.n2; b.n3 += pt.n3; b.mfma += pt.mfma;
-		b.parts.push_back(pi);
-		if (b.np + 5 > nbp || h.lds - b.used < 64) open.erase(std::find(open.begin(), open.end(), bi));
 	}
+	if (bins.empty())
+		for (size_t pi : pord) {
+			const PrPart& pt = parts[pi];
+			bins.push_back(PrBin{ pt.need, pt.np, pt.n1, pt.n2, pt.n3, pt.mfma, { pi } });
+		}
 	// records: header | pieces | stage-1 tiles | stage-2 tiles | stage-3 tiles | row maps, every chunk
 	// moved up by the LDS of the chunks before it in its unit
 	for (PrBin& b : bins) {
@@ -942,11 +972,24 @@ int pair_default_cap()
 	return v > 0 ? v : 96;
 }
 
+// workgroups of the unit launch resident at once on this device (CUs x occupancy of the kernel)
+static int pair_resident_slots(int lds)
+{
+	int per_cu = 0;
+	switch (lds) {
+	case 2048: CTG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair_units<2048>, PR_THREADS, 0)); break;
+	case 3072: CTG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair_units<3072>, PR_THREADS, 0)); break;
+	default: CTG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pair_units<4608>, PR_THREADS, 0)); break;
+	}
+	return std::max(1, per_cu) * num_compute_units();
+}
+
 bool build_pair_plan(PairPlan& pp, const Meta& th, const Meta& W, const Meta& L, const Meta& R, const double* w, hipStream_t s)
 {
 	PairHost h;
 	h.mfma_cap = pair_default_cap();
 	h.lds = pair_default_lds();
+	h.slots = pair_resident_slots(h.lds);
 	if (!plan_pair_tables(h, th, W, L, R, 0, 0, 0, 0)) return false;
 	pp.upload(h, w, s);
 	return true;

@@ -432,7 +432,8 @@ struct PairPlan
 // append one problem (operands at the given element offsets of the concatenated buffers); false
 // when a class W_D exceeds PR_RMAX x PR_KMAX or a single row and column of a pair exceeds the LDS
 bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L, const Meta& R, int64_t x_off, int64_t w_off, int64_t l_off, int64_t r_off);
-bool build_pair_plan(PairPlan& pp, const Meta& th, const Meta& W, const Meta& L, const Meta& R, const double* w, hipStream_t s);
+// lds: the unit's LDS budget in doubles (2048, 3072 or 4608; 0: pair_default_lds())
+bool build_pair_plan(PairPlan& pp, const Meta& th, const Meta& W, const Meta& L, const Meta& R, const double* w, hipStream_t s, int lds = 0);
 bool build_pair_batch(PairPlan& pp, const std::vector<FusedProblem>& probs, const double* w, hipStream_t s);
 // host replay of the tables with the kernels' index arithmetic (no device); 0 when every check holds
 int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r, int64_t nr, const double* l, int64_t nl, const double* w, int64_t nw, double* y);

@@ -1242,7 +1242,47 @@ MatvecPlan* build_matvec_plan_dev(const Meta& theta, DevTensor&& w, DevTensor&& 
 			p->nunits = p->fused.nunits;
 		}
 		else if (mode == CT_MATVEC_PAIR_FP64) {
-			if (!build_pair_plan(p->pair, theta, p->w.meta, p->l.meta, p->r.meta, p->w.d, s))
+			// the unit's LDS budget: timed at plan creation over the three budgets (small budgets
+			// win for small sectors: more, lighter units), unless CT_PAIR_LDS fixes it
+			int lds = 0;
+			if (autotune && !getenv("CT_PAIR_LDS")) {
+				double* x = (double*)dev_alloc((size_t)std::max<int64_t>(theta.nelem, 1) * sizeof(double));
+				double* y = (double*)dev_alloc((size_t)std::max<int64_t>(theta.nelem, 1) * sizeof(double));
+				launch_fill(x, 1.0, theta.nelem, s);
+				hipEvent_t e0 = nullptr, e1 = nullptr;
+				CTG_HIP(hipEventCreate(&e0));
+				CTG_HIP(hipEventCreate(&e1));
+				float best = 1e30f;
+				for (int cand : { 4608, 3072, 2048 }) {
+					PairPlan c;
+					try {
+						if (!build_pair_plan(c, theta, p->w.meta, p->l.meta, p->r.meta, p->w.d, s, cand)) { c.release(); continue; }
+						launch_pair_matvec(c, x, p->r.d, p->l.d, y, s);
+						CTG_HIP(hipEventRecord(e0, s));
+						for (int rep = 0; rep < 3; rep++) launch_pair_matvec(c, x, p->r.d, p->l.d, y, s);
+						CTG_HIP(hipEventRecord(e1, s));
+						CTG_HIP(hipEventSynchronize(e1));
+						float ms = 0;
+						CTG_HIP(hipEventElapsedTime(&ms, e0, e1));
+						if (ms < best) { best = ms; lds = cand; }
+					}
+					catch (...) {
+						(void)hipStreamSynchronize(s);
+						c.release();
+						dev_free(x);
+						dev_free(y);
+						(void)hipEventDestroy(e0);
+						(void)hipEventDestroy(e1);
+						throw;
+					}
+					c.release();
+				}
+				dev_free(x);
+				dev_free(y);
+				CTG_HIP(hipEventDestroy(e0));
+				CTG_HIP(hipEventDestroy(e1));
+			}
+			if (!build_pair_plan(p->pair, theta, p->w.meta, p->l.meta, p->r.meta, p->w.d, s, lds))
 				throw HipError{ hipErrorNotSupported, "sector-pair matvec: a class W_D or a single pair row and column exceeds the unit limits" };
 			p->stage_flops[0] = p->pair.flops;
 			// units read theta, R, L and W_D and write the partial blocks; the reduction reads them and writes y

@@ -984,11 +984,11 @@ static int pair_resident_slots(int lds)
 	return std::max(1, per_cu) * num_compute_units();
 }
 
-bool build_pair_plan(PairPlan& pp, const Meta& th, const Meta& W, const Meta& L, const Meta& R, const double* w, hipStream_t s)
+bool build_pair_plan(PairPlan& pp, const Meta& th, const Meta& W, const Meta& L, const Meta& R, const double* w, hipStream_t s, int lds)
 {
 	PairHost h;
 	h.mfma_cap = pair_default_cap();
-	h.lds = pair_default_lds();
+	h.lds = lds > 0 ? lds : pair_default_lds();
 	h.slots = pair_resident_slots(h.lds);
 	if (!plan_pair_tables(h, th, W, L, R, 0, 0, 0, 0)) return false;
 	pp.upload(h, w, s);

@@ -37,6 +37,29 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
 	return block_sum(v, sh);
 }
 
+// the sums of two partial arrays (each as sum_partials would give it), their loads in one round trip
+__device__ __forceinline__ void sum_partials2(const double* __restrict__ pa, int Ga, const double* __restrict__ pb, int Gb, double* sh, double& ra, double& rb)
+{
+	double a = 0, b = 0;
+	for (int i = threadIdx.x; i < Ga || i < Gb; i += LZ_THREADS) {
+		if (i < Ga) a += pa[i];
+		if (i < Gb) b += pb[i];
+	}
+	#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		a += __shfl_xor(a, off, 64);
+		b += __shfl_xor(b, off, 64);
+	}
+	if ((threadIdx.x & 63) == 0) {
+		sh[threadIdx.x >> 6] = a;
+		sh[4 + (threadIdx.x >> 6)] = b;
+	}
+	__syncthreads();
+	ra = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+	rb = (sh[4] + sh[5]) + (sh[6] + sh[7]);
+	__syncthreads();
+}
+
 // scal layout: [0, maxiter) alpha, [maxiter, 2 maxiter) beta, [2 maxiter] numiter, [2 maxiter + 1] stop flag
 __global__ __launch_bounds__(LZ_THREADS) void k_lz_dot_partial(const double* __restrict__ x, const double* __restrict__ y, int64_t n,
 	double* __restrict__ part, const double* __restrict__ stop)
@@ -159,9 +182,12 @@ __global__ __launch_bounds__(LZ_THREADS) void k_lz_step(const double* uj, double
 		yv[k] = (in && upd) ? y[i] : 0.0;
 		bv[k] = (in && upd && j > 0) ? vjm1[i] : 0.0;
 	}
+	// beta^2 and <u_j, y> from their partials, loaded together
+	double nsum = 1.0, dsum = 0.0;
+	sum_partials2(npart, j > 0 ? Gn : 0, dpart, Gd, sh, nsum, dsum);
 	double beta = 1.0;
 	if (j > 0) {
-		beta = sqrt(sum_partials(npart, Gn, sh));
+		beta = sqrt(nsum);
 		// every block has read the stop flag before block 0 may set it (it is only read at kernel entry)
 		if (beta < thresh) {
 			if (blockIdx.x == 0 && threadIdx.x == 0) { scal[maxiter + j - 1] = beta; scal[2 * maxiter] = j; scal[2 * maxiter + 1] = 1; }
@@ -169,7 +195,7 @@ __global__ __launch_bounds__(LZ_THREADS) void k_lz_step(const double* uj, double
 		}
 	}
 	const double inv = 1 / beta;
-	const double alpha = sum_partials(dpart, Gd, sh) * inv * inv;
+	const double alpha = dsum * inv * inv;
 	if (blockIdx.x == 0 && threadIdx.x == 0) {
 		scal[j] = alpha;
 		if (j > 0) scal[maxiter + j - 1] = beta;

@@ -439,13 +439,21 @@ bool build_pair_batch(PairPlan& pp, const std::vector<FusedProblem>& probs, cons
 int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r, int64_t nr, const double* l, int64_t nl, const double* w, int64_t nw, double* y);
 int pair_default_cap();
 int pair_default_lds();
-void launch_pair_units(const PairPlan& pp, const double* x, const double* r, const double* l, hipStream_t s);
+// udot (optional, pp.nunits entries): per unit, its partial entries times x at their y indices
+void launch_pair_units(const PairPlan& pp, const double* x, const double* r, const double* l, hipStream_t s, double* udot = nullptr);
 void launch_pair_reduce(const PairPlan& pp, double* y, hipStream_t s);
 void launch_pair_matvec(const PairPlan& pp, const double* x, const double* r, const double* l, double* y, hipStream_t s);
 // the matvec with the Lanczos alpha partials of <y, x> from its partial-sum launch: dpart[0 .. G),
 // G = pair_dot_partials(pp)
 int pair_dot_partials(const PairPlan& pp);
 void launch_pair_matvec_dot(const PairPlan& pp, const double* x, const double* r, const double* l, double* y, double* dpart, hipStream_t s);
+// one deferred-normalisation Lanczos step on a sector-pair plan after launch_pair_units(.., dpart):
+// beta from npart (the previous step's workgroup partials), alpha from the unit partials dpart, y
+// summed from the partial blocks, v_j and u_{j+1} written, npart_out[0 .. G) the partials of
+// |u_{j+1}|^2; buffers of pair_step_partials(pp) entries hold either partial set
+int pair_step_partials(const PairPlan& pp);
+void launch_pair_lz_step(const PairPlan& pp, const double* uj, double* unext, double* vj_out, const double* vjm1, int j, int maxiter,
+	const double* dpart, const double* npart, double thresh, double* npart_out, double* scal, hipStream_t s);
 // the same launches with the partial blocks in a file-scope device array of PR_SCRATCH_DOUBLES
 // (no allocation; concurrent calls on one GPU share the array and race)
 constexpr int64_t PR_SCRATCH_DOUBLES = (int64_t)1 << 22;
@@ -478,6 +486,12 @@ struct DevMatvecDot
 {
 	std::function<void(const double* x, double* y, double* part)> apply;
 	int partials = 0;
+	// optional (deferred recurrence): the matvec's first launch writing the alpha partials to part,
+	// then a step that forms y itself (units / step replace apply and the generic step kernel)
+	std::function<void(const double* x, double* part)> units;
+	int step_partials = 0;   // entries of either partial set of units / step
+	std::function<void(const double* uj, double* unext, double* vj_out, const double* vjm1, int j, int maxiter, const double* dpart,
+		const double* npart, double thresh, double* npart_out, double* scal)> step;
 };
 
 // Lanczos eigensystem (src/util/krylov.c:24-89 and :172-251): lambda[0:numeig] ascending,

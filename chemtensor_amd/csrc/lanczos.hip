@@ -316,6 +316,14 @@ static void enqueue_recurrence(LanczosWork& lw, int64_t n, hipStream_t s, const 
 		double* nparts[2] = { lw.part + lw.pcap, lw.part + 2 * (size_t)lw.pcap };
 		for (int j = 0; j < maxiter; j++) {
 			const double* x = (j == 0) ? V : lw.u;
+			if (mvd && mvd->step) {
+				// two launches per step: the matvec's units (with the alpha partials), then the step
+				// that sums y from the partial blocks as it updates
+				mvd->units(x, part);
+				mvd->step(x, lw.u, V + (int64_t)j * n, j > 0 ? V + (int64_t)(j - 1) * n : V, j, maxiter, (const double*)part,
+					(const double*)nparts[(j + 1) & 1], thresh, nparts[j & 1], scal);
+				continue;
+			}
 			int Ga = G;
 			if (mvd) {
 				mvd->apply(x, w, part);
@@ -363,7 +371,7 @@ int lanczos_eigensystem(LanczosWork& lw, int64_t n, hipStream_t s, const DevMatv
 	if (sync_stop) deferred = false;   // the host callback sees normalised Krylov vectors
 	if (maxiter < 1 || numeig < 1 || numeig > maxiter || n < 1) return CT_ERR_ARG;
 	if (matvec_dot && matvec_dot->partials < 1) matvec_dot = nullptr;
-	lw.reserve(n, maxiter, numeig, matvec_dot ? matvec_dot->partials : 1024);
+	lw.reserve(n, maxiter, numeig, matvec_dot ? std::max(matvec_dot->partials, matvec_dot->step_partials) : 1024);
 	if (use_graph) {
 		if (!lw.vs) lw.vs = (double*)dev_alloc((size_t)n * sizeof(double));
 		launch_copy(vstart, lw.vs, n, s);
@@ -432,6 +440,14 @@ static bool graph_enabled()
 	return !(e && e[0] == '0');
 }
 
+// sector-pair plans fold the partial-block sum into the Lanczos step (CT_PAIR_LZ_FUSED=0: the
+// matvec's own partial-sum launch and the generic step)
+static bool pair_lz_fused()
+{
+	const char* e = getenv("CT_PAIR_LZ_FUSED");
+	return !(e && e[0] == '0');
+}
+
 int lanczos_ground_state(MatvecPlan& p, const double* vstart, int maxiter, double* lambda, double* u)
 {
 	// the plain form replays thousands of per-block launches; it stays eager
@@ -441,6 +457,15 @@ int lanczos_ground_state(MatvecPlan& p, const double* vstart, int maxiter, doubl
 	if (p.mode == CT_MATVEC_PAIR_FP64) {
 		mvd.partials = pair_dot_partials(p.pair);
 		mvd.apply = [&p](const double* x, double* y, double* part) { launch_pair_matvec_dot(p.pair, x, p.r.d, p.l.d, y, part, p.stream); };
+		if (pair_lz_fused()) {
+			// deferred recurrence: two launches per step (units with the alpha partials, then the step)
+			mvd.step_partials = pair_step_partials(p.pair);
+			mvd.units = [&p](const double* x, double* part) { launch_pair_units(p.pair, x, p.r.d, p.l.d, p.stream, part); };
+			mvd.step = [&p](const double* uj, double* unext, double* vj_out, const double* vjm1, int j, int maxiter, const double* dpart,
+				const double* npart, double thresh, double* npart_out, double* scal) {
+				launch_pair_lz_step(p.pair, uj, unext, vj_out, vjm1, j, maxiter, dpart, npart, thresh, npart_out, scal, p.stream);
+			};
+		}
 	}
 	return lanczos_eigensystem(p.lz, p.theta_meta.nelem, p.stream,
 		[&p](const double* x, double* y) { matvec_apply(p, x, y, nullptr); }, vstart, maxiter, 1, lambda, u, graph, false,

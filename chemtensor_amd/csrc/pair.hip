@@ -42,8 +42,9 @@ typedef double pr_double4 __attribute__((ext_vector_type(4)));
 //   stage-2 tiles at off2 (8 words each): wbase wld|nk4<<16 kb tc | mag(tc) S1 j0 t2out
 //     (W_D rows of the tile from LDS wbase at stride wld, nk4 k-steps of four, the chunk's row map at
 //     kb, columns j0 .. j0+15 of the chunk's S1 = ta tc columns (a c), output rows at t2out, stride S1)
-//   stage-3 tiles at off3 (16 words each): lbase dX t2b dQ*S1 | dM ta nka vr|vc<<16 | S1 j0 tc dC |
-//                                          mag(tc) pbase(2) dQ*dC
+//   stage-3 tiles at off3 (16 words each): lbase dX t2b dQ*S1 | dM ta|nka<<16 ydelta vr|vc<<16 |
+//                                          S1 j0 tc dC | mag(tc) pbase(2) dQ*dC
+//     (ydelta: the y index of a partial entry minus its index in the partial blocks)
 //   row maps after the tiles (even offsets; 2 words per T1 row k < K4 of a chunk): LDS offset of row k
 //                                          at a = 0, c = 0, and its stride in a (0, 0 for absent rows:
 //                                          the zero region), staged by the chunk's kind-5 piece
@@ -80,7 +81,8 @@ __device__ __forceinline__ void pr_fresh(uint4& d)
 
 template <int LDS>
 __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __restrict__ recs, const int rec_words, const double* __restrict__ x,
-	const double* __restrict__ rr, const double* __restrict__ ll, const double* __restrict__ wd, double* __restrict__ part, double* __restrict__ dump)
+	const double* __restrict__ rr, const double* __restrict__ ll, const double* __restrict__ wd, double* __restrict__ part, double* __restrict__ dump,
+	double* __restrict__ udot)
 {
 	constexpr int NB = LDS / PR_THREADS;   // staging pieces per wavefront
 	static_assert(NB <= 64, "one piece descriptor per lane");
@@ -201,9 +203,11 @@ __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __res
 	__syncthreads();
 
 	// stage 3: partial y(X,Q,C)[x, (q c)] = L(o,A,M,X)[(a m), x]^T T2(A,M,Q,C)[(a m), (q c)]
+	double dsum = 0.0;
 	for (int t = wave, tl = 0; t < n3; t += 4, tl++) {
 		const int lbase = pr_lane(d3a.x, tl), dX = pr_lane(d3a.y, tl), t2b = pr_lane(d3a.z, tl), dQS1 = pr_lane(d3a.w, tl);
-		const int dM = pr_lane(d3b.x, tl), ta = pr_lane(d3b.y, tl), nka = pr_lane(d3b.z, tl), vrc = pr_lane(d3b.w, tl);
+		const int dM = pr_lane(d3b.x, tl), tanka = pr_lane(d3b.y, tl), ydelta = pr_lane(d3b.z, tl), vrc = pr_lane(d3b.w, tl);
+		const int ta = tanka & 0xffff, nka = tanka >> 16;
 		const int S1 = pr_lane(d3c.x, tl), j0 = pr_lane(d3c.y, tl), tc3 = pr_lane(d3c.z, tl), dC = pr_lane(d3c.w, tl);
 		const int vr = vrc & 0xffff, vc = vrc >> 16;
 		const uint32_t mag3 = (uint32_t)pr_lane(d3d.x, tl);
@@ -228,11 +232,25 @@ __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __res
 			}
 		}
 		if (vj) {
-			double* __restrict__ out = part + pbase + (uint32_t)(__mul24(q, dC) + c);
+			const int64_t o0 = pbase + (uint32_t)(__mul24(q, dC) + c);
+			double* __restrict__ out = part + o0;
 			#pragma unroll
 			for (int r4 = 0; r4 < 4; r4++)
-				if (kq + 4 * r4 < vr) out[(uint32_t)__mul24(kq + 4 * r4, dQdC)] = acc[r4];
+				if (kq + 4 * r4 < vr) {
+					out[(uint32_t)__mul24(kq + 4 * r4, dQdC)] = acc[r4];
+					if (udot) dsum += acc[r4] * x[o0 + ydelta + (uint32_t)__mul24(kq + 4 * r4, dQdC)];
+				}
 		}
+	}
+	// the unit's share of <x, H x>: its partial entries times x at their y indices, summed over the
+	// workgroup in a fixed order
+	if (udot) {
+		__shared__ double dsh[4];
+		#pragma unroll
+		for (int off = 32; off > 0; off >>= 1) dsum += __shfl_xor(dsum, off, 64);
+		if (lane == 0) dsh[wave] = dsum;
+		__syncthreads();
+		if (tid == 0) udot[blockIdx.x] = (dsh[0] + dsh[1]) + (dsh[2] + dsh[3]);
 	}
 }
 
@@ -287,6 +305,90 @@ __global__ __launch_bounds__(256) void k_pair_reduce_dot(const PrRed* __restrict
 	if (lane == 0) sh[threadIdx.x >> 6] = d;
 	__syncthreads();
 	if (threadIdx.x == 0) dpart[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// One step of the deferred-normalisation Lanczos recurrence on a sector-pair plan (the vector part
+// of the recurrence's step, as the generic step kernel, with y read from the partial blocks): the
+// unit launch wrote the partial blocks of y = H u_j and the per-unit partials of <u_j, y>; here
+// beta_{j-1} = sqrt(sum npart) and alpha_j = sum dpart / beta^2 (both sums in a fixed order, the
+// same in every workgroup), and per element of one reduction item: y = the sum of its slots in slot
+// order, v_j = u_j / beta, u_{j+1} = y / beta - alpha v_j - beta v_{j-1}, and the partials of
+// |u_{j+1}|^2 per workgroup. The final step (j = maxiter - 1) writes alpha and v_j only.
+__global__ __launch_bounds__(256) void k_pair_lz_step(const PrRed* __restrict__ items, const int nitems, const double* __restrict__ part,
+	const double* __restrict__ uj, double* __restrict__ unext, double* __restrict__ vj_out, const double* __restrict__ vjm1, int j, int maxiter,
+	const double* __restrict__ dpart, int Gd, const double* __restrict__ npart, int Gn, double thresh, double* __restrict__ npart_out,
+	double* __restrict__ scal)
+{
+	__shared__ double sh[8];
+	if (scal[2 * maxiter + 1] != 0) return;
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + wave));
+	const bool upd = j < maxiter - 1;
+	int64_t i = -1;
+	double yv = 0.0, uv = 0.0, bv = 0.0;
+	if (w < nitems) {
+		const PrRed it = items[w];
+		if (lane < it.n) {
+			i = it.y + lane;
+			uv = uj[i];
+			if (upd) {
+				yv = pr_slot_sum(part + it.p + lane, it.size, it.nslots);
+				if (j > 0) bv = vjm1[i];
+			}
+		}
+	}
+	// beta^2 and <u_j, y>: per-thread strided sums, wavefront butterflies, then the four wavefronts
+	double a = 0.0, b = 0.0;
+	const int Ga = j > 0 ? Gn : 0;
+	for (int k = tid; k < Ga || k < Gd; k += 256) {
+		if (k < Ga) a += npart[k];
+		if (k < Gd) b += dpart[k];
+	}
+	#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		a += __shfl_xor(a, off, 64);
+		b += __shfl_xor(b, off, 64);
+	}
+	if (lane == 0) {
+		sh[wave] = a;
+		sh[4 + wave] = b;
+	}
+	__syncthreads();
+	const double nsum = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+	const double dsum = (sh[4] + sh[5]) + (sh[6] + sh[7]);
+	__syncthreads();
+	double beta = 1.0;
+	if (j > 0) {
+		beta = sqrt(nsum);
+		// every workgroup has read the stop flag before workgroup 0 may set it (read at entry only)
+		if (beta < thresh) {
+			if (blockIdx.x == 0 && tid == 0) { scal[maxiter + j - 1] = beta; scal[2 * maxiter] = j; scal[2 * maxiter + 1] = 1; }
+			return;
+		}
+	}
+	const double inv = 1 / beta;
+	const double alpha = dsum * inv * inv;
+	if (blockIdx.x == 0 && tid == 0) {
+		scal[j] = alpha;
+		if (j > 0) scal[maxiter + j - 1] = beta;
+	}
+	double acc = 0.0;
+	if (i >= 0) {
+		const double v = inv * uv;
+		if (j > 0) vj_out[i] = v;
+		if (upd) {
+			const double wi = (j > 0) ? inv * yv - (alpha * v + beta * bv) : yv - alpha * v;
+			unext[i] = wi;
+			acc = wi * wi;
+		}
+	}
+	if (upd) {
+		#pragma unroll
+		for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+		if (lane == 0) sh[wave] = acc;
+		__syncthreads();
+		if (tid == 0) npart_out[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+	}
 }
 
 //________________________________________________________________________________________________________________________
@@ -501,7 +603,7 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 							for (int ni = 0; ni < cdiv(Nq, 16); ni++) {
 								const int64_t rel = (int64_t)16 * mi * dQ * dC + c0;
 								const uint32_t d[PR_T3] = { (uint32_t)(lds_l[i] + 16 * mi), (uint32_t)dX, (uint32_t)(lt2 + vq[i].rbase * S1), (uint32_t)(dQ * S1),
-									(uint32_t)dM, (uint32_t)tA, (uint32_t)cdiv(tA, 4), (uint32_t)std::min(16, dX - 16 * mi) | ((uint32_t)std::min(16, Nq - 16 * ni) << 16),
+									(uint32_t)dM, (uint32_t)tA | ((uint32_t)cdiv(tA, 4) << 16), 0u, (uint32_t)std::min(16, dX - 16 * mi) | ((uint32_t)std::min(16, Nq - 16 * ni) << 16),
 									(uint32_t)S1, (uint32_t)(16 * ni), (uint32_t)tC, (uint32_t)dC, magic(tC),
 									(uint32_t)rel, (uint32_t)((uint64_t)rel >> 32), (uint32_t)(dQ * dC) };
 								pt.yref.push_back(PrSlotRef{ vq[i].yblock, A, ia, 0, 0 });
@@ -708,6 +810,10 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 				const int64_t pb = slot + (int64_t)((uint64_t)w[0] | ((uint64_t)w[1] << 32));
 				w[0] = (uint32_t)pb;
 				w[1] = (uint32_t)((uint64_t)pb >> 32);
+				// y index of partial entry o: o - slot + (x_off + offset of the y block)
+				const int64_t yd = x_off + th.offset[yb] - slot;
+				if (yd < INT32_MIN || yd > INT32_MAX) return false;
+				w[-7] = (uint32_t)(int32_t)yd;
 			}
 			h.part_elems += size;
 			nslots[(size_t)yb]++;
@@ -749,6 +855,7 @@ int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r,
 	}
 	std::vector<double> part((size_t)h.part_elems, 0.0);
 	std::vector<uint8_t> pwr((size_t)h.part_elems, 0);
+	std::vector<int64_t> pyi((size_t)h.part_elems, -1);   // the y index a stage-3 tile gives each partial entry
 	std::vector<double> sm((size_t)LDS);
 	std::vector<uint8_t> sw((size_t)LDS);
 	const int rec_words = std::max(PR_HDR, (h.max_words + 3) / 4 * 4);
@@ -858,7 +965,8 @@ int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r,
 		for (int t = 0; t < n3; t++) {
 			const uint32_t* td = g + off3 + PR_T3 * t;
 			const int lbase = (int)td[0], dX = (int)td[1], t2b = (int)td[2], dQS1 = (int)td[3], dM = (int)td[4];
-			const int ta3 = (int)td[5], nka = (int)td[6], vr = (int)(td[7] & 0xffffu), vc = (int)(td[7] >> 16), S1 = (int)td[8], j0 = (int)td[9];
+			const int ta3 = (int)(td[5] & 0xffffu), nka = (int)(td[5] >> 16), ydelta = (int)(int32_t)td[6];
+			const int vr = (int)(td[7] & 0xffffu), vc = (int)(td[7] >> 16), S1 = (int)td[8], j0 = (int)td[9];
 			const int tc3 = (int)td[10], dC = (int)td[11], dQdC = (int)td[15];
 			const int64_t pbase = (int64_t)((uint64_t)td[13] | ((uint64_t)td[14] << 32));
 			if (vr < 1 || vr > 16 || vc < 1 || vc > 16 || nka * 4 < ta3 || tc3 < 1 || ta3 * tc3 != S1 || dQS1 % S1) return 12;
@@ -879,6 +987,7 @@ int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r,
 						const int64_t o = pbase + (int64_t)q * dC + c + (int64_t)i * dQdC;
 						if (o < 0 || o >= h.part_elems || pwr[(size_t)o]++) return 13;
 						part[(size_t)o] = acc;
+						pyi[(size_t)o] = o + ydelta;
 					}
 				}
 		}
@@ -891,7 +1000,11 @@ int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r,
 		if (it.nslots > 0 && (it.p < 0 || it.p + (int64_t)(it.nslots - 1) * it.size + it.n > h.part_elems)) return 20;
 		for (int e = 0; e < it.n; e++) {
 			double acc = 0.0;
-			for (int s = 0; s < it.nslots; s++) acc += part[(size_t)(it.p + (int64_t)s * it.size + e)];
+			for (int s = 0; s < it.nslots; s++) {
+				const int64_t o = it.p + (int64_t)s * it.size + e;
+				if (pyi[(size_t)o] != it.y + e) return 23;   // the units' dot indices agree with the reduction
+				acc += part[(size_t)o];
+			}
 			if (yw[(size_t)(it.y + e)]++) return 21;
 			if (y) y[it.y + e] = acc;
 		}
@@ -1006,16 +1119,16 @@ bool build_pair_batch(PairPlan& pp, const std::vector<FusedProblem>& probs, cons
 	return true;
 }
 
-void launch_pair_units(const PairPlan& pp, const double* x, const double* r, const double* l, hipStream_t s)
+void launch_pair_units(const PairPlan& pp, const double* x, const double* r, const double* l, hipStream_t s, double* udot)
 {
 	if (pp.nunits <= 0) return;
 	const dim3 grid((unsigned)pp.nunits), block(PR_THREADS);
 	const uint32_t* recs = pp.recs;
 	const double* wd = pp.wd;
 	switch (pp.lds) {
-	case 2048: hipLaunchKernelGGL(k_pair_units<2048>, grid, block, 0, s, recs, pp.rec_words, x, r, l, wd, pp.part, pp.part + pp.part_elems); break;
-	case 3072: hipLaunchKernelGGL(k_pair_units<3072>, grid, block, 0, s, recs, pp.rec_words, x, r, l, wd, pp.part, pp.part + pp.part_elems); break;
-	case 4608: hipLaunchKernelGGL(k_pair_units<4608>, grid, block, 0, s, recs, pp.rec_words, x, r, l, wd, pp.part, pp.part + pp.part_elems); break;
+	case 2048: hipLaunchKernelGGL(k_pair_units<2048>, grid, block, 0, s, recs, pp.rec_words, x, r, l, wd, pp.part, pp.part + pp.part_elems, udot); break;
+	case 3072: hipLaunchKernelGGL(k_pair_units<3072>, grid, block, 0, s, recs, pp.rec_words, x, r, l, wd, pp.part, pp.part + pp.part_elems, udot); break;
+	case 4608: hipLaunchKernelGGL(k_pair_units<4608>, grid, block, 0, s, recs, pp.rec_words, x, r, l, wd, pp.part, pp.part + pp.part_elems, udot); break;
 	default: throw HipError{ hipErrorInvalidValue, "sector-pair units: unsupported LDS budget" };
 	}
 	CTG_HIP(hipGetLastError());
@@ -1036,6 +1149,17 @@ void launch_pair_matvec(const PairPlan& pp, const double* x, const double* r, co
 }
 
 int pair_dot_partials(const PairPlan& pp) { return std::max(1, (pp.nred + 3) / 4); }
+
+int pair_step_partials(const PairPlan& pp) { return std::max(std::max(1, pp.nunits), (pp.nred + 3) / 4); }
+
+void launch_pair_lz_step(const PairPlan& pp, const double* uj, double* unext, double* vj_out, const double* vjm1, int j, int maxiter,
+	const double* dpart, const double* npart, double thresh, double* npart_out, double* scal, hipStream_t s)
+{
+	const int G = std::max(1, (pp.nred + 3) / 4);
+	hipLaunchKernelGGL(k_pair_lz_step, dim3((unsigned)G), dim3(256), 0, s, (const PrRed*)pp.red, pp.nred, (const double*)pp.part, uj, unext, vj_out, vjm1,
+		j, maxiter, dpart, std::max(0, pp.nunits), npart, G, thresh, npart_out, scal);
+	CTG_HIP(hipGetLastError());
+}
 
 void launch_pair_matvec_dot(const PairPlan& pp, const double* x, const double* r, const double* l, double* y, double* dpart, hipStream_t s)
 {
@@ -1085,9 +1209,9 @@ void launch_pair_matvec_noalloc(const PairPlan& pp, const double* x, const doubl
 		const dim3 grid((unsigned)pp.nunits), block(PR_THREADS);
 		double* dump = part + pp.part_elems;
 		switch (pp.lds) {
-		case 2048: hipLaunchKernelGGL(k_pair_units<2048>, grid, block, 0, s, (const uint32_t*)pp.recs, pp.rec_words, x, r, l, (const double*)pp.wd, part, dump); break;
-		case 3072: hipLaunchKernelGGL(k_pair_units<3072>, grid, block, 0, s, (const uint32_t*)pp.recs, pp.rec_words, x, r, l, (const double*)pp.wd, part, dump); break;
-		case 4608: hipLaunchKernelGGL(k_pair_units<4608>, grid, block, 0, s, (const uint32_t*)pp.recs, pp.rec_words, x, r, l, (const double*)pp.wd, part, dump); break;
+		case 2048: hipLaunchKernelGGL(k_pair_units<2048>, grid, block, 0, s, (const uint32_t*)pp.recs, pp.rec_words, x, r, l, (const double*)pp.wd, part, dump, nullptr); break;
+		case 3072: hipLaunchKernelGGL(k_pair_units<3072>, grid, block, 0, s, (const uint32_t*)pp.recs, pp.rec_words, x, r, l, (const double*)pp.wd, part, dump, nullptr); break;
+		case 4608: hipLaunchKernelGGL(k_pair_units<4608>, grid, block, 0, s, (const uint32_t*)pp.recs, pp.rec_words, x, r, l, (const double*)pp.wd, part, dump, nullptr); break;
 		default: throw HipError{ hipErrorInvalidValue, "sector-pair units: unsupported LDS budget" };
 		}
 		CTG_HIP(hipGetLastError());

@@ -73,6 +73,7 @@ def lib():
             "block_sparse_tensor_norm2": (C.c_double, [P]),
             "block_sparse_tensor_dot": (None, [P, C.c_int, P, C.c_int, C.c_int, bst]),
             "block_sparse_tensor_transpose": (None, [IP, P, bst]),
+            "block_sparse_tensor_cyclic_partial_trace": (None, [P, C.c_int, bst]),
             "apply_local_hamiltonian": (None, [P, P, P, P, bst]),
             "mpo_merge_tensor_pair": (None, [P, P, bst]),
             "ct_matvec_plan_create": (C.c_int, [P, P, P, P, C.c_int, C.POINTER(P)]),
@@ -229,6 +230,14 @@ def transpose(perm, t):
     r = HostTensor()
     p = np.asarray(perm, dtype=np.int32)
     lib().block_sparse_tensor_transpose(p.ctypes.data_as(IP), _ptr(t), r.sptr)
+    r.live = True
+    return r
+
+
+def cyclic_partial_trace(t, ndim_trace):
+    """block_sparse_tensor_cyclic_partial_trace (bst.c:1560) on the GPU"""
+    r = HostTensor()
+    lib().block_sparse_tensor_cyclic_partial_trace(_ptr(t), int(ndim_trace), r.sptr)
     r.live = True
     return r
 

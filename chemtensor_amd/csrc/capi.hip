@@ -72,6 +72,32 @@ void block_sparse_tensor_transpose(const int* perm, const struct block_sparse_te
 	}
 }
 
+void block_sparse_tensor_cyclic_partial_trace(const struct block_sparse_tensor* t, const int ndim_trace, struct block_sparse_tensor* r)
+{
+	try {
+		if (!t || !r || ndim_trace < 0 || t->ndim < 2 * ndim_trace) throw HipError{ hipErrorInvalidValue, "invalid arguments" };
+		if (t->ndim == 2 * ndim_trace && ndim_trace > 0) throw HipError{ hipErrorNotSupported, "a trace over every axis (degree-0 result)" };
+		for (int i = 0; i < ndim_trace; i++) {
+			const int k = t->ndim - ndim_trace + i;
+			if (t->dim_logical[i] != t->dim_logical[k] || t->axis_dir[i] != -t->axis_dir[k])
+				throw HipError{ hipErrorInvalidValue, "traced axes differ in dimension or do not have opposite directions" };
+			for (ct_long j = 0; j < t->dim_logical[i]; j++)
+				if (t->qnums_logical[i][j] != t->qnums_logical[k][j]) throw HipError{ hipErrorInvalidValue, "traced axes differ in quantum numbers" };
+		}
+		hipStream_t st = default_stream();
+		DevTensor dt = upload(t, st);
+		if (ndim_trace == 0) {
+			download(dt, r, st);
+			return;
+		}
+		DevTensor dr = dev_cyclic_partial_trace(dt, ndim_trace, st);
+		download(dr, r, st);
+	}
+	catch (const HipError& e) {
+		fatal("block_sparse_tensor_cyclic_partial_trace", e.where.c_str());
+	}
+}
+
 void mpo_merge_tensor_pair(const struct block_sparse_tensor* a0, const struct block_sparse_tensor* a1, struct block_sparse_tensor* a)
 {
 	try {

@@ -99,6 +99,11 @@ void block_sparse_tensor_deserialize_entries(struct block_sparse_tensor* t, cons
 void block_sparse_tensor_dot(const struct block_sparse_tensor* s, const enum tensor_axis_range axrange_s, const struct block_sparse_tensor* t, const enum tensor_axis_range axrange_t, const int ndim_mult, struct block_sparse_tensor* r);
 /// bst.c:785-870 (block_sparse_tensor.h:99), on the GPU. Allocates r.
 void block_sparse_tensor_transpose(const int* perm, const struct block_sparse_tensor* t, struct block_sparse_tensor* r);
+/// bst.c:1560-1612 (block_sparse_tensor.h:57): trace of the ndim_trace leading axes against the
+/// ndim_trace trailing ones (equal dimensions and quantum numbers, opposite directions), on the GPU
+/// (one workgroup per output block sums the diagonal entries of every contributing block). A trace
+/// over every axis (degree-0 result) is not supported. Allocates r.
+void block_sparse_tensor_cyclic_partial_trace(const struct block_sparse_tensor* t, const int ndim_trace, struct block_sparse_tensor* r);
 
 //________________________________________________________________________________________________________________________
 //

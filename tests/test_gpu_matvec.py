@@ -80,6 +80,57 @@ def test_transpose_fixture(golden):
     assert np.array_equal(g.serialize(), o.serialize())
 
 
+def test_cyclic_partial_trace_fixture(golden):
+    """test/tensor/test_block_sparse_tensor.c:163-230 on the GPU: the reference's complex64 tensor
+    (7 axes, trace over 2) split into real and imaginary parts (the trace is linear), each traced by
+    block_sparse_tensor_cyclic_partial_trace against the reference result (tolerance of the reference
+    test, 5e-6 for single-precision data) and against the oracle's trace (1e-12)."""
+    z = golden("test_block_sparse_tensor_cyclic_partial_trace")
+    qn = [z[f"A_qnums{i}"].astype(np.int32) for i in range(7)]
+    ad = z["A_axis_dir"].astype(np.int32)
+    for part_t, part_r in zip((z["D_t"].real, z["D_t"].imag), (z["D_t_tr"].real, z["D_t_tr"].imag)):
+        t = orc.BST.from_dense(np.ascontiguousarray(part_t, dtype=np.float64), ad, qn)
+        g = ct.cyclic_partial_trace(t, 2)
+        o = orc.partial_trace(t, 2)
+        want = o.to_dense()
+        v = g.serialize()
+        assert v.size == o.nelem()
+        o.deserialize(v)
+        got = o.to_dense()
+        assert np.max(np.abs(got - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
+        ref = orc.BST.from_dense(np.ascontiguousarray(part_r, dtype=np.float64), ad[2:5], qn[2:5]).to_dense()
+        assert np.max(np.abs(got - ref)) < 5e-6
+
+
+@pytest.mark.parametrize("nt,seed", [(1, 3), (2, 4), (0, 5)])
+def test_cyclic_partial_trace_random(nt, seed):
+    """Random block-sparse tensors with traced outer axes of dimension > 1 (several blocks per traced
+    axis) against the oracle's trace at 1e-13, and a dense einsum closed form."""
+    rng = np.random.default_rng(seed)
+    dims_t = [int(d) for d in rng.integers(3, 6, size=nt)]
+    dims_r = [7, 6]
+    qn_t = [rng.integers(-1, 2, size=d).astype(np.int32) for d in dims_t]
+    qn_r = [rng.integers(-1, 2, size=d).astype(np.int32) for d in dims_r]
+    dims = dims_t + dims_r + dims_t
+    ad = np.array([1] * nt + [1, -1] + [-1] * nt, dtype=np.int32)
+    t = orc.BST.zeros(dims, ad, qn_t + qn_r + qn_t).fill_random(seed)
+    g = ct.cyclic_partial_trace(t, nt)
+    o = orc.partial_trace(t, nt)
+    want = o.to_dense()
+    v = g.serialize()
+    assert v.size == o.nelem()
+    o.deserialize(v)
+    assert np.max(np.abs(o.to_dense() - want)) < 1e-13
+    d = t.to_dense()
+    if nt == 1:
+        dense = np.einsum("iabi->ab", d)
+    elif nt == 2:
+        dense = np.einsum("ijabij->ab", d)
+    else:
+        dense = d
+    assert np.max(np.abs(o.to_dense() - dense)) < 1e-13
+
+
 def test_mpo_merge_matches_oracle():
     mpo = orc.MPO.hubbard(6, 1.0, 4.0, 0.3)
     for i in range(5):

@@ -169,7 +169,7 @@ def bench_dmrg_precision(ct, seed):
 
 
 def bench_molecular(ct, n, chi, sweeps, seed, split):
-    """configuration 3: two-site DMRG on a synthetic spin molecular Hamiltonian (n spatial orbitals,
+    """Two-site DMRG on a synthetic spin molecular Hamiltonian (n spatial orbitals,
     sector N = n, 2 Sz = 0), FP64 matvecs and split-FP32 matvecs from the same seeded start state;
     energy per sweep and the split form's energy error per sweep"""
     t, v = ct.molecular_coefficients(n, seed)
@@ -216,7 +216,7 @@ def split_mfma_use(plan, stage_ms):
 
 
 def bench_molecular_matvec(ct, n, chi, seed, steps, warmup):
-    """configuration 3's compute-heavy kernel: one H_eff matvec at the centre bond pair of the spin
+    """The molecular workload's compute-heavy kernel: one H_eff matvec at the centre bond pair of the spin
     molecular Hamiltonian (n orbitals, h2 = [712, 16, 16, 712] at n = 20, the MPO bond between them 862),
     chi on both outer bonds, bond quantum numbers as construct_random_mps draws them in the sector
     N = n, 2Sz = 0, theta / L / R N(0,1); every FP64 form that plans it and the split-FP32 form"""
@@ -307,8 +307,8 @@ def bench_serve(ct, nreq, rate, L_range, chi_range, seed, budget, max_active, ti
     lat = np.array([r.t_done - r.t_submit for r in done]) if done else np.zeros(1)
     log(f"serve: {len(done)}/{nreq} solved in {makespan:.1f}s, p50 {np.percentile(lat, 50):.2f}s p99 {np.percentile(lat, 99):.2f}s")
     return {
-        "sample": (f"bounded sample: {nreq} of the 2000 requests configuration 5 states (run --serve 2000 for the full configuration)"
-                   if nreq < 2000 else "configuration 5 as stated (2000 requests)"),
+        "sample": (f"bounded sample: {nreq} of the 2000 requests of the serving workload (run --serve 2000 for all of them)"
+                   if nreq < 2000 else "the serving workload in full (2000 requests)"),
         "workload": f"{nreq} DMRG ground-state solves, Poisson arrivals at {rate}/s, XXZ (Delta~U[0,2]) or Hubbard (U~U[0,8]), "
                     f"L~U{{{L_range[0]}..{L_range[1]}}}, chi~U{{{chi_range[0]}..{chi_range[1]}}}, stop at |dE_sweep| < 1e-10",
         "requests": nreq, "solved": len(done), "failed": int(sum(1 for r in res if r.status < 0)),

@@ -206,9 +206,12 @@ def test_split_form_stage2_options_match_oracle(stage2, monkeypatch):
         assert err < TOL
 
 
-def test_apply_local_hamiltonian_dropin_with_reference_structs():
-    """The oracle's structs (reference layout) go straight into the product's drop-in entry point."""
-    theta, h2, left, right, _ = orc.matvec_problem("hubbard", 16, 48, 77)
+@pytest.mark.parametrize("model,L,chi", [("hubbard", 16, 48), ("hubbard", 32, 256), ("xxz", 32, 512), ("hubbard", 32, 512)])
+def test_apply_local_hamiltonian_dropin_with_reference_structs(model, L, chi):
+    """The oracle's structs (reference layout) go straight into the product's drop-in entry point
+    (operands copied to the device on every call), up to the headline size."""
+    say("drop-in", model, L, chi)
+    theta, h2, left, right, _ = orc.matvec_problem(model, L, chi, 77)
     b = ct.apply_local_hamiltonian(theta, h2, left, right)
     want = orc.apply_local_hamiltonian(theta, h2, left, right)
     dims_b, ad_b, qn_b = b.meta()

@@ -1,4 +1,4 @@
-"""Two-site DMRG on the GPU with the spin molecular Hamiltonian MPO (configuration 3's operator at
+"""Two-site DMRG on the GPU with the spin molecular Hamiltonian MPO (the seeded-integral operator at
 small orbital counts) against exact diagonalisation of the same Hamiltonian in the target sector."""
 
 import sys
@@ -86,8 +86,8 @@ def test_dmrg_spin_molecular_matches_oracle():
 
 @pytest.mark.parametrize("n,chi", [(8, 64), (12, 128), (20, 512)])
 def test_molecular_matvec_forms_match_oracle(n, chi):
-    """The H_eff matvec with the spin molecular MPO (configuration 3's operator; at n = 20 the centre
-    pair h2 is [712, 16, 16, 712] and chi = 512, the stated size) in every form that plans it, against
+    """The H_eff matvec with the spin molecular MPO (seeded integrals; at n = 20 the centre pair h2 is
+    [712, 16, 16, 712] and chi = 512, the molecular workload's size) in every form that plans it, against
     the oracle's apply_local_hamiltonian on the same seeded inputs: 1e-12 relative; the product's
     generator gives the oracle generator's inputs bit for bit."""
     from oracle import oracle as orc

@@ -80,7 +80,7 @@ def test_spin_molecular_mpo_bond_dimensions():
 
 
 def test_spin_molecular_mpo_centre_bond_n20():
-    # configuration 3's operator: 862 at the centre bond, the reference's count 2 + 80 + 780
+    # the n = 20 operator: 862 at the centre bond, the reference's count 2 + 80 + 780
     t, v = ct.molecular_coefficients(20, 5)
     mpo = ct.MPO.spin_molecular(t, v)
     dims = [mpo.s.a[0].dim_logical[0]] + [mpo.s.a[i].dim_logical[3] for i in range(20)]

@@ -363,7 +363,7 @@ def main():
                     help="only run a plain device copy of MIB MiB ten times (scale for HBM counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
+        print(f"bench.py measures one GPU only: the library has no multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
 
     import chemtensor_amd as ct

@@ -62,7 +62,7 @@ def test_dmrg_spin_molecular_split_fp32_matches_fp64(monkeypatch):
 
 
 def test_dmrg_spin_molecular_matches_oracle():
-    """Configuration 3's operator through both sweeps: the product's spin molecular MPO (reference
+    """The seeded-integral molecular operator through both sweeps: the product's spin molecular MPO (reference
     struct layout) drives the product's dmrg_twosite and the oracle's (a restatement of dmrg.c:262-399)
     from the same seeded start state. n = 8 orbitals, N = 8, 2Sz = 0, chi = 64 (reached on the
     inner bonds): energies within 1e-10, bond dimensions equal, entropies within 1e-8."""

@@ -298,6 +298,7 @@ int ct_matvec_pair_check(const struct block_sparse_tensor* a, const struct block
 		PairHost h;
 		h.mfma_cap = mfma_cap > 0 ? mfma_cap : pair_default_cap();
 		h.lds = pair_default_lds();
+		h.slots = pair_default_slots(h.slots);
 		if (!plan_pair_tables(h, ma, mw, ml, mr, 0, 0, 0, 0)) return CT_ERR_UNSUPPORTED;
 		std::vector<double> x((size_t)ma.nelem), vw((size_t)mw.nelem), vl((size_t)ml.nelem), vr((size_t)mr.nelem);
 		block_sparse_tensor_serialize_entries(a, x.data());
@@ -321,6 +322,7 @@ int ct_matvec_pair_stats(const struct block_sparse_tensor* a, const struct block
 		PairHost h;
 		h.mfma_cap = mfma_cap > 0 ? mfma_cap : pair_default_cap();
 		h.lds = pair_default_lds();
+		h.slots = pair_default_slots(h.slots);
 		if (!plan_pair_tables(h, ma, mw, ml, mr, 0, 0, 0, 0)) return CT_ERR_UNSUPPORTED;
 		double cmax = 0, csum = 0;
 		for (double c : h.cost) { cmax = std::max(cmax, c); csum += c; }

@@ -438,6 +438,7 @@ bool build_pair_batch(PairPlan& pp, const std::vector<FusedProblem>& probs, cons
 // host replay of the tables with the kernels' index arithmetic (no device); 0 when every check holds
 int pair_replay(const PairHost& h, const double* x, int64_t nx, const double* r, int64_t nr, const double* l, int64_t nl, const double* w, int64_t nw, double* y);
 int pair_default_cap();
+int pair_default_slots(int device_slots);
 int pair_default_lds();
 // udot (optional, pp.nunits entries): per unit, its partial entries times x at their y indices
 void launch_pair_units(const PairPlan& pp, const double* x, const double* r, const double* l, hipStream_t s, double* udot = nullptr);

@@ -1077,6 +1077,14 @@ int pair_default_lds()
 	return (v == 2048 || v == 3072 || v == 4608 || (getenv("CT_PAIR_LDS_ANY") && v > 0 && v <= 65536)) ? v : PR_LDS;
 }
 
+// resident units the packing aims for: the device's (or 1024 without a device), CT_PAIR_SLOTS overrides
+int pair_default_slots(int device_slots)
+{
+	const char* e = getenv("CT_PAIR_SLOTS");
+	const int v = e ? atoi(e) : 0;
+	return v > 0 ? v : device_slots;
+}
+
 // MFMAs per unit above which a pair is cut into chunks (CT_PAIR_CAP overrides)
 int pair_default_cap()
 {
@@ -1102,7 +1110,7 @@ bool build_pair_plan(PairPlan& pp, const Meta& th, const Meta& W, const Meta& L,
 	PairHost h;
 	h.mfma_cap = pair_default_cap();
 	h.lds = lds > 0 ? lds : pair_default_lds();
-	h.slots = pair_resident_slots(h.lds);
+	h.slots = pair_default_slots(pair_resident_slots(h.lds));
 	if (!plan_pair_tables(h, th, W, L, R, 0, 0, 0, 0)) return false;
 	pp.upload(h, w, s);
 	return true;

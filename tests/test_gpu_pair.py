@@ -124,3 +124,18 @@ def test_pair_lanczos_step_from_partials(model, L, chi, monkeypatch):
     for lam, u in vals[1:]:
         assert lam == vals[0][0]
         assert np.array_equal(u, vals[0][1])
+
+
+@pytest.mark.parametrize("model,L,chi,seed", [("hubbard", 32, 128, 41), ("hubbard", 32, 256, 42), ("xxz", 20, 64, 43)])
+def test_pair_packed_units_match_oracle(model, L, chi, seed, monkeypatch):
+    """Units holding several packed chunks (the packing target set below the chunk count) against
+    the oracle at 1e-12, bit-reproducible, for every LDS budget."""
+    theta, h2, left, right = orc.matvec_problem(model, L, chi, seed)[:4]
+    for lds in ("2048", "3072", "4608"):
+        monkeypatch.setenv("CT_PAIR_LDS", lds)
+        monkeypatch.setenv("CT_PAIR_SLOTS", "100000000")
+        chunks = int(ct.matvec_pair_stats(theta, h2, left, right)["units"])
+        monkeypatch.setenv("CT_PAIR_SLOTS", str(max(1, chunks * 2 // 3)))
+        units = int(ct.matvec_pair_stats(theta, h2, left, right)["units"])
+        say("packed pair", model, L, chi, "lds", lds, "chunks", chunks, "units", units)
+        run(theta, h2, left, right)

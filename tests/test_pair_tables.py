@@ -32,6 +32,28 @@ def test_pair_tables_chunking(cap):
     check(*orc.matvec_problem("xxz", 24, 128, 22)[:4], cap=cap)
 
 
+@pytest.mark.parametrize("model,L,chi,seed,cap", [("hubbard", 32, 128, 31, 0), ("xxz", 20, 64, 32, 0), ("hubbard", 32, 256, 33, 0),
+                                                  ("hubbard", 8, 24, 34, 0), ("hubbard", 32, 128, 35, 24), ("xxz", 24, 128, 36, 16)])
+def test_pair_tables_packed_units(model, L, chi, seed, cap, monkeypatch):
+    """Chunks packed into units (several chunks per workgroup, their LDS regions moved up) when the
+    chunks exceed the resident units but fit them packed: the target is set to 2/3 and 1/2 of the
+    chunk count, the packing is checked to have happened, and the replay (liveness of the moved
+    regions included) must reproduce the oracle's product."""
+    theta, h2, left, right = orc.matvec_problem(model, L, chi, seed)[:4]
+    monkeypatch.setenv("CT_PAIR_SLOTS", "100000000")
+    chunks = int(ct.matvec_pair_stats(theta, h2, left, right, cap)["units"])
+    packed = False
+    for slots in (chunks * 2 // 3, chunks // 2, chunks - max(1, chunks // 10)):
+        monkeypatch.setenv("CT_PAIR_SLOTS", str(max(1, slots)))
+        units = int(ct.matvec_pair_stats(theta, h2, left, right, cap)["units"])
+        # packed into the target, or (when the chunks cannot fit it) left one per unit
+        assert units <= max(1, slots) or units == chunks
+        packed = packed or units < chunks
+        check(theta, h2, left, right, cap=cap)
+    if model == "hubbard":   # small sectors: chunks well below the LDS budget always pack
+        assert packed
+
+
 def test_pair_tables_physical_environments():
     """Environments built by contraction from DMRG states (oracle), every bond of the chain,
     including the edges where bond dimensions are 1."""

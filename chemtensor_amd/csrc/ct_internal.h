@@ -397,6 +397,7 @@ void launch_fused_matvec(const FusedPlan& f, const double* x, const double* w, c
 // its contribution to each output block y(X,Q,C) as a partial block; a reduction launch sums the
 // partial blocks of every y block in (A, row chunk) order.
 constexpr int PR_THREADS = 256;
+constexpr int PR_XCDS = 8;                            // XCDs of the device: workgroup i runs on XCD i mod 8
 constexpr int PR_LDS = 4608;                          // doubles of operand / intermediate LDS per workgroup (36 KB)
 constexpr int PR_REC_WORDS = 512;                     // largest unit record (4-byte words)
 constexpr int PR_RMAX = 32, PR_KMAX = 32;             // largest class W_D (rows x columns)
@@ -405,6 +406,7 @@ struct PairHost
 {
 	std::vector<std::vector<uint32_t>> recs;   // one record per unit
 	std::vector<double> cost;                  // estimated cost per unit (MFMA count + staging)
+	std::vector<int> xcd;                      // per unit: the XCD it is placed on
 	std::vector<int64_t> wd_idx;               // class W_D tables (R16 x K4 each) as W element indices, -1: zero
 	std::vector<PrRed> red;
 	int64_t part_elems = 0;

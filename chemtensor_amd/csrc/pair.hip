@@ -417,6 +417,7 @@ struct PrPart
 	size_t kpiece = 0;                             // word of pcs holding the row-map piece's source
 	std::vector<uint32_t> pcs, t1s, t2s, t3s, kt;
 	std::vector<PrSlotRef> yref;                   // per stage-3 tile: its y block, A and row chunk
+	int32_t A = 0;                                 // the pair's left sector
 };
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -562,6 +563,7 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 					const int need = std::max(off, lt2 + cl.R16 * S1);
 					if (need > h.lds || need > 0xffff) return false;
 					PrPart pt;
+					pt.A = (int32_t)A;
 					pt.need = need;
 					std::vector<uint32_t>& t1s = pt.t1s;
 					std::vector<uint32_t>& t2s = pt.t2s;
@@ -658,14 +660,29 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 			}
 		}
 	}
-	// units: the chunks, largest LDS need first, each into the first unit that still holds it within
-	// an LDS budget and the piece, tile and MFMA limits (first fit decreasing), else into a new unit.
-	// Packing pays only when it saves a round of workgroups: when the chunks exceed the device's
-	// resident units (h.slots) but fit them once packed, the budget is the smallest that does (small
-	// units finish sooner); otherwise every chunk is a unit of its own.
-	std::vector<size_t> pord(parts.size());
-	std::iota(pord.begin(), pord.end(), (size_t)0);
-	std::stable_sort(pord.begin(), pord.end(), [&](size_t i, size_t j) { return parts[i].need > parts[j].need; });
+	// XCD groups: workgroup i runs on XCD i mod PR_XCDS, each with its own L2. The left sectors A are
+	// dealt to the XCDs by cost (largest first, to the least loaded), and the units of a group are
+	// placed on its XCD (PairPlan::upload), so the theta and L rows of a sector are fetched into one
+	// L2 rather than into every XCD's
+	std::map<int32_t, double> cost_a;
+	for (const PrPart& pt : parts) cost_a[pt.A] += (double)pt.mfma + (double)pt.np / 4.0 + 4.0;
+	std::vector<std::pair<double, int32_t>> by_cost;
+	for (const auto& kv : cost_a) by_cost.emplace_back(kv.second, kv.first);
+	std::stable_sort(by_cost.begin(), by_cost.end(), [](const std::pair<double, int32_t>& x, const std::pair<double, int32_t>& y) { return x.first > y.first; });
+	std::vector<double> xcd_load(PR_XCDS, 0.0);
+	std::map<int32_t, int> xcd_of_a;
+	for (const auto& ca : by_cost) {
+		const int x = (int)(std::min_element(xcd_load.begin(), xcd_load.end()) - xcd_load.begin());
+		xcd_load[(size_t)x] += ca.first;
+		xcd_of_a[ca.second] = x;
+	}
+	// units per group: the chunks, largest LDS need first, each into the first unit that still holds
+	// it within an LDS budget and the piece, tile and MFMA limits (first fit decreasing), else into a
+	// new unit. Packing pays only when it saves a round of workgroups: when a group's chunks exceed
+	// its share of the device's resident units (h.slots / PR_XCDS) but fit them once packed, the
+	// budget is the smallest that does (small units finish sooner); otherwise every chunk is a unit of
+	// its own.
+	std::vector<size_t> pord;
 	struct PrBin { int used, np, n1, n2, n3, mfma; std::vector<size_t> parts; };
 	auto pack = [&](int budget, size_t limit) {
 		std::vector<PrBin> bins;
@@ -695,32 +712,72 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 		}
 		return bins;
 	};
-	std::vector<PrBin> bins;
-	const size_t slots = (size_t)std::max(1, h.slots);
-	int64_t total = 0;
-	int lo = PR_WB;   // a budget below a chunk's need leaves that chunk alone in its unit
-	for (const PrPart& pt : parts) total += pt.need - PR_WB;
-	if (parts.size() > slots && total <= (int64_t)slots * (h.lds - PR_WB)) {
-		bins = pack(h.lds, slots);
-		if (!bins.empty()) {
-			// the smallest budget that still packs into the resident units
+	// the chunks of pord into units for 'limit' resident units: packed with the smallest budget that
+	// fits when they exceed it, one per unit when they already fit; empty when they cannot fit
+	auto fit = [&](size_t limit) {
+		std::vector<PrBin> gb;
+		int64_t total = 0;
+		int lo = PR_WB;   // a budget below a chunk's need leaves that chunk alone in its unit
+		for (size_t pi : pord) total += parts[pi].need - PR_WB;
+		if (pord.size() > limit) {
+			if (total > (int64_t)limit * (h.lds - PR_WB)) return gb;
+			gb = pack(h.lds, limit);
+			if (gb.empty()) return gb;
 			int hi = h.lds;
 			while (hi - lo > 32) {
 				const int mid = (lo + hi) / 2;
-				std::vector<PrBin> t = pack(mid, slots);
+				std::vector<PrBin> t = pack(mid, limit);
 				if (t.empty()) lo = mid;
-				else { hi = mid; bins = std::move(t); }
+				else { hi = mid; gb = std::move(t); }
 			}
+			return gb;
 		}
-	}
-	if (bins.empty())
 		for (size_t pi : pord) {
 			const PrPart& pt = parts[pi];
-			bins.push_back(PrBin{ pt.need, pt.np, pt.n1, pt.n2, pt.n3, pt.mfma, { pi } });
+			gb.push_back(PrBin{ pt.need, pt.np, pt.n1, pt.n2, pt.n3, pt.mfma, { pi } });
 		}
+		return gb;
+	};
+	auto by_need = [&]() { std::stable_sort(pord.begin(), pord.end(), [&](size_t i, size_t j) { return parts[i].need > parts[j].need; }); };
+	// one round with every unit on its sector's XCD when each XCD's chunks fit its share of the
+	// resident units; otherwise the chunks of all sectors together, packed to fit the device when
+	// that saves a round, or one per unit (several rounds: placement by sector would leave XCDs idle)
+	std::vector<PrBin> bins;
+	std::vector<int> bin_xcd;
+	const size_t slots_all = (size_t)std::max(1, h.slots);
+	const size_t slots_x = (size_t)std::max(1, h.slots / PR_XCDS);
+	bool placed = true;
+	for (int x = 0; x < PR_XCDS && placed; x++) {
+		pord.clear();
+		for (size_t i = 0; i < parts.size(); i++)
+			if (xcd_of_a[parts[i].A] == x) pord.push_back(i);
+		by_need();
+		std::vector<PrBin> gb = fit(slots_x);
+		if (gb.empty() && !pord.empty()) { placed = false; break; }
+		for (PrBin& b : gb) {
+			bins.push_back(std::move(b));
+			bin_xcd.push_back(x);
+		}
+	}
+	if (!placed) {
+		bins.clear();
+		bin_xcd.clear();
+		pord.resize(parts.size());
+		std::iota(pord.begin(), pord.end(), (size_t)0);
+		by_need();
+		bins = fit(slots_all);
+		if (bins.empty())
+			for (size_t pi : pord) {
+				const PrPart& pt = parts[pi];
+				bins.push_back(PrBin{ pt.need, pt.np, pt.n1, pt.n2, pt.n3, pt.mfma, { pi } });
+			}
+		bin_xcd.assign(bins.size(), -1);
+	}
 	// records: header | pieces | stage-1 tiles | stage-2 tiles | stage-3 tiles | row maps, every chunk
 	// moved up by the LDS of the chunks before it in its unit
-	for (PrBin& b : bins) {
+	for (size_t bi = 0; bi < bins.size(); bi++) {
+		PrBin& b = bins[bi];
+		h.xcd.push_back(bin_xcd[bi]);
 		std::vector<int> delta(b.parts.size());
 		for (size_t q = 0, at = 0; q < b.parts.size(); q++) { delta[q] = (int)at; at += (size_t)(parts[b.parts[q]].need - PR_WB); }
 		auto moved = [&](const std::vector<uint32_t>& v, int stride, std::initializer_list<int> words, int dl) {
@@ -1034,14 +1091,36 @@ void PairPlan::upload(const PairHost& h, const double* w, hipStream_t s)
 	release();
 	nunits = (int)h.recs.size();
 	rec_words = std::max(PR_HDR, (h.max_words + 3) / 4 * 4);
-	// units in descending cost order: the largest start first
-	std::vector<size_t> ord(h.recs.size());
-	std::iota(ord.begin(), ord.end(), (size_t)0);
-	std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return h.cost[a] > h.cost[b]; });
+	// units on their XCD (workgroup i runs on XCD i mod PR_XCDS) when the plan placed them: each XCD's
+	// units in descending cost order (the largest start first), the XCDs' queues interleaved
+	bool placed = h.xcd.size() == h.recs.size();
+	for (int x : h.xcd) placed = placed && x >= 0;
+	std::vector<std::vector<size_t>> q(placed ? PR_XCDS : 1);   // unplaced: one queue in cost order
+	for (size_t u = 0; u < h.recs.size(); u++) q[placed ? (size_t)h.xcd[u] : 0].push_back(u);
+	size_t longest = 0;
+	for (auto& v : q) {
+		std::stable_sort(v.begin(), v.end(), [&](size_t a, size_t b) { return h.cost[a] > h.cost[b]; });
+		longest = std::max(longest, v.size());
+	}
+	// a shorter queue is padded with empty units (no pieces or tiles; their alpha partial is zero),
+	// so that every unit keeps its XCD
+	const size_t EMPTY = (size_t)-1;
+	std::vector<size_t> ord;
+	ord.reserve(q.size() * longest);
+	for (size_t j = 0; j < longest; j++)
+		for (size_t x = 0; x < q.size(); x++) ord.push_back(j < q[x].size() ? q[x][j] : EMPTY);
+	while (!ord.empty() && ord.back() == EMPTY) ord.pop_back();
+	nunits = (int)ord.size();
 	std::vector<uint32_t> flat((size_t)nunits * (size_t)rec_words + PR_REC_PAD, 0u);
 	for (size_t u = 0; u < ord.size(); u++) {
+		uint32_t* dst = flat.data() + u * (size_t)rec_words;
+		if (ord[u] == EMPTY) {
+			// header only: no pieces, tiles or y blocks; every part offset right after the header
+			dst[10] = dst[11] = dst[12] = dst[13] = (uint32_t)PR_HDR;
+			continue;
+		}
 		const auto& r = h.recs[ord[u]];
-		std::copy(r.begin(), r.end(), flat.begin() + (ptrdiff_t)(u * (size_t)rec_words));
+		std::copy(r.begin(), r.end(), dst);
 	}
 	recs = pr_upload(flat, s);
 	red = pr_upload(h.red, s);

@@ -92,42 +92,34 @@ __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __res
 	const int i16 = lane & 15, kq = lane >> 4;
 	const uint32_t* __restrict__ g = recs + (size_t)blockIdx.x * (size_t)rec_words;
 
+	// lane j of wavefront w holds the descriptor of piece w + 4 j: loaded at the fixed offset PR_HDR
+	// before the header arrives (the record table is padded so that the NB slots of the last record
+	// stay in bounds; lanes past NB repeat slot NB - 1, slots past npiece are never used)
+	uint4 pd = reinterpret_cast<const uint4*>(g + PR_HDR)[wave + 4 * min(lane, NB - 1)];
+	__builtin_amdgcn_sched_barrier(0);
 	const int n1 = pr_s(g[0]), n3 = pr_s(g[1]), npiece = pr_s(g[2]), n2 = pr_s(g[3]);
 	const int off1 = pr_s(g[10]), off3 = pr_s(g[11]), off2 = pr_s(g[12]);
 
-	// lane j of wavefront w holds the descriptor of piece w + 4 j: loaded at the fixed offset PR_HDR
-	// without waiting for the header (the record table is padded so that the NB slots of the last
-	// record stay in bounds; slots past npiece are never used)
-	uint4 pd = { 0u, 0u, 0u, 0u };
-	if (lane < NB) pd = reinterpret_cast<const uint4*>(g + PR_HDR)[wave + 4 * lane];
-	// the wavefront's stage-1 tiles w + 4 j and stage-3 tiles w + 4 j, one per lane, in the same round trip
-	uint4 d1a = { 0u, 0u, 0u, 0u }, d1b = d1a, d2a = d1a, d2b = d1a, d3a = d1a, d3b = d1a, d3c = d1a, d3d = d1a;
-	if (wave + 4 * lane < n1) {
-		const uint4* d = reinterpret_cast<const uint4*>(g + off1 + PR_T1 * (wave + 4 * lane));
+	// the wavefront's stage-1, stage-2 and stage-3 tiles w + 4 j, one per lane; lanes past a table
+	// read the header instead (16 words, in bounds), so every lane issues every load: the wait for
+	// pd before the staging then counts exactly the eight loads issued after it, and these
+	// descriptors arrive while the staging loads are in flight
+	uint4 d1a, d1b, d2a, d2b, d3a, d3b, d3c, d3d;
+	{
+		const int tj = wave + 4 * lane;
+		const uint4* d = reinterpret_cast<const uint4*>(g + (tj < n1 ? off1 + PR_T1 * tj : 0));
 		d1a = d[0];
 		d1b = d[1];
-	}
-	if (wave + 4 * lane < n2) {
-		const uint4* d = reinterpret_cast<const uint4*>(g + off2 + PR_T2 * (wave + 4 * lane));
+		d = reinterpret_cast<const uint4*>(g + (tj < n2 ? off2 + PR_T2 * tj : 0));
 		d2a = d[0];
 		d2b = d[1];
-	}
-	if (wave + 4 * lane < n3) {
-		const uint4* d = reinterpret_cast<const uint4*>(g + off3 + PR_T3 * (wave + 4 * lane));
+		d = reinterpret_cast<const uint4*>(g + (tj < n3 ? off3 + PR_T3 * tj : 0));
 		d3a = d[0];
 		d3b = d[1];
 		d3c = d[2];
 		d3d = d[3];
 	}
 	pr_fresh(pd);
-	pr_fresh(d1a);
-	pr_fresh(d1b);
-	pr_fresh(d2a);
-	pr_fresh(d2b);
-	pr_fresh(d3a);
-	pr_fresh(d3b);
-	pr_fresh(d3c);
-	pr_fresh(d3d);
 
 	// staging: every load of the wavefront's pieces is issued before the first LDS store
 	if (tid < PR_ZERO) sm[tid] = 0.0;
@@ -160,6 +152,14 @@ __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __res
 		for (int i = 0; i < NB; i++)
 			if (wave + 4 * i < npiece) sm[dst[i]] = v[i];
 	}
+	pr_fresh(d1a);
+	pr_fresh(d1b);
+	pr_fresh(d2a);
+	pr_fresh(d2b);
+	pr_fresh(d3a);
+	pr_fresh(d3b);
+	pr_fresh(d3c);
+	pr_fresh(d3d);
 	__syncthreads();
 
 	// stage 1: T1 of each route in its own layout [(a p)][(n c)] = theta[(a p), b] R[b, (n c)]

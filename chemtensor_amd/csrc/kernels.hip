@@ -113,6 +113,14 @@ __device__ __forceinline__ void term_cache_start(TermCache& c, const GemmTerm* _
 	if (tk.t1 - tk.t0 > 1) term_cache_load(c, terms, tk.t0, tk.t1, lane);
 }
 
+// identity DPP moves: the cache words get registers written by a VALU op, so that later readlanes
+// wait for nothing (the wait for the cache load happens here, once)
+__device__ __forceinline__ void term_cache_settle(TermCache& c)
+{
+	#pragma unroll
+	for (int i = 0; i < 12; i++) c.w[i] = (uint32_t)__builtin_amdgcn_mov_dpp((int)c.w[i], 0xE4, 0xF, 0xF, false);
+}
+
 __device__ __forceinline__ void wave_tile_f64(const GemmTask& tk, const GemmTerm* __restrict__ terms, const GemmTerm& first, int m0, int n0, int lane, GemmBases bs)
 {
 	const int i16 = lane & 15, kq = lane >> 4;
@@ -697,10 +705,13 @@ __global__ __launch_bounds__(256) void k_fill(double* __restrict__ p, double v, 
 //
 // LDS-tiled form for large tasks: one workgroup per 64 x 64 output tile, each wavefront a 32 x 32
 // quadrant (2 x 2 MFMA 16x16x4 f64 sub-tiles). The k range of every term is cut into panels of 16;
-// the A panel (64 rows x 16) and the B panel (16 x 64 columns) of the next panel are loaded into
-// registers while the current panel's MFMAs run from LDS (double buffer, one barrier per panel).
-// Entries past a term's K or past the task's rows / columns are zeros. The k-steps run in the
-// order of the wavefront forms (terms in order, k ascending in steps of four).
+// the A panels (64 rows x 16) and B panels (16 x 64 columns) of the next two panels are in flight
+// in two register sets while the current panel's MFMAs run from LDS (LDS double buffer, one barrier
+// per panel). Every lane issues every panel load (entries past a term's K or past the task's rows /
+// columns read an in-bounds entry and are replaced by zeros; past the last panel the last one is
+// loaded again and dropped), so the wait for one register set counts exactly the eight loads of the
+// other. The k-steps run in the order of the wavefront forms (terms in order, k ascending in steps
+// of four).
 constexpr int LK = 16;                 // k-panel
 constexpr int LA_LD = LDS_TILE;        // A panel stored k-major: entry (row, k) at k LA_LD + row
 constexpr int LB_LD = LDS_TILE;        // B panel: entry (k, col) at k LB_LD + col
@@ -713,7 +724,9 @@ struct LdsPanelLoad
 __global__ __launch_bounds__(256) void k_grouped_gemm_f64_lds(const GemmTerm* __restrict__ terms, const WorkItem* __restrict__ items, GemmBases bs)
 {
 	// both panels k-major, unpadded (the MFMA fragments read 16 consecutive rows / columns per k):
-	// 32 KB, five workgroups per CU
+	// 32 KB; the two register sets of panels in flight (100 VGPRs) allow four workgroups per CU
+	// (five with one set in flight were slower; padding the panels to stride 80 against the read
+	// bank conflicts made no difference)
 	__shared__ double sa[2][LK * LA_LD];
 	__shared__ double sb[2][LK * LB_LD];
 	const int tid = threadIdx.x, lane = tid & 63;
@@ -728,9 +741,14 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64_lds(const GemmTerm* __
 	const bool va = arow < tk.M, vb = bcol < tk.N;
 	TermCache tc;
 	term_cache_start(tc, terms, tk, lane);
-	// the panel (t, k0) into registers
+	term_cache_settle(tc);
+	// the panel (t, k0) into registers: row / column 0 and k = 0 stand in for entries outside the
+	// term (every term present here has M, N, K >= 1)
 	auto load_panel = [&](int t, int k0, LdsPanelLoad& r) {
-		if (t - tc.base >= 64) term_cache_load(tc, terms, t, tk.t1, lane);
+		if (t - tc.base >= 64) {
+			term_cache_load(tc, terms, t, tk.t1, lane);
+			term_cache_settle(tc);
+		}
 		const TermF f = term_fields(tc, t, it.first, tk.t0);
 		const double* __restrict__ A = rebase(bs.A, f.A);
 		const double* __restrict__ B = rebase(bs.B, f.B);
@@ -739,12 +757,16 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64_lds(const GemmTerm* __
 		#pragma unroll
 		for (int i = 0; i < 4; i++) {
 			const int k = k0 + ka + i;
-			r.a[i] = (va && k < f.K) ? A[ao + (int64_t)k * f.sa_k] : 0.0;
+			const bool ok = va && k < f.K;
+			const double v = A[ao + (int64_t)(ok ? k : 0) * f.sa_k];
+			r.a[i] = ok ? v : 0.0;
 		}
 		#pragma unroll
 		for (int i = 0; i < 4; i++) {
 			const int k = k0 + kb + 4 * i;
-			r.b[i] = (vb && k < f.K) ? B[bo + (int64_t)k * f.sb_k] : 0.0;
+			const bool ok = vb && k < f.K;
+			const double v = B[bo + (int64_t)(ok ? k : 0) * f.sb_k];
+			r.b[i] = ok ? v : 0.0;
 		}
 	};
 	auto store_panel = [&](int buf, const LdsPanelLoad& r) {
@@ -757,12 +779,24 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64_lds(const GemmTerm* __
 	int t = tk.t0, k0 = 0;
 	// K of term t (the cache moved to cover t first)
 	auto term_k = [&](int tt) {
-		if (tt - tc.base >= 64) term_cache_load(tc, terms, tt, tk.t1, lane);
+		if (tt - tc.base >= 64) {
+			term_cache_load(tc, terms, tt, tk.t1, lane);
+			term_cache_settle(tc);
+		}
 		return term_K_at(tc, tt, it.first, tk.t0);
 	};
 	auto advance = [&]() {
 		k0 += LK;
 		while (t < tk.t1 && k0 >= term_k(t)) { t++; k0 = 0; }
+	};
+	// the next panel into r (or the last one again, dropped, once the panels are exhausted)
+	int tp = t, kp = 0;
+	auto issue = [&](LdsPanelLoad& r) {
+		const bool real = t < tk.t1;
+		if (real) { tp = t; kp = k0; }
+		load_panel(tp, kp, r);
+		if (real) advance();
+		return real;
 	};
 	// a term with K = 0 contributes nothing: skip to the first panel with entries
 	while (t < tk.t1 && term_k(t) <= 0) t++;
@@ -778,18 +812,7 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64_lds(const GemmTerm* __
 	const int nm = rows <= 0 ? 0 : rows <= 16 ? 1 : 2, nn = cols <= 0 ? 0 : cols <= 16 ? 1 : 2;
 	auto mainloop = [&](auto cm, auto cn) {
 		constexpr int NM = decltype(cm)::value, NN = decltype(cn)::value;
-		LdsPanelLoad r;
-		load_panel(t, k0, r);
-		advance();
-		store_panel(0, r);
-		__syncthreads();
-		int buf = 0;
-		while (true) {
-			const bool more = t < tk.t1;
-			if (more) {
-				load_panel(t, k0, r);
-				advance();
-			}
+		auto compute = [&](int buf) {
 			const double* __restrict__ pa = sa[buf];
 			const double* __restrict__ pb = sb[buf];
 			if constexpr (NM > 0 && NN > 0) {
@@ -812,8 +835,26 @@ __global__ __launch_bounds__(256) void k_grouped_gemm_f64_lds(const GemmTerm* __
 					}
 				}
 			}
-			if (!more) break;
-			store_panel(buf ^ 1, r);
+		};
+		LdsPanelLoad r0, r1;
+		issue(r0);
+		bool m1 = issue(r1);
+		store_panel(0, r0);
+		__syncthreads();
+		int buf = 0;
+		while (true) {
+			// LDS buf: panel p; r1: panel p + 1 (when m1); r0 receives panel p + 2
+			const bool m2 = issue(r0);
+			compute(buf);
+			if (!m1) break;
+			store_panel(buf ^ 1, r1);
+			__syncthreads();
+			buf ^= 1;
+			// LDS buf: panel p + 1; r0: panel p + 2 (when m2); r1 receives panel p + 3
+			m1 = issue(r1);
+			compute(buf);
+			if (!m2) break;
+			store_panel(buf ^ 1, r0);
 			__syncthreads();
 			buf ^= 1;
 		}

@@ -168,9 +168,22 @@ __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __res
 		const int nk = pr_lane(d1b.x, tl), vr = pr_lane(d1b.y, tl), vc = pr_lane(d1b.z, tl), sbase = pr_lane(d1b.w, tl);
 		const int ar = (i16 < vr ? abase + __mul24(i16, sth) : abase) + kq;
 		const int bc = (i16 < vc ? bbase + i16 : bbase) + __mul24(kq, Nr);
+		// the operands of k-step s + 1 are read before the MFMA of step s is issued
 		pr_double4 acc = { 0, 0, 0, 0 };
-		for (int s = 0; s < nk; s++)
-			acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sm[ar + 4 * s], sm[bc + __mul24(4 * s, Nr)], acc, 0, 0, 0);
+		if (nk > 0) {
+			const int bstep = 4 * Nr;
+			int ao = ar, bo = bc;
+			double av = sm[ao], bv = sm[bo];
+			for (int s = 1; s < nk; s++) {
+				ao += 4;
+				bo += bstep;
+				const double an = sm[ao], bn = sm[bo];
+				acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+				av = an;
+				bv = bn;
+			}
+			acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+		}
 		#pragma unroll
 		for (int r4 = 0; r4 < 4; r4++) {
 			const int i = kq + 4 * r4;
@@ -192,10 +205,22 @@ __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __res
 		const int a = vj ? pr_q(j2, mag2) : 0, c = vj ? j2 - a * tc2 : 0;
 		const int2* __restrict__ kmap = reinterpret_cast<const int2*>(sm + kbb) + kq;
 		const int wrow = wbase + __mul24(i16, wld) + kq;
+		// software pipelined: the row-map entry of k-step ks + 2 and the operands of step ks + 1 are
+		// read before the MFMA of step ks is issued (the entry index clamped to the last step)
 		pr_double4 acc = { 0, 0, 0, 0 };
-		for (int ks = 0; ks < nk4; ks++) {
-			const int2 e = kmap[4 * ks];
-			acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sm[wrow + 4 * ks], sm[e.x + __mul24(a, e.y) + c], acc, 0, 0, 0);
+		if (nk4 > 0) {
+			const int2 e0 = kmap[0];
+			double wv = sm[wrow], tv = sm[e0.x + __mul24(a, e0.y) + c];
+			int2 en = kmap[4 * min(1, nk4 - 1)];
+			for (int ks = 1; ks < nk4; ks++) {
+				const int2 e2 = kmap[4 * min(ks + 1, nk4 - 1)];
+				const double wn = sm[wrow + 4 * ks], tn = sm[en.x + __mul24(a, en.y) + c];
+				acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wv, tv, acc, 0, 0, 0);
+				wv = wn;
+				tv = tn;
+				en = e2;
+			}
+			acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wv, tv, acc, 0, 0, 0);
 		}
 		#pragma unroll
 		for (int r4 = 0; r4 < 4; r4++) sm[vj ? t2o + __mul24(kq + 4 * r4, S1) + j2 : PR_DUMP] = acc[r4];
@@ -220,16 +245,32 @@ __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __res
 		const int jofs = t2b + __mul24(q, S1) + c;
 		// k runs over (m, a): L row (a dM + m) of the staged [a m x] panel, T2 row (m dQ + q) column
 		// a tc + c; lanes past the chunk's rows a read the zero region
+		// the k-steps (m, s) in one loop, m outer, the operands of the next step read before the MFMA
+		// of the current one is issued
 		pr_double4 acc = { 0, 0, 0, 0 };
 		const int lstep = __mul24(4 * dM, dX), bstep = 4 * tc3;
-		for (int m = 0; m < dM; m++) {
-			int la = ax + __mul24(kq * dM + m, dX), lb = jofs + __mul24(m, dQS1) + __mul24(kq, tc3);
-			for (int s = 0; s < nka; s++) {
-				const bool vk = 4 * s + kq < ta;
-				acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sm[vk ? la : 0], sm[vk ? lb : 0], acc, 0, 0, 0);
+		const int nsteps = __mul24(dM, nka);
+		if (nsteps > 0) {
+			int s = 0, lam = ax + __mul24(kq * dM, dX), lbm = jofs + __mul24(kq, tc3), la = lam, lb = lbm;
+			double av = sm[kq < ta ? la : 0], bv = sm[kq < ta ? lb : 0];
+			for (int i = 1; i < nsteps; i++) {
+				s++;
 				la += lstep;
 				lb += bstep;
+				if (s == nka) {
+					s = 0;
+					lam += dX;
+					lbm += dQS1;
+					la = lam;
+					lb = lbm;
+				}
+				const bool vk = 4 * s + kq < ta;
+				const double an = sm[vk ? la : 0], bn = sm[vk ? lb : 0];
+				acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+				av = an;
+				bv = bn;
 			}
+			acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
 		}
 		if (vj) {
 			const int64_t o0 = pbase + (uint32_t)(__mul24(q, dC) + c);

@@ -119,6 +119,7 @@ __global__ __launch_bounds__(PR_THREADS) void k_pair_units(const uint32_t* __res
 		d3c = d[2];
 		d3d = d[3];
 	}
+	__builtin_amdgcn_sched_barrier(0);
 	pr_fresh(pd);
 
 	// staging: every load of the wavefront's pieces is issued before the first LDS store

@@ -1133,15 +1133,21 @@ void PairPlan::upload(const PairHost& h, const double* w, hipStream_t s)
 	release();
 	nunits = (int)h.recs.size();
 	rec_words = std::max(PR_HDR, (h.max_words + 3) / 4 * 4);
-	// units on their XCD (workgroup i runs on XCD i mod PR_XCDS) when the plan placed them: each XCD's
-	// units in descending cost order (the largest start first), the XCDs' queues interleaved
+	// units on their XCD (workgroup i runs on XCD i mod PR_XCDS) when the plan placed them, the XCDs'
+	// queues interleaved; an unplaced plan's units in descending cost order (the largest start first)
 	bool placed = h.xcd.size() == h.recs.size();
 	for (int x : h.xcd) placed = placed && x >= 0;
 	std::vector<std::vector<size_t>> q(placed ? PR_XCDS : 1);   // unplaced: one queue in cost order
 	for (size_t u = 0; u < h.recs.size(); u++) q[placed ? (size_t)h.xcd[u] : 0].push_back(u);
 	size_t longest = 0;
+	// placed units start in one round: an XCD dispatches consecutive workgroups of its queue to its
+	// CUs in turn, so the queue runs in passes of one unit per CU, descending and ascending in cost
+	// by turns, and every CU receives a similar sum of costs
+	const size_t cus = (size_t)std::max(1, num_compute_units() / PR_XCDS);
 	for (auto& v : q) {
 		std::stable_sort(v.begin(), v.end(), [&](size_t a, size_t b) { return h.cost[a] > h.cost[b]; });
+		if (placed)
+			for (size_t p0 = cus; p0 < v.size(); p0 += 2 * cus) std::reverse(v.begin() + (ptrdiff_t)p0, v.begin() + (ptrdiff_t)std::min(v.size(), p0 + cus));
 		longest = std::max(longest, v.size());
 	}
 	// a shorter queue is padded with empty units (no pieces or tiles; their alpha partial is zero),

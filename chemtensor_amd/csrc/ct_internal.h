@@ -405,7 +405,7 @@ struct PrRed { int64_t y, p; int32_t n, size, nslots, pad; };   // up to 64 entr
 struct PairHost
 {
 	std::vector<std::vector<uint32_t>> recs;   // one record per unit
-	std::vector<double> cost;                  // estimated cost per unit (MFMA count + staging)
+	std::vector<double> cost;                  // estimated time per unit (tiles of the busiest wavefront per stage, staging)
 	std::vector<int> xcd;                      // per unit: the XCD it is placed on
 	std::vector<int64_t> wd_idx;               // class W_D tables (R16 x K4 each) as W element indices, -1: zero
 	std::vector<PrRed> red;

@@ -23,6 +23,7 @@
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -706,8 +707,20 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 	// dealt to the XCDs by cost (largest first, to the least loaded), and the units of a group are
 	// placed on its XCD (PairPlan::upload), so the theta and L rows of a sector are fetched into one
 	// L2 rather than into every XCD's
+	// estimated unit time (µs, fitted to per-unit stage timings on MI355X): each stage lasts as long
+	// as the tiles of its busiest wavefront (tile t runs on wavefront t mod 4), the staging as the
+	// pieces per wavefront; stage 3 also grows with its k-steps
+	auto steps3 = [](const PrPart& pt) {
+		double st = 0.0;
+		for (size_t t = 0; t + PR_T3 <= pt.t3s.size(); t += PR_T3) st += (double)pt.t3s[t + 4] * (double)(pt.t3s[t + 5] >> 16);
+		return st;
+	};
+	auto unit_cost = [](double np, double n1, double n2, double n3, double st3) {
+		auto w = [](double n) { return std::ceil(n / 4.0); };
+		return 3.0 + 0.24 * w(np) + 0.32 * w(n1) + 0.59 * w(n2) + 0.26 * w(n3) + 0.065 * w(st3);
+	};
 	std::map<int32_t, double> cost_a;
-	for (const PrPart& pt : parts) cost_a[pt.A] += (double)pt.mfma + (double)pt.np / 4.0 + 4.0;
+	for (const PrPart& pt : parts) cost_a[pt.A] += 2.0 + 0.06 * pt.np + 0.08 * pt.n1 + 0.1475 * pt.n2 + 0.065 * pt.n3 + 0.016 * steps3(pt);
 	std::vector<std::pair<double, int32_t>> by_cost;
 	for (const auto& kv : cost_a) by_cost.emplace_back(kv.second, kv.first);
 	std::stable_sort(by_cost.begin(), by_cost.end(), [](const std::pair<double, int32_t>& x, const std::pair<double, int32_t>& y) { return x.first > y.first; });
@@ -878,7 +891,9 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 		rec[0] = (uint32_t)b.n1; rec[1] = (uint32_t)b.n3; rec[2] = (uint32_t)b.np; rec[3] = (uint32_t)b.n2;
 		h.max_words = std::max(h.max_words, (int)rec.size());
 		h.max_lds = std::max(h.max_lds, b.used);
-		h.cost.push_back((double)b.mfma + (double)b.np / 4.0 + 4.0);
+		double st3 = 0.0;
+		for (size_t pi : b.parts) st3 += steps3(parts[pi]);
+		h.cost.push_back(unit_cost(b.np, b.n1, b.n2, b.n3, st3));
 		h.recs.push_back(std::move(rec));
 	}
 	// partial slots: one per (y block, A, row chunk), grouped by y block in (A, row chunk) order;

@@ -909,6 +909,8 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 	});
 	std::vector<int64_t> slot_base((size_t)th.nblocks, -1);
 	std::vector<int> nslots((size_t)th.nblocks, 0);
+	// per y block and XCD: the stage-3 tiles that write its partial blocks from units on that XCD
+	std::vector<int> votes((size_t)th.nblocks * PR_XCDS, 0);
 	for (size_t k = 0; k < order.size();) {
 		const int64_t yb = refs[order[k]].yblock;
 		const int64_t size = th.block_size(yb);
@@ -920,6 +922,7 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 			const int ia0 = rs.ia;
 			for (; k < order.size() && refs[order[k]].yblock == yb && refs[order[k]].A == A0 && refs[order[k]].ia == ia0; k++) {
 				const PrSlotRef& ru = refs[order[k]];
+				if (ru.unit < h.xcd.size() && h.xcd[ru.unit] >= 0) votes[(size_t)yb * PR_XCDS + (size_t)h.xcd[ru.unit]]++;
 				uint32_t* w = &h.recs[ru.unit][(size_t)ru.word];
 				const int64_t pb = slot + (int64_t)((uint64_t)w[0] | ((uint64_t)w[1] << 32));
 				w[0] = (uint32_t)pb;
@@ -933,10 +936,16 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 			nslots[(size_t)yb]++;
 		}
 	}
-	// reduction items over every y block in serialise order (blocks without slots are zeroed)
+	// reduction items over every y block in serialise order (blocks without slots are zeroed), each
+	// queued for the XCD whose units wrote most of its partial blocks (still in that XCD's L2); the
+	// four items of workgroup b (it runs on XCD b mod PR_XCDS) come from that XCD's queue while it
+	// lasts, else from the longest other queue
+	std::vector<std::vector<PrRed>> rq(PR_XCDS);
 	for (ct_long k = 0; k < th.nblocks; k++) {
 		if (!th.present(k)) continue;
 		const int64_t size = th.block_size(k);
+		const int* vk = &votes[(size_t)k * PR_XCDS];
+		const size_t x = (size_t)(std::max_element(vk, vk + PR_XCDS) - vk);
 		for (int64_t e0 = 0; e0 < size; e0 += 64) {
 			PrRed it;
 			it.y = x_off + th.offset[k] + e0;
@@ -945,9 +954,24 @@ bool plan_pair_tables(PairHost& h, const Meta& th, const Meta& W, const Meta& L,
 			it.size = (int32_t)size;
 			it.nslots = nslots[(size_t)k];
 			it.pad = 0;
-			h.red.push_back(it);
+			rq[x].push_back(it);
 		}
 	}
+	std::vector<size_t> head(PR_XCDS, 0);
+	size_t left = 0;
+	for (auto& v : rq) left += v.size();
+	for (size_t b = 0; left > 0; b++)
+		for (int i = 0; i < 4 && left > 0; i++) {
+			size_t x = b % PR_XCDS;
+			if (head[x] == rq[x].size()) {
+				size_t best = 0;
+				for (size_t y = 1; y < PR_XCDS; y++)
+					if (rq[y].size() - head[y] > rq[best].size() - head[best]) best = y;
+				x = best;
+			}
+			h.red.push_back(rq[x][head[x]++]);
+			left--;
+		}
 	return true;
 }
 

@@ -40,10 +40,22 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
 // the sums of two partial arrays (each as sum_partials would give it), their loads in one round trip
 __device__ __forceinline__ void sum_partials2(const double* __restrict__ pa, int Ga, const double* __restrict__ pb, int Gb, double* sh, double& ra, double& rb)
 {
+	// each thread sums entries threadIdx.x + k LZ_THREADS in k order, four loads in flight at a time
+	// (entries past the ends add zeros, which leave the sums unchanged)
 	double a = 0, b = 0;
-	for (int i = threadIdx.x; i < Ga || i < Gb; i += LZ_THREADS) {
-		if (i < Ga) a += pa[i];
-		if (i < Gb) b += pb[i];
+	for (int i0 = threadIdx.x; i0 < Ga || i0 < Gb; i0 += 4 * LZ_THREADS) {
+		double va[4], vb[4];
+		#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int i = i0 + u * LZ_THREADS;
+			va[u] = i < Ga ? pa[i] : 0.0;
+			vb[u] = i < Gb ? pb[i] : 0.0;
+		}
+		#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			a += va[u];
+			b += vb[u];
+		}
 	}
 	#pragma unroll
 	for (int off = 32; off > 0; off >>= 1) {

@@ -381,11 +381,23 @@ __global__ __launch_bounds__(256) void k_pair_lz_step(const PrRed* __restrict__ 
 		}
 	}
 	// beta^2 and <u_j, y>: per-thread strided sums, wavefront butterflies, then the four wavefronts
+	// (thread tid sums entries tid + 256 k in k order, four loads in flight at a time; entries past
+	// the ends add zeros, which leave the sums unchanged)
 	double a = 0.0, b = 0.0;
 	const int Ga = j > 0 ? Gn : 0;
-	for (int k = tid; k < Ga || k < Gd; k += 256) {
-		if (k < Ga) a += npart[k];
-		if (k < Gd) b += dpart[k];
+	for (int k0 = tid; k0 < Ga || k0 < Gd; k0 += 1024) {
+		double va[4], vb[4];
+		#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int k = k0 + 256 * u;
+			va[u] = k < Ga ? npart[k] : 0.0;
+			vb[u] = k < Gd ? dpart[k] : 0.0;
+		}
+		#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			a += va[u];
+			b += vb[u];
+		}
 	}
 	#pragma unroll
 	for (int off = 32; off > 0; off >>= 1) {
